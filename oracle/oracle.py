@@ -1,0 +1,299 @@
+"""Python face of the CPU FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this module, and only as the checker.  The product path
+(``mipt_amd``) never imports it.
+
+Evolution, measurement and the random streams come from ``liboracle.so``
+(oracle/mipt_oracle.cpp, complex128).  The dense linear algebra of the
+entropy side (RDM of a kept set, Hermitian eigenvalues, log2 entropies, TMI)
+is restated here with numpy, following (paths relative to the reference
+repository root):
+
+* cpp/include/mipt/tmi/linalg.hpp:885-1026  basis offsets / Schmidt gather
+* cpp/include/mipt/tmi/linalg.hpp:125-141   entropy cut-offs (lambda<=1e-15, |S|<1e-12)
+* cpp/include/mipt/tmi/compute.hpp:64-113, 186-227  TMI regions and formula
+* cpp/include/mipt/free_energy.hpp:274-287  S(L/2): kept = {0 .. n/2-1}
+* cpp/include/mipt/ancilla_mi.hpp:24-52     probe entropy (nats)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and _ref/libref.so when the reference is present)."""
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_splitmix64.restype = ctypes.c_uint64
+        L.orc_splitmix64.argtypes = [ctypes.c_uint64]
+        L.orc_trajectory_seed.restype = ctypes.c_uint64
+        L.orc_trajectory_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_linspace.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_layer_bonds.restype = ctypes.c_int
+        L.orc_layer_bonds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip]
+        L.orc_total_bonds.restype = ctypes.c_int
+        L.orc_total_bonds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        for name in ("orc_mt_build_layers", "orc_philox_build_layers"):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint64, _dp, _u8p, _dp]
+        L.orc_apply_dense.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, _dp]
+        L.orc_measure_collapse.restype = ctypes.c_int
+        L.orc_measure_collapse.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _dp]
+        L.orc_measure_layer_joint.restype = ctypes.c_double
+        L.orc_measure_layer_joint.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip]
+        L.orc_run_layers.restype = ctypes.c_int
+        L.orc_run_layers.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _u8p, _dp,
+                                     ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.orc_rdm.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, _dp]
+        L.orc_jacobi_eigvalsh.restype = ctypes.c_int
+        L.orc_jacobi_eigvalsh.argtypes = [_dp, ctypes.c_int, _dp]
+        L.orc_entropy_bits_from_eigs.restype = ctypes.c_double
+        L.orc_entropy_bits_from_eigs.argtypes = [_dp, ctypes.c_int]
+        L.orc_rho1.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_rho1_entropy_nats.restype = ctypes.c_double
+        L.orc_rho1_entropy_nats.argtypes = [_dp]
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The reference's own host code compiled into oracle/_ref (None if absent)."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_measure_collapse_host.restype = ctypes.c_int
+        R.ref_measure_collapse_host.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp]
+        R.ref_entropy_subsystem_rdm.restype = ctypes.c_double
+        R.ref_entropy_subsystem_rdm.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int]
+        R.ref_entropy_hermitian.restype = ctypes.c_double
+        R.ref_entropy_hermitian.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        R.ref_entropy_from_eigenvalue.restype = ctypes.c_double
+        R.ref_entropy_from_eigenvalue.argtypes = [ctypes.c_double]
+        R.ref_finish_entropy.restype = ctypes.c_double
+        R.ref_finish_entropy.argtypes = [ctypes.c_double]
+        R.ref_hermitian_eigenvalues.restype = ctypes.c_int
+        R.ref_hermitian_eigenvalues.argtypes = [_dp, ctypes.c_int, _dp]
+        R.ref_splitmix64.restype = ctypes.c_uint64
+        R.ref_splitmix64.argtypes = [ctypes.c_uint64]
+        R.ref_trajectory_seed.restype = ctypes.c_uint64
+        R.ref_trajectory_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        R.ref_linspace.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        R.ref_tensor_product.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp]
+        _REF = R
+    return _REF
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+# ----------------------------------------------------------------- streams
+
+def layer_bonds(n: int, layer: int, closed: bool = True):
+    q0 = np.zeros(n, np.int32)
+    q1 = np.zeros(n, np.int32)
+    nb = lib().orc_layer_bonds(n, layer, int(closed), _i(q0), _i(q1))
+    return [(int(q0[b]), int(q1[b])) for b in range(nb)]
+
+
+def total_bonds(n: int, layers: int, closed: bool = True) -> int:
+    return lib().orc_total_bonds(n, layers, int(closed))
+
+
+def build_layers(n: int, layers: int, p: float, seed: int, closed: bool = True, stream: str = "philox"):
+    """Gates (complex128 [bonds,4,4]), flags (uint8 [layers,n]), uniforms ([layers,n]).
+
+    stream "philox": the counter-based stream the batched engine draws on device.
+    stream "mt": the reference-compatible mt19937 host stream (circuit.hpp:129-140).
+    """
+    nb = total_bonds(n, layers, closed)
+    gates = np.zeros((nb, 4, 4), np.complex128)
+    flags = np.zeros((layers, n), np.uint8)
+    unif = np.zeros((layers, n), np.float64)
+    f = lib().orc_philox_build_layers if stream == "philox" else lib().orc_mt_build_layers
+    f(n, layers, float(p), int(closed), ctypes.c_uint64(seed & (2**64 - 1)),
+      gates.ctypes.data_as(_dp), flags.ctypes.data_as(_u8p), _d(unif))
+    return gates, flags, unif
+
+
+def philox4x32_10(key, ctr):
+    k = np.asarray(key, np.uint32)
+    c = np.asarray(ctr, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(k.ctypes.data_as(_u32p), c.ctypes.data_as(_u32p), out.ctypes.data_as(_u32p))
+    return [int(x) for x in out]
+
+
+# ----------------------------------------------------------------- evolution
+
+def zero_state(n: int) -> np.ndarray:
+    psi = np.zeros(1 << n, np.complex128)
+    psi[0] = 1.0
+    return psi
+
+
+def apply_dense(psi: np.ndarray, n: int, targets, m: np.ndarray) -> None:
+    t = np.asarray(targets, np.int32)
+    mm = np.ascontiguousarray(m, np.complex128)
+    lib().orc_apply_dense(psi.ctypes.data_as(_dp), n, _i(t), len(t), mm.ctypes.data_as(_dp))
+
+
+def measure_collapse(psi: np.ndarray, n: int, q: int, u: float):
+    o = ctypes.c_int(0)
+    pr = ctypes.c_double(0.0)
+    st = lib().orc_measure_collapse(psi.ctypes.data_as(_dp), n, q, float(u), ctypes.byref(o), ctypes.byref(pr))
+    return st, o.value, pr.value
+
+
+def run_layers(psi: np.ndarray, n: int, gates, flags, unif, layer_begin: int = 0, layer_end: int | None = None,
+               closed: bool = True, mode: int = 0, chunk_bits: int = 16):
+    """Evolve psi in place; returns (outcomes[layers,n] (-1 unmeasured), p1[layers,n])."""
+    layers = flags.shape[0]
+    if layer_end is None:
+        layer_end = layers
+    outcomes = np.full((layers, n), -1, np.int32)
+    p1 = np.full((layers, n), -1.0, np.float64)
+    g = np.ascontiguousarray(gates, np.complex128)
+    fl = np.ascontiguousarray(flags, np.uint8)
+    un = np.ascontiguousarray(unif, np.float64)
+    st = lib().orc_run_layers(psi.ctypes.data_as(_dp), n, layer_begin, layer_end, int(closed), g.ctypes.data_as(_dp),
+                              fl.ctypes.data_as(_u8p), _d(un), mode, chunk_bits, _i(outcomes), _d(p1))
+    if st != 0:
+        raise RuntimeError(f"oracle run_layers status {st}")
+    return outcomes, p1
+
+
+def simulate(n: int, layers: int, p: float, seed: int, closed: bool = True, stream: str = "philox", mode: int = 0):
+    gates, flags, unif = build_layers(n, layers, p, seed, closed, stream)
+    psi = zero_state(n)
+    outcomes, p1 = run_layers(psi, n, gates, flags, unif, closed=closed, mode=mode)
+    return psi, outcomes, p1, (gates, flags, unif)
+
+
+# ----------------------------------------------------------------- entropies
+
+LOG2_EPS = 1.0e-15
+
+
+def schmidt_matrix(psi: np.ndarray, n: int, kept) -> np.ndarray:
+    """M[r, c] = psi[env_off[c] | row_off[r]]: bit j of r is kept[j], env ascending."""
+    kept = list(kept)
+    env = [q for q in range(n) if q not in kept]
+    t = psi.reshape([2] * n)  # axis a <-> qubit n-1-a
+    axes = [n - 1 - q for q in reversed(kept)] + [n - 1 - q for q in reversed(env)]
+    return np.ascontiguousarray(np.transpose(t, axes)).reshape(1 << len(kept), 1 << len(env))
+
+
+def rdm(psi: np.ndarray, n: int, kept) -> np.ndarray:
+    m = schmidt_matrix(psi, n, kept)
+    return m @ m.conj().T
+
+
+def entropy_from_eigs(w) -> float:
+    w = np.asarray(w, np.float64)
+    w = w[w > LOG2_EPS]
+    s = float(-(w * np.log2(w)).sum())
+    return 0.0 if abs(s) < 1.0e-12 else s
+
+
+def renyi2_from_purity(purity: float) -> float:
+    if not purity > 0.0 or purity >= 1.0:
+        return 0.0
+    s = -np.log2(purity)
+    return 0.0 if abs(s) < 1.0e-12 else float(s)
+
+
+def entropy(psi: np.ndarray, n: int, kept, order: int = 1) -> float:
+    if len(kept) == 0 or len(kept) == n:
+        return 0.0
+    rho = rdm(psi, n, kept)
+    if order == 2:
+        return renyi2_from_purity(float((np.abs(rho) ** 2).sum()))
+    rho = 0.5 * (rho + rho.conj().T)
+    return entropy_from_eigs(np.linalg.eigvalsh(rho))
+
+
+def cyclic_range(begin: int, count: int, n: int):
+    return [(begin + i) % n for i in range(count)]
+
+
+def tmi_regions(n: int, block: int, cycle: int = 0):
+    """A, B, C, D (compute.hpp:89-113); D is the rest of the ring."""
+    A = cyclic_range(cycle, block, n)
+    B = cyclic_range(cycle + block, block, n)
+    C = cyclic_range(cycle + 2 * block, block, n)
+    used = set(A + B + C)
+    D = [(cycle + 3 * block + i) % n for i in range(n - 3 * block)]
+    assert not used.intersection(D)
+    return A, B, C, D
+
+
+def tmi(psi: np.ndarray, n: int, block: int | None = None, cycle: int = 0, order: int = 1) -> float:
+    """I3 = S_A+S_B+S_C+S_D-S_AB-S_AC-S_BC in bits (compute.hpp:186-227)."""
+    if block is None:
+        block = n // 4
+    A, B, C, D = tmi_regions(n, block, cycle)
+    s = lambda k: entropy(psi, n, k, order)
+    return s(A) + s(B) + s(C) + s(D) - s(A + B) - s(A + C) - s(B + C)
+
+
+def half_chain_entropy(psi: np.ndarray, n: int, order: int = 1) -> float:
+    return entropy(psi, n, list(range(n // 2)), order)
+
+
+def rho1(psi: np.ndarray, nq: int, qubit: int):
+    out = np.zeros(4, np.float64)
+    lib().orc_rho1(psi.ctypes.data_as(_dp), nq, qubit, _d(out))
+    return out
+
+
+def rho1_entropy_nats(r) -> float:
+    r = np.ascontiguousarray(r, np.float64)
+    return lib().orc_rho1_entropy_nats(_d(r))
+
+
+def bloch_from_rho1(r):
+    tr = r[0] + r[1]
+    return np.array([2 * r[2] / tr, -2 * r[3] / tr, (r[0] - r[1]) / tr])
+
+
+def jacobi_eigvalsh(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.complex128)
+    n = a.shape[0]
+    w = np.zeros(n)
+    st = lib().orc_jacobi_eigvalsh(a.ctypes.data_as(_dp), n, _d(w))
+    if st != 0:
+        raise RuntimeError("oracle Jacobi did not converge")
+    return w
