@@ -1,0 +1,201 @@
+"""Benchmark: circuit layers/s of the batched MIPT engine at L=20 x 4096
+trajectories (16 p values in [0, 0.3] x 256, seeds 1..4096, T=40 layers,
+closed ring, Haar U(4), complex64 state), next to the plain
+one-trajectory-per-launch GPU path and the FP64 CPU oracle, with S(L/2)/TMI
+errors against the oracle on a replayed subset.
+
+A step = one full run of the batch: draw every gate, flag and uniform on
+device, then evolve all 4096 trajectories through all 40 layers (gate sweeps,
+Born probabilities, collapse and renormalisation), states resident in HBM.
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "circuit layers/s at L=20 x 4096 traj, batched vs plain; S/TMI err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def hip_runtime():
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    return h
+
+
+def linspace(first, last, count):
+    # cpp/include/mipt/types.hpp:23-38
+    if count == 1:
+        return [first]
+    step = (last - first) / (count - 1)
+    v = [first + step * i for i in range(count)]
+    v[-1] = last
+    return v
+
+
+def _oracle_traj(args):
+    """CPU FP64 oracle run of one trajectory of the workload (cpu_baseline leg)."""
+    L, layers, p, seed, want_obs = args
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    psi, out, p1, (g, f, u) = O.simulate(L, layers, p, seed, True, "philox", 0)
+    dt = time.perf_counter() - t0
+    obs = None
+    if want_obs:
+        obs = (O.half_chain_entropy(psi, L), O.tmi(psi, L))
+    return dt, psi.astype(np.complex128), out, p1, u, f, obs
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=20)
+    ap.add_argument("--layers", type=int, default=40)
+    ap.add_argument("--per-p", type=int, default=256)
+    ap.add_argument("--np", type=int, default=16)
+    ap.add_argument("--plain", type=int, default=4, help="trajectories timed on the plain path")
+    ap.add_argument("--replay", type=int, default=8, help="trajectories checked against the FP64 oracle")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg (profiling runs)")
+    ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--profile-every", type=int, default=16)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; trajectories shard across GPUs without "
+              "communication until a final per-p merge, which is not built here", file=sys.stderr)
+        return 2
+
+    import mipt_amd as M
+
+    L, layers = args.L, args.layers
+    ps = linspace(0.0, 0.3, args.np)
+    specs = []
+    for pi, p in enumerate(ps):
+        for r in range(args.per_p):
+            specs.append(M.Spec(L, layers, p, len(specs) + 1, True))
+    ntraj = len(specs)
+    b = M.Batch(specs, tile_bits=args.tile_bits)
+    hip = hip_runtime()
+    stream = ctypes.c_void_p(b.stream)
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+
+    def step():
+        b.reset()
+        b.generate()
+        b.evolve()
+
+    for _ in range(args.warmup):
+        step()
+    b.synchronize()
+    b.set_profiling(args.profile_every)
+    b.synchronize()
+    hip.hipEventRecord(e0, stream)
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    hip.hipEventRecord(e1, stream)
+    hip.hipEventSynchronize(e1)
+    b.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    ms_per_step = ms.value / args.steps
+    layer_trajs = ntraj * layers
+    value = layer_trajs / (ms_per_step / 1e3)
+    st = b.pass_stats()
+    achieved = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
+
+    # plain one-trajectory-per-launch path on the same trajectories' device draws
+    plain_lps = None
+    if args.plain > 0:
+        eng = M.PlainEngine(False)
+        idx = [int(i * ntraj / args.plain) for i in range(args.plain)]
+        data = [b.export_layers(i) for i in idx]
+        M.run_plain_trajectory(eng, L, *data[0], True, 4, 0, 10)  # warm
+        t = time.perf_counter()
+        for g, f, u in data:
+            M.run_plain_trajectory(eng, L, g, f, u, True, 4, 0, 10)
+        dt = time.perf_counter() - t
+        plain_lps = len(data) * layers / dt
+        eng.close()
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "layers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (device Philox draws; seeds 1..%d)" % ntraj,
+        "config": {"workload": f"batched p-sweep: L={L} ring, T={layers}, {args.np} p in [0,0.3] x {args.per_p} "
+                               f"= {ntraj} trajectories, Haar U(4), complex64 state",
+                   "L": L, "layers": layers, "trajectories": ntraj, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "pass_kernel (gate sweep)", "timed_launches": st["timed"],
+                     "avg_launch_ms": round(st["avg_ms"], 4), "algorithmic_bytes_per_launch": st["bytes_avg"]},
+        "plain_layers_per_s": None if plain_lps is None else round(plain_lps, 1),
+        "batched_vs_plain": None if plain_lps is None else round(value / plain_lps, 2),
+    }
+
+    # ---- oracle leg: CPU baseline + S/TMI/outcome parity on a replayed subset
+    if not args.no_cpu and args.replay > 0:
+        idx = [int(i * ntraj / args.replay) for i in range(args.replay)]
+        cores = max(1, min(len(idx), len(os.sched_getaffinity(0)), 16))
+        jobs = [(L, layers, specs[i].p, specs[i].seed, True) for i in idx]
+        t = time.perf_counter()
+        with mp.get_context("spawn").Pool(cores) as pool:
+            res = pool.map(_oracle_traj, jobs)
+        wall = time.perf_counter() - t
+        evo = sum(r[0] for r in res)  # evolution seconds summed over processes
+        cpu_lps = len(idx) * layers / (evo / cores)
+        s_err = tmi_err = amp_err = 0.0
+        mism = flagged_n = 0
+        for i, (dt, psi, out, p1, u, f, obs) in zip(idx, res):
+            got, flg, _, _ = b.outcomes(i)
+            near = np.abs(u - p1) < 1e-6
+            mism += int(np.sum((got != out) & (f == 1) & ~near))
+            flagged_n += int(np.sum(flg))
+            ob = b.observables(i)
+            s_err = max(s_err, abs(ob["s_half"] - obs[0]))
+            tmi_err = max(tmi_err, abs(ob["tmi"] - obs[1]))
+            if not flg.any():
+                amp_err = max(amp_err, float(np.abs(b.get_state(i).astype(np.complex128) - psi).max()))
+        result["cpu_baseline"] = {"value": round(cpu_lps, 2), "unit": "layers/s", "cores": cores, "kind": "port",
+                                  "sample": f"{len(idx)} trajectories of the workload (L={L}, T={layers}) evolved "
+                                            f"in complex128 by the oracle, one process per trajectory"}
+        result["parity"] = {"replayed": len(idx), "outcome_mismatches": mism, "flagged_draws": flagged_n,
+                            "s_half_err_max": s_err, "tmi_err_max": tmi_err, "amp_err_max": amp_err}
+        result["batched_vs_cpu"] = round(value / cpu_lps, 1)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,198 @@
+/*
+ * libmipt_hip: MI355X (gfx950) qubit MIPT trajectory engine, C ABI.
+ *
+ * Conventions shared by every entry point:
+ *   - Return 0 on success.  Negative values are argument/numeric errors
+ *     (-1 invalid argument, -2 invalid branch norms or zero state,
+ *      -3 invalid conditional probability, -4 invalid branch norm,
+ *      -5 unsupported request, -6 out of memory); positive values are
+ *     hipError_t codes, 100000+status rocSOLVER/rocBLAS failures and
+ *     102000+info an FP64 eigensolve that did not converge.
+ *     mipt_hip_last_error() describes the last failure of the calling thread.
+ *   - State vectors are device pointers to interleaved complex numbers
+ *     (complex64 for *_f32, complex128 for *_f64); qubit q is index bit q.
+ *   - Host buffers are caller-owned.  Calls are synchronous unless stated.
+ *   - Entropies of TMI/half-chain subsystems are in bits; probe entropies in
+ *     nats (the units of the reference).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   cpp/include/mipt/cuda/tmi_cuda_svd.hpp:20-60     entropy / Renyi-2 C ABI
+ *   cpp/include/mipt/cuda/tmi_cuda_rdm.hpp:20-33     subsystem RDM export
+ *   cpp/include/mipt/cuda/free_energy_measure.hpp:13-29  sequential measurement
+ *   cpp/include/mipt/cuda/ancilla_rdm.hpp:19-29      probe rho1
+ *   cpp/include/mipt/cusv/engine.hpp:100-582         per-trajectory engine
+ *   cpp/include/mipt/circuit.hpp:240-282, 409-443    Haar trajectory loop
+ */
+#ifndef MIPT_HIP_H
+#define MIPT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *mipt_hip_last_error(void);
+/* Library version string. */
+const char *mipt_hip_version(void);
+
+/* ------------------------------------------------------------------ drop-ins */
+
+/* Replaces mipt_cuda_measure_collapse_f32/_f64
+ * (cpp/include/mipt/cuda/free_energy_measure.hpp:13-27): P(1) from FP64 branch
+ * norms of the whole state, outcome = 1 iff uniform_01 < P(1), collapse by
+ * 1/sqrt(raw branch norm).  realized_probability = conditional P of the outcome. */
+int mipt_hip_measure_collapse_f32(void *device_state_vector, int n_qubits, int measured_qubit, double uniform_01,
+                                  int *outcome, double *realized_probability);
+int mipt_hip_measure_collapse_f64(void *device_state_vector, int n_qubits, int measured_qubit, double uniform_01,
+                                  int *outcome, double *realized_probability);
+/* Replaces mipt_cuda_measure_collapse_last_error (free_energy_measure.hpp:29). */
+const char *mipt_hip_measure_collapse_last_error(void);
+
+/* Replace mipt_cuda_entropy_svd_f32/_f64 (cpp/include/mipt/cuda/tmi_cuda_svd.hpp:20-34):
+ * von Neumann entropy (bits) of the kept modes (little-endian order) of a pure
+ * device state.  RDM by an FP32-MFMA Gram for _f32, FP64 for _f64; FP64
+ * Hermitian eigenvalues in both.  fermion_trace must be 0 (qubit trace). */
+int mipt_hip_entropy_svd_f32(const void *device_state_vector, int n_qubits, const int *host_kept_modes,
+                             int kept_count, int fermion_trace, double *entropy_out);
+int mipt_hip_entropy_svd_f64(const void *device_state_vector, int n_qubits, const int *host_kept_modes,
+                             int kept_count, int fermion_trace, double *entropy_out);
+
+/* Replace mipt_cuda_renyi2_f32/_f64 (tmi_cuda_svd.hpp:36-58): S_2 = -log2 Tr rho^2 (bits). */
+int mipt_hip_renyi2_f32(const void *device_state_vector, int n_qubits, const int *host_kept_modes, int kept_count,
+                        int fermion_trace, double *entropy_out);
+int mipt_hip_renyi2_f64(const void *device_state_vector, int n_qubits, const int *host_kept_modes, int kept_count,
+                        int fermion_trace, double *entropy_out);
+
+/* Replace mipt_cuda_rdm_subsystem_f32/_f64 (cpp/include/mipt/cuda/tmi_cuda_rdm.hpp:20-33):
+ * rho[2*(r*D+c)+{0,1}] = Re/Im rho[r,c], D = 2^kept_count, kept_count <= 12. */
+int mipt_hip_rdm_subsystem_f32(const void *device_state_vector, int n_qubits, const int *host_kept_modes,
+                               int kept_count, int fermion_trace, double *host_rho_ri_row_major);
+int mipt_hip_rdm_subsystem_f64(const void *device_state_vector, int n_qubits, const int *host_kept_modes,
+                               int kept_count, int fermion_trace, double *host_rho_ri_row_major);
+
+/* Replace mipt_cuda_rho1_complex_f32/_f64 (cpp/include/mipt/cuda/ancilla_rdm.hpp:19-29):
+ * host_rho_ri = {rho00, rho11, Re rho01, Im rho01}, FP64 accumulation. */
+int mipt_hip_rho1_complex_f32(const void *device_state_vector, int n_qubits, int retained_qubit,
+                              double *host_rho_ri);
+int mipt_hip_rho1_complex_f64(const void *device_state_vector, int n_qubits, int retained_qubit,
+                              double *host_rho_ri);
+
+/* TMI I3 = S_A+S_B+S_C+S_D-S_AB-S_AC-S_BC (bits) of cyclic blocks of `block`
+ * sites starting at `cycle` (cpp/include/mipt/tmi/compute.hpp:89-113, 186-227);
+ * D is the rest of the ring.  S(L/2) of kept {0..n/2-1}
+ * (cpp/include/mipt/free_energy.hpp:274-287) is the entropy entry point above. */
+int mipt_hip_tmi_f32(const void *device_state_vector, int n_qubits, int chain_sites, int block, int cycle,
+                     double *tmi_out, double *terms_out /* 7: A,B,C,D,AB,AC,BC; may be NULL */);
+
+/* ---------------------------------------------------------- plain engine */
+/* One trajectory at a time, one launch per blocked op, a host round trip per
+ * measured layer: the semantics and structure of cusv::Engine
+ * (cpp/include/mipt/cusv/engine.hpp:100-582). */
+typedef struct mipt_hip_engine mipt_hip_engine;
+
+/* stream may be NULL (the null stream). */
+int mipt_hip_engine_create(int use_fp64, void *hip_stream, mipt_hip_engine **out);
+int mipt_hip_engine_destroy(mipt_hip_engine *engine);
+/* Operate on a caller-owned device buffer (engine.hpp:131-140). */
+int mipt_hip_engine_adopt(mipt_hip_engine *engine, void *device_state_vector, int n_qubits);
+/* Engine-owned |0...0> of n qubits (engine.hpp:143-160). */
+int mipt_hip_engine_reset(mipt_hip_engine *engine, int n_qubits);
+void *mipt_hip_engine_data(mipt_hip_engine *engine);
+/* Dense 2^k x 2^k row-major matrix (interleaved complex128, narrowed to the
+ * state precision) on k <= 5 targets; bit j of the matrix index = targets[j]
+ * (engine.hpp:236-277). */
+int mipt_hip_engine_apply_matrix(mipt_hip_engine *engine, const int *targets, int k, const double *matrix_ri);
+/* A layer of two-qubit gates (q0 = bit 0 of each 4x4) fused into Kronecker
+ * blocks of at most block_targets qubits (cpp/include/mipt/cusv/gates.hpp:295-333). */
+int mipt_hip_engine_apply_bonds(mipt_hip_engine *engine, const int *q0, const int *q1, const double *gates_ri,
+                                int count, int block_targets);
+/* Measures `sites` (ascending) and collapses.  mode 0: sequential per-site
+ * draws uniforms[j] (outcome 1 iff u < P(1)); mode 1: joint draw per chunk of
+ * chunk_bits sites with uniforms[chunk] (engine.hpp:293-368).  p1 (mode 0,
+ * may be NULL) receives each conditional P(1); surprisal = -log P(outcomes). */
+int mipt_hip_engine_measure_layer(mipt_hip_engine *engine, const int *sites, int count, const double *uniforms,
+                                  int mode, int chunk_bits, int *outcomes, double *p1, double *surprisal);
+int mipt_hip_engine_to_host(mipt_hip_engine *engine, void *host_buffer);
+int mipt_hip_engine_set_from_host(mipt_hip_engine *engine, const void *host_buffer);
+int mipt_hip_engine_squared_norm(mipt_hip_engine *engine, double *out);
+
+/* ---------------------------------------------------------- batched engine */
+/* Thousands of trajectories (mixed chain lengths) evolved on device-resident
+ * states with fused tile sweeps; no host round trip inside the evolution. */
+typedef struct mipt_hip_batch mipt_hip_batch;
+
+typedef struct mipt_hip_traj_spec
+{
+    int L;              /* chain sites (gates act on 0..L-1) */
+    int layers;         /* brickwork layers (2 x periods) */
+    double p;           /* measurement probability per site per layer */
+    uint64_t seed;      /* counter-based stream key */
+    int closed;         /* 1: ring (wrap bond on odd layers), 0: open chain */
+    int gate_family;    /* 1 = Haar U(4) (cpp/include/mipt/types.hpp:155-186) */
+    int probe_site;     /* -1: none; else a probe qubit at index L starts Bell-paired with this site */
+    int p_zero_layers;  /* the first p_zero_layers layers are unmeasured (Gullans-Huse scrambling) */
+    int shared_gates;   /* -1: own gates; else use trajectory shared_gates's gate stream */
+} mipt_hip_traj_spec;
+
+typedef struct mipt_hip_batch_config
+{
+    int tile_bits;      /* 0 = autotuned choice; else 12..14 */
+    int max_hist_bits;  /* measured sites per fused histogram chunk (<= 10); 0 = default */
+    int threads;        /* reserved: 0 */
+} mipt_hip_batch_config;
+
+int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *hip_stream, mipt_hip_batch **out);
+int mipt_hip_batch_destroy(mipt_hip_batch *batch);
+/* Allocates device-resident states and streams for `count` trajectories. */
+int mipt_hip_batch_setup(mipt_hip_batch *batch, const mipt_hip_traj_spec *specs, int count);
+/* Draws every gate, flag and uniform on device from the counter-based stream
+ * (Philox4x32-10 keyed by the seed; asynchronous on the batch stream). */
+int mipt_hip_batch_generate(mipt_hip_batch *batch);
+/* Replay: explicit unitaries (complex128 [bonds][4][4] interleaved, bond order),
+ * flags ([layers][L] 0/1) and per-site uniforms ([layers][L]) for one trajectory. */
+int mipt_hip_batch_set_layers(mipt_hip_batch *batch, int traj, const double *gates_ri, const uint8_t *flags,
+                              const double *uniforms);
+/* Initial state (complex64 interleaved, 2^(L+probe) amplitudes); default |0...0>
+ * (or the probe Bell pair). */
+int mipt_hip_batch_set_state(mipt_hip_batch *batch, int traj, const float *psi_ri);
+int mipt_hip_batch_get_state(mipt_hip_batch *batch, int traj, float *psi_ri);
+/* Device pointer of a trajectory's state (valid until destroy). */
+void *mipt_hip_batch_state_ptr(mipt_hip_batch *batch, int traj);
+/* Evolves every trajectory through all of its layers (asynchronous). */
+int mipt_hip_batch_evolve(mipt_hip_batch *batch);
+/* Evolves layers [layer_begin, layer_end) of all trajectories (asynchronous). */
+int mipt_hip_batch_evolve_range(mipt_hip_batch *batch, int layer_begin, int layer_end);
+int mipt_hip_batch_synchronize(mipt_hip_batch *batch);
+/* outcomes [layers][L]: -1 unmeasured, else 0/1; flagged [layers][L]: 1 where
+ * |u - P(1)| < 1e-6; p1 [layers][L] conditional P(1) (may be NULL). */
+int mipt_hip_batch_outcomes(mipt_hip_batch *batch, int traj, int *outcomes, uint8_t *flagged, double *p1,
+                            double *surprisal);
+/* Per-trajectory observables: S(L/2) in bits, TMI (bits, cycle 0, block
+ * tmi_block or L/4 if 0), probe entropy (nats) and Bloch vector (NaN when
+ * the trajectory has no probe). */
+int mipt_hip_batch_observables(mipt_hip_batch *batch, int traj, int tmi_block, double *s_half, double *tmi,
+                               double *probe_entropy, double *bloch3);
+/* The stream the batch launches on. */
+void *mipt_hip_batch_stream(mipt_hip_batch *batch);
+/* Re-arms the in-kernel initial states and clears per-run records, so the
+ * same trajectories can be run again (generate + evolve). */
+int mipt_hip_batch_reset(mipt_hip_batch *batch);
+/* Copies a trajectory's device-drawn unitaries (complex128 [bonds][4][4]),
+ * flags ([layers][L]) and uniforms ([layers][L]) to host (any may be NULL). */
+int mipt_hip_batch_export_layers(mipt_hip_batch *batch, int traj, double *gates_ri, uint8_t *flags,
+                                 double *uniforms);
+/* Gate-sweep timing: every sample_every-th sweep kernel is bracketed by HIP
+ * events on the batch stream (0 = off); resets the statistics. */
+int mipt_hip_batch_set_profiling(mipt_hip_batch *batch, int sample_every);
+/* Average duration (ms) of the timed sweeps, how many were timed, their mean
+ * algorithmic bytes (one state read + write per layer, split over the layer's
+ * sweeps), all sweeps launched and their total algorithmic bytes. */
+int mipt_hip_batch_pass_stats(mipt_hip_batch *batch, double *avg_ms, int *timed, double *bytes_avg,
+                              long *launches, double *bytes_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIPT_HIP_H */

@@ -1,0 +1,426 @@
+"""mipt_amd: MI355X-native qubit MIPT trajectory engine (host mirror of the C ABI).
+
+The compute path is ``libmipt_hip.so`` (hand-written gfx950 HIP kernels behind
+the C ABI declared in ``include/mipt_hip.h``).  This module binds it with
+ctypes and mirrors the reference's host-side interfaces for the hot path
+(paths relative to the reference repository root):
+
+* ``PlainEngine``     <- ``cusv::Engine`` (cpp/include/mipt/cusv/engine.hpp:100-582)
+* ``Batch``           <- the serial loops of ``run_1d_sim_tmi`` / ``run_1d``
+                          (cpp/include/mipt/tmi/runner.hpp:269-378,
+                          cpp/apps/mipt.cpp:111-190), batched on device
+* module functions    <- the device C ABI (cpp/include/mipt/cuda/tmi_cuda_svd.hpp,
+                          tmi_cuda_rdm.hpp, free_energy_measure.hpp, ancilla_rdm.hpp)
+
+Errors raise ``MiptHipError`` carrying the status code and the library's
+last-error text.  There is no CPU fallback: if the shared library is missing
+the import of any entry point fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmipt_hip.so")
+
+_LIB = None
+
+
+class MiptHipError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what} failed with status {status}: {last_error()}")
+        self.status = status
+
+
+def lib():
+    """The loaded libmipt_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m mipt_amd.build` "
+                          "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp, u8p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), \
+        ctypes.POINTER(ctypes.c_uint8)
+    fp = ctypes.POINTER(ctypes.c_float)
+    i = ctypes.c_int
+    L.mipt_hip_last_error.restype = ctypes.c_char_p
+    L.mipt_hip_version.restype = ctypes.c_char_p
+    L.mipt_hip_measure_collapse_last_error.restype = ctypes.c_char_p
+    for name in ("mipt_hip_measure_collapse_f32", "mipt_hip_measure_collapse_f64"):
+        getattr(L, name).argtypes = [vp, i, i, ctypes.c_double, ip, dp]
+    for name in ("mipt_hip_entropy_svd_f32", "mipt_hip_entropy_svd_f64", "mipt_hip_renyi2_f32",
+                 "mipt_hip_renyi2_f64", "mipt_hip_rdm_subsystem_f32", "mipt_hip_rdm_subsystem_f64"):
+        getattr(L, name).argtypes = [vp, i, ip, i, i, dp]
+    for name in ("mipt_hip_rho1_complex_f32", "mipt_hip_rho1_complex_f64"):
+        getattr(L, name).argtypes = [vp, i, i, dp]
+    L.mipt_hip_tmi_f32.argtypes = [vp, i, i, i, i, dp, dp]
+    L.mipt_hip_engine_create.argtypes = [i, vp, ctypes.POINTER(vp)]
+    L.mipt_hip_engine_destroy.argtypes = [vp]
+    L.mipt_hip_engine_adopt.argtypes = [vp, vp, i]
+    L.mipt_hip_engine_reset.argtypes = [vp, i]
+    L.mipt_hip_engine_data.argtypes = [vp]
+    L.mipt_hip_engine_data.restype = vp
+    L.mipt_hip_engine_apply_matrix.argtypes = [vp, ip, i, dp]
+    L.mipt_hip_engine_apply_bonds.argtypes = [vp, ip, ip, dp, i, i]
+    L.mipt_hip_engine_measure_layer.argtypes = [vp, ip, i, dp, i, i, ip, dp, dp]
+    L.mipt_hip_engine_to_host.argtypes = [vp, vp]
+    L.mipt_hip_engine_set_from_host.argtypes = [vp, vp]
+    L.mipt_hip_engine_squared_norm.argtypes = [vp, dp]
+    L.mipt_hip_batch_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    L.mipt_hip_batch_destroy.argtypes = [vp]
+    L.mipt_hip_batch_setup.argtypes = [vp, vp, i]
+    L.mipt_hip_batch_generate.argtypes = [vp]
+    L.mipt_hip_batch_set_layers.argtypes = [vp, i, dp, u8p, dp]
+    L.mipt_hip_batch_set_state.argtypes = [vp, i, fp]
+    L.mipt_hip_batch_get_state.argtypes = [vp, i, fp]
+    L.mipt_hip_batch_state_ptr.argtypes = [vp, i]
+    L.mipt_hip_batch_state_ptr.restype = vp
+    L.mipt_hip_batch_evolve.argtypes = [vp]
+    L.mipt_hip_batch_evolve_range.argtypes = [vp, i, i]
+    L.mipt_hip_batch_synchronize.argtypes = [vp]
+    L.mipt_hip_batch_outcomes.argtypes = [vp, i, ip, u8p, dp, dp]
+    L.mipt_hip_batch_observables.argtypes = [vp, i, i, dp, dp, dp, dp]
+    L.mipt_hip_batch_stream.argtypes = [vp]
+    L.mipt_hip_batch_stream.restype = vp
+    L.mipt_hip_batch_reset.argtypes = [vp]
+    L.mipt_hip_batch_export_layers.argtypes = [vp, i, dp, u8p, dp]
+    L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
+    L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    if _LIB is None:
+        return ""
+    return _LIB.mipt_hip_last_error().decode(errors="replace")
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise MiptHipError(status, what)
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ------------------------------------------------------------------ drop-ins
+def measure_collapse(state_ptr: int, n_qubits: int, qubit: int, uniform: float, fp64: bool = False):
+    """mipt_cuda_measure_collapse_* (free_energy_measure.hpp:13-27): (outcome, P(outcome))."""
+    o = ctypes.c_int()
+    p = ctypes.c_double()
+    f = lib().mipt_hip_measure_collapse_f64 if fp64 else lib().mipt_hip_measure_collapse_f32
+    _check(f(ctypes.c_void_p(state_ptr), n_qubits, qubit, float(uniform), ctypes.byref(o), ctypes.byref(p)),
+           "measure_collapse")
+    return o.value, p.value
+
+
+def entropy(state_ptr: int, n_qubits: int, kept, fp64: bool = False, order: int = 1) -> float:
+    """mipt_cuda_entropy_svd_* / renyi2_* (tmi_cuda_svd.hpp:20-58), bits."""
+    k = np.ascontiguousarray(kept, np.int32)
+    out = ctypes.c_double()
+    if order == 2:
+        f = lib().mipt_hip_renyi2_f64 if fp64 else lib().mipt_hip_renyi2_f32
+    else:
+        f = lib().mipt_hip_entropy_svd_f64 if fp64 else lib().mipt_hip_entropy_svd_f32
+    _check(f(ctypes.c_void_p(state_ptr), n_qubits, _ip(k), len(k), 0, ctypes.byref(out)), "entropy")
+    return out.value
+
+
+def rdm_subsystem(state_ptr: int, n_qubits: int, kept, fp64: bool = False) -> np.ndarray:
+    """mipt_cuda_rdm_subsystem_* (tmi_cuda_rdm.hpp:20-33): complex128 [D, D]."""
+    k = np.ascontiguousarray(kept, np.int32)
+    d = 1 << len(k)
+    out = np.zeros((d, d), np.complex128)
+    f = lib().mipt_hip_rdm_subsystem_f64 if fp64 else lib().mipt_hip_rdm_subsystem_f32
+    _check(f(ctypes.c_void_p(state_ptr), n_qubits, _ip(k), len(k), 0,
+             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "rdm_subsystem")
+    return out
+
+
+def rho1(state_ptr: int, n_qubits: int, qubit: int, fp64: bool = False) -> np.ndarray:
+    """mipt_cuda_rho1_complex_* (ancilla_rdm.hpp:19-29): {rho00, rho11, Re rho01, Im rho01}."""
+    out = np.zeros(4)
+    f = lib().mipt_hip_rho1_complex_f64 if fp64 else lib().mipt_hip_rho1_complex_f32
+    _check(f(ctypes.c_void_p(state_ptr), n_qubits, qubit, _dp(out)), "rho1")
+    return out
+
+
+def tmi(state_ptr: int, n_qubits: int, chain_sites: int | None = None, block: int | None = None, cycle: int = 0):
+    """I3 in bits with the seven terms (compute.hpp:186-227)."""
+    L = n_qubits if chain_sites is None else chain_sites
+    b = L // 4 if block is None else block
+    out = ctypes.c_double()
+    terms = np.zeros(7)
+    _check(lib().mipt_hip_tmi_f32(ctypes.c_void_p(state_ptr), n_qubits, L, b, cycle, ctypes.byref(out), _dp(terms)),
+           "tmi")
+    return out.value, terms
+
+
+# ------------------------------------------------------------------ plain engine
+class PlainEngine:
+    """One trajectory, one launch per blocked op, a host round trip per measured
+    layer: cusv::Engine (cpp/include/mipt/cusv/engine.hpp:100-582)."""
+
+    def __init__(self, fp64: bool = False, stream: int | None = None):
+        self.fp64 = fp64
+        h = ctypes.c_void_p()
+        _check(lib().mipt_hip_engine_create(int(fp64), ctypes.c_void_p(stream or 0), ctypes.byref(h)),
+               "engine_create")
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().mipt_hip_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, n: int):
+        _check(lib().mipt_hip_engine_reset(self._h, n), "engine_reset")
+        self.n = n
+
+    def adopt(self, ptr: int, n: int):
+        _check(lib().mipt_hip_engine_adopt(self._h, ctypes.c_void_p(ptr), n), "engine_adopt")
+        self.n = n
+
+    @property
+    def data(self) -> int:
+        return lib().mipt_hip_engine_data(self._h)
+
+    def apply_matrix(self, targets, m):
+        t = np.ascontiguousarray(targets, np.int32)
+        mm = np.ascontiguousarray(m, np.complex128)
+        _check(lib().mipt_hip_engine_apply_matrix(self._h, _ip(t), len(t), mm.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "apply_matrix")
+
+    def apply_bonds(self, bonds, gates, block_targets: int = 4):
+        """bonds: [(q0, q1)]; gates: complex128 [nb, 4, 4] (bit 0 = q0)."""
+        if len(bonds) == 0:
+            return
+        q0 = np.ascontiguousarray([b[0] for b in bonds], np.int32)
+        q1 = np.ascontiguousarray([b[1] for b in bonds], np.int32)
+        g = np.ascontiguousarray(gates, np.complex128)
+        _check(lib().mipt_hip_engine_apply_bonds(self._h, _ip(q0), _ip(q1), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                 len(bonds), block_targets), "apply_bonds")
+
+    def measure_layer(self, sites, uniforms, mode: int = 0, chunk_bits: int = 10):
+        """mode 0: sequential per-site draws; mode 1: joint chunk draws (engine.hpp:293-368).
+        Returns (outcomes, p1 (mode 0), surprisal)."""
+        s = np.ascontiguousarray(sites, np.int32)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        o = np.zeros(len(s), np.int32)
+        p1 = np.zeros(len(s), np.float64)
+        sp = ctypes.c_double()
+        _check(lib().mipt_hip_engine_measure_layer(self._h, _ip(s), len(s), _dp(u), mode, chunk_bits, _ip(o), _dp(p1),
+                                                   ctypes.byref(sp)), "measure_layer")
+        return o, p1, sp.value
+
+    def to_host(self) -> np.ndarray:
+        out = np.zeros(1 << self.n, np.complex128 if self.fp64 else np.complex64)
+        _check(lib().mipt_hip_engine_to_host(self._h, out.ctypes.data_as(ctypes.c_void_p)), "to_host")
+        return out
+
+    def set_from_host(self, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, np.complex128 if self.fp64 else np.complex64)
+        _check(lib().mipt_hip_engine_set_from_host(self._h, a.ctypes.data_as(ctypes.c_void_p)), "set_from_host")
+
+    def squared_norm(self) -> float:
+        out = ctypes.c_double()
+        _check(lib().mipt_hip_engine_squared_norm(self._h, ctypes.byref(out)), "squared_norm")
+        return out.value
+
+
+def layer_bonds(n: int, layer: int, closed: bool = True):
+    """Brickwork bonds of a layer (cpp/include/mipt/circuits/haar.hpp:226-243)."""
+    start = layer & 1
+    stop = n if (closed and start == 1 and n > 2) else n - 1
+    return [(i, (i + 1) % n) for i in range(start, stop, 2)]
+
+
+def run_plain_trajectory(engine: PlainEngine, n: int, gates, flags, uniforms, closed: bool = True,
+                         block_targets: int = 4, mode: int = 0, chunk_bits: int = 10):
+    """The try_cusv loop (cpp/include/mipt/circuit.hpp:240-282) on the plain
+    engine: per layer, blocked bond gates, then the measured sites.
+    gates: complex128 [bonds,4,4]; flags/uniforms: [layers, n].  Returns outcomes [layers, n] (-1 unmeasured)."""
+    engine.reset(n)
+    layers = flags.shape[0]
+    outcomes = np.full((layers, n), -1, np.int32)
+    g = 0
+    for t in range(layers):
+        bonds = layer_bonds(n, t, closed)
+        engine.apply_bonds(bonds, gates[g:g + len(bonds)], block_targets)
+        g += len(bonds)
+        sites = np.nonzero(flags[t])[0]
+        if len(sites):
+            u = uniforms[t, sites] if mode == 0 else uniforms[t, :]
+            o, _, _ = engine.measure_layer(sites, u, mode, chunk_bits)
+            outcomes[t, sites] = o
+    return outcomes
+
+
+# ------------------------------------------------------------------ batched engine
+class TrajSpec(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int), ("layers", ctypes.c_int), ("p", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("closed", ctypes.c_int), ("gate_family", ctypes.c_int), ("probe_site", ctypes.c_int),
+                ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("tile_bits", ctypes.c_int), ("max_hist_bits", ctypes.c_int), ("threads", ctypes.c_int)]
+
+
+@dataclass
+class Spec:
+    L: int
+    layers: int
+    p: float
+    seed: int
+    closed: bool = True
+    probe_site: int = -1
+    p_zero_layers: int = 0
+    shared_gates: int = -1
+
+    def to_c(self) -> TrajSpec:
+        return TrajSpec(self.L, self.layers, float(self.p), self.seed & (2**64 - 1), int(self.closed), 1,
+                        self.probe_site, self.p_zero_layers, self.shared_gates)
+
+
+class Batch:
+    """Device-resident batch of trajectories (mixed chain lengths allowed)."""
+
+    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None):
+        cfg = BatchConfig(tile_bits, 0, 0)
+        h = ctypes.c_void_p()
+        _check(lib().mipt_hip_batch_create(ctypes.byref(cfg), ctypes.c_void_p(stream or 0), ctypes.byref(h)),
+               "batch_create")
+        self._h = h
+        self.specs = list(specs)
+        arr = (TrajSpec * len(self.specs))(*[s.to_c() for s in self.specs])
+        _check(lib().mipt_hip_batch_setup(self._h, arr, len(self.specs)), "batch_setup")
+
+    def close(self):
+        if self._h:
+            lib().mipt_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return lib().mipt_hip_batch_stream(self._h) or 0
+
+    def generate(self):
+        _check(lib().mipt_hip_batch_generate(self._h), "batch_generate")
+
+    def set_layers(self, traj: int, gates, flags, uniforms):
+        g = np.ascontiguousarray(gates, np.complex128)
+        f = np.ascontiguousarray(flags, np.uint8)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        _check(lib().mipt_hip_batch_set_layers(self._h, traj, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(u)),
+               "batch_set_layers")
+
+    def nq(self, traj: int) -> int:
+        s = self.specs[traj]
+        return s.L + (1 if s.probe_site >= 0 else 0)
+
+    def set_state(self, traj: int, psi):
+        a = np.ascontiguousarray(psi, np.complex64)
+        _check(lib().mipt_hip_batch_set_state(self._h, traj, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "batch_set_state")
+
+    def get_state(self, traj: int) -> np.ndarray:
+        out = np.zeros(1 << self.nq(traj), np.complex64)
+        _check(lib().mipt_hip_batch_get_state(self._h, traj, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "batch_get_state")
+        return out
+
+    def state_ptr(self, traj: int) -> int:
+        return lib().mipt_hip_batch_state_ptr(self._h, traj)
+
+    def evolve(self, layer_begin: int | None = None, layer_end: int | None = None):
+        if layer_begin is None:
+            _check(lib().mipt_hip_batch_evolve(self._h), "batch_evolve")
+        else:
+            _check(lib().mipt_hip_batch_evolve_range(self._h, layer_begin, layer_end), "batch_evolve_range")
+
+    def synchronize(self):
+        _check(lib().mipt_hip_batch_synchronize(self._h), "batch_synchronize")
+
+    def reset(self):
+        """Re-arm in-kernel initial states so the same trajectories run again."""
+        _check(lib().mipt_hip_batch_reset(self._h), "batch_reset")
+
+    def export_layers(self, traj: int):
+        """Device-drawn (gates complex128 [bonds,4,4], flags [layers,L], uniforms [layers,L])."""
+        s = self.specs[traj]
+        nb = 0
+        for t in range(s.layers):
+            nb += len(layer_bonds(s.L, t, s.closed))
+        g = np.zeros((nb, 4, 4), np.complex128)
+        f = np.zeros((s.layers, s.L), np.uint8)
+        u = np.zeros((s.layers, s.L), np.float64)
+        _check(lib().mipt_hip_batch_export_layers(self._h, traj, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                  f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(u)),
+               "batch_export_layers")
+        return g, f, u
+
+    def set_profiling(self, sample_every: int):
+        _check(lib().mipt_hip_batch_set_profiling(self._h, sample_every), "batch_set_profiling")
+
+    def pass_stats(self):
+        """(avg sweep ms over timed launches, timed count, mean algorithmic bytes, launches, total bytes)."""
+        avg = ctypes.c_double()
+        n = ctypes.c_int()
+        b = ctypes.c_double()
+        la = ctypes.c_long()
+        bt = ctypes.c_double()
+        _check(lib().mipt_hip_batch_pass_stats(self._h, ctypes.byref(avg), ctypes.byref(n), ctypes.byref(b),
+                                               ctypes.byref(la), ctypes.byref(bt)), "batch_pass_stats")
+        return {"avg_ms": avg.value, "timed": n.value, "bytes_avg": b.value, "launches": la.value,
+                "bytes_total": bt.value}
+
+    def outcomes(self, traj: int):
+        s = self.specs[traj]
+        o = np.zeros((s.layers, s.L), np.int32)
+        f = np.zeros((s.layers, s.L), np.uint8)
+        p1 = np.zeros((s.layers, s.L), np.float64)
+        sp = ctypes.c_double()
+        _check(lib().mipt_hip_batch_outcomes(self._h, traj, _ip(o), f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                             _dp(p1), ctypes.byref(sp)), "batch_outcomes")
+        return o, f, p1, sp.value
+
+    def observables(self, traj: int, tmi_block: int = 0, want_tmi: bool = True):
+        sh = ctypes.c_double()
+        tm = ctypes.c_double(float("nan"))
+        pe = ctypes.c_double()
+        bl = np.zeros(3)
+        _check(lib().mipt_hip_batch_observables(self._h, traj, tmi_block, ctypes.byref(sh),
+                                                ctypes.byref(tm) if want_tmi else None, ctypes.byref(pe), _dp(bl)),
+               "batch_observables")
+        return {"s_half": sh.value, "tmi": tm.value, "probe_entropy": pe.value, "bloch": bl}
+
+
+__all__ = ["lib", "last_error", "MiptHipError", "measure_collapse", "entropy", "rdm_subsystem", "rho1", "tmi",
+           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch"]

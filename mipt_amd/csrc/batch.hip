@@ -1,0 +1,1693 @@
+// Batched MIPT trajectory engine for gfx950.
+//
+// Reference behaviour (paths relative to the reference repository root):
+//   cpp/include/mipt/circuit.hpp:240-282   per-trajectory layer loop
+//   cpp/include/mipt/circuits/haar.hpp:39-85, 226-305  Haar bonds, layers, flags
+//   cpp/include/mipt/free_energy_measure.hpp:19-95     sequential Born sampling
+//   cpp/include/mipt/cusv/engine.hpp:293-368           joint-chunk = sequential (chain rule)
+//   cpp/include/mipt/probed.hpp:25-33                  probe Bell pair h(site); cx(site, n)
+//
+// Design (MI355X-first, not a translation):
+//   * every state lives in HBM for the whole run (complex64, qubit q = bit q);
+//   * chains whose state fits one workgroup's LDS (<= 14 qubits) run all their
+//     layers in ONE launch, one workgroup per trajectory;
+//   * longer chains are swept in tiles of 2^k amplitudes (k = 6 + RB + wave
+//     bits): each tile keeps the low c physical bits plus a window of high bits
+//     local, stages its amplitudes through LDS so that up to RB/2 bonds at a time
+//     sit in registers, and applies the brickwork layer's bonds there.  A layer
+//     takes as many sweeps as windows are needed to cover its bonds (2 for L=20);
+//   * Born probabilities of the layer's measured sites are accumulated in FP64
+//     in the last sweep (per-tile partials, fixed order), a finalize kernel
+//     turns them into sequential outcomes, and the collapse with FP64
+//     renormalisation is folded into the next layer's first sweep;
+//   * trajectories are processed in groups whose states fit the 256 MiB
+//     Infinity Cache so the second sweep of a layer re-reads from the MALL.
+#include "batch.hpp"
+#include "state_ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+namespace mipt_hip
+{
+
+// ============================================================== generation
+// Gates of every (trajectory, layer, bond): Haar U(4) from the counter-based
+// stream, narrowed to complex64 row-major.
+__global__ void gen_gates_kernel(const TrajDesc *__restrict__ desc, const int *__restrict__ gen_list, int count,
+                                 const std::uint32_t *__restrict__ bond_prefix_layers, float2 *__restrict__ gates,
+                                 std::uint64_t total_items, const std::uint64_t *__restrict__ item_prefix)
+{
+    const std::uint64_t id = blockIdx.x * static_cast<std::uint64_t>(blockDim.x) + threadIdx.x;
+    if (id >= total_items) return;
+    // binary search the trajectory owning this item
+    int lo = 0, hi = count - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) / 2;
+        if (item_prefix[mid] <= id) lo = mid;
+        else hi = mid - 1;
+    }
+    const TrajDesc d = desc[gen_list[lo]];
+    const std::uint64_t local = id - item_prefix[lo];  // gate index within the trajectory
+    // locate (layer, bond)
+    const int nb_even = layer_bond_count(d.L, 0, d.closed);
+    const int nb_odd = layer_bond_count(d.L, 1, d.closed);
+    const std::uint64_t per_period = static_cast<std::uint64_t>(nb_even + nb_odd);
+    const std::uint64_t period = local / per_period;
+    std::uint64_t rem = local % per_period;
+    int layer = static_cast<int>(2 * period);
+    if (rem >= static_cast<std::uint64_t>(nb_even))
+    {
+        rem -= nb_even;
+        layer += 1;
+    }
+    double qr[16], qi[16];
+    haar_u4_philox(d.seed, static_cast<std::uint32_t>(layer), static_cast<std::uint32_t>(rem), qr, qi);
+    float2 *g = gates + (d.gate_off + local) * 16;
+    for (int e = 0; e < 16; ++e) g[e] = make_float2(static_cast<float>(qr[e]), static_cast<float>(qi[e]));
+    (void)bond_prefix_layers;
+}
+
+// Flags and measurement uniforms of every (trajectory, layer).
+__global__ void gen_flags_kernel(const TrajDesc *__restrict__ desc, int count, std::uint32_t *__restrict__ flags,
+                                 double *__restrict__ uniforms, std::uint64_t total_items,
+                                 const std::uint64_t *__restrict__ layer_prefix)
+{
+    const std::uint64_t id = blockIdx.x * static_cast<std::uint64_t>(blockDim.x) + threadIdx.x;
+    if (id >= total_items) return;
+    int lo = 0, hi = count - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) / 2;
+        if (layer_prefix[mid] <= id) lo = mid;
+        else hi = mid - 1;
+    }
+    const TrajDesc d = desc[lo];
+    const int t = static_cast<int>(id - layer_prefix[lo]);
+    std::uint32_t mask = 0;
+    for (int s = 0; s < d.L; ++s)
+    {
+        double a, b, m, unused;
+        philox_uniform_pair(d.seed, static_cast<std::uint32_t>(t), static_cast<std::uint32_t>(s), 0, STREAM_FLAG, a, b);
+        const double p = (t < d.p_zero_layers) ? 0.0 : d.p;
+        const bool f = (p <= 0.0) ? false : (p >= 1.0 ? true : (a < p));
+        philox_uniform_pair(d.seed, static_cast<std::uint32_t>(t), static_cast<std::uint32_t>(s), 0, STREAM_MEAS, m, unused);
+        uniforms[d.unif_off + static_cast<std::size_t>(t) * d.L + s] = f ? m : 0.0;
+        if (f) mask |= 1u << s;
+    }
+    flags[d.layer_off + t] = mask;
+}
+
+// ============================================================== small chains
+// One workgroup per trajectory, the whole state in LDS, all requested layers
+// in one launch.  Bonds one at a time (thread per group of 4 amplitudes),
+// measured sites sequentially with fixed-order FP64 block reductions.
+constexpr int SMALL_THREADS = 256;
+
+__device__ inline double block_sum(double v, double *red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = SMALL_THREADS / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
+{
+    if (d.probe_site >= 0)
+    {
+        const std::uint64_t bell = (std::uint64_t{1} << d.probe_site) | (std::uint64_t{1} << d.L);
+        const float h = 0.70710678118654752440f;
+        return (i == 0 || i == bell) ? make_float2(h, 0.f) : make_float2(0.f, 0.f);
+    }
+    return i == 0 ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
+}
+
+__device__ inline std::uint64_t gate_prefix(const TrajDesc &d, int t)
+{
+    const int nb_even = layer_bond_count(d.L, 0, d.closed);
+    const int nb_odd = layer_bond_count(d.L, 1, d.closed);
+    return static_cast<std::uint64_t>((t + 1) / 2) * nb_even + static_cast<std::uint64_t>(t / 2) * nb_odd;
+}
+
+__global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
+    DevArena a, const int *__restrict__ traj_list, int layer_begin, int layer_end, int init)
+{
+    extern __shared__ float2 st[];
+    __shared__ double red[SMALL_THREADS];
+    const int traj = traj_list[blockIdx.x];
+    const TrajDesc d = a.desc[traj];
+    const int nq = d.nq;
+    const std::uint32_t dim = 1u << nq;
+    float2 *psi = a.states + d.state_off;
+    const bool gen = init && a.init[traj];
+    for (std::uint32_t i = threadIdx.x; i < dim; i += SMALL_THREADS) st[i] = gen ? initial_amp(d, i) : psi[i];
+    __syncthreads();
+    double surpr = 0.0;
+    int status = 0;
+    for (int t = layer_begin; t < layer_end; ++t)
+    {
+        const int nb = layer_bond_count(d.L, t, d.closed);
+        const std::uint64_t gbase = d.gate_off + gate_prefix(d, t);
+        for (int b = 0; b < nb; ++b)
+        {
+            int q0, q1;
+            layer_bond(d.L, t, b, q0, q1);
+            const float2 *g = a.gates + (gbase + b) * 16;
+            float ur[16], ui[16];
+            for (int e = 0; e < 16; ++e)
+            {
+                ur[e] = g[e].x;
+                ui[e] = g[e].y;
+            }
+            const int lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+            for (std::uint32_t grp = threadIdx.x; grp < (dim >> 2); grp += SMALL_THREADS)
+            {
+                std::uint32_t base = ((grp >> lo) << (lo + 1)) | (grp & ((1u << lo) - 1));
+                base = ((base >> hi) << (hi + 1)) | (base & ((1u << hi) - 1));
+                const std::uint32_t i1 = base | (1u << q0), i2 = base | (1u << q1), i3 = i1 | i2;
+                const float2 x0 = st[base], x1 = st[i1], x2 = st[i2], x3 = st[i3];
+                float2 y[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                {
+                    float yr = ur[4 * r] * x0.x - ui[4 * r] * x0.y;
+                    float yi = ur[4 * r] * x0.y + ui[4 * r] * x0.x;
+                    yr = fmaf(ur[4 * r + 1], x1.x, fmaf(-ui[4 * r + 1], x1.y, yr));
+                    yi = fmaf(ur[4 * r + 1], x1.y, fmaf(ui[4 * r + 1], x1.x, yi));
+                    yr = fmaf(ur[4 * r + 2], x2.x, fmaf(-ui[4 * r + 2], x2.y, yr));
+                    yi = fmaf(ur[4 * r + 2], x2.y, fmaf(ui[4 * r + 2], x2.x, yi));
+                    yr = fmaf(ur[4 * r + 3], x3.x, fmaf(-ui[4 * r + 3], x3.y, yr));
+                    yi = fmaf(ur[4 * r + 3], x3.y, fmaf(ui[4 * r + 3], x3.x, yi));
+                    y[r] = make_float2(yr, yi);
+                }
+                st[base] = y[0];
+                st[i1] = y[1];
+                st[i2] = y[2];
+                st[i3] = y[3];
+            }
+            __syncthreads();
+        }
+        const std::uint32_t fl = a.flags[d.layer_off + t];
+        std::uint32_t outc = 0, flg = 0;
+        for (int s = 0; s < d.L; ++s)
+        {
+            if (!((fl >> s) & 1u)) continue;
+            double w = 0.0, w1 = 0.0;
+            for (std::uint32_t i = threadIdx.x; i < dim; i += SMALL_THREADS)
+            {
+                const float2 v = st[i];
+                const double x = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+                w += x;
+                if ((i >> s) & 1u) w1 += x;
+            }
+            w = block_sum(w, red);
+            w1 = block_sum(w1, red);
+            double p = (w > 0.0) ? w1 / w : 0.0;
+            p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
+            const double u = a.uniforms[d.unif_off + static_cast<std::size_t>(t) * d.L + s];
+            const int o = u < p ? 1 : 0;
+            const double branch = o ? w1 : w - w1;
+            if (!(w > 0.0) || !(branch > 0.0)) status = E_NORM;
+            if (threadIdx.x == 0) a.p1[d.unif_off + static_cast<std::size_t>(t) * d.L + s] = p;
+            if (o) outc |= 1u << s;
+            if (fabs(u - p) < 1e-6) flg |= 1u << s;
+            surpr += -log((o ? p : 1.0 - p));
+            const float scale = static_cast<float>(1.0 / sqrt(branch > 0.0 ? branch : 1.0));
+            for (std::uint32_t i = threadIdx.x; i < dim; i += SMALL_THREADS)
+            {
+                float2 v = st[i];
+                if (((i >> s) & 1u) == static_cast<std::uint32_t>(o))
+                {
+                    v.x *= scale;
+                    v.y *= scale;
+                }
+                else
+                    v = make_float2(0.f, 0.f);
+                st[i] = v;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+        {
+            a.outcomes[d.layer_off + t] = outc;
+            a.flagged[d.layer_off + t] = flg;
+            a.coll[d.layer_off + t] = Collapse{0, 0, 1.0, 1.0};
+        }
+    }
+    for (std::uint32_t i = threadIdx.x; i < dim; i += SMALL_THREADS) psi[i] = st[i];
+    if (threadIdx.x == 0)
+    {
+        a.surprisal[traj] += surpr;
+        if (status) a.status[traj] = status;
+    }
+}
+
+// ============================================================== tiled sweeps
+// u = in-tile index (k bits).  u bits 0..c-1 <-> physical bits 0..c-1; u bits
+// c..k-1 <-> physical bits s..s+k-c-1 (the window); the other physical bits
+// enumerate the tiles.  A "mapping" assigns each u bit to a register bit (RB
+// of them, so a thread holds 2^RB amplitudes), a lane bit (6) or a wave bit.
+// M0 (used for global loads/stores): lanes = u 0..5, registers = u 6..6+RB-1,
+// waves = the rest, so every load/store instruction covers runs of 2^c
+// consecutive amplitudes.
+struct StagePlan
+{
+    std::int8_t reg_u[6];
+    std::int8_t lane_u[6];
+    std::int8_t wave_u[4];
+    std::int8_t nb;        // bonds in this stage
+    std::int8_t bond[3];   // bond number (within the layer) on register pair i: (0,1), (2,3), (4,5); -1 none
+    std::int8_t use_lds;   // 0: M0 registers (no exchange)
+    std::int8_t pad[3];
+};
+
+struct PassPlan
+{
+    std::uint64_t rest_mask;  // physical bits enumerating tiles
+    int ntiles;
+    int k, c, s;
+    int nstages;
+    StagePlan st[6];
+    int collapse;      // apply the previous layer's pending collapse at load
+    int init;          // generate the initial state instead of loading
+    int hist;          // accumulate the chunk histogram after the gates
+    int store;         // write the tile back
+    int chunk;         // histogram chunk index (sites chunk*MAX_HIST_BITS ...)
+    int cond;          // restrict the histogram to the decided outcomes of the layer
+};
+
+struct PassArgs
+{
+    DevArena a;
+    const int *traj_list;  // group trajectories
+    int layer;
+    PassPlan plan;
+};
+
+__device__ __forceinline__ std::uint32_t swz(std::uint32_t u)
+{
+    return u ^ (((u >> 4) ^ (u >> 8) ^ (u >> 12)) & 15u);
+}
+
+// LDS word of in-tile index u: the low nibble is XORed with the higher
+// nibbles so that lanes walking any 6 u bits with distinct residues mod 4 hit
+// distinct banks.  swz is GF(2)-linear: swz(a ^ b) = swz(a) ^ swz(b).
+// 4x4 complex gate (row-major, index 2*b(q1)+b(q0)) on register bits (RA <- q0, RC <- q1).
+template <int NV, int RA, int RC>
+__device__ __forceinline__ void gate_on_regs(float2 (&v)[NV], const float2 *__restrict__ g)
+{
+    float ur[16], ui[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+    {
+        const float2 x = g[e];
+        ur[e] = x.x;
+        ui[e] = x.y;
+    }
+#pragma unroll
+    for (int base = 0; base < NV; ++base)
+    {
+        if (base & ((1 << RA) | (1 << RC))) continue;
+        const int i1 = base | (1 << RA), i2 = base | (1 << RC), i3 = i1 | i2;
+        const float2 x0 = v[base], x1 = v[i1], x2 = v[i2], x3 = v[i3];
+        float2 y[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            float yr = ur[4 * r] * x0.x;
+            float yi = ur[4 * r] * x0.y;
+            yr = fmaf(-ui[4 * r], x0.y, yr);
+            yi = fmaf(ui[4 * r], x0.x, yi);
+            yr = fmaf(ur[4 * r + 1], x1.x, yr);
+            yi = fmaf(ur[4 * r + 1], x1.y, yi);
+            yr = fmaf(-ui[4 * r + 1], x1.y, yr);
+            yi = fmaf(ui[4 * r + 1], x1.x, yi);
+            yr = fmaf(ur[4 * r + 2], x2.x, yr);
+            yi = fmaf(ur[4 * r + 2], x2.y, yi);
+            yr = fmaf(-ui[4 * r + 2], x2.y, yr);
+            yi = fmaf(ui[4 * r + 2], x2.x, yi);
+            yr = fmaf(ur[4 * r + 3], x3.x, yr);
+            yi = fmaf(ur[4 * r + 3], x3.y, yi);
+            yr = fmaf(-ui[4 * r + 3], x3.y, yr);
+            yi = fmaf(ui[4 * r + 3], x3.x, yi);
+            y[r] = make_float2(yr, yi);
+        }
+        v[base] = y[0];
+        v[i1] = y[1];
+        v[i2] = y[2];
+        v[i3] = y[3];
+    }
+}
+
+// Bonds of a stage sit on register pairs (0,1), (2,3), (4,5); -1 = none.
+template <int NV>
+__device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
+{
+    static_assert(NV == 64, "register tiles hold 64 amplitudes");
+    if (sp.bond[0] >= 0) gate_on_regs<NV, 0, 1>(v, gl + 16 * sp.bond[0]);
+    if (sp.bond[1] >= 0) gate_on_regs<NV, 2, 3>(v, gl + 16 * sp.bond[1]);
+    if (sp.bond[2] >= 0) gate_on_regs<NV, 4, 5>(v, gl + 16 * sp.bond[2]);
+}
+
+// LDS address pieces of a mapping: thread part T (lane and wave bits) and the
+// per-register-bit XOR terms Q[r] = swz(1 << reg_u[r]).
+template <int WB>
+__device__ __forceinline__ std::uint32_t thread_term(const std::int8_t *lane_u, const std::int8_t *wave_u, int lane,
+                                                     int wave)
+{
+    std::uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) o |= static_cast<std::uint32_t>((lane >> b) & 1) << lane_u[b];
+#pragma unroll
+    for (int b = 0; b < WB; ++b) o |= static_cast<std::uint32_t>((wave >> b) & 1) << wave_u[b];
+    return swz(o);
+}
+
+template <int NV>
+__device__ __forceinline__ void lds_write(float2 *lds, const float2 (&v)[NV], std::uint32_t T, const std::uint32_t (&Q)[6])
+{
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+    {
+        std::uint32_t a = T;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            if ((j >> r) & 1) a ^= Q[r];
+        lds[a] = v[j];
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void lds_read(const float2 *lds, float2 (&v)[NV], std::uint32_t T, const std::uint32_t (&Q)[6])
+{
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+    {
+        std::uint32_t a = T;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            if ((j >> r) & 1) a ^= Q[r];
+        v[j] = lds[a];
+    }
+}
+
+// One sweep over the tiles of a group of trajectories.
+template <int RB, int NT>
+__global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
+{
+    constexpr int NV = 1 << RB;
+    constexpr int WB = (NT == 256) ? 2 : (NT == 128 ? 1 : 0);
+    static_assert(RB == 6, "register tiles hold 64 amplitudes");
+    extern __shared__ float2 lds[];
+    const PassPlan &pl = args.plan;
+    const DevArena &a = args.a;
+    const int k = pl.k;
+    double *hred = reinterpret_cast<double *>(lds + (1u << k));
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int tl = blockIdx.x / pl.ntiles;
+    const int tile = blockIdx.x - tl * pl.ntiles;
+    const int traj = args.traj_list[tl];
+    const TrajDesc d = a.desc[traj];
+    const int t = args.layer;
+    const int c = pl.c, s = pl.s;
+    const std::uint64_t B = pdep64(static_cast<std::uint64_t>(tile), pl.rest_mask);
+    float2 *tp = a.states + d.state_off + B;  // tile base (uniform)
+    const std::uint32_t lowmask = (1u << c) - 1u;
+    // M0 element offset of (lane, wave, register j): thread part + j * jstride
+    const std::uint32_t jshift = static_cast<std::uint32_t>(s + 6 - c);
+    const std::uint32_t toff = (static_cast<std::uint32_t>(lane) & lowmask) |
+                               ((static_cast<std::uint32_t>(lane) >> c) << s) |
+                               (static_cast<std::uint32_t>(wave) << (jshift + RB));
+
+    // ---- load (M0), with the previous layer's collapse or the initial state
+    std::uint64_t cm = 0, cv = 0;
+    float scale = 1.f;
+    if (pl.collapse && t > 0)
+    {
+        const Collapse cc = a.coll[d.layer_off + t - 1];
+        cm = cc.mask;
+        cv = cc.val;
+        scale = static_cast<float>(cc.scale);
+    }
+    const bool tile_zero = (B & cm) != (cv & cm & pl.rest_mask);
+    const std::uint32_t cml = static_cast<std::uint32_t>(cm & ~pl.rest_mask);
+    const std::uint32_t cvl = static_cast<std::uint32_t>(cv & ~pl.rest_mask);
+    const bool gen_init = pl.init && a.init[traj];
+    float2 v[NV];
+    if (gen_init)
+    {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] = initial_amp(d, B | (toff + (static_cast<std::uint32_t>(j) << jshift)));
+    }
+    else if (tile_zero)
+    {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] = make_float2(0.f, 0.f);
+    }
+    else
+    {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+        {
+            const std::uint32_t o = toff + (static_cast<std::uint32_t>(j) << jshift);
+            float2 x = tp[o];
+            const float f = ((o & cml) == cvl) ? scale : 0.f;
+            x.x *= f;
+            x.y *= f;
+            v[j] = x;
+        }
+    }
+
+    // ---- stages
+    const float2 *gl = a.gates + (d.gate_off + gate_prefix(d, t)) * 16;
+    // current mapping: M0 (registers = u bits 6.., lanes = u bits 0..5)
+    std::uint32_t Tcur = swz(static_cast<std::uint32_t>(lane) | (static_cast<std::uint32_t>(wave) << (6 + RB)));
+    std::uint32_t Qcur[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << (6 + r));
+    bool in_m0 = true;
+    bool lds_dirty = false;  // a previous LDS read must finish before the next write
+    for (int si = 0; si < pl.nstages; ++si)
+    {
+        const StagePlan &sp = pl.st[si];
+        if (sp.use_lds)
+        {
+            if (lds_dirty) __syncthreads();
+            lds_write<NV>(lds, v, Tcur, Qcur);
+            __syncthreads();
+            Tcur = thread_term<WB>(sp.lane_u, sp.wave_u, lane, wave);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << sp.reg_u[r]);
+            lds_read<NV>(lds, v, Tcur, Qcur);
+            lds_dirty = true;
+            in_m0 = false;
+        }
+        apply_stage_gates<NV>(v, sp, gl);
+    }
+    // ---- back to M0 through LDS (also leaves the tile in LDS for the histogram)
+    if (!in_m0 || pl.hist)
+    {
+        if (lds_dirty) __syncthreads();
+        lds_write<NV>(lds, v, Tcur, Qcur);
+        __syncthreads();
+        if (!in_m0)
+        {
+            Tcur = swz(static_cast<std::uint32_t>(lane) | (static_cast<std::uint32_t>(wave) << (6 + RB)));
+#pragma unroll
+            for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << (6 + r));
+            lds_read<NV>(lds, v, Tcur, Qcur);
+        }
+    }
+
+    // ---- FP64 histogram partial of this tile over the chunk's local sites
+    if (pl.hist)
+    {
+        auto local_phys = [&](std::uint32_t u) -> std::uint32_t { return (u & lowmask) | ((u >> c) << s); };
+        const std::uint32_t fl = a.flags[d.layer_off + t];
+        // chunk sites: set bits number [chunk*MAX, chunk*MAX + MAX) of fl
+        std::uint32_t chunk_sites = 0;
+        int seen = 0;
+        for (int q = 0; q < d.L; ++q)
+        {
+            if (!((fl >> q) & 1u)) continue;
+            if (seen >= pl.chunk * MAX_HIST_BITS && seen < (pl.chunk + 1) * MAX_HIST_BITS) chunk_sites |= 1u << q;
+            ++seen;
+        }
+        std::uint32_t loc_u = 0;  // u bits of local chunk sites
+        for (int q = 0; q < d.L; ++q)
+        {
+            if (!((chunk_sites >> q) & 1u)) continue;
+            if (q < c) loc_u |= 1u << q;
+            else if (q >= s && q < s + (k - c)) loc_u |= 1u << (q - s + c);
+        }
+        std::uint64_t condm = 0, condv = 0;
+        if (pl.cond)
+        {
+            const Collapse cc = a.coll[d.layer_off + t];
+            condm = cc.mask;
+            condv = cc.val;
+        }
+        const bool tile_out = (B & condm) != (condv & condm & pl.rest_mask);
+        const std::uint32_t cdm = static_cast<std::uint32_t>(condm & ~pl.rest_mask);
+        const std::uint32_t cdv = static_cast<std::uint32_t>(condv & ~pl.rest_mask);
+        const int mloc = __popc(loc_u);
+        const std::uint32_t bins = 1u << mloc;
+        const std::uint32_t full = (1u << k) - 1u;
+        const std::uint32_t rmask = full & ~loc_u;
+        const std::uint32_t R = 1u << (k - mloc);
+        double *out = a.partials + (static_cast<std::size_t>(tl) * pl.ntiles + tile) * HSTRIDE;
+        if (bins >= static_cast<std::uint32_t>(NT))
+        {
+            for (std::uint32_t b = threadIdx.x; b < bins; b += NT)
+            {
+                double acc = 0.0;
+                if (!tile_out)
+                {
+                    const std::uint32_t bb = static_cast<std::uint32_t>(pdep64(b, loc_u));
+                    std::uint32_t x = 0;
+                    for (std::uint32_t e = 0; e < R; ++e)
+                    {
+                        const std::uint32_t u = bb | x;
+                        if ((local_phys(u) & cdm) == cdv)
+                        {
+                            const float2 z = lds[swz(u)];
+                            acc = fma(static_cast<double>(z.x), static_cast<double>(z.x), acc);
+                            acc = fma(static_cast<double>(z.y), static_cast<double>(z.y), acc);
+                        }
+                        x = next_submask(x, rmask);
+                    }
+                }
+                out[b] = acc;
+            }
+        }
+        else
+        {
+            const std::uint32_t P = NT / bins;  // parts per bin
+            const std::uint32_t b = threadIdx.x / P, part = threadIdx.x % P;
+            const std::uint32_t per = R / P;
+            double acc = 0.0;
+            if (!tile_out)
+            {
+                const std::uint32_t bb = static_cast<std::uint32_t>(pdep64(b, loc_u));
+                std::uint32_t x = static_cast<std::uint32_t>(pdep64(part * per, rmask));
+                for (std::uint32_t e = 0; e < per; ++e)
+                {
+                    const std::uint32_t u = bb | x;
+                    if ((local_phys(u) & cdm) == cdv)
+                    {
+                        const float2 z = lds[swz(u)];
+                        acc = fma(static_cast<double>(z.x), static_cast<double>(z.x), acc);
+                        acc = fma(static_cast<double>(z.y), static_cast<double>(z.y), acc);
+                    }
+                    x = next_submask(x, rmask);
+                }
+            }
+            hred[threadIdx.x] = acc;
+            __syncthreads();
+            if (threadIdx.x < bins)
+            {
+                double sum = 0.0;
+                for (std::uint32_t pp = 0; pp < P; ++pp) sum += hred[threadIdx.x * P + pp];
+                out[threadIdx.x] = sum;
+            }
+        }
+    }
+
+    // ---- store (M0)
+    if (pl.store)
+    {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) tp[toff + (static_cast<std::uint32_t>(j) << jshift)] = v[j];
+    }
+}
+
+// Finalize one chunk of a layer's measurements per trajectory: fixed-order
+// reduction of the tile partials into the chunk's joint histogram, then the
+// sequential Born decisions (outcome 1 iff u < P(1)), the near-threshold flags
+// and the collapse record for the next sweep.
+struct FinalizeArgs
+{
+    DevArena a;
+    const int *traj_list;
+    int layer;
+    int chunk;
+    std::uint64_t rest_mask;
+    int ntiles, k, c, s;
+};
+
+__global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f)
+{
+    __shared__ double h[HSTRIDE];
+    const DevArena &a = f.a;
+    const int traj = f.traj_list[blockIdx.x];
+    const TrajDesc d = a.desc[traj];
+    const int t = f.layer;
+    const std::uint32_t fl = a.flags[d.layer_off + t];
+    // chunk sites in ascending order
+    int sites[MAX_HIST_BITS];
+    int m = 0, seen = 0;
+    for (int q = 0; q < d.L; ++q)
+    {
+        if (!((fl >> q) & 1u)) continue;
+        if (seen >= f.chunk * MAX_HIST_BITS && seen < (f.chunk + 1) * MAX_HIST_BITS) sites[m++] = q;
+        ++seen;
+    }
+    Collapse prev = (f.chunk == 0) ? Collapse{0, 0, 1.0, 1.0} : a.coll[d.layer_off + t];
+    if (m == 0)
+    {
+        if (threadIdx.x == 0 && f.chunk == 0)
+        {
+            a.coll[d.layer_off + t] = prev;
+            a.outcomes[d.layer_off + t] = 0;
+            a.flagged[d.layer_off + t] = 0;
+        }
+        return;
+    }
+    // local (u bit) vs tile (physical bit) parts of each global bin bit
+    std::uint32_t loc_u = 0;
+    std::uint64_t non_phys = 0;
+    for (int j = 0; j < m; ++j)
+    {
+        const int q = sites[j];
+        if (q < f.c) loc_u |= 1u << q;
+        else if (q >= f.s && q < f.s + (f.k - f.c)) loc_u |= 1u << (q - f.s + f.c);
+        else non_phys |= std::uint64_t{1} << q;
+    }
+    const std::uint32_t bins = 1u << m;
+    for (std::uint32_t g = threadIdx.x; g < bins; g += blockDim.x)
+    {
+        // bin bit j <-> sites[j]; split into local-bin index and tile pattern
+        std::uint32_t bloc = 0;
+        std::uint64_t pat = 0;
+        int jl = 0;
+        for (int j = 0; j < m; ++j)
+        {
+            const int q = sites[j];
+            const bool local = !((non_phys >> q) & 1u);
+            const std::uint32_t bit = (g >> j) & 1u;
+            if (local)
+            {
+                // local-bin bit order = ascending u bit order of the local sites
+                (void)jl;
+            }
+            else if (bit)
+                pat |= std::uint64_t{1} << q;
+        }
+        // local-bin index: bits of g for local sites, ordered by their u bit
+        int idx = 0;
+        for (int ub = 0; ub < f.k; ++ub)
+        {
+            if (!((loc_u >> ub) & 1u)) continue;
+            const int q = (ub < f.c) ? ub : (ub - f.c + f.s);
+            int j = 0;
+            while (sites[j] != q) ++j;
+            if ((g >> j) & 1u) bloc |= 1u << idx;
+            ++idx;
+        }
+        double sum = 0.0;
+        for (int tile = 0; tile < f.ntiles; ++tile)
+        {
+            const std::uint64_t Bt = pdep64(static_cast<std::uint64_t>(tile), f.rest_mask);
+            if ((Bt & non_phys) != pat) continue;
+            sum += a.partials[(static_cast<std::size_t>(blockIdx.x) * f.ntiles + tile) * HSTRIDE + bloc];
+        }
+        h[g] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double *u = a.uniforms + d.unif_off + static_cast<std::size_t>(t) * d.L;
+    double uu[MAX_HIST_BITS], pp[MAX_HIST_BITS];
+    for (int j = 0; j < m; ++j) uu[j] = u[sites[j]];
+    double total = 0.0;
+    for (std::uint32_t g = 0; g < bins; ++g) total += h[g];
+    int status = 0;
+    const std::uint32_t chosen = sequential_from_hist(h, m, uu, pp, &status);
+    const double w = h[chosen];
+    std::uint32_t outc = (f.chunk == 0) ? 0u : a.outcomes[d.layer_off + t];
+    std::uint32_t flg = (f.chunk == 0) ? 0u : a.flagged[d.layer_off + t];
+    Collapse cc = prev;
+    for (int j = 0; j < m; ++j)
+    {
+        const int q = sites[j];
+        a.p1[d.unif_off + static_cast<std::size_t>(t) * d.L + q] = pp[j];
+        cc.mask |= std::uint64_t{1} << q;
+        if ((chosen >> j) & 1u)
+        {
+            cc.val |= std::uint64_t{1} << q;
+            outc |= 1u << q;
+        }
+        if (fabs(uu[j] - pp[j]) < 1e-6) flg |= 1u << q;
+    }
+    if (status != 0 || !(w > 0.0) || !(total > 0.0))
+    {
+        a.status[traj] = E_NORM;
+        cc.scale = 1.0;
+    }
+    else
+    {
+        cc.weight = w;
+        cc.scale = 1.0 / sqrt(w);
+        a.surprisal[traj] += -log(w / total);
+    }
+    a.coll[d.layer_off + t] = cc;
+    a.outcomes[d.layer_off + t] = outc;
+    a.flagged[d.layer_off + t] = flg;
+}
+
+// Marks the layer's collapse as applied (identity), after the closing sweep.
+__global__ void reset_collapse_kernel(DevArena a, const int *__restrict__ traj_list, int n, int layer)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const TrajDesc d = a.desc[traj_list[i]];
+    if (layer >= 0 && layer < d.layers) a.coll[d.layer_off + layer] = Collapse{0, 0, 1.0, 1.0};
+}
+
+__global__ void fill_int_kernel(int *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ============================================================== host side
+namespace
+{
+struct Config
+{
+    int rb, nt, c;
+    int k() const { return 6 + rb + (nt == 256 ? 2 : nt == 128 ? 1 : 0); }
+};
+
+// Build the sweeps of one layer parity for a chain of L sites in nq qubits.
+struct LayerPlan
+{
+    std::vector<PassPlan> passes;
+};
+
+int ilog2i(int x)
+{
+    int r = 0;
+    while ((1 << r) < x) ++r;
+    return r;
+}
+
+void set_m0(StagePlan &sp, int rb, int wb)
+{
+    for (int r = 0; r < 6; ++r) sp.reg_u[r] = static_cast<std::int8_t>(r < rb ? 6 + r : 0);
+    for (int b = 0; b < 6; ++b) sp.lane_u[b] = static_cast<std::int8_t>(b);
+    for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<std::int8_t>(b < wb ? 6 + rb + b : 0);
+}
+
+// Choose 6 lane bits among `free` so that their low-nibble contributions
+// under swz() span all 16 banks-pairs where possible.
+void pick_lanes(std::vector<int> &freeb, std::int8_t lane_u[6])
+{
+    std::vector<int> chosen;
+    int covered = 0;  // residues mod 4 seen
+    // first pass: one bit per residue class
+    for (int res = 0; res < 4 && chosen.size() < 6; ++res)
+        for (std::size_t i = 0; i < freeb.size(); ++i)
+            if ((freeb[i] & 3) == res && std::find(chosen.begin(), chosen.end(), freeb[i]) == chosen.end())
+            {
+                chosen.push_back(freeb[i]);
+                covered |= 1 << res;
+                break;
+            }
+    for (std::size_t i = 0; i < freeb.size() && chosen.size() < 6; ++i)
+        if (std::find(chosen.begin(), chosen.end(), freeb[i]) == chosen.end()) chosen.push_back(freeb[i]);
+    for (int b = 0; b < 6; ++b) lane_u[b] = static_cast<std::int8_t>(chosen[b]);
+    std::vector<int> rest;
+    for (int x : freeb)
+        if (std::find(chosen.begin(), chosen.end(), x) == chosen.end()) rest.push_back(x);
+    freeb.swap(rest);
+    (void)covered;
+}
+
+int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, LayerPlan &out)
+{
+    out.passes.clear();
+    const int k = cf.k();
+    const int c = cf.c;
+    const int wbits = k - c;
+    const int wb = cf.nt == 256 ? 2 : (cf.nt == 128 ? 1 : 0);
+    if (nq <= k) return E_UNSUPPORTED;
+    // windows over the physical bits c..nq-1, consecutive windows overlapping
+    // by one bit so that every bond (i, i+1) lies in some window
+    std::vector<int> starts;
+    int st = c;
+    while (true)
+    {
+        if (st + wbits >= nq)
+        {
+            starts.push_back(std::max(c, nq - wbits));
+            break;
+        }
+        starts.push_back(st);
+        st = st + wbits - 1;
+    }
+    const int nb = layer_bond_count(L, parity, closed);
+    std::vector<int> b0(nb), b1(nb);
+    for (int b = 0; b < nb; ++b) layer_bond(L, parity, b, b0[b], b1[b]);
+    auto local = [&](int p, int q) { return q < c || (q >= starts[p] && q < starts[p] + wbits); };
+    std::vector<std::vector<int>> assigned(starts.size());
+    // bonds local to exactly one window first, then balance the rest
+    std::vector<int> flexible;
+    for (int b = 0; b < nb; ++b)
+    {
+        std::vector<int> cand;
+        for (std::size_t p = 0; p < starts.size(); ++p)
+            if (local(static_cast<int>(p), b0[b]) && local(static_cast<int>(p), b1[b])) cand.push_back(static_cast<int>(p));
+        if (cand.empty())
+        {
+            set_error("tile plan: bond (%d,%d) is local to no window (L=%d k=%d)", b0[b], b1[b], L, k);
+            return E_UNSUPPORTED;
+        }
+        if (cand.size() == 1) assigned[cand[0]].push_back(b);
+        else flexible.push_back(b);
+    }
+    for (int b : flexible)
+    {
+        int best = -1;
+        for (std::size_t p = 0; p < starts.size(); ++p)
+            if (local(static_cast<int>(p), b0[b]) && local(static_cast<int>(p), b1[b]) &&
+                (best < 0 || assigned[p].size() < assigned[best].size()))
+                best = static_cast<int>(p);
+        assigned[best].push_back(b);
+    }
+    const int per_stage = cf.rb / 2;
+    for (std::size_t p = 0; p < starts.size(); ++p)
+    {
+        PassPlan pp{};
+        const int s = starts[p];
+        pp.k = k;
+        pp.c = c;
+        pp.s = s;
+        std::uint64_t local_mask = (std::uint64_t{1} << c) - 1;
+        local_mask |= ((std::uint64_t{1} << wbits) - 1) << s;
+        pp.rest_mask = ((std::uint64_t{1} << nq) - 1) & ~local_mask;
+        pp.ntiles = 1 << (nq - k);
+        auto u_of = [&](int q) { return q < c ? q : q - s + c; };
+        // bonds that already sit on M0 register pairs
+        std::vector<int> m0_bonds, other;
+        for (int b : assigned[p])
+        {
+            const int ua = u_of(b0[b]), ub = u_of(b1[b]);
+            const int ra = ua - 6, rc = ub - 6;
+            if (ra >= 0 && rc == ra + 1 && (ra % 2) == 0 && rc < cf.rb && static_cast<int>(m0_bonds.size()) < per_stage)
+                m0_bonds.push_back(b);
+            else
+                other.push_back(b);
+        }
+        std::vector<StagePlan> stages;
+        if (!m0_bonds.empty())
+        {
+            StagePlan sp{};
+            set_m0(sp, cf.rb, wb);
+            sp.use_lds = 0;
+            sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
+            for (std::size_t i = 0; i < m0_bonds.size(); ++i)
+                sp.bond[(u_of(b0[m0_bonds[i]]) - 6) / 2] = static_cast<std::int8_t>(m0_bonds[i]);
+            sp.nb = static_cast<std::int8_t>(m0_bonds.size());
+            stages.push_back(sp);
+        }
+        for (std::size_t i = 0; i < other.size(); i += per_stage)
+        {
+            StagePlan sp{};
+            sp.use_lds = 1;
+            sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
+            std::vector<int> used;
+            int nbs = 0;
+            for (std::size_t j = i; j < other.size() && j < i + per_stage; ++j, ++nbs)
+            {
+                const int b = other[j];
+                sp.reg_u[2 * nbs] = static_cast<std::int8_t>(u_of(b0[b]));
+                sp.reg_u[2 * nbs + 1] = static_cast<std::int8_t>(u_of(b1[b]));
+                used.push_back(u_of(b0[b]));
+                used.push_back(u_of(b1[b]));
+                sp.bond[nbs] = static_cast<std::int8_t>(b);
+            }
+            sp.nb = static_cast<std::int8_t>(nbs);
+            std::vector<int> freeb;
+            for (int u = 0; u < k; ++u)
+                if (std::find(used.begin(), used.end(), u) == used.end()) freeb.push_back(u);
+            // fill the remaining register bits first from high u bits (keeps
+            // lanes on bank-friendly bits), then lanes, then waves
+            int r = 2 * nbs;
+            while (r < cf.rb)
+            {
+                // take the largest free bit
+                const int x = freeb.back();
+                freeb.pop_back();
+                sp.reg_u[r++] = static_cast<std::int8_t>(x);
+            }
+            for (int rr = cf.rb; rr < 6; ++rr) sp.reg_u[rr] = 0;
+            pick_lanes(freeb, sp.lane_u);
+            for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<std::int8_t>(b < wb ? freeb[b] : 0);
+            stages.push_back(sp);
+        }
+        if (stages.size() > 6)
+        {
+            set_error("tile plan: too many stages");
+            return E_UNSUPPORTED;
+        }
+        pp.nstages = static_cast<int>(stages.size());
+        for (std::size_t i = 0; i < stages.size(); ++i) pp.st[i] = stages[i];
+        pp.store = 1;
+        out.passes.push_back(pp);
+    }
+    return OK;
+}
+} // namespace
+
+struct Batch
+{
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    mipt_hip_batch_config cfg{};
+    std::vector<mipt_hip_traj_spec> specs;
+    std::vector<TrajDesc> desc;
+    std::vector<int> gen_list;  // trajectories that own their gates
+    DevArena a;
+    int *d_traj_list = nullptr;  // all trajectory indices (grouped order)
+    std::vector<int> order;      // grouped order of trajectories
+    std::size_t states_elems = 0, gates_count = 0, layer_items = 0, unif_items = 0;
+    std::vector<std::uint32_t> h_flags;  // host mirror (for overflow planning)
+    bool flags_on_host = false;
+    std::vector<char> initialized;  // per trajectory: state already set by caller
+
+    // Sampled HIP-event timing of the gate sweeps (pass kernels).
+    int prof_every = 0;
+    long prof_counter = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, double>> ev_used;  // (event pair index, algorithmic bytes)
+    long gate_launches = 0;
+    double gate_bytes = 0.0;
+
+    ~Batch()
+    {
+        release();
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    }
+
+    int timed_launch_begin(double bytes, int &slot)
+    {
+        slot = -1;
+        ++gate_launches;
+        gate_bytes += bytes;
+        if (prof_every <= 0 || (prof_counter++ % prof_every) != 0) return OK;
+        const int pair = static_cast<int>(ev_used.size());
+        while (ev_pool.size() < static_cast<std::size_t>(2 * pair + 2))
+        {
+            hipEvent_t e;
+            MIPT_HIP_CHECK(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        MIPT_HIP_CHECK(hipEventRecord(ev_pool[2 * pair], stream));
+        ev_used.push_back({pair, bytes});
+        slot = pair;
+        return OK;
+    }
+
+    int timed_launch_end(int slot)
+    {
+        if (slot < 0) return OK;
+        MIPT_HIP_CHECK(hipEventRecord(ev_pool[2 * slot + 1], stream));
+        return OK;
+    }
+
+    void release()
+    {
+        void *ptrs[] = {a.states, a.gates, a.desc, a.flags, a.outcomes, a.flagged, a.uniforms,
+                        a.p1, a.coll, a.surprisal, a.status, a.init, a.partials, d_traj_list};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        a = DevArena{};
+        d_traj_list = nullptr;
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        own_stream = false;
+    }
+
+    Config config_for(int nq) const
+    {
+        (void)nq;
+        Config cf{6, 256, 6};
+        if (cfg.tile_bits == 13) cf = Config{6, 128, 5};
+        if (cfg.tile_bits == 12) cf = Config{6, 64, 4};
+        return cf;
+    }
+
+    int setup(const mipt_hip_traj_spec *sp, int count)
+    {
+        void *ptrs[] = {a.states, a.gates, a.desc, a.flags, a.outcomes, a.flagged, a.uniforms,
+                        a.p1, a.coll, a.surprisal, a.status, a.init, a.partials, d_traj_list};
+        for (void *p : ptrs)
+            if (p) MIPT_HIP_CHECK(hipFree(p));
+        a = DevArena{};
+        d_traj_list = nullptr;
+        specs.assign(sp, sp + count);
+        desc.resize(count);
+        gen_list.clear();
+        std::size_t so = 0, go = 0, lo = 0, uo = 0;
+        for (int i = 0; i < count; ++i)
+        {
+            const auto &s = specs[i];
+            if (s.L < 2 || s.L > 30 || (s.L % 2) != 0 || s.layers < 0 || s.gate_family != 1 ||
+                (s.probe_site >= s.L) || s.L + (s.probe_site >= 0 ? 1 : 0) > 30 ||
+                (s.shared_gates >= 0 && (s.shared_gates >= count || s.shared_gates == i)))
+            {
+                set_error("batch_setup: unsupported spec %d (L=%d layers=%d family=%d)", i, s.L, s.layers, s.gate_family);
+                return E_ARG;
+            }
+            TrajDesc &d = desc[i];
+            d.L = s.L;
+            d.nq = s.L + (s.probe_site >= 0 ? 1 : 0);
+            d.layers = s.layers;
+            d.closed = s.closed ? 1 : 0;
+            d.probe_site = s.probe_site;
+            d.p_zero_layers = s.p_zero_layers;
+            d.p = s.p;
+            d.seed = s.seed;
+            d.state_off = so;
+            so += std::size_t{1} << d.nq;
+            d.layer_off = static_cast<std::uint32_t>(lo);
+            lo += s.layers;
+            d.unif_off = static_cast<std::uint32_t>(uo);
+            uo += static_cast<std::size_t>(s.layers) * s.L;
+            if (s.shared_gates < 0)
+            {
+                d.gate_off = go;
+                std::size_t nb = 0;
+                for (int t = 0; t < s.layers; ++t) nb += layer_bond_count(s.L, t, d.closed);
+                go += nb;
+                gen_list.push_back(i);
+            }
+        }
+        for (int i = 0; i < count; ++i)
+            if (specs[i].shared_gates >= 0)
+            {
+                const auto &o = specs[specs[i].shared_gates];
+                if (o.shared_gates >= 0 || o.L != specs[i].L || o.layers < specs[i].layers || o.closed != specs[i].closed)
+                {
+                    set_error("batch_setup: shared_gates of %d must name an owning trajectory of the same shape", i);
+                    return E_ARG;
+                }
+                desc[i].gate_off = desc[specs[i].shared_gates].gate_off;
+            }
+        states_elems = so;
+        gates_count = go;
+        layer_items = lo;
+        unif_items = uo;
+        MIPT_HIP_CHECK(hipMalloc(&a.states, std::max<std::size_t>(so, 1) * sizeof(float2)));
+        MIPT_HIP_CHECK(hipMalloc(&a.gates, std::max<std::size_t>(go, 1) * 16 * sizeof(float2)));
+        MIPT_HIP_CHECK(hipMalloc(&a.desc, count * sizeof(TrajDesc)));
+        MIPT_HIP_CHECK(hipMalloc(&a.flags, std::max<std::size_t>(lo, 1) * 4));
+        MIPT_HIP_CHECK(hipMalloc(&a.outcomes, std::max<std::size_t>(lo, 1) * 4));
+        MIPT_HIP_CHECK(hipMalloc(&a.flagged, std::max<std::size_t>(lo, 1) * 4));
+        MIPT_HIP_CHECK(hipMalloc(&a.coll, std::max<std::size_t>(lo, 1) * sizeof(Collapse)));
+        MIPT_HIP_CHECK(hipMalloc(&a.uniforms, std::max<std::size_t>(uo, 1) * 8));
+        MIPT_HIP_CHECK(hipMalloc(&a.p1, std::max<std::size_t>(uo, 1) * 8));
+        MIPT_HIP_CHECK(hipMalloc(&a.surprisal, count * 8));
+        MIPT_HIP_CHECK(hipMalloc(&a.status, count * 4));
+        MIPT_HIP_CHECK(hipMalloc(&a.init, count * 4));
+        hipLaunchKernelGGL(fill_int_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, a.init, count, 1);
+        MIPT_HIP_CHECK(hipGetLastError());
+        MIPT_HIP_CHECK(hipMemcpyAsync(a.desc, desc.data(), count * sizeof(TrajDesc), hipMemcpyHostToDevice, stream));
+        MIPT_HIP_CHECK(hipMemsetAsync(a.surprisal, 0, count * 8, stream));
+        MIPT_HIP_CHECK(hipMemsetAsync(a.status, 0, count * 4, stream));
+        MIPT_HIP_CHECK(hipMemsetAsync(a.p1, 0, std::max<std::size_t>(uo, 1) * 8, stream));
+        MIPT_HIP_CHECK(hipMemsetAsync(a.outcomes, 0, std::max<std::size_t>(lo, 1) * 4, stream));
+        MIPT_HIP_CHECK(hipMemsetAsync(a.flagged, 0, std::max<std::size_t>(lo, 1) * 4, stream));
+        // grouped order: by qubit count, then index
+        order.resize(count);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            const TrajDesc &a0 = desc[x], &b0 = desc[y];
+            if (a0.nq != b0.nq) return a0.nq < b0.nq;
+            if (a0.L != b0.L) return a0.L < b0.L;
+            if (a0.closed != b0.closed) return a0.closed < b0.closed;
+            return a0.layers < b0.layers;
+        });
+        MIPT_HIP_CHECK(hipMalloc(&d_traj_list, count * sizeof(int)));
+        MIPT_HIP_CHECK(hipMemcpyAsync(d_traj_list, order.data(), count * sizeof(int), hipMemcpyHostToDevice, stream));
+        h_flags.assign(lo, 0);
+        flags_on_host = false;
+        initialized.assign(count, 0);
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        return OK;
+    }
+
+    int generate()
+    {
+        const int count = static_cast<int>(desc.size());
+        if (count == 0) return OK;
+        // gates
+        std::vector<std::uint64_t> item_prefix(gen_list.size() + 1, 0);
+        for (std::size_t i = 0; i < gen_list.size(); ++i)
+        {
+            const TrajDesc &d = desc[gen_list[i]];
+            std::size_t nb = 0;
+            for (int t = 0; t < d.layers; ++t) nb += layer_bond_count(d.L, t, d.closed);
+            item_prefix[i + 1] = item_prefix[i] + nb;
+        }
+        std::vector<std::uint64_t> layer_prefix(count + 1, 0);
+        for (int i = 0; i < count; ++i) layer_prefix[i + 1] = layer_prefix[i] + desc[i].layers;
+        std::uint64_t *d_ip = nullptr, *d_lp = nullptr;
+        int *d_gl = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&d_ip, item_prefix.size() * 8));
+        MIPT_HIP_CHECK(hipMalloc(&d_lp, layer_prefix.size() * 8));
+        MIPT_HIP_CHECK(hipMalloc(&d_gl, std::max<std::size_t>(gen_list.size(), 1) * 4));
+        MIPT_HIP_CHECK(hipMemcpyAsync(d_ip, item_prefix.data(), item_prefix.size() * 8, hipMemcpyHostToDevice, stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(d_lp, layer_prefix.data(), layer_prefix.size() * 8, hipMemcpyHostToDevice, stream));
+        if (!gen_list.empty())
+            MIPT_HIP_CHECK(hipMemcpyAsync(d_gl, gen_list.data(), gen_list.size() * 4, hipMemcpyHostToDevice, stream));
+        const std::uint64_t ng = item_prefix.back();
+        if (ng)
+        {
+            hipLaunchKernelGGL(gen_gates_kernel, dim3(static_cast<unsigned>((ng + 255) / 256)), dim3(256), 0, stream,
+                               a.desc, d_gl, static_cast<int>(gen_list.size()), nullptr, a.gates, ng, d_ip);
+            MIPT_HIP_CHECK(hipGetLastError());
+        }
+        const std::uint64_t nl = layer_prefix.back();
+        if (nl)
+        {
+            hipLaunchKernelGGL(gen_flags_kernel, dim3(static_cast<unsigned>((nl + 255) / 256)), dim3(256), 0, stream,
+                               a.desc, count, a.flags, a.uniforms, nl, d_lp);
+            MIPT_HIP_CHECK(hipGetLastError());
+        }
+        MIPT_HIP_CHECK(hipMemcpyAsync(h_flags.data(), a.flags, layer_items * 4, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        flags_on_host = true;
+        MIPT_HIP_CHECK(hipFree(d_ip));
+        MIPT_HIP_CHECK(hipFree(d_lp));
+        MIPT_HIP_CHECK(hipFree(d_gl));
+        return OK;
+    }
+
+    int set_layers(int traj, const double *g, const std::uint8_t *flags, const double *unif)
+    {
+        if (traj < 0 || traj >= static_cast<int>(desc.size()) || specs[traj].shared_gates >= 0)
+        {
+            set_error("set_layers: bad trajectory (or it shares another's gates)");
+            return E_ARG;
+        }
+        const TrajDesc &d = desc[traj];
+        std::size_t nb = 0;
+        for (int t = 0; t < d.layers; ++t) nb += layer_bond_count(d.L, t, d.closed);
+        std::vector<float2> gf(nb * 16);
+        for (std::size_t i = 0; i < nb * 16; ++i)
+            gf[i] = make_float2(static_cast<float>(g[2 * i]), static_cast<float>(g[2 * i + 1]));
+        std::vector<std::uint32_t> fm(d.layers, 0);
+        for (int t = 0; t < d.layers; ++t)
+            for (int s = 0; s < d.L; ++s)
+                if (flags[static_cast<std::size_t>(t) * d.L + s]) fm[t] |= 1u << s;
+        for (std::size_t i = 0; i < static_cast<std::size_t>(d.layers) * d.L; ++i)
+            if (flags[i] && !(unif[i] >= 0.0 && unif[i] < 1.0))
+            {
+                set_error("set_layers: uniform out of [0,1)");
+                return E_ARG;
+            }
+        MIPT_HIP_CHECK(hipMemcpyAsync(a.gates + d.gate_off * 16, gf.data(), gf.size() * sizeof(float2),
+                                      hipMemcpyHostToDevice, stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(a.flags + d.layer_off, fm.data(), fm.size() * 4, hipMemcpyHostToDevice, stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(a.uniforms + d.unif_off, unif, static_cast<std::size_t>(d.layers) * d.L * 8,
+                                      hipMemcpyHostToDevice, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        std::copy(fm.begin(), fm.end(), h_flags.begin() + d.layer_off);
+        flags_on_host = true;
+        return OK;
+    }
+
+    int set_state(int traj, const float *psi)
+    {
+        if (traj < 0 || traj >= static_cast<int>(desc.size()) || !psi) { set_error("set_state: bad arguments"); return E_ARG; }
+        const TrajDesc &d = desc[traj];
+        MIPT_HIP_CHECK(hipMemcpyAsync(a.states + d.state_off, psi, (std::size_t{1} << d.nq) * 8, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, stream, a.init + traj, 1, 0);
+        MIPT_HIP_CHECK(hipGetLastError());
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        initialized[traj] = 1;
+        return OK;
+    }
+
+    int get_state(int traj, float *psi)
+    {
+        if (traj < 0 || traj >= static_cast<int>(desc.size()) || !psi) { set_error("get_state: bad arguments"); return E_ARG; }
+        const TrajDesc &d = desc[traj];
+        MIPT_HIP_CHECK(hipMemcpyAsync(psi, a.states + d.state_off, (std::size_t{1} << d.nq) * 8, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        return OK;
+    }
+
+    int ensure_partials(std::size_t doubles)
+    {
+        if (doubles <= a.partial_cap) return OK;
+        if (a.partials) MIPT_HIP_CHECK(hipFree(a.partials));
+        a.partials = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&a.partials, doubles * 8));
+        a.partial_cap = doubles;
+        return OK;
+    }
+
+    template <int RB, int NT>
+    int launch_pass(const PassPlan &pl, const int *list, int ntraj, int layer)
+    {
+        PassArgs pa{a, list, layer, pl};
+        const std::size_t lds = (std::size_t{1} << pl.k) * sizeof(float2) + NT * sizeof(double);
+        static bool attr_set = false;
+        if (!attr_set)
+        {
+            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>((std::size_t{1} << 14) * sizeof(float2) + 256 * sizeof(double))));
+            attr_set = true;
+        }
+        hipLaunchKernelGGL((pass_kernel<RB, NT>), dim3(static_cast<unsigned>(ntraj * pl.ntiles)), dim3(NT), lds, stream, pa);
+        MIPT_HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+
+    int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer)
+    {
+        if (cf.nt == 256) return launch_pass<6, 256>(pl, list, ntraj, layer);
+        if (cf.nt == 128) return launch_pass<6, 128>(pl, list, ntraj, layer);
+        return launch_pass<6, 64>(pl, list, ntraj, layer);
+    }
+
+    // Tiled evolution of a group of same-size trajectories (nq > 14).
+    int evolve_group_tiled(const std::vector<int> &grp, int grp_pos, int t0, int t1)
+    {
+        const TrajDesc &d0 = desc[grp[0]];
+        const Config cf = config_for(d0.nq);
+        LayerPlan lp[2];
+        MIPT_HIP_TRY(build_layer_plan(d0.L, d0.nq, d0.closed, 0, cf, lp[0]));
+        MIPT_HIP_TRY(build_layer_plan(d0.L, d0.nq, d0.closed, 1, cf, lp[1]));
+        const int ntraj = static_cast<int>(grp.size());
+        const int *list = d_traj_list + grp_pos;
+        MIPT_HIP_TRY(ensure_partials(static_cast<std::size_t>(ntraj) * (std::size_t{1} << (d0.nq - cf.k())) * HSTRIDE));
+        // maximum measured count per layer over the group (host flags)
+        for (int t = t0; t < t1; ++t)
+        {
+            const LayerPlan &P = lp[t & 1];
+            int maxm = 0;
+            for (int i : grp) maxm = std::max(maxm, __builtin_popcount(h_flags[desc[i].layer_off + t]));
+            const int chunks = (maxm + MAX_HIST_BITS - 1) / MAX_HIST_BITS;
+            for (std::size_t p = 0; p < P.passes.size(); ++p)
+            {
+                PassPlan pl = P.passes[p];
+                pl.collapse = (p == 0) ? 1 : 0;
+                pl.init = (p == 0 && t == 0) ? 1 : 0;
+                pl.hist = (p + 1 == P.passes.size() && chunks > 0) ? 1 : 0;
+                pl.chunk = 0;
+                pl.cond = 0;
+                pl.store = 1;
+                // algorithmic bytes: one read + one write of the states per
+                // layer, shared among the layer's sweeps
+                const double bytes = 16.0 * static_cast<double>(std::size_t{1} << d0.nq) * ntraj /
+                                     static_cast<double>(P.passes.size());
+                int slot;
+                MIPT_HIP_TRY(timed_launch_begin(bytes, slot));
+                MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
+                MIPT_HIP_TRY(timed_launch_end(slot));
+            }
+            const PassPlan &last = P.passes.back();
+            FinalizeArgs fa{a, list, t, 0, last.rest_mask, last.ntiles, last.k, last.c, last.s};
+            hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, stream, fa);
+            MIPT_HIP_CHECK(hipGetLastError());
+            for (int ch = 1; ch < chunks; ++ch)
+            {
+                PassPlan pl = last;
+                pl.nstages = 0;
+                pl.collapse = 0;
+                pl.init = 0;
+                pl.hist = 1;
+                pl.chunk = ch;
+                pl.cond = 1;
+                pl.store = 0;
+                MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
+                FinalizeArgs fc{a, list, t, ch, last.rest_mask, last.ntiles, last.k, last.c, last.s};
+                hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, stream, fc);
+                MIPT_HIP_CHECK(hipGetLastError());
+            }
+        }
+        // apply the last layer's pending collapse
+        if (t1 > t0)
+        {
+            PassPlan pl = lp[(t1 - 1) & 1].passes[0];
+            pl.nstages = 0;
+            pl.collapse = 1;
+            pl.init = 0;
+            pl.hist = 0;
+            pl.store = 1;
+            MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t1));
+            hipLaunchKernelGGL(reset_collapse_kernel, dim3((ntraj + 255) / 256), dim3(256), 0, stream, a, list, ntraj,
+                               t1 - 1);
+            MIPT_HIP_CHECK(hipGetLastError());
+        }
+        return OK;
+    }
+
+    // t1 < 0: every trajectory through its own depth.
+    int evolve(int t0, int t1_req)
+    {
+        if (!flags_on_host)
+        {
+            set_error("evolve: call generate() or set_layers() first");
+            return E_ARG;
+        }
+        const int count = static_cast<int>(order.size());
+        int pos = 0;
+        while (pos < count)
+        {
+            const int nq = desc[order[pos]].nq;
+            int end = pos;
+            while (end < count && desc[order[end]].nq == nq && desc[order[end]].L == desc[order[pos]].L &&
+                   desc[order[end]].closed == desc[order[pos]].closed &&
+                   (t1_req >= 0 || desc[order[end]].layers == desc[order[pos]].layers))
+                ++end;
+            const int t1 = t1_req >= 0 ? t1_req : desc[order[pos]].layers;
+            if (t1 <= t0)
+            {
+                pos = end;
+                continue;
+            }
+            if (nq <= SMALL_MAX_QUBITS)
+            {
+                const std::size_t lds = (std::size_t{1} << nq) * sizeof(float2);
+                static bool attr = false;
+                if (!attr)
+                {
+                    MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&small_evolve_kernel),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       static_cast<int>((std::size_t{1} << SMALL_MAX_QUBITS) * sizeof(float2))));
+                    attr = true;
+                }
+                hipLaunchKernelGGL(small_evolve_kernel, dim3(end - pos), dim3(SMALL_THREADS), lds, stream, a,
+                                   d_traj_list + pos, t0, t1, t0 == 0 ? 1 : 0);
+                MIPT_HIP_CHECK(hipGetLastError());
+            }
+            else
+            {
+                // MALL-sized groups
+                const std::size_t bytes = (std::size_t{1} << nq) * 8;
+                const std::size_t budget = std::size_t{160} << 20;
+                int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
+                for (int q = pos; q < end; q += g)
+                {
+                    const int qe = std::min(end, q + g);
+                    std::vector<int> grp(order.begin() + q, order.begin() + qe);
+                    MIPT_HIP_TRY(evolve_group_tiled(grp, q, t0, t1));
+                }
+            }
+            pos = end;
+        }
+        if (t1_req < 0 || t1_req > t0)
+        {
+            hipLaunchKernelGGL(fill_int_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, a.init, count, 0);
+            MIPT_HIP_CHECK(hipGetLastError());
+            for (auto &x : initialized) x = 1;
+        }
+        return OK;
+    }
+
+    // Re-arm in-kernel initial states and clear per-run records (a new run of
+    // the same trajectories).
+    int reset()
+    {
+        const int count = static_cast<int>(desc.size());
+        if (!count) return OK;
+        hipLaunchKernelGGL(fill_int_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, a.init, count, 1);
+        MIPT_HIP_CHECK(hipGetLastError());
+        MIPT_HIP_CHECK(hipMemsetAsync(a.surprisal, 0, count * 8, stream));
+        MIPT_HIP_CHECK(hipMemsetAsync(a.status, 0, count * 4, stream));
+        std::fill(initialized.begin(), initialized.end(), 0);
+        return OK;
+    }
+
+    int export_layers(int traj, double *g, std::uint8_t *flags, double *unif)
+    {
+        if (traj < 0 || traj >= static_cast<int>(desc.size())) { set_error("export_layers: bad trajectory"); return E_ARG; }
+        const TrajDesc &d = desc[traj];
+        std::size_t nb = 0;
+        for (int t = 0; t < d.layers; ++t) nb += layer_bond_count(d.L, t, d.closed);
+        if (g)
+        {
+            std::vector<float2> gf(nb * 16);
+            MIPT_HIP_CHECK(hipMemcpyAsync(gf.data(), a.gates + d.gate_off * 16, gf.size() * sizeof(float2),
+                                          hipMemcpyDeviceToHost, stream));
+            MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+            for (std::size_t i = 0; i < gf.size(); ++i)
+            {
+                g[2 * i] = gf[i].x;
+                g[2 * i + 1] = gf[i].y;
+            }
+        }
+        std::vector<std::uint32_t> fm(d.layers);
+        MIPT_HIP_CHECK(hipMemcpyAsync(fm.data(), a.flags + d.layer_off, d.layers * 4, hipMemcpyDeviceToHost, stream));
+        if (unif)
+            MIPT_HIP_CHECK(hipMemcpyAsync(unif, a.uniforms + d.unif_off, static_cast<std::size_t>(d.layers) * d.L * 8,
+                                          hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        if (flags)
+            for (int t = 0; t < d.layers; ++t)
+                for (int q = 0; q < d.L; ++q) flags[static_cast<std::size_t>(t) * d.L + q] = (fm[t] >> q) & 1u;
+        return OK;
+    }
+
+    int pass_stats(double *avg_ms, int *timed, double *bytes_avg, long *launches, double *bytes_total)
+    {
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        double ms_sum = 0.0, b_sum = 0.0;
+        for (const auto &u : ev_used)
+        {
+            float ms = 0.f;
+            MIPT_HIP_CHECK(hipEventElapsedTime(&ms, ev_pool[2 * u.first], ev_pool[2 * u.first + 1]));
+            ms_sum += ms;
+            b_sum += u.second;
+        }
+        const int n = static_cast<int>(ev_used.size());
+        if (avg_ms) *avg_ms = n ? ms_sum / n : 0.0;
+        if (timed) *timed = n;
+        if (bytes_avg) *bytes_avg = n ? b_sum / n : 0.0;
+        if (launches) *launches = gate_launches;
+        if (bytes_total) *bytes_total = gate_bytes;
+        return OK;
+    }
+
+    int outcomes(int traj, int *out, std::uint8_t *flagged, double *p1, double *surprisal)
+    {
+        if (traj < 0 || traj >= static_cast<int>(desc.size())) { set_error("outcomes: bad trajectory"); return E_ARG; }
+        const TrajDesc &d = desc[traj];
+        std::vector<std::uint32_t> o(d.layers), f(d.layers), fl(d.layers);
+        MIPT_HIP_CHECK(hipMemcpyAsync(o.data(), a.outcomes + d.layer_off, d.layers * 4, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(f.data(), a.flagged + d.layer_off, d.layers * 4, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(fl.data(), a.flags + d.layer_off, d.layers * 4, hipMemcpyDeviceToHost, stream));
+        std::vector<double> pp(static_cast<std::size_t>(d.layers) * d.L);
+        if (!pp.empty())
+            MIPT_HIP_CHECK(hipMemcpyAsync(pp.data(), a.p1 + d.unif_off, pp.size() * 8, hipMemcpyDeviceToHost, stream));
+        double sp = 0;
+        int st = 0;
+        MIPT_HIP_CHECK(hipMemcpyAsync(&sp, a.surprisal + traj, 8, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(&st, a.status + traj, 4, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int t = 0; t < d.layers; ++t)
+            for (int s = 0; s < d.L; ++s)
+            {
+                const std::size_t i = static_cast<std::size_t>(t) * d.L + s;
+                const bool meas = (fl[t] >> s) & 1u;
+                if (out) out[i] = meas ? static_cast<int>((o[t] >> s) & 1u) : -1;
+                if (flagged) flagged[i] = meas ? static_cast<std::uint8_t>((f[t] >> s) & 1u) : 0;
+                if (p1) p1[i] = meas ? pp[i] : -1.0;
+            }
+        if (surprisal) *surprisal = sp;
+        if (st != 0)
+        {
+            set_error("trajectory %d hit a zero-weight measurement branch", traj);
+            return st;
+        }
+        return OK;
+    }
+};
+
+} // namespace mipt_hip
+
+using namespace mipt_hip;
+
+struct mipt_hip_batch : Batch
+{
+};
+
+extern "C" {
+
+int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *stream, mipt_hip_batch **out)
+{
+    if (!out) { set_error("batch_create: null out"); return E_ARG; }
+    auto *b = new (std::nothrow) mipt_hip_batch();
+    if (!b) return E_NOMEM;
+    if (config) b->cfg = *config;
+    if (b->cfg.tile_bits != 0 && (b->cfg.tile_bits < 12 || b->cfg.tile_bits > 14))
+    {
+        delete b;
+        set_error("batch_create: tile_bits must be 0 or 12..14");
+        return E_ARG;
+    }
+    if (stream) b->stream = static_cast<hipStream_t>(stream);
+    else
+    {
+        const hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e != hipSuccess)
+        {
+            delete b;
+            set_error("hipStreamCreate: %s", hipGetErrorString(e));
+            return static_cast<int>(e);
+        }
+        b->own_stream = true;
+    }
+    *out = b;
+    return OK;
+}
+
+int mipt_hip_batch_destroy(mipt_hip_batch *b)
+{
+    if (b) (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return OK;
+}
+
+int mipt_hip_batch_setup(mipt_hip_batch *b, const mipt_hip_traj_spec *specs, int count)
+{
+    if (!b || !specs || count <= 0) { set_error("batch_setup: bad arguments"); return E_ARG; }
+    return b->setup(specs, count);
+}
+
+int mipt_hip_batch_generate(mipt_hip_batch *b)
+{
+    if (!b) { set_error("batch_generate: null batch"); return E_ARG; }
+    return b->generate();
+}
+
+int mipt_hip_batch_set_layers(mipt_hip_batch *b, int traj, const double *g, const uint8_t *f, const double *u)
+{
+    if (!b || !g || !f || !u) { set_error("batch_set_layers: null argument"); return E_ARG; }
+    return b->set_layers(traj, g, f, u);
+}
+
+int mipt_hip_batch_set_state(mipt_hip_batch *b, int traj, const float *psi)
+{
+    if (!b) { set_error("batch_set_state: null batch"); return E_ARG; }
+    return b->set_state(traj, psi);
+}
+
+int mipt_hip_batch_get_state(mipt_hip_batch *b, int traj, float *psi)
+{
+    if (!b) { set_error("batch_get_state: null batch"); return E_ARG; }
+    return b->get_state(traj, psi);
+}
+
+void *mipt_hip_batch_state_ptr(mipt_hip_batch *b, int traj)
+{
+    if (!b || traj < 0 || traj >= static_cast<int>(b->desc.size())) return nullptr;
+    return b->a.states + b->desc[traj].state_off;
+}
+
+int mipt_hip_batch_evolve(mipt_hip_batch *b)
+{
+    if (!b) { set_error("batch_evolve: null batch"); return E_ARG; }
+    return b->evolve(0, -1);
+}
+
+int mipt_hip_batch_evolve_range(mipt_hip_batch *b, int t0, int t1)
+{
+    if (!b || t0 < 0 || t1 < t0) { set_error("batch_evolve_range: bad arguments"); return E_ARG; }
+    for (const auto &d : b->desc)
+        if (t1 > d.layers) { set_error("batch_evolve_range: layer beyond a trajectory's depth"); return E_ARG; }
+    return b->evolve(t0, t1);
+}
+
+int mipt_hip_batch_synchronize(mipt_hip_batch *b)
+{
+    if (!b) { set_error("batch_synchronize: null batch"); return E_ARG; }
+    MIPT_HIP_CHECK(hipStreamSynchronize(b->stream));
+    return OK;
+}
+
+int mipt_hip_batch_outcomes(mipt_hip_batch *b, int traj, int *o, uint8_t *f, double *p1, double *s)
+{
+    if (!b) { set_error("batch_outcomes: null batch"); return E_ARG; }
+    return b->outcomes(traj, o, f, p1, s);
+}
+
+void *mipt_hip_batch_stream(mipt_hip_batch *b) { return b ? static_cast<void *>(b->stream) : nullptr; }
+
+int mipt_hip_batch_reset(mipt_hip_batch *b)
+{
+    if (!b) { set_error("batch_reset: null batch"); return E_ARG; }
+    return b->reset();
+}
+
+int mipt_hip_batch_export_layers(mipt_hip_batch *b, int traj, double *g, uint8_t *f, double *u)
+{
+    if (!b) { set_error("batch_export_layers: null batch"); return E_ARG; }
+    return b->export_layers(traj, g, f, u);
+}
+
+int mipt_hip_batch_set_profiling(mipt_hip_batch *b, int sample_every)
+{
+    if (!b || sample_every < 0) { set_error("batch_set_profiling: bad arguments"); return E_ARG; }
+    b->prof_every = sample_every;
+    b->prof_counter = 0;
+    b->ev_used.clear();
+    b->gate_launches = 0;
+    b->gate_bytes = 0.0;
+    return OK;
+}
+
+int mipt_hip_batch_pass_stats(mipt_hip_batch *b, double *avg_ms, int *timed, double *bytes_avg, long *launches,
+                              double *bytes_total)
+{
+    if (!b) { set_error("batch_pass_stats: null batch"); return E_ARG; }
+    return b->pass_stats(avg_ms, timed, bytes_avg, launches, bytes_total);
+}
+
+int mipt_hip_batch_observables(mipt_hip_batch *b, int traj, int tmi_block, double *s_half, double *tmi,
+                               double *probe_entropy, double *bloch3)
+{
+    if (!b || traj < 0 || traj >= static_cast<int>(b->desc.size())) { set_error("batch_observables: bad arguments"); return E_ARG; }
+    const TrajDesc &d = b->desc[traj];
+    const float2 *psi = b->a.states + d.state_off;
+    if (s_half)
+    {
+        std::vector<int> kept(d.L / 2);
+        std::iota(kept.begin(), kept.end(), 0);
+        MIPT_HIP_TRY(entropy_bits_f32(psi, d.nq, kept.data(), static_cast<int>(kept.size()), b->stream, s_half));
+    }
+    if (tmi)
+    {
+        const int block = tmi_block > 0 ? tmi_block : d.L / 4;
+        MIPT_HIP_TRY(tmi_bits_f32(psi, d.nq, d.L, block, 0, b->stream, tmi, nullptr));
+    }
+    if (probe_entropy || bloch3)
+    {
+        if (d.probe_site < 0)
+        {
+            if (probe_entropy) *probe_entropy = NAN;
+            if (bloch3) bloch3[0] = bloch3[1] = bloch3[2] = NAN;
+        }
+        else
+        {
+            double r[4];
+            MIPT_HIP_TRY(rho1_f32(psi, d.nq, d.L, b->stream, r));
+            const double tr = r[0] + r[1];
+            const double a0 = r[0] / tr, a1 = r[1] / tr, x = r[2] / tr, y = r[3] / tr;
+            if (bloch3)
+            {
+                bloch3[0] = 2 * x;
+                bloch3[1] = -2 * y;
+                bloch3[2] = a0 - a1;
+            }
+            if (probe_entropy)
+            {
+                // cpp/include/mipt/ancilla_mi.hpp:24-52 (nats)
+                const double disc = std::min(1.0, std::max(0.0, std::sqrt(std::max(0.0, (a0 - a1) * (a0 - a1) + 4.0 * (x * x + y * y)))));
+                const double l0 = 0.5 * (1 + disc), l1 = 0.5 * (1 - disc);
+                *probe_entropy = (l0 > 0 ? -l0 * std::log(l0) : 0.0) + (l1 > 0 ? -l1 * std::log(l1) : 0.0);
+            }
+        }
+    }
+    return OK;
+}
+
+} // extern "C"

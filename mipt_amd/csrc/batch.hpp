@@ -1,0 +1,63 @@
+// Batched trajectory engine: shared types of batch.hip and entropy.hip.
+#pragma once
+
+#include "common.hpp"
+#include "mipt_hip.h"
+
+#include <vector>
+
+namespace mipt_hip
+{
+
+constexpr int MAX_HIST_BITS = 10;          // measured sites per fused histogram chunk
+constexpr int HSTRIDE = 1 << MAX_HIST_BITS;  // doubles per tile partial
+constexpr int SMALL_MAX_QUBITS = 14;       // whole state LDS-resident (128 KiB)
+
+// Read-only per-trajectory description (device).
+struct TrajDesc
+{
+    std::uint64_t state_off;  // complex64 element offset in the state arena
+    std::uint64_t gate_off;   // gate (16 x complex64) offset in the gate arena
+    std::uint32_t layer_off;  // offset into per-layer arrays (flags, outcomes, collapse)
+    std::uint32_t unif_off;   // offset into per-(layer, site) arrays (uniforms, p1)
+    int L, nq, layers, closed;
+    int probe_site, p_zero_layers;
+    double p;
+    std::uint64_t seed;
+};
+
+// Pending collapse of one layer's measurements (phys bit q = qubit q).
+struct Collapse
+{
+    std::uint64_t mask;
+    std::uint64_t val;
+    double scale;     // 1/sqrt(raw joint weight of the outcomes)
+    double weight;    // raw joint weight of the decided chunks
+};
+
+// Device-resident arena of a batch.
+struct DevArena
+{
+    float2 *states = nullptr;
+    float2 *gates = nullptr;
+    TrajDesc *desc = nullptr;
+    std::uint32_t *flags = nullptr;      // [layer_off + t] site mask
+    std::uint32_t *outcomes = nullptr;   // [layer_off + t] outcome mask
+    std::uint32_t *flagged = nullptr;    // [layer_off + t] near-threshold mask
+    double *uniforms = nullptr;          // [unif_off + t*L + s]
+    double *p1 = nullptr;                // [unif_off + t*L + s]
+    Collapse *coll = nullptr;            // [layer_off + t]
+    double *surprisal = nullptr;         // [traj]
+    int *status = nullptr;               // [traj]
+    int *init = nullptr;                 // [traj] 1: build the initial state in-kernel at layer 0
+    double *partials = nullptr;          // [group traj][tile][HSTRIDE]
+    std::size_t partial_cap = 0;         // doubles
+};
+
+// Entropy/observable helpers implemented in entropy.hip, used by the batch.
+int entropy_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
+int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream, double *out,
+                 double *terms);
+int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4);
+
+} // namespace mipt_hip

@@ -1,0 +1,697 @@
+// Reduced density matrices, entanglement entropies, TMI and the probe rho1.
+//
+// Reference behaviour (paths relative to the reference repository root):
+//   cpp/include/mipt/tmi/linalg.hpp:885-1026     kept/env offsets, Schmidt gather
+//   cpp/include/mipt/tmi/linalg.hpp:125-141, 201-208  entropy cut-offs, Renyi-2
+//   cpp/include/mipt/cuda/schmidt_gram.cuh:270-315, 538-724  gather + herk
+//   cpp/src/cuda/tmi_cuda_svd.cu:663-798, 801-851 entropy C ABI
+//   cpp/src/cuda/tmi_cuda_rdm.cu:59-144          RDM export (row-major interleaved)
+//   cpp/include/mipt/tmi/compute.hpp:89-113, 186-227  TMI regions and formula
+//   cpp/src/cuda/ancilla_rdm.cu:228-297          probe rho1
+//
+// gfx950 design: the Schmidt matrix M (2^kept x 2^env) is gathered once into
+// split real/imaginary FP32 planes; rho = M M^H is computed as the two real
+// GEMMs rho_r = Ar Ar^T + Ai Ai^T and rho_i = Ai Ar^T - Ar Ai^T on MFMA,
+// lower-triangle 64x64 tiles only, split-K, then a fixed-order FP64 split-K
+// reduction.  The products run on FP64 MFMA (exact products of the complex64
+// inputs, FP64 accumulation) by default, or on FP32 MFMA with FP64 promotion
+// every 16-deep K step.  Eigenvalues of the FP64 rho come from rocSOLVER
+// zheevd; the -sum lambda log2 lambda is FP64.
+#include "batch.hpp"
+#include "state_ops.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <numeric>
+#include <vector>
+
+namespace mipt_hip
+{
+namespace
+{
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int GT = 64;         // output tile
+constexpr int GK = 16;         // K step staged through LDS
+constexpr int KCHUNK = 16;     // FP32 accumulation depth before FP64 promotion
+
+// M[r, c] = psi[env_off(c) | row_off(r)], written as planes Ar[r][c], Ai[r][c].
+template <typename C>
+__global__ void gather_kernel(const C *__restrict__ psi, const std::uint64_t *__restrict__ row_off, int kept, int env,
+                              std::uint64_t env_mask, float *__restrict__ ar, float *__restrict__ ai)
+{
+    const std::uint64_t cols = std::uint64_t{1} << env;
+    const std::uint64_t total = cols << kept;
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t id = blockIdx.x * static_cast<std::uint64_t>(blockDim.x) + threadIdx.x; id < total; id += stride)
+    {
+        const std::uint64_t r = id >> env;
+        const std::uint64_t c = id & (cols - 1);
+        const C v = psi[pdep64(c, env_mask) | row_off[r]];
+        ar[id] = static_cast<float>(v.x);
+        ai[id] = static_cast<float>(v.y);
+    }
+}
+
+// Lower-triangle tile (bi >= bj) of rho over K range [k0, k1) of split `sp`;
+// FP64 partial written to part[sp][tile][64][64] as (re, im).
+__global__ void __launch_bounds__(256) gram_mfma_kernel(const float *__restrict__ ar, const float *__restrict__ ai,
+                                                        int m, int K, int ksplit, int ntiles_tri,
+                                                        double2 *__restrict__ part)
+{
+    __shared__ float sA_r[GT][GK + 1], sA_i[GT][GK + 1], sB_r[GT][GK + 1], sB_i[GT][GK + 1];
+    const int tile = blockIdx.x % ntiles_tri;
+    const int sp = blockIdx.x / ntiles_tri;
+    // tile -> (bi, bj) with bj <= bi
+    int bi = static_cast<int>((sqrt(8.0 * tile + 1.0) - 1.0) / 2.0);
+    while ((bi + 1) * (bi + 2) / 2 <= tile) ++bi;
+    while (bi * (bi + 1) / 2 > tile) --bi;
+    const int bj = tile - bi * (bi + 1) / 2;
+    const int kper = (K + ksplit - 1) / ksplit;
+    const int k0 = sp * kper, k1 = min(K, k0 + kper);
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wi = wave >> 1, wj = wave & 1;  // 32x32 quadrant of the 64x64 tile
+    floatx4 accr[2][2], acci[2][2];
+    double dr[2][2][4], di[2][2][4];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+        {
+            accr[x][y] = floatx4{0, 0, 0, 0};
+            acci[x][y] = floatx4{0, 0, 0, 0};
+            for (int e = 0; e < 4; ++e) dr[x][y][e] = di[x][y][e] = 0.0;
+        }
+    const int lrow = threadIdx.x >> 2;       // 0..63
+    const int lk = (threadIdx.x & 3) * 4;    // 0,4,8,12
+    int since = 0;
+    for (int kk = k0; kk < k1; kk += GK)
+    {
+        // stage 64 rows x 16 k of the i-block and j-block
+        const int gi = bi * GT + lrow, gj = bj * GT + lrow;
+        for (int e = 0; e < 4; ++e)
+        {
+            const int k = kk + lk + e;
+            const bool okk = k < k1;
+            sA_r[lrow][lk + e] = (gi < m && okk) ? ar[static_cast<std::size_t>(gi) * K + k] : 0.f;
+            sA_i[lrow][lk + e] = (gi < m && okk) ? ai[static_cast<std::size_t>(gi) * K + k] : 0.f;
+            sB_r[lrow][lk + e] = (gj < m && okk) ? ar[static_cast<std::size_t>(gj) * K + k] : 0.f;
+            sB_i[lrow][lk + e] = (gj < m && okk) ? ai[static_cast<std::size_t>(gj) * K + k] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GK; ks += 4)
+        {
+            const int kq = ks + (lane >> 4);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+            {
+                const int ra = wi * 32 + x * 16 + (lane & 15);
+                const float a_r = sA_r[ra][kq], a_i = sA_i[ra][kq];
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                {
+                    const int rb = wj * 32 + y * 16 + (lane & 15);
+                    const float b_r = sB_r[rb][kq], b_i = sB_i[rb][kq];
+                    accr[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b_r, accr[x][y], 0, 0, 0);
+                    accr[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_i, accr[x][y], 0, 0, 0);
+                    acci[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b_r, acci[x][y], 0, 0, 0);
+                    acci[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_r, b_i, acci[x][y], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+        since += GK;
+        if (since >= KCHUNK || kk + GK >= k1)
+        {
+            for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y)
+                    for (int e = 0; e < 4; ++e)
+                    {
+                        dr[x][y][e] += static_cast<double>(accr[x][y][e]);
+                        di[x][y][e] += static_cast<double>(acci[x][y][e]);
+                        accr[x][y][e] = 0.f;
+                        acci[x][y][e] = 0.f;
+                    }
+            since = 0;
+        }
+    }
+    // D layout of 16x16x4: lane holds rows 4*(lane/16)+e, column lane%16
+    double2 *out = part + (static_cast<std::size_t>(sp) * ntiles_tri + tile) * GT * GT;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+            for (int e = 0; e < 4; ++e)
+            {
+                const int r = wi * 32 + x * 16 + 4 * (lane >> 4) + e;
+                const int c = wj * 32 + y * 16 + (lane & 15);
+                out[r * GT + c] = make_double2(dr[x][y][e], di[x][y][e]);
+            }
+}
+
+// Same tiling with exact products: the complex64 inputs are widened to FP64
+// and multiplied on FP64 MFMA (v_mfma_f64_16x16x4f64) with FP64 accumulation,
+// so rho carries only the rounding of the complex64 state itself.
+__global__ void __launch_bounds__(256) gram_mfma64_kernel(const float *__restrict__ ar, const float *__restrict__ ai,
+                                                          int m, int K, int ksplit, int ntiles_tri,
+                                                          double2 *__restrict__ part)
+{
+    __shared__ float sA_r[GT][GK + 1], sA_i[GT][GK + 1], sB_r[GT][GK + 1], sB_i[GT][GK + 1];
+    const int tile = blockIdx.x % ntiles_tri;
+    const int sp = blockIdx.x / ntiles_tri;
+    int bi = static_cast<int>((sqrt(8.0 * tile + 1.0) - 1.0) / 2.0);
+    while ((bi + 1) * (bi + 2) / 2 <= tile) ++bi;
+    while (bi * (bi + 1) / 2 > tile) --bi;
+    const int bj = tile - bi * (bi + 1) / 2;
+    const int kper = (K + ksplit - 1) / ksplit;
+    const int k0 = sp * kper, k1 = min(K, k0 + kper);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wi = wave >> 1, wj = wave & 1;
+    doublex4 accr[2][2], acci[2][2];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+        {
+            accr[x][y] = doublex4{0, 0, 0, 0};
+            acci[x][y] = doublex4{0, 0, 0, 0};
+        }
+    const int lrow = threadIdx.x >> 2;
+    const int lk = (threadIdx.x & 3) * 4;
+    for (int kk = k0; kk < k1; kk += GK)
+    {
+        const int gi = bi * GT + lrow, gj = bj * GT + lrow;
+        for (int e = 0; e < 4; ++e)
+        {
+            const int k = kk + lk + e;
+            const bool okk = k < k1;
+            sA_r[lrow][lk + e] = (gi < m && okk) ? ar[static_cast<std::size_t>(gi) * K + k] : 0.f;
+            sA_i[lrow][lk + e] = (gi < m && okk) ? ai[static_cast<std::size_t>(gi) * K + k] : 0.f;
+            sB_r[lrow][lk + e] = (gj < m && okk) ? ar[static_cast<std::size_t>(gj) * K + k] : 0.f;
+            sB_i[lrow][lk + e] = (gj < m && okk) ? ai[static_cast<std::size_t>(gj) * K + k] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GK; ks += 4)
+        {
+            const int kq = ks + (lane >> 4);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+            {
+                const int ra = wi * 32 + x * 16 + (lane & 15);
+                const double a_r = sA_r[ra][kq], a_i = sA_i[ra][kq];
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                {
+                    const int rb = wj * 32 + y * 16 + (lane & 15);
+                    const double b_r = sB_r[rb][kq], b_i = sB_i[rb][kq];
+                    accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_r, accr[x][y], 0, 0, 0);
+                    accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_i, accr[x][y], 0, 0, 0);
+                    acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_r, acci[x][y], 0, 0, 0);
+                    acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r, b_i, acci[x][y], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of f64 16x16x4: lane holds rows lane/16 + 4*e, column lane%16
+    double2 *out = part + (static_cast<std::size_t>(sp) * ntiles_tri + tile) * GT * GT;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+            for (int e = 0; e < 4; ++e)
+            {
+                const int r = wi * 32 + x * 16 + (lane >> 4) + 4 * e;
+                const int c = wj * 32 + y * 16 + (lane & 15);
+                out[r * GT + c] = make_double2(accr[x][y][e], acci[x][y][e]);
+            }
+}
+
+// Sum the split-K partials in order into rho (m x m, row-major double2),
+// mirroring the lower triangle and forcing a real diagonal.
+__global__ void gram_reduce_kernel(const double2 *__restrict__ part, int m, int ksplit, int ntiles_tri,
+                                   double2 *__restrict__ rho)
+{
+    const std::size_t id = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    if (id >= static_cast<std::size_t>(m) * m) return;
+    const int r = static_cast<int>(id / m), c = static_cast<int>(id % m);
+    const bool lower = r >= c;
+    const int rr = lower ? r : c, cc = lower ? c : r;
+    const int bi = rr / GT, bj = cc / GT;
+    const int tile = bi * (bi + 1) / 2 + bj;
+    const int lr = rr % GT, lc = cc % GT;
+    double sr = 0.0, si = 0.0;
+    for (int sp = 0; sp < ksplit; ++sp)
+    {
+        const double2 v = part[(static_cast<std::size_t>(sp) * ntiles_tri + tile) * GT * GT + lr * GT + lc];
+        sr += v.x;
+        si += v.y;
+    }
+    if (r == c) si = 0.0;
+    rho[id] = lower ? make_double2(sr, si) : make_double2(sr, -si);
+}
+
+// FP64 Gram for complex128 states (drop-in _f64 path): rho[r][c] = sum_k M[r,k] conj(M[c,k]).
+__global__ void gram_f64_kernel(const double2 *__restrict__ psi, const std::uint64_t *__restrict__ row_off, int m,
+                                int env, std::uint64_t env_mask, double2 *__restrict__ rho)
+{
+    const std::size_t id = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    if (id >= static_cast<std::size_t>(m) * m) return;
+    const int r = static_cast<int>(id / m), c = static_cast<int>(id % m);
+    if (c > r) return;
+    const std::uint64_t cols = std::uint64_t{1} << env;
+    double sr = 0.0, si = 0.0;
+    for (std::uint64_t k = 0; k < cols; ++k)
+    {
+        const std::uint64_t e = pdep64(k, env_mask);
+        const double2 a = psi[e | row_off[r]], b = psi[e | row_off[c]];
+        sr += a.x * b.x + a.y * b.y;
+        si += a.y * b.x - a.x * b.y;
+    }
+    if (r == c) si = 0.0;
+    rho[static_cast<std::size_t>(r) * m + c] = make_double2(sr, si);
+    rho[static_cast<std::size_t>(c) * m + r] = make_double2(sr, -si);
+}
+
+__global__ void purity_partial_kernel(const double2 *__restrict__ rho, std::size_t n2, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n2;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x)
+    {
+        const double2 v = rho[i];
+        acc += v.x * v.x + v.y * v.y;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+template <typename C>
+__global__ void rho1_kernel(const C *__restrict__ psi, int n, int q, double *__restrict__ part)
+{
+    __shared__ double red[4][256];
+    const std::uint64_t env = std::uint64_t{1} << (n - 1);
+    const std::uint64_t low = (std::uint64_t{1} << q) - 1;
+    double a00 = 0, a11 = 0, re = 0, im = 0;
+    for (std::uint64_t e = blockIdx.x * static_cast<std::uint64_t>(blockDim.x) + threadIdx.x; e < env;
+         e += static_cast<std::uint64_t>(gridDim.x) * blockDim.x)
+    {
+        const std::uint64_t i0 = (e & low) | ((e & ~low) << 1);
+        const C a = psi[i0], b = psi[i0 | (std::uint64_t{1} << q)];
+        const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+        a00 += ar * ar + ai * ai;
+        a11 += br * br + bi * bi;
+        re += ar * br + ai * bi;
+        im += ai * br - ar * bi;
+    }
+    red[0][threadIdx.x] = a00;
+    red[1][threadIdx.x] = a11;
+    red[2][threadIdx.x] = re;
+    red[3][threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            for (int v = 0; v < 4; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < 4; ++v) part[blockIdx.x * 4 + v] = red[v][0];
+}
+
+// ------------------------------------------------------------ host helpers
+struct Workspace
+{
+    float *ar = nullptr, *ai = nullptr;
+    std::size_t plane_cap = 0, plane_cap_i = 0;
+    double2 *part = nullptr;
+    std::size_t part_cap = 0;
+    double2 *rho = nullptr;
+    std::size_t rho_cap = 0;
+    double *evals = nullptr, *e = nullptr;
+    std::size_t ev_cap = 0, e_cap = 0;
+    int *info = nullptr;
+    std::uint64_t *row_off = nullptr;
+    std::size_t row_cap = 0;
+    double *red = nullptr;
+    rocblas_handle handle = nullptr;
+};
+
+// Process-lifetime per-thread workspace (never freed: the reference keeps its
+// workspaces thread_local too, and freeing at thread exit could run after the
+// HIP runtime is gone).
+thread_local Workspace *t_ws = nullptr;
+
+Workspace &ws()
+{
+    if (!t_ws) t_ws = new Workspace();
+    return *t_ws;
+}
+
+template <typename T>
+int grow(T *&p, std::size_t &cap, std::size_t need)
+{
+    if (need <= cap) return OK;
+    if (p) MIPT_HIP_CHECK(hipFree(p));
+    p = nullptr;
+    MIPT_HIP_CHECK(hipMalloc(&p, need * sizeof(T)));
+    cap = need;
+    return OK;
+}
+
+int validate_kept(int n, const int *kept, int count, std::uint64_t &kmask)
+{
+    kmask = 0;
+    if (!kept || count < 0 || count > 30 || n < 1 || n > 34 || count > n) return E_ARG;
+    for (int j = 0; j < count; ++j)
+    {
+        if (kept[j] < 0 || kept[j] >= n || ((kmask >> kept[j]) & 1u)) return E_ARG;
+        kmask |= std::uint64_t{1} << kept[j];
+    }
+    return OK;
+}
+
+// Gram arithmetic of the complex64 path: 64 (default) = FP64 MFMA with exact
+// products; 32 = FP32 MFMA promoted to FP64 every 16-deep K step
+// (MIPT_HIP_GRAM_PRECISION=32).  The FP32 form measured up to 1e-5 bits of
+// entropy error on 256x256 half-chain RDMs, so it is not the default.
+int gram_precision()
+{
+    static const int p = [] {
+        const char *e = std::getenv("MIPT_HIP_GRAM_PRECISION");
+        if (!e || !*e) return 64;
+        const int v = std::atoi(e);
+        return v == 32 ? 32 : 64;
+    }();
+    return p;
+}
+
+// rho (FP64, device, row-major m x m) of the kept modes of psi.
+int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int &m_out)
+{
+    std::uint64_t kmask;
+    if (validate_kept(n, kept, count, kmask) != OK)
+    {
+        set_error("invalid kept-mode list");
+        return E_ARG;
+    }
+    Workspace &w = ws();
+    const int m = 1 << count;
+    const int env = n - count;
+    const std::uint64_t env_mask = ((n >= 64) ? ~std::uint64_t{0} : ((std::uint64_t{1} << n) - 1)) & ~kmask;
+    std::vector<std::uint64_t> roff(m, 0);
+    for (int r = 0; r < m; ++r)
+        for (int j = 0; j < count; ++j)
+            if ((r >> j) & 1) roff[r] |= std::uint64_t{1} << kept[j];
+    MIPT_HIP_TRY(grow(w.row_off, w.row_cap, static_cast<std::size_t>(m)));
+    MIPT_HIP_CHECK(hipMemcpyAsync(w.row_off, roff.data(), m * 8, hipMemcpyHostToDevice, stream));
+    MIPT_HIP_TRY(grow(w.rho, w.rho_cap, static_cast<std::size_t>(m) * m));
+    if (f64)
+    {
+        const std::size_t mm = static_cast<std::size_t>(m) * m;
+        hipLaunchKernelGGL(gram_f64_kernel, dim3(static_cast<unsigned>((mm + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<const double2 *>(psi), w.row_off, m, env, env_mask, w.rho);
+        MIPT_HIP_CHECK(hipGetLastError());
+    }
+    else
+    {
+        const std::size_t K = std::size_t{1} << env;
+        const std::size_t plane = static_cast<std::size_t>(m) * K;
+        MIPT_HIP_TRY(grow(w.ar, w.plane_cap, plane));
+        MIPT_HIP_TRY(grow(w.ai, w.plane_cap_i, plane));
+        const unsigned gblocks = static_cast<unsigned>(std::min<std::size_t>((plane + 255) / 256, 65536));
+        hipLaunchKernelGGL(gather_kernel<float2>, dim3(gblocks), dim3(256), 0, stream, static_cast<const float2 *>(psi),
+                           w.row_off, count, env, env_mask, w.ar, w.ai);
+        MIPT_HIP_CHECK(hipGetLastError());
+        const int nb = (m + GT - 1) / GT;
+        const int ntri = nb * (nb + 1) / 2;
+        int ksplit = 1;
+        while (ntri * ksplit < 1024 && static_cast<std::size_t>(ksplit) * 2 * KCHUNK <= K) ksplit *= 2;
+        MIPT_HIP_TRY(grow(w.part, w.part_cap, static_cast<std::size_t>(ksplit) * ntri * GT * GT));
+        if (gram_precision() == 32)
+            hipLaunchKernelGGL(gram_mfma_kernel, dim3(static_cast<unsigned>(ntri * ksplit)), dim3(256), 0, stream,
+                               w.ar, w.ai, m, static_cast<int>(K), ksplit, ntri, w.part);
+        else
+            hipLaunchKernelGGL(gram_mfma64_kernel, dim3(static_cast<unsigned>(ntri * ksplit)), dim3(256), 0, stream,
+                               w.ar, w.ai, m, static_cast<int>(K), ksplit, ntri, w.part);
+        MIPT_HIP_CHECK(hipGetLastError());
+        const std::size_t mm = static_cast<std::size_t>(m) * m;
+        hipLaunchKernelGGL(gram_reduce_kernel, dim3(static_cast<unsigned>((mm + 255) / 256)), dim3(256), 0, stream,
+                           w.part, m, ksplit, ntri, w.rho);
+        MIPT_HIP_CHECK(hipGetLastError());
+    }
+    m_out = m;
+    return OK;
+}
+
+// Eigenvalues of w.rho (m x m) into host vector.
+int eigvalsh(int m, hipStream_t stream, std::vector<double> &lam)
+{
+    Workspace &w = ws();
+    if (!w.handle)
+    {
+        const rocblas_status s = rocblas_create_handle(&w.handle);
+        if (s != rocblas_status_success)
+        {
+            set_error("rocblas_create_handle failed (%d)", static_cast<int>(s));
+            return E_SOLVER + static_cast<int>(s);
+        }
+    }
+    rocblas_set_stream(w.handle, stream);
+    MIPT_HIP_TRY(grow(w.evals, w.ev_cap, static_cast<std::size_t>(m)));
+    MIPT_HIP_TRY(grow(w.e, w.e_cap, static_cast<std::size_t>(m)));
+    if (!w.info) MIPT_HIP_CHECK(hipMalloc(&w.info, sizeof(int)));
+    const rocblas_status s = rocsolver_zheevd(w.handle, rocblas_evect_none, rocblas_fill_lower, m,
+                                              reinterpret_cast<rocblas_double_complex *>(w.rho), m, w.evals, w.e, w.info);
+    if (s != rocblas_status_success)
+    {
+        set_error("rocsolver_zheevd failed (%d)", static_cast<int>(s));
+        return E_SOLVER + static_cast<int>(s);
+    }
+    int info = 0;
+    lam.resize(m);
+    MIPT_HIP_CHECK(hipMemcpyAsync(lam.data(), w.evals, m * 8, hipMemcpyDeviceToHost, stream));
+    MIPT_HIP_CHECK(hipMemcpyAsync(&info, w.info, 4, hipMemcpyDeviceToHost, stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+    if (info != 0)
+    {
+        set_error("FP64 zheevd did not converge (info=%d)", info);
+        return E_EIG + info;
+    }
+    return OK;
+}
+
+// cpp/include/mipt/tmi/linalg.hpp:125-141
+double entropy_from_eigs(const std::vector<double> &lam)
+{
+    double s = 0.0;
+    for (double l : lam)
+        if (l > 1.0e-15) s += -l * std::log2(l);
+    return std::abs(s) < 1.0e-12 ? 0.0 : s;
+}
+
+int entropy_impl(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int order,
+                 double *out)
+{
+    if (!psi || !out)
+    {
+        set_error("entropy: null argument");
+        return E_ARG;
+    }
+    if (count == 0 || count == n)
+    {
+        std::uint64_t km;
+        if (validate_kept(n, kept, count, km) != OK && count != 0)
+        {
+            set_error("invalid kept-mode list");
+            return E_ARG;
+        }
+        *out = 0.0;
+        return OK;
+    }
+    int m = 0;
+    MIPT_HIP_TRY(build_rho(psi, f64, n, kept, count, stream, m));
+    if (order == 2)
+    {
+        Workspace &w = ws();
+        const std::size_t n2 = static_cast<std::size_t>(m) * m;
+        const int blocks = static_cast<int>(std::min<std::size_t>(1024, (n2 + 255) / 256));
+        if (!w.red) MIPT_HIP_CHECK(hipMalloc(&w.red, 1024 * sizeof(double)));
+        hipLaunchKernelGGL(purity_partial_kernel, dim3(blocks), dim3(256), 0, stream, w.rho, n2, w.red);
+        MIPT_HIP_CHECK(hipGetLastError());
+        std::vector<double> part(blocks);
+        MIPT_HIP_CHECK(hipMemcpyAsync(part.data(), w.red, blocks * 8, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        double purity = 0.0;
+        for (double v : part) purity += v;  // fixed block order
+        // cpp/include/mipt/tmi/linalg.hpp:201-208
+        if (!(purity > 0.0) || purity >= 1.0) *out = 0.0;
+        else
+        {
+            const double s = -std::log2(purity);
+            *out = std::abs(s) < 1.0e-12 ? 0.0 : s;
+        }
+        return OK;
+    }
+    std::vector<double> lam;
+    MIPT_HIP_TRY(eigvalsh(m, stream, lam));
+    *out = entropy_from_eigs(lam);
+    return OK;
+}
+
+template <typename C>
+int rho1_impl(const void *psi, int n, int q, hipStream_t stream, double *out4)
+{
+    if (!psi || !out4 || n < 1 || n >= 63 || q < 0 || q >= n)
+    {
+        set_error("rho1: bad arguments");
+        return E_ARG;
+    }
+    Workspace &w = ws();
+    if (!w.red) MIPT_HIP_CHECK(hipMalloc(&w.red, 1024 * sizeof(double)));
+    const std::uint64_t env = std::uint64_t{1} << (n - 1);
+    const int blocks = static_cast<int>(std::min<std::uint64_t>(256, (env + 255) / 256));
+    hipLaunchKernelGGL(rho1_kernel<C>, dim3(blocks), dim3(256), 0, stream, static_cast<const C *>(psi), n, q, w.red);
+    MIPT_HIP_CHECK(hipGetLastError());
+    std::vector<double> part(static_cast<std::size_t>(blocks) * 4);
+    MIPT_HIP_CHECK(hipMemcpyAsync(part.data(), w.red, part.size() * 8, hipMemcpyDeviceToHost, stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+    for (int v = 0; v < 4; ++v) out4[v] = 0.0;
+    for (int b = 0; b < blocks; ++b)
+        for (int v = 0; v < 4; ++v) out4[v] += part[static_cast<std::size_t>(b) * 4 + v];
+    return OK;
+}
+
+int tmi_impl(const void *psi, bool f64, int n, int L, int block, int cycle, hipStream_t stream, double *out,
+             double *terms)
+{
+    if (!psi || !out || L < 4 || L > n || block < 1 || 3 * block > L || cycle < 0 || cycle >= L)
+    {
+        set_error("tmi: bad arguments");
+        return E_ARG;
+    }
+    auto range = [&](int begin, int cnt) {
+        std::vector<int> v;
+        for (int i = 0; i < cnt; ++i) v.push_back((begin + i) % L);
+        return v;
+    };
+    const auto A = range(cycle, block), B = range(cycle + block, block), Cc = range(cycle + 2 * block, block);
+    const auto D = range(cycle + 3 * block, L - 3 * block);
+    auto cat = [](std::vector<int> x, const std::vector<int> &y) {
+        x.insert(x.end(), y.begin(), y.end());
+        return x;
+    };
+    const std::vector<int> sets[7] = {A, B, Cc, D, cat(A, B), cat(A, Cc), cat(B, Cc)};
+    double s[7];
+    for (int i = 0; i < 7; ++i)
+        MIPT_HIP_TRY(entropy_impl(psi, f64, n, sets[i].data(), static_cast<int>(sets[i].size()), stream, 1, &s[i]));
+    *out = s[0] + s[1] + s[2] + s[3] - s[4] - s[5] - s[6];
+    if (terms)
+        for (int i = 0; i < 7; ++i) terms[i] = s[i];
+    return OK;
+}
+
+int rdm_export(const void *psi, bool f64, int n, const int *kept, int count, double *host)
+{
+    if (!host || count > 12)
+    {
+        set_error("rdm_subsystem: bad arguments (kept <= 12)");
+        return E_ARG;
+    }
+    int m = 0;
+    if (count == 0)
+    {
+        host[0] = 1.0;
+        host[1] = 0.0;
+        return OK;
+    }
+    MIPT_HIP_TRY(build_rho(psi, f64, n, kept, count, nullptr, m));
+    MIPT_HIP_CHECK(hipMemcpyAsync(host, ws().rho, static_cast<std::size_t>(m) * m * 16, hipMemcpyDeviceToHost, nullptr));
+    MIPT_HIP_CHECK(hipStreamSynchronize(nullptr));
+    return OK;
+}
+} // namespace
+
+int entropy_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out)
+{
+    return entropy_impl(psi, false, n, kept, kept_count, stream, 1, out);
+}
+
+int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream, double *out,
+                 double *terms)
+{
+    return tmi_impl(psi, false, n, chain_sites, block, cycle, stream, out, terms);
+}
+
+int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4)
+{
+    return rho1_impl<float2>(psi, n, qubit, stream, out4);
+}
+
+} // namespace mipt_hip
+
+using namespace mipt_hip;
+
+extern "C" {
+
+static int check_fermion(int fermion_trace)
+{
+    if (fermion_trace != 0)
+    {
+        set_error("fermion_trace is not supported (qubit trace only)");
+        return E_UNSUPPORTED;
+    }
+    return OK;
+}
+
+int mipt_hip_entropy_svd_f32(const void *s, int n, const int *k, int c, int f, double *out)
+{
+    MIPT_HIP_TRY(check_fermion(f));
+    return entropy_impl(s, false, n, k, c, nullptr, 1, out);
+}
+int mipt_hip_entropy_svd_f64(const void *s, int n, const int *k, int c, int f, double *out)
+{
+    MIPT_HIP_TRY(check_fermion(f));
+    return entropy_impl(s, true, n, k, c, nullptr, 1, out);
+}
+int mipt_hip_renyi2_f32(const void *s, int n, const int *k, int c, int f, double *out)
+{
+    MIPT_HIP_TRY(check_fermion(f));
+    return entropy_impl(s, false, n, k, c, nullptr, 2, out);
+}
+int mipt_hip_renyi2_f64(const void *s, int n, const int *k, int c, int f, double *out)
+{
+    MIPT_HIP_TRY(check_fermion(f));
+    return entropy_impl(s, true, n, k, c, nullptr, 2, out);
+}
+int mipt_hip_rdm_subsystem_f32(const void *s, int n, const int *k, int c, int f, double *rho)
+{
+    MIPT_HIP_TRY(check_fermion(f));
+    return rdm_export(s, false, n, k, c, rho);
+}
+int mipt_hip_rdm_subsystem_f64(const void *s, int n, const int *k, int c, int f, double *rho)
+{
+    MIPT_HIP_TRY(check_fermion(f));
+    return rdm_export(s, true, n, k, c, rho);
+}
+int mipt_hip_rho1_complex_f32(const void *s, int n, int q, double *rho)
+{
+    return rho1_impl<float2>(s, n, q, nullptr, rho);
+}
+int mipt_hip_rho1_complex_f64(const void *s, int n, int q, double *rho)
+{
+    return rho1_impl<double2>(s, n, q, nullptr, rho);
+}
+int mipt_hip_tmi_f32(const void *s, int n, int L, int block, int cycle, double *out, double *terms)
+{
+    return tmi_impl(s, false, n, L, block, cycle, nullptr, out, terms);
+}
+
+} // extern "C"

@@ -1,0 +1,503 @@
+// Plain per-trajectory engine (one launch per blocked op, a host round trip
+// per measured layer) and the sequential-measurement drop-in.  Structure and
+// semantics of cusv::Engine (cpp/include/mipt/cusv/engine.hpp:100-582) and of
+// mipt_cuda_measure_collapse_* (cpp/src/cuda/free_energy_measure.cu:127-299),
+// written for gfx950 with deterministic FP64 histograms.
+#include "mipt_hip.h"
+#include "state_ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+namespace mipt_hip
+{
+static thread_local char g_error[512] = {0};
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+}
+
+const char *last_error() { return g_error; }
+
+namespace
+{
+int grid_for(std::uint64_t work, int threads)
+{
+    const std::uint64_t b = (work + threads - 1) / threads;
+    return static_cast<int>(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+// Deterministic conditional histogram of one state on the engine's stream.
+struct HistScratch
+{
+    double *partial = nullptr;
+    std::size_t partial_cap = 0;
+    double *hist = nullptr;
+    std::size_t hist_cap = 0;
+    std::uint64_t *masks = nullptr;  // off, mmask, cmask, cval
+    ~HistScratch()
+    {
+        if (partial) (void)hipFree(partial);
+        if (hist) (void)hipFree(hist);
+        if (masks) (void)hipFree(masks);
+    }
+};
+
+template <typename Real>
+int conditional_hist(const void *psi, int n, std::uint64_t mmask, std::uint64_t cmask, std::uint64_t cval,
+                     hipStream_t stream, HistScratch &s, std::vector<double> &out)
+{
+    const int m = __builtin_popcountll(mmask);
+    const std::uint64_t bins = std::uint64_t{1} << m;
+    const std::uint64_t rest = std::uint64_t{1} << (n - m);
+    const std::uint64_t per_block = static_cast<std::uint64_t>(HIST_THREADS) * HIST_PER_THREAD;
+    const int bpb = static_cast<int>((rest + per_block - 1) / per_block);
+    const std::size_t need_partial = static_cast<std::size_t>(bins) * bpb;
+    if (need_partial > s.partial_cap)
+    {
+        if (s.partial) MIPT_HIP_CHECK(hipFree(s.partial));
+        s.partial = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&s.partial, need_partial * sizeof(double)));
+        s.partial_cap = need_partial;
+    }
+    if (bins > s.hist_cap)
+    {
+        if (s.hist) MIPT_HIP_CHECK(hipFree(s.hist));
+        s.hist = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&s.hist, bins * sizeof(double)));
+        s.hist_cap = bins;
+    }
+    if (!s.masks) MIPT_HIP_CHECK(hipMalloc(&s.masks, 4 * sizeof(std::uint64_t)));
+    const std::uint64_t hm[4] = {0, mmask, cmask, cval};
+    MIPT_HIP_CHECK(hipMemcpyAsync(s.masks, hm, sizeof(hm), hipMemcpyHostToDevice, stream));
+    const dim3 g1(static_cast<unsigned>(bins * bpb), 1);
+    hipLaunchKernelGGL(hist_partial_kernel<Real>, g1, dim3(HIST_THREADS), 0, stream,
+                       static_cast<const typename CxT<Real>::type *>(psi), s.masks + 0, n, s.masks + 1, s.masks + 2,
+                       s.masks + 3, bpb, s.partial, static_cast<int>(need_partial));
+    MIPT_HIP_CHECK(hipGetLastError());
+    const dim3 g2(static_cast<unsigned>((bins + 255) / 256), 1);
+    hipLaunchKernelGGL(hist_final_kernel, g2, dim3(256), 0, stream, s.partial, static_cast<int>(need_partial), bpb,
+                       s.masks + 1, s.hist, static_cast<int>(bins));
+    MIPT_HIP_CHECK(hipGetLastError());
+    out.resize(bins);
+    MIPT_HIP_CHECK(hipMemcpyAsync(out.data(), s.hist, bins * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+    return OK;
+}
+
+template <typename Real>
+int collapse(void *psi, int n, std::uint64_t cmask, std::uint64_t cval, double scale, hipStream_t stream)
+{
+    const std::uint64_t dim = std::uint64_t{1} << n;
+    hipLaunchKernelGGL(collapse_kernel<Real>, dim3(grid_for(dim, 256)), dim3(256), 0, stream,
+                       static_cast<typename CxT<Real>::type *>(psi), dim, cmask, cval, static_cast<Real>(scale));
+    MIPT_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+// Never freed (like the reference's thread_local device buffers): freeing at
+// thread exit could run after the HIP runtime is torn down.
+thread_local HistScratch *t_measure_scratch = nullptr;
+thread_local char t_measure_error[512] = {0};
+
+template <typename Real>
+int measure_collapse_impl(void *state, int n, int q, double u, int *outcome, double *prob)
+{
+    t_measure_error[0] = '\0';
+    if (state == nullptr || outcome == nullptr || prob == nullptr || n < 1 || n >= 63 || q < 0 || q >= n ||
+        !(u >= 0.0 && u < 1.0))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "Invalid sequential-measurement argument.");
+        set_error("%s", t_measure_error);
+        return E_ARG;
+    }
+    if (!t_measure_scratch) t_measure_scratch = new HistScratch();
+    std::vector<double> h;
+    int st = conditional_hist<Real>(state, n, std::uint64_t{1} << q, 0, 0, nullptr, *t_measure_scratch, h);
+    if (st != 0)
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "%s", last_error());
+        return st;
+    }
+    const double total = h[0] + h[1];
+    const double one = h[1];
+    if (!(total > 0.0) || !std::isfinite(total) || !std::isfinite(one) || one < 0.0 || one > total * (1.0 + 1.0e-10))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "Invalid branch norms: total=%.17g, one=%.17g",
+                      total, one);
+        set_error("%s", t_measure_error);
+        return E_NORM;
+    }
+    const double p1 = std::fmin(1.0, std::fmax(0.0, one / total));
+    *outcome = u < p1 ? 1 : 0;
+    *prob = *outcome ? p1 : 1.0 - p1;
+    const double branch = *outcome ? one : total - one;
+    if (!(*prob > 0.0) || !std::isfinite(*prob))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error),
+                      "Sampled an outcome with invalid conditional probability %.17g", *prob);
+        set_error("%s", t_measure_error);
+        return E_PROB;
+    }
+    if (!(branch > 0.0) || !std::isfinite(branch))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "Sampled an outcome with invalid branch norm %.17g",
+                      branch);
+        set_error("%s", t_measure_error);
+        return E_BRANCH;
+    }
+    const std::uint64_t mask = std::uint64_t{1} << q;
+    st = collapse<Real>(state, n, mask, *outcome ? mask : 0, 1.0 / std::sqrt(branch), nullptr);
+    if (st == 0)
+    {
+        const hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess)
+        {
+            set_error("collapse synchronization: %s", hipGetErrorString(e));
+            st = static_cast<int>(e);
+        }
+    }
+    if (st != 0) std::snprintf(t_measure_error, sizeof(t_measure_error), "%s", last_error());
+    return st;
+}
+} // namespace
+
+// ------------------------------------------------------------------ engine
+struct PlainEngine
+{
+    bool fp64 = false;
+    int n = 0;
+    void *state = nullptr;
+    void *owned = nullptr;
+    hipStream_t stream = nullptr;
+    void *mat_dev = nullptr;
+    DenseTargets *tg_dev = nullptr;
+    HistScratch hs;
+    std::vector<double> hist;
+
+    ~PlainEngine()
+    {
+        if (owned) (void)hipFree(owned);
+        if (mat_dev) (void)hipFree(mat_dev);
+        if (tg_dev) (void)hipFree(tg_dev);
+    }
+
+    std::size_t elem() const { return fp64 ? 16 : 8; }
+    std::size_t dim() const { return std::size_t{1} << n; }
+
+    int ensure_scratch()
+    {
+        if (!mat_dev) MIPT_HIP_CHECK(hipMalloc(&mat_dev, 256 * 256 * 16));
+        if (!tg_dev) MIPT_HIP_CHECK(hipMalloc(&tg_dev, sizeof(DenseTargets)));
+        return OK;
+    }
+
+    template <typename Real>
+    int launch_dense(int k, std::uint64_t groups)
+    {
+        using C = typename CxT<Real>::type;
+        auto *psi = static_cast<C *>(state);
+        const auto *m = static_cast<const C *>(mat_dev);
+        const dim3 grid(grid_for(groups, 256));
+        switch (k)
+        {
+        case 1: hipLaunchKernelGGL((apply_dense_kernel<Real, 1>), grid, dim3(256), 0, stream, psi, groups, tg_dev, m); break;
+        case 2: hipLaunchKernelGGL((apply_dense_kernel<Real, 2>), grid, dim3(256), 0, stream, psi, groups, tg_dev, m); break;
+        case 3: hipLaunchKernelGGL((apply_dense_kernel<Real, 3>), grid, dim3(256), 0, stream, psi, groups, tg_dev, m); break;
+        case 4: hipLaunchKernelGGL((apply_dense_kernel<Real, 4>), grid, dim3(256), 0, stream, psi, groups, tg_dev, m); break;
+        case 5: hipLaunchKernelGGL((apply_dense_kernel<Real, 5>), grid, dim3(256), 0, stream, psi, groups, tg_dev, m); break;
+        default: set_error("dense apply supports 1..5 targets, got %d", k); return E_UNSUPPORTED;
+        }
+        MIPT_HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+
+    // H2D of the matrix and targets per call, as the reference engine does.
+    int apply_matrix(const int *targets, int k, const double *m_ri)
+    {
+        if (!state || k < 1 || k > 5 || k > n) { set_error("apply_matrix: bad arguments"); return E_ARG; }
+        MIPT_HIP_TRY(ensure_scratch());
+        DenseTargets tg{};
+        tg.k = k;
+        std::uint64_t tmask = 0;
+        for (int j = 0; j < k; ++j)
+        {
+            if (targets[j] < 0 || targets[j] >= n || ((tmask >> targets[j]) & 1)) { set_error("apply_matrix: bad target"); return E_ARG; }
+            tmask |= std::uint64_t{1} << targets[j];
+            tg.sorted[j] = targets[j];
+        }
+        std::sort(tg.sorted, tg.sorted + k);
+        const int d = 1 << k;
+        for (int a = 0; a < d; ++a)
+            for (int j = 0; j < k; ++j)
+                if ((a >> j) & 1) tg.off[a] |= std::uint64_t{1} << targets[j];
+        const std::size_t cnt = static_cast<std::size_t>(d) * d;
+        MIPT_HIP_CHECK(hipMemcpyAsync(tg_dev, &tg, sizeof(tg), hipMemcpyHostToDevice, stream));
+        if (fp64)
+        {
+            MIPT_HIP_CHECK(hipMemcpyAsync(mat_dev, m_ri, cnt * 16, hipMemcpyHostToDevice, stream));
+            MIPT_HIP_TRY(launch_dense<double>(k, dim() >> k));
+        }
+        else
+        {
+            std::vector<float> narrow(2 * cnt);
+            for (std::size_t i = 0; i < 2 * cnt; ++i) narrow[i] = static_cast<float>(m_ri[i]);
+            MIPT_HIP_CHECK(hipMemcpyAsync(mat_dev, narrow.data(), cnt * 8, hipMemcpyHostToDevice, stream));
+            MIPT_HIP_TRY(launch_dense<float>(k, dim() >> k));
+        }
+        // The host staging buffers must outlive the async copies.
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        return OK;
+    }
+
+    // Kronecker blocking of consecutive disjoint bonds up to block_targets
+    // (cpp/include/mipt/cusv/gates.hpp:295-333; tensor order of
+    // cpp/include/mipt/cusv/matrix.hpp:214-235: later ops take the high bits).
+    int apply_bonds(const int *q0, const int *q1, const double *gates, int count, int block_targets)
+    {
+        int i = 0;
+        while (i < count)
+        {
+            std::vector<int> targets = {q0[i], q1[i]};
+            std::vector<double> m(gates + 32 * i, gates + 32 * (i + 1));
+            int j = i + 1;
+            while (j < count && static_cast<int>(targets.size()) + 2 <= block_targets)
+            {
+                if (std::find(targets.begin(), targets.end(), q0[j]) != targets.end() ||
+                    std::find(targets.begin(), targets.end(), q1[j]) != targets.end())
+                    break;
+                const int dl = 1 << targets.size();
+                std::vector<double> out(static_cast<std::size_t>(dl * 4) * (dl * 4) * 2, 0.0);
+                const double *hi = gates + 32 * j;
+                for (int hr = 0; hr < 4; ++hr)
+                    for (int hc = 0; hc < 4; ++hc)
+                    {
+                        const double hvr = hi[2 * (hr * 4 + hc)], hvi = hi[2 * (hr * 4 + hc) + 1];
+                        for (int lr = 0; lr < dl; ++lr)
+                            for (int lc = 0; lc < dl; ++lc)
+                            {
+                                const double lvr = m[2 * (lr * dl + lc)], lvi = m[2 * (lr * dl + lc) + 1];
+                                const std::size_t o = 2 * (static_cast<std::size_t>(hr * dl + lr) * (dl * 4) + (hc * dl + lc));
+                                out[o] = hvr * lvr - hvi * lvi;
+                                out[o + 1] = hvr * lvi + hvi * lvr;
+                            }
+                    }
+                m.swap(out);
+                targets.push_back(q0[j]);
+                targets.push_back(q1[j]);
+                ++j;
+            }
+            MIPT_HIP_TRY(apply_matrix(targets.data(), static_cast<int>(targets.size()), m.data()));
+            i = j;
+        }
+        return OK;
+    }
+
+    template <typename Real>
+    int measure_layer_t(const int *sites, int count, const double *uniforms, int mode, int chunk_bits, int *outcomes,
+                        double *p1, double *surprisal)
+    {
+        double surpr = 0.0;
+        std::uint64_t cmask = 0, cval = 0;
+        double weight = 0.0, total0 = -1.0;
+        int chunk = 0;
+        for (int begin = 0; begin < count; begin += chunk_bits, ++chunk)
+        {
+            const int end = std::min(count, begin + chunk_bits);
+            const int m = end - begin;
+            std::uint64_t mmask = 0;
+            for (int j = begin; j < end; ++j) mmask |= std::uint64_t{1} << sites[j];
+            MIPT_HIP_TRY(conditional_hist<Real>(state, n, mmask, cmask, cval, stream, hs, hist));
+            double total = 0.0;
+            for (double v : hist) total += v;
+            if (total0 < 0) total0 = total;
+            if (!(total > 0.0) || !std::isfinite(total)) { set_error("Measurement layer found a zero-norm state vector."); return E_NORM; }
+            std::uint32_t chosen = 0;
+            if (mode == 0)
+            {
+                int st = 0;
+                double pp[64];
+                chosen = sequential_from_hist(hist.data(), m, uniforms + begin, pp, &st);
+                if (st) { set_error("Measurement layer found a zero-weight branch."); return st; }
+                if (p1) for (int j = 0; j < m; ++j) p1[begin + j] = pp[j];
+            }
+            else
+            {
+                const double draw = uniforms[chunk] * total;
+                double running = 0.0;
+                chosen = static_cast<std::uint32_t>(hist.size() - 1);
+                for (std::size_t b = 0; b < hist.size(); ++b)
+                {
+                    running += hist[b];
+                    if (draw < running) { chosen = static_cast<std::uint32_t>(b); break; }
+                }
+                if (!(hist[chosen] > 0.0))
+                {
+                    chosen = static_cast<std::uint32_t>(std::max_element(hist.begin(), hist.end()) - hist.begin());
+                    if (!(hist[chosen] > 0.0)) { set_error("Measurement layer has no outcome with positive weight."); return E_PROB; }
+                }
+            }
+            weight = hist[chosen];
+            surpr += -std::log(weight / total);
+            for (int j = 0; j < m; ++j)
+            {
+                const int bit = (chosen >> j) & 1u;
+                outcomes[begin + j] = bit;
+                cmask |= std::uint64_t{1} << sites[begin + j];
+                if (bit) cval |= std::uint64_t{1} << sites[begin + j];
+            }
+        }
+        if (count > 0)
+        {
+            if (!(weight > 0.0)) { set_error("Measurement selected a zero-weight branch."); return E_BRANCH; }
+            MIPT_HIP_TRY(collapse<Real>(state, n, cmask, cval, 1.0 / std::sqrt(weight), stream));
+        }
+        if (surprisal) *surprisal = surpr;
+        return OK;
+    }
+};
+
+} // namespace mipt_hip
+
+using namespace mipt_hip;
+
+struct mipt_hip_engine : PlainEngine
+{
+};
+
+extern "C" {
+
+const char *mipt_hip_last_error(void) { return mipt_hip::last_error(); }
+const char *mipt_hip_version(void) { return "mipt_hip 0.1 gfx950"; }
+
+int mipt_hip_measure_collapse_f32(void *s, int n, int q, double u, int *o, double *p)
+{
+    return measure_collapse_impl<float>(s, n, q, u, o, p);
+}
+int mipt_hip_measure_collapse_f64(void *s, int n, int q, double u, int *o, double *p)
+{
+    return measure_collapse_impl<double>(s, n, q, u, o, p);
+}
+const char *mipt_hip_measure_collapse_last_error(void) { return t_measure_error; }
+
+int mipt_hip_engine_create(int use_fp64, void *stream, mipt_hip_engine **out)
+{
+    if (!out) { set_error("engine_create: null out"); return E_ARG; }
+    auto *e = new (std::nothrow) mipt_hip_engine();
+    if (!e) return E_NOMEM;
+    e->fp64 = use_fp64 != 0;
+    e->stream = static_cast<hipStream_t>(stream);
+    *out = e;
+    return OK;
+}
+
+int mipt_hip_engine_destroy(mipt_hip_engine *e)
+{
+    delete e;
+    return OK;
+}
+
+int mipt_hip_engine_adopt(mipt_hip_engine *e, void *state, int n)
+{
+    if (!e || !state || n < 1 || n > 34) { set_error("engine_adopt: bad arguments"); return E_ARG; }
+    if (e->owned) { MIPT_HIP_CHECK(hipFree(e->owned)); e->owned = nullptr; }
+    e->state = state;
+    e->n = n;
+    return OK;
+}
+
+int mipt_hip_engine_reset(mipt_hip_engine *e, int n)
+{
+    if (!e || n < 1 || n > 34) { set_error("engine_reset: bad arguments"); return E_ARG; }
+    if (!e->owned || e->n != n)
+    {
+        if (e->owned) MIPT_HIP_CHECK(hipFree(e->owned));
+        e->owned = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&e->owned, (std::size_t{1} << n) * (e->fp64 ? 16 : 8)));
+    }
+    e->state = e->owned;
+    e->n = n;
+    const std::uint64_t dim = std::uint64_t{1} << n;
+    if (e->fp64)
+        hipLaunchKernelGGL(set_basis_kernel<double>, dim3(grid_for(dim, 256)), dim3(256), 0, e->stream,
+                           static_cast<double2 *>(e->state), dim, std::uint64_t{0});
+    else
+        hipLaunchKernelGGL(set_basis_kernel<float>, dim3(grid_for(dim, 256)), dim3(256), 0, e->stream,
+                           static_cast<float2 *>(e->state), dim, std::uint64_t{0});
+    MIPT_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+void *mipt_hip_engine_data(mipt_hip_engine *e) { return e ? e->state : nullptr; }
+
+int mipt_hip_engine_apply_matrix(mipt_hip_engine *e, const int *targets, int k, const double *m)
+{
+    if (!e || !targets || !m) { set_error("apply_matrix: null argument"); return E_ARG; }
+    return e->apply_matrix(targets, k, m);
+}
+
+int mipt_hip_engine_apply_bonds(mipt_hip_engine *e, const int *q0, const int *q1, const double *g, int count,
+                                int block_targets)
+{
+    if (!e || !e->state || count < 0 || (count && (!q0 || !q1 || !g)) || block_targets < 2 || block_targets > 5)
+    {
+        set_error("apply_bonds: bad arguments");
+        return E_ARG;
+    }
+    return e->apply_bonds(q0, q1, g, count, block_targets);
+}
+
+int mipt_hip_engine_measure_layer(mipt_hip_engine *e, const int *sites, int count, const double *u, int mode,
+                                  int chunk_bits, int *outcomes, double *p1, double *surprisal)
+{
+    if (!e || !e->state || count < 0 || (count && (!sites || !u || !outcomes)) || (mode != 0 && mode != 1) ||
+        chunk_bits < 1 || chunk_bits > 16 || (mode == 0 && chunk_bits > 16))
+    {
+        set_error("measure_layer: bad arguments");
+        return E_ARG;
+    }
+    for (int j = 0; j < count; ++j)
+    {
+        if (sites[j] < 0 || sites[j] >= e->n || (j && sites[j] <= sites[j - 1])) { set_error("measure_layer: sites must be ascending and in range"); return E_ARG; }
+        if (mode == 0 && !(u[j] >= 0.0 && u[j] < 1.0)) { set_error("measure_layer: uniform out of [0,1)"); return E_ARG; }
+    }
+    if (surprisal) *surprisal = 0.0;
+    if (count == 0) return OK;
+    return e->fp64 ? e->measure_layer_t<double>(sites, count, u, mode, chunk_bits, outcomes, p1, surprisal)
+                   : e->measure_layer_t<float>(sites, count, u, mode, chunk_bits, outcomes, p1, surprisal);
+}
+
+int mipt_hip_engine_to_host(mipt_hip_engine *e, void *host)
+{
+    if (!e || !e->state || !host) { set_error("to_host: bad arguments"); return E_ARG; }
+    MIPT_HIP_CHECK(hipMemcpyAsync(host, e->state, e->dim() * e->elem(), hipMemcpyDeviceToHost, e->stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return OK;
+}
+
+int mipt_hip_engine_set_from_host(mipt_hip_engine *e, const void *host)
+{
+    if (!e || !e->state || !host) { set_error("set_from_host: bad arguments"); return E_ARG; }
+    MIPT_HIP_CHECK(hipMemcpyAsync(e->state, host, e->dim() * e->elem(), hipMemcpyHostToDevice, e->stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return OK;
+}
+
+int mipt_hip_engine_squared_norm(mipt_hip_engine *e, double *out)
+{
+    if (!e || !e->state || !out) { set_error("squared_norm: bad arguments"); return E_ARG; }
+    std::vector<double> h;
+    MIPT_HIP_TRY(e->fp64 ? conditional_hist<double>(e->state, e->n, 0, 0, 0, e->stream, e->hs, h)
+                         : conditional_hist<float>(e->state, e->n, 0, 0, 0, e->stream, e->hs, h));
+    *out = h[0];
+    return OK;
+}
+
+} // extern "C"

@@ -168,10 +168,27 @@ void apply_dense(cd *psi, int n, const int *targets, int k, const cd *m)
     }
 }
 
+// Two-qubit specialisation of apply_dense (same arithmetic order).
 void apply_bond(cd *psi, int n, int q0, int q1, const cd *u)
 {
-    const int t[2] = {q0, q1};
-    apply_dense(psi, n, t, 2, u);
+    const std::uint64_t dim = std::uint64_t{1} << n;
+    const std::uint64_t m0 = std::uint64_t{1} << q0, m1 = std::uint64_t{1} << q1;
+    for (std::uint64_t base = 0; base < dim; ++base)
+    {
+        if (base & (m0 | m1)) continue;
+        const cd in[4] = {psi[base], psi[base | m0], psi[base | m1], psi[base | m0 | m1]};
+        cd out[4];
+        for (int r = 0; r < 4; ++r)
+        {
+            cd acc = 0.0;
+            for (int c = 0; c < 4; ++c) acc += u[r * 4 + c] * in[c];
+            out[r] = acc;
+        }
+        psi[base] = out[0];
+        psi[base | m0] = out[1];
+        psi[base | m1] = out[2];
+        psi[base | m0 | m1] = out[3];
+    }
 }
 
 // ---------------------------------------------------------------- measurement

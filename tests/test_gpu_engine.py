@@ -1,0 +1,303 @@
+"""GPU parity of the HIP engine against the CPU FP64 oracle (replay mode).
+
+Bar (stated per test): measurement outcomes bit-exact except draws within
+1e-6 of the Born threshold (flagged); complex64 amplitudes within 1e-5 of the
+complex128 oracle; FP64 paths within 1e-11.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FLAG_EPS = 1e-6
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mipt_amd
+    mipt_amd.lib()
+    return mipt_amd
+
+
+class DevState:
+    """A device state owned by a plain engine (the library's own HIP runtime;
+    no second GPU runtime is loaded into the test process)."""
+
+    def __init__(self, M, psi, fp64=False):
+        n = int(np.log2(len(psi)))
+        self.e = M.PlainEngine(fp64)
+        self.e.reset(n)
+        self.e.set_from_host(np.asarray(psi))
+
+    def data_ptr(self):
+        return self.e.data
+
+
+@pytest.fixture(scope="module")
+def gpu(M):
+    """The library module; device buffers come from its own HIP runtime."""
+    return M
+
+
+def dev_state(M, psi, fp64=False):
+    return DevState(M, psi, fp64)
+
+
+def host_state(t, fp64=False):
+    return t.e.to_host()
+
+
+def check_outcomes(got, want, p1_oracle, unif, flags, label):
+    """Outcomes equal the oracle's except draws within FLAG_EPS of P(1)."""
+    m = flags == 1
+    near = np.abs(unif - p1_oracle) < FLAG_EPS
+    bad = m & (got != want) & ~near
+    assert not bad.any(), f"{label}: outcome mismatch at {np.argwhere(bad)[:5]}"
+    assert np.all((got >= 0) == m), f"{label}: measured-site pattern differs"
+
+
+# ------------------------------------------------------------------ drop-ins
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_measure_collapse_kat(M, gpu, fp64):
+    # tests/core_tests.cpp:120-157 through the device C ABI
+    print(f"measure_collapse KAT fp64={fp64}", flush=True)
+    t = dev_state(gpu, np.array([1, 2, 3, 4], np.complex128), fp64)
+    o, p = M.measure_collapse(t.data_ptr(), 2, 0, 0.5, fp64)
+    assert o == 1 and abs(p - 2 / 3) < 1e-14
+    o2, p2 = M.measure_collapse(t.data_ptr(), 2, 1, 0.9, fp64)
+    assert o2 == 0 and abs(p2 - 0.2) < 1e-14
+    psi = host_state(t, fp64)
+    assert abs(np.vdot(psi, psi).real - 1) < (1e-14 if fp64 else 1e-6)
+    with pytest.raises(M.MiptHipError):
+        M.measure_collapse(t.data_ptr(), 2, 2, 0.5, fp64)
+
+
+@pytest.mark.parametrize("n", [6, 11])
+def test_measure_collapse_matches_oracle(M, gpu, n):
+    rng = np.random.default_rng(n)
+    for trial in range(6):
+        psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        psi /= np.linalg.norm(psi)
+        q = int(rng.integers(0, n))
+        u = float(rng.random())
+        t = dev_state(gpu, psi, True)
+        o, p = M.measure_collapse(t.data_ptr(), n, q, u, True)
+        ref = psi.copy()
+        st, oo, pp = O.measure_collapse(ref, n, q, u)
+        assert o == oo and abs(p - pp) < 1e-13
+        assert np.abs(host_state(t, True) - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,kept", [(8, [0, 1, 2]), (8, [3, 7, 1]), (10, [0, 1, 2, 3, 4]), (12, [5, 0, 9, 2]),
+                                    (14, list(range(7))), (16, list(range(8))), (16, [1]), (13, [12, 0, 6])])
+def test_entropy_rdm_renyi_match_oracle(M, gpu, n, kept):
+    print(f"entropy n={n} kept={kept}", flush=True)
+    psi, _, _, _ = O.simulate(n, 2 * n, 0.1, 1000 + n)
+    want = O.entropy(psi, n, kept)
+    want2 = O.entropy(psi, n, kept, order=2)
+    t32 = dev_state(gpu, psi)
+    t64 = dev_state(gpu, psi, True)
+    # FP32-MFMA Gram of a complex64 state: within 1e-5 bits of FP64
+    assert abs(M.entropy(t32.data_ptr(), n, kept) - want) < 1e-5
+    assert abs(M.entropy(t64.data_ptr(), n, kept, fp64=True) - want) < 1e-10
+    assert abs(M.entropy(t32.data_ptr(), n, kept, order=2) - want2) < 1e-5
+    assert abs(M.entropy(t64.data_ptr(), n, kept, fp64=True, order=2) - want2) < 1e-10
+    if len(kept) <= 8:
+        rho = O.rdm(psi, n, kept)
+        assert np.abs(M.rdm_subsystem(t64.data_ptr(), n, kept, fp64=True) - rho).max() < 1e-12
+        assert np.abs(M.rdm_subsystem(t32.data_ptr(), n, kept) - rho).max() < 2e-6
+
+
+def test_entropy_closed_forms_gpu(M, gpu):
+    # Bell pairs across the half cut -> exactly 3 bits; product state -> 0
+    n = 8
+    psi = np.zeros(1 << n, np.complex128)
+    for bits in range(8):
+        idx = 0
+        for (a, b), v in zip([(0, 7), (3, 4), (1, 5)], [(bits >> j) & 1 for j in range(3)]):
+            if v:
+                idx |= (1 << a) | (1 << b)
+        psi[idx] = 1
+    psi /= np.linalg.norm(psi)
+    t = dev_state(gpu, psi)
+    assert abs(M.entropy(t.data_ptr(), n, [0, 1, 2, 3]) - 3.0) < 1e-6
+    z = dev_state(gpu, O.zero_state(n))
+    assert M.entropy(z.data_ptr(), n, [0, 1, 2, 3]) == 0.0
+    assert M.entropy(z.data_ptr(), n, [0, 1, 2, 3], order=2) == 0.0
+    ghz = np.zeros(16, np.complex128)
+    ghz[0] = ghz[15] = 1 / math.sqrt(2)
+    g = dev_state(gpu, ghz)
+    i3, terms = M.tmi(g.data_ptr(), 4, 4, 1)
+    assert abs(i3 - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_tmi_and_rho1_match_oracle(M, gpu, n):
+    psi, _, _, _ = O.simulate(n, 2 * n, 0.05, 77 + n)
+    t = dev_state(gpu, psi)
+    i3, terms = M.tmi(t.data_ptr(), n)
+    assert abs(i3 - O.tmi(psi, n)) < 1e-5
+    for q in (0, n // 2, n - 1):
+        r = M.rho1(t.data_ptr(), n, q)
+        assert np.abs(r - O.rho1(psi, n, q)).max() < 1e-6
+
+
+# ------------------------------------------------------------------ plain engine
+
+@pytest.mark.parametrize("fp64,block", [(False, 2), (False, 4), (True, 4), (True, 2)])
+def test_plain_engine_bond_apply(M, fp64, block):
+    n = 10
+    gates, _, _ = O.build_layers(n, 2, 0.0, 5)
+    rng = np.random.default_rng(0)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    e = M.PlainEngine(fp64)
+    e.reset(n)
+    e.set_from_host(psi)
+    bonds = M.layer_bonds(n, 1, True)
+    e.apply_bonds(bonds, gates[:len(bonds)], block)
+    ref = psi.copy()
+    for (q0, q1), g in zip(bonds, gates):
+        O.apply_dense(ref, n, [q0, q1], g)
+    tol = 1e-12 if fp64 else 2e-6
+    assert np.abs(e.to_host() - ref).max() < tol
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_plain_trajectory_replay(M, fp64):
+    n, layers, p = 10, 16, 0.3
+    print(f"plain replay fp64={fp64}", flush=True)
+    psi, want, p1, (g, f, u) = O.simulate(n, layers, p, 42)
+    e = M.PlainEngine(fp64)
+    got = M.run_plain_trajectory(e, n, g, f, u, True, 4, 0, 10)
+    check_outcomes(got, want, p1, u, f, "plain")
+    tol = 1e-11 if fp64 else 1e-5
+    assert np.abs(e.to_host() - psi).max() < tol
+
+
+def test_plain_joint_mode_matches_oracle(M):
+    # engine.hpp:293-368 joint chunk sampling (one uniform per chunk)
+    n, layers = 8, 10
+    gates, flags, _ = O.build_layers(n, layers, 0.4, 9)
+    rng = np.random.default_rng(1)
+    unif = rng.random((layers, n))
+    ref = O.zero_state(n)
+    want, _ = O.run_layers(ref, n, gates, flags, unif, mode=1, chunk_bits=3)
+    e = M.PlainEngine(True)
+    got = M.run_plain_trajectory(e, n, gates, flags, unif, True, 4, 1, 3)
+    assert np.array_equal(got, want)
+    assert np.abs(e.to_host() - ref).max() < 1e-11
+
+
+# ------------------------------------------------------------------ batched engine
+
+def replay_batch(M, cases, tile_bits=0):
+    """cases: list of (n, layers, p, seed, closed); returns (batch, oracle results)."""
+    specs = [M.Spec(n, layers, p, seed, closed) for (n, layers, p, seed, closed) in cases]
+    b = M.Batch(specs, tile_bits=tile_bits)
+    refs = []
+    for i, (n, layers, p, seed, closed) in enumerate(cases):
+        psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed, closed)
+        b.set_layers(i, g, f, u)
+        refs.append((psi, out, p1, f, u))
+    b.evolve()
+    b.synchronize()
+    return b, refs
+
+
+@pytest.mark.parametrize("tile_bits", [0, 13])
+def test_batch_replay_mixed_lengths(M, tile_bits):
+    cases = [(4, 8, 0.3, 1, True), (8, 12, 0.2, 2, True), (12, 14, 0.16, 3, False), (14, 10, 0.3, 4, True),
+             (16, 8, 0.2, 5, True), (16, 9, 0.5, 6, False), (18, 6, 0.3, 7, True)]
+    print(f"batch replay tile_bits={tile_bits}", flush=True)
+    b, refs = replay_batch(M, cases, tile_bits)
+    for i, (psi, out, p1, f, u) in enumerate(refs):
+        got, flagged, gp1, _ = b.outcomes(i)
+        check_outcomes(got, out, p1, u, f, f"traj {i} L={cases[i][0]}")
+        m = f == 1
+        assert np.abs(gp1[m] - p1[m]).max(initial=0) < 1e-5
+        assert np.all(flagged[m] == (np.abs(u[m] - gp1[m]) < FLAG_EPS))
+        if not np.any(flagged):
+            assert np.abs(b.get_state(i) - psi).max() < 1e-5, f"traj {i}"
+
+
+def test_batch_overflow_chunks_and_edge_rates(M):
+    # p=1 measures every site (16 > one 10-site histogram chunk); p=0 none.
+    cases = [(16, 6, 1.0, 11, True), (16, 6, 0.0, 12, True), (10, 6, 1.0, 13, True), (20, 4, 0.6, 14, True)]
+    b, refs = replay_batch(M, cases)
+    for i, (psi, out, p1, f, u) in enumerate(refs):
+        got, flagged, _, _ = b.outcomes(i)
+        check_outcomes(got, out, p1, u, f, f"traj {i}")
+        if not np.any(flagged):
+            assert np.abs(b.get_state(i) - psi).max() < 1e-5
+
+
+def test_batch_device_stream_matches_oracle_stream(M):
+    # gates, flags and uniforms drawn on device (Philox) equal the oracle's
+    cases = [(10, 10, 0.2, 101), (16, 8, 0.25, 102), (18, 6, 0.16, 103)]
+    specs = [M.Spec(n, layers, p, seed) for (n, layers, p, seed) in cases]
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    for i, (n, layers, p, seed) in enumerate(cases):
+        psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed)
+        got, flagged, _, _ = b.outcomes(i)
+        check_outcomes(got, out, p1, u, f, f"gen {i}")
+        if not np.any(flagged):
+            assert np.abs(b.get_state(i) - psi).max() < 1e-5
+
+
+def test_batch_observables_match_oracle(M):
+    cases = [(12, 12, 0.1, 21, True), (16, 16, 0.1, 22, True), (20, 6, 0.1, 23, True)]
+    b, refs = replay_batch(M, cases)
+    for i, (psi, out, p1, f, u) in enumerate(refs):
+        n = cases[i][0]
+        ob = b.observables(i)
+        assert abs(ob["s_half"] - O.half_chain_entropy(psi, n)) < 1e-5
+        assert abs(ob["tmi"] - O.tmi(psi, n)) < 1e-5
+        assert math.isnan(ob["probe_entropy"])
+
+
+def test_batch_probe_bell_pair(M):
+    # probe qubit at index L Bell-paired with site L/2 (probed.hpp:25-33): with
+    # p = 0 the probe stays maximally entangled: S = ln 2, Bloch = 0.
+    specs = [M.Spec(8, 8, 0.0, 31, True, probe_site=4), M.Spec(16, 4, 0.0, 32, True, probe_site=8),
+             M.Spec(10, 10, 0.3, 33, True, probe_site=5)]
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    for i in range(2):
+        ob = b.observables(i, want_tmi=False)
+        assert abs(ob["probe_entropy"] - math.log(2)) < 1e-6
+        assert np.abs(ob["bloch"]).max() < 1e-6
+    # measured case against the oracle (probe qubit = index L, never gated)
+    n = 10
+    g, f, u = O.build_layers(n, 10, 0.3, 33)
+    psi = np.zeros(1 << (n + 1), np.complex128)
+    psi[0] = psi[(1 << 5) | (1 << n)] = 1 / math.sqrt(2)
+    # evolve the (n+1)-qubit state: gates act on 0..n-1 only
+    outs = np.full((10, n), -1, np.int32)
+    gi = 0
+    for t in range(10):
+        for (q0, q1) in O.layer_bonds(n, t, True):
+            O.apply_dense(psi, n + 1, [q0, q1], g[gi])
+            gi += 1
+        for s in range(n):
+            if f[t, s]:
+                st, o, _ = O.measure_collapse(psi, n + 1, s, u[t, s])
+                outs[t, s] = o
+    got, flagged, _, _ = b.outcomes(2)
+    if not flagged.any():
+        assert np.array_equal(got, outs)
+        r = O.rho1(psi, n + 1, n)
+        ob = b.observables(2, want_tmi=False)
+        assert abs(ob["probe_entropy"] - O.rho1_entropy_nats(r)) < 1e-5
+        assert np.abs(ob["bloch"] - O.bloch_from_rho1(r)).max() < 1e-5
