@@ -15,8 +15,8 @@ import argparse
 import ctypes
 import json
 import math
-import multiprocessing as mp
 import os
+from concurrent.futures import ThreadPoolExecutor
 import sys
 import time
 
@@ -48,17 +48,26 @@ def linspace(first, last, count):
     return v
 
 
-def _oracle_traj(args):
-    """CPU FP64 oracle run of one trajectory of the workload (cpu_baseline leg)."""
-    L, layers, p, seed, want_obs = args
+def _oracle_evolve(args):
+    """CPU FP64 oracle run of one trajectory of the workload (cpu_baseline leg).
+    Runs in a thread: the oracle's C calls release the GIL."""
+    L, layers, p, seed = args
     from oracle import oracle as O
-    t0 = time.perf_counter()
     psi, out, p1, (g, f, u) = O.simulate(L, layers, p, seed, True, "philox", 0)
-    dt = time.perf_counter() - t0
-    obs = None
-    if want_obs:
-        obs = (O.half_chain_entropy(psi, L), O.tmi(psi, L))
-    return dt, psi.astype(np.complex128), out, p1, u, f, obs
+    return psi, out, p1, u, f
+
+
+def _oracle_observables(args):
+    psi, L = args
+    from oracle import oracle as O
+    return O.half_chain_entropy(psi, L), O.tmi(psi, L)
+
+
+T_START = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def main(argv=None):
@@ -74,7 +83,8 @@ def main(argv=None):
     ap.add_argument("--replay", type=int, default=8, help="trajectories checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg (profiling runs)")
     ap.add_argument("--tile-bits", type=int, default=0)
-    ap.add_argument("--profile-every", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--profile-every", type=int, default=7, help="odd, so both sweeps of a layer are sampled")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -91,7 +101,8 @@ def main(argv=None):
         for r in range(args.per_p):
             specs.append(M.Spec(L, layers, p, len(specs) + 1, True))
     ntraj = len(specs)
-    b = M.Batch(specs, tile_bits=args.tile_bits)
+    log(f"setting up {ntraj} trajectories (L={L}, T={layers})")
+    b = M.Batch(specs, tile_bits=args.tile_bits, threads=args.threads)
     hip = hip_runtime()
     stream = ctypes.c_void_p(b.stream)
     e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
@@ -103,15 +114,17 @@ def main(argv=None):
         b.generate()
         b.evolve()
 
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         step()
-    b.synchronize()
+        b.synchronize()
+        log(f"warmup step {w + 1}/{args.warmup} done")
     b.set_profiling(args.profile_every)
     b.synchronize()
     hip.hipEventRecord(e0, stream)
     t_wall = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         step()
+        log(f"timed step {k + 1}/{args.steps} issued")
     hip.hipEventRecord(e1, stream)
     hip.hipEventSynchronize(e1)
     b.synchronize()
@@ -124,6 +137,7 @@ def main(argv=None):
     st = b.pass_stats()
     achieved = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
 
+    log(f"timed steps done: {ms_per_step:.1f} ms/step, {value:.0f} layers/s")
     # plain one-trajectory-per-launch path on the same trajectories' device draws
     plain_lps = None
     if args.plain > 0:
@@ -137,6 +151,7 @@ def main(argv=None):
         dt = time.perf_counter() - t
         plain_lps = len(data) * layers / dt
         eng.close()
+        log(f"plain path: {plain_lps:.0f} layers/s")
 
     result = {
         "metric": METRIC,
@@ -166,16 +181,22 @@ def main(argv=None):
     if not args.no_cpu and args.replay > 0:
         idx = [int(i * ntraj / args.replay) for i in range(args.replay)]
         cores = max(1, min(len(idx), len(os.sched_getaffinity(0)), 16))
-        jobs = [(L, layers, specs[i].p, specs[i].seed, True) for i in idx]
+        jobs = [(L, layers, specs[i].p, specs[i].seed) for i in idx]
+        from oracle import oracle as O
+        O.lib()
+        log(f"oracle leg: {len(jobs)} trajectories on {cores} threads")
         t = time.perf_counter()
-        with mp.get_context("spawn").Pool(cores) as pool:
-            res = pool.map(_oracle_traj, jobs)
+        with ThreadPoolExecutor(cores) as pool:
+            res = list(pool.map(_oracle_evolve, jobs))
         wall = time.perf_counter() - t
-        evo = sum(r[0] for r in res)  # evolution seconds summed over processes
-        cpu_lps = len(idx) * layers / (evo / cores)
+        cpu_lps = len(idx) * layers / wall
+        log(f"oracle evolution done in {wall:.1f} s ({cpu_lps:.2f} layers/s)")
+        with ThreadPoolExecutor(cores) as pool:
+            obs_all = list(pool.map(_oracle_observables, [(r[0], L) for r in res]))
+        log("oracle observables done")
         s_err = tmi_err = amp_err = 0.0
         mism = flagged_n = 0
-        for i, (dt, psi, out, p1, u, f, obs) in zip(idx, res):
+        for i, (psi, out, p1, u, f), obs in zip(idx, res, obs_all):
             got, flg, _, _ = b.outcomes(i)
             near = np.abs(u - p1) < 1e-6
             mism += int(np.sum((got != out) & (f == 1) & ~near))
@@ -187,7 +208,7 @@ def main(argv=None):
                 amp_err = max(amp_err, float(np.abs(b.get_state(i).astype(np.complex128) - psi).max()))
         result["cpu_baseline"] = {"value": round(cpu_lps, 2), "unit": "layers/s", "cores": cores, "kind": "port",
                                   "sample": f"{len(idx)} trajectories of the workload (L={L}, T={layers}) evolved "
-                                            f"in complex128 by the oracle, one process per trajectory"}
+                                            f"in complex128 by the oracle, one thread per trajectory"}
         result["parity"] = {"replayed": len(idx), "outcome_mismatches": mism, "flagged_draws": flagged_n,
                             "s_half_err_max": s_err, "tmi_err_max": tmi_err, "amp_err_max": amp_err}
         result["batched_vs_cpu"] = round(value / cpu_lps, 1)

@@ -305,8 +305,8 @@ class Spec:
 class Batch:
     """Device-resident batch of trajectories (mixed chain lengths allowed)."""
 
-    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None):
-        cfg = BatchConfig(tile_bits, 0, 0)
+    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None, threads: int = 0):
+        cfg = BatchConfig(tile_bits, 0, threads)
         h = ctypes.c_void_p()
         _check(lib().mipt_hip_batch_create(ctypes.byref(cfg), ctypes.c_void_p(stream or 0), ctypes.byref(h)),
                "batch_create")

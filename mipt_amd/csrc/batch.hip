@@ -355,10 +355,11 @@ __device__ __forceinline__ void gate_on_regs(float2 (&v)[NV], const float2 *__re
 template <int NV>
 __device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
 {
-    static_assert(NV == 64, "register tiles hold 64 amplitudes");
+    static_assert(NV == 32 || NV == 64, "register tiles hold 32 or 64 amplitudes");
     if (sp.bond[0] >= 0) gate_on_regs<NV, 0, 1>(v, gl + 16 * sp.bond[0]);
     if (sp.bond[1] >= 0) gate_on_regs<NV, 2, 3>(v, gl + 16 * sp.bond[1]);
-    if (sp.bond[2] >= 0) gate_on_regs<NV, 4, 5>(v, gl + 16 * sp.bond[2]);
+    if constexpr (NV == 64)
+        if (sp.bond[2] >= 0) gate_on_regs<NV, 4, 5>(v, gl + 16 * sp.bond[2]);
 }
 
 // LDS address pieces of a mapping: thread part T (lane and wave bits) and the
@@ -403,13 +404,132 @@ __device__ __forceinline__ void lds_read(const float2 *lds, float2 (&v)[NV], std
     }
 }
 
-// One sweep over the tiles of a group of trajectories.
-template <int RB, int NT>
+// Finalize one chunk of a layer's measurements for one trajectory (one
+// workgroup, after the histogram sweep): fixed-order reduction of the tile
+// partials into the chunk's joint histogram h over its m sites (bin bit j <->
+// j-th chunk site, ascending), a tree of marginals M_j[x] = sum of h over bins
+// whose low j bits are x, then the sequential Born walk: site j's conditional
+// P(1) = M_{j+1}[prefix | 1<<j] / M_j[prefix], outcome 1 iff u_j < P(1)
+// (cpp/include/mipt/free_energy_measure.hpp:59), near-threshold flags, and the
+// collapse record (mask, outcomes, 1/sqrt(raw joint weight)) for the next sweep.
+struct Geom
+{
+    std::uint64_t rest_mask;
+    int ntiles, k, c, s;
+};
+
+__device__ void finalize_chunk(const DevArena &a, int traj, int tl, int t, int chunk, const Geom g, double *lds_d)
+{
+    const TrajDesc d = a.desc[traj];
+    const std::uint32_t fl = a.flags[d.layer_off + t];
+    int sites[MAX_HIST_BITS];
+    int m = 0, seen = 0;
+    for (int q = 0; q < d.L; ++q)
+    {
+        if (!((fl >> q) & 1u)) continue;
+        if (seen >= chunk * MAX_HIST_BITS && seen < (chunk + 1) * MAX_HIST_BITS) sites[m++] = q;
+        ++seen;
+    }
+    if (m == 0)
+    {
+        if (threadIdx.x == 0 && chunk == 0)
+        {
+            a.coll[d.layer_off + t] = Collapse{0, 0, 1.0, 1.0};
+            a.outcomes[d.layer_off + t] = 0;
+            a.flagged[d.layer_off + t] = 0;
+        }
+        return;
+    }
+    // global-bin bits of local sites (ascending u bit = ascending site) and of tile sites
+    std::uint32_t gloc = 0, gnon = 0;
+    std::uint64_t non_phys = 0;
+    for (int j = 0; j < m; ++j)
+    {
+        const int q = sites[j];
+        const bool local = q < g.c || (q >= g.s && q < g.s + (g.k - g.c));
+        if (local) gloc |= 1u << j;
+        else
+        {
+            gnon |= 1u << j;
+            non_phys |= std::uint64_t{1} << q;
+        }
+    }
+    const std::uint32_t bins = 1u << m;
+    const std::uint32_t lbins = 1u << __popc(gloc);
+    double *M = lds_d;  // level j at M[2^j .. 2^(j+1)), level m = h
+    double *h = M + bins;
+    for (std::uint32_t x = threadIdx.x; x < bins; x += blockDim.x) h[x] = 0.0;
+    __syncthreads();
+    const double *part = a.partials + static_cast<std::size_t>(tl) * g.ntiles * HSTRIDE;
+    for (int tile = 0; tile < g.ntiles; ++tile)
+    {
+        const std::uint64_t Bt = pdep64(static_cast<std::uint64_t>(tile), g.rest_mask);
+        const std::uint32_t pat = static_cast<std::uint32_t>(pdep64(pext64(Bt, non_phys), gnon));
+        for (std::uint32_t b = threadIdx.x; b < lbins; b += blockDim.x)
+            h[static_cast<std::uint32_t>(pdep64(b, gloc)) | pat] += part[static_cast<std::size_t>(tile) * HSTRIDE + b];
+    }
+    __syncthreads();
+    for (int j = m - 1; j >= 0; --j)
+    {
+        const std::uint32_t n = 1u << j;
+        for (std::uint32_t x = threadIdx.x; x < n; x += blockDim.x) M[n + x] = M[2 * n + x] + M[2 * n + x + n];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double *u = a.uniforms + d.unif_off + static_cast<std::size_t>(t) * d.L;
+    std::uint32_t prefix = 0;
+    int status = 0;
+    std::uint32_t outc = (chunk == 0) ? 0u : a.outcomes[d.layer_off + t];
+    std::uint32_t flg = (chunk == 0) ? 0u : a.flagged[d.layer_off + t];
+    Collapse cc = (chunk == 0) ? Collapse{0, 0, 1.0, 1.0} : a.coll[d.layer_off + t];
+    for (int j = 0; j < m; ++j)
+    {
+        const std::uint32_t n = 1u << j;
+        const double w = M[n + prefix];           // marginal of the decided prefix
+        const double w1 = M[2 * n + (prefix | n)];  // ... with site j = 1
+        if (!(w > 0.0)) status = E_NORM;
+        double p1 = (w > 0.0) ? w1 / w : 0.0;
+        p1 = p1 < 0.0 ? 0.0 : (p1 > 1.0 ? 1.0 : p1);
+        const int q = sites[j];
+        const double uj = u[q];
+        a.p1[d.unif_off + static_cast<std::size_t>(t) * d.L + q] = p1;
+        cc.mask |= std::uint64_t{1} << q;
+        if (uj < p1)
+        {
+            prefix |= n;
+            cc.val |= std::uint64_t{1} << q;
+            outc |= 1u << q;
+        }
+        if (fabs(uj - p1) < 1e-6) flg |= 1u << q;
+    }
+    const double total = M[1];
+    const double w = h[prefix];
+    if (status != 0 || !(w > 0.0) || !(total > 0.0))
+    {
+        a.status[traj] = E_NORM;
+        cc.scale = 1.0;
+    }
+    else
+    {
+        cc.weight = w;
+        cc.scale = 1.0 / sqrt(w);
+        a.surprisal[traj] += -log(w / total);
+    }
+    a.coll[d.layer_off + t] = cc;
+    a.outcomes[d.layer_off + t] = outc;
+    a.flagged[d.layer_off + t] = flg;
+}
+
+// One sweep over the tiles of a group of trajectories.  AUX = 0: a gate sweep
+// (the hot kernel); AUX = 1: a sweep without gates (the closing collapse, or
+// the histogram of an extra chunk of a layer with more than MAX_HIST_BITS
+// measured sites), kept as its own kernel so profiles separate the two.
+template <int RB, int NT, int AUX>
 __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
 {
     constexpr int NV = 1 << RB;
     constexpr int WB = (NT == 256) ? 2 : (NT == 128 ? 1 : 0);
-    static_assert(RB == 6, "register tiles hold 64 amplitudes");
+    static_assert(RB == 5 || RB == 6, "register tiles hold 32 or 64 amplitudes");
     extern __shared__ float2 lds[];
     const PassPlan &pl = args.plan;
     const DevArena &a = args.a;
@@ -481,7 +601,7 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
     for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << (6 + r));
     bool in_m0 = true;
     bool lds_dirty = false;  // a previous LDS read must finish before the next write
-    for (int si = 0; si < pl.nstages; ++si)
+    for (int si = 0; si < (AUX ? 0 : pl.nstages); ++si)
     {
         const StagePlan &sp = pl.st[si];
         if (sp.use_lds)
@@ -615,137 +735,20 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
     }
 }
 
-// Finalize one chunk of a layer's measurements per trajectory: fixed-order
-// reduction of the tile partials into the chunk's joint histogram, then the
-// sequential Born decisions (outcome 1 iff u < P(1)), the near-threshold flags
-// and the collapse record for the next sweep.
+// One workgroup per trajectory of the group: finalize_chunk after a
+// histogram sweep (the kernel boundary publishes every tile's partials).
 struct FinalizeArgs
 {
     DevArena a;
     const int *traj_list;
-    int layer;
-    int chunk;
-    std::uint64_t rest_mask;
-    int ntiles, k, c, s;
+    int layer, chunk;
+    Geom g;
 };
 
 __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f)
 {
-    __shared__ double h[HSTRIDE];
-    const DevArena &a = f.a;
-    const int traj = f.traj_list[blockIdx.x];
-    const TrajDesc d = a.desc[traj];
-    const int t = f.layer;
-    const std::uint32_t fl = a.flags[d.layer_off + t];
-    // chunk sites in ascending order
-    int sites[MAX_HIST_BITS];
-    int m = 0, seen = 0;
-    for (int q = 0; q < d.L; ++q)
-    {
-        if (!((fl >> q) & 1u)) continue;
-        if (seen >= f.chunk * MAX_HIST_BITS && seen < (f.chunk + 1) * MAX_HIST_BITS) sites[m++] = q;
-        ++seen;
-    }
-    Collapse prev = (f.chunk == 0) ? Collapse{0, 0, 1.0, 1.0} : a.coll[d.layer_off + t];
-    if (m == 0)
-    {
-        if (threadIdx.x == 0 && f.chunk == 0)
-        {
-            a.coll[d.layer_off + t] = prev;
-            a.outcomes[d.layer_off + t] = 0;
-            a.flagged[d.layer_off + t] = 0;
-        }
-        return;
-    }
-    // local (u bit) vs tile (physical bit) parts of each global bin bit
-    std::uint32_t loc_u = 0;
-    std::uint64_t non_phys = 0;
-    for (int j = 0; j < m; ++j)
-    {
-        const int q = sites[j];
-        if (q < f.c) loc_u |= 1u << q;
-        else if (q >= f.s && q < f.s + (f.k - f.c)) loc_u |= 1u << (q - f.s + f.c);
-        else non_phys |= std::uint64_t{1} << q;
-    }
-    const std::uint32_t bins = 1u << m;
-    for (std::uint32_t g = threadIdx.x; g < bins; g += blockDim.x)
-    {
-        // bin bit j <-> sites[j]; split into local-bin index and tile pattern
-        std::uint32_t bloc = 0;
-        std::uint64_t pat = 0;
-        int jl = 0;
-        for (int j = 0; j < m; ++j)
-        {
-            const int q = sites[j];
-            const bool local = !((non_phys >> q) & 1u);
-            const std::uint32_t bit = (g >> j) & 1u;
-            if (local)
-            {
-                // local-bin bit order = ascending u bit order of the local sites
-                (void)jl;
-            }
-            else if (bit)
-                pat |= std::uint64_t{1} << q;
-        }
-        // local-bin index: bits of g for local sites, ordered by their u bit
-        int idx = 0;
-        for (int ub = 0; ub < f.k; ++ub)
-        {
-            if (!((loc_u >> ub) & 1u)) continue;
-            const int q = (ub < f.c) ? ub : (ub - f.c + f.s);
-            int j = 0;
-            while (sites[j] != q) ++j;
-            if ((g >> j) & 1u) bloc |= 1u << idx;
-            ++idx;
-        }
-        double sum = 0.0;
-        for (int tile = 0; tile < f.ntiles; ++tile)
-        {
-            const std::uint64_t Bt = pdep64(static_cast<std::uint64_t>(tile), f.rest_mask);
-            if ((Bt & non_phys) != pat) continue;
-            sum += a.partials[(static_cast<std::size_t>(blockIdx.x) * f.ntiles + tile) * HSTRIDE + bloc];
-        }
-        h[g] = sum;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const double *u = a.uniforms + d.unif_off + static_cast<std::size_t>(t) * d.L;
-    double uu[MAX_HIST_BITS], pp[MAX_HIST_BITS];
-    for (int j = 0; j < m; ++j) uu[j] = u[sites[j]];
-    double total = 0.0;
-    for (std::uint32_t g = 0; g < bins; ++g) total += h[g];
-    int status = 0;
-    const std::uint32_t chosen = sequential_from_hist(h, m, uu, pp, &status);
-    const double w = h[chosen];
-    std::uint32_t outc = (f.chunk == 0) ? 0u : a.outcomes[d.layer_off + t];
-    std::uint32_t flg = (f.chunk == 0) ? 0u : a.flagged[d.layer_off + t];
-    Collapse cc = prev;
-    for (int j = 0; j < m; ++j)
-    {
-        const int q = sites[j];
-        a.p1[d.unif_off + static_cast<std::size_t>(t) * d.L + q] = pp[j];
-        cc.mask |= std::uint64_t{1} << q;
-        if ((chosen >> j) & 1u)
-        {
-            cc.val |= std::uint64_t{1} << q;
-            outc |= 1u << q;
-        }
-        if (fabs(uu[j] - pp[j]) < 1e-6) flg |= 1u << q;
-    }
-    if (status != 0 || !(w > 0.0) || !(total > 0.0))
-    {
-        a.status[traj] = E_NORM;
-        cc.scale = 1.0;
-    }
-    else
-    {
-        cc.weight = w;
-        cc.scale = 1.0 / sqrt(w);
-        a.surprisal[traj] += -log(w / total);
-    }
-    a.coll[d.layer_off + t] = cc;
-    a.outcomes[d.layer_off + t] = outc;
-    a.flagged[d.layer_off + t] = flg;
+    __shared__ double M[2 << MAX_HIST_BITS];
+    finalize_chunk(f.a, f.traj_list[blockIdx.x], blockIdx.x, f.layer, f.chunk, f.g, M);
 }
 
 // Marks the layer's collapse as applied (identity), after the closing sweep.
@@ -1021,13 +1024,16 @@ struct Batch
         own_stream = false;
     }
 
+    // Tile variants: 14 -> 2^14-amplitude tiles, 256 threads x 64 registers
+    // (one workgroup per CU); 13 -> 2^13 tiles, 256 x 32 (two per CU), or
+    // 128 x 64 with threads = 128; 12 -> 64 x 64.
     Config config_for(int nq) const
     {
         (void)nq;
-        Config cf{6, 256, 6};
-        if (cfg.tile_bits == 13) cf = Config{6, 128, 5};
-        if (cfg.tile_bits == 12) cf = Config{6, 64, 4};
-        return cf;
+        if (cfg.tile_bits == 14) return Config{6, 256, 6};
+        if (cfg.tile_bits == 13) return cfg.threads == 128 ? Config{6, 128, 5} : Config{5, 256, 5};
+        if (cfg.tile_bits == 12) return Config{6, 64, 4};
+        return Config{5, 256, 5};
     }
 
     int setup(const mipt_hip_traj_spec *sp, int count)
@@ -1242,7 +1248,7 @@ struct Batch
         return OK;
     }
 
-    template <int RB, int NT>
+    template <int RB, int NT, int AUX>
     int launch_pass(const PassPlan &pl, const int *list, int ntraj, int layer)
     {
         PassArgs pa{a, list, layer, pl};
@@ -1250,21 +1256,27 @@ struct Batch
         static bool attr_set = false;
         if (!attr_set)
         {
-            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT>),
+            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                static_cast<int>((std::size_t{1} << 14) * sizeof(float2) + 256 * sizeof(double))));
             attr_set = true;
         }
-        hipLaunchKernelGGL((pass_kernel<RB, NT>), dim3(static_cast<unsigned>(ntraj * pl.ntiles)), dim3(NT), lds, stream, pa);
+        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX>), dim3(static_cast<unsigned>(ntraj * pl.ntiles)), dim3(NT), lds,
+                           stream, pa);
         MIPT_HIP_CHECK(hipGetLastError());
         return OK;
     }
 
     int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer)
     {
-        if (cf.nt == 256) return launch_pass<6, 256>(pl, list, ntraj, layer);
-        if (cf.nt == 128) return launch_pass<6, 128>(pl, list, ntraj, layer);
-        return launch_pass<6, 64>(pl, list, ntraj, layer);
+        const bool aux = pl.nstages == 0;
+        if (cf.nt == 256 && cf.rb == 5)
+            return aux ? launch_pass<5, 256, 1>(pl, list, ntraj, layer) : launch_pass<5, 256, 0>(pl, list, ntraj, layer);
+        if (cf.nt == 256)
+            return aux ? launch_pass<6, 256, 1>(pl, list, ntraj, layer) : launch_pass<6, 256, 0>(pl, list, ntraj, layer);
+        if (cf.nt == 128)
+            return aux ? launch_pass<6, 128, 1>(pl, list, ntraj, layer) : launch_pass<6, 128, 0>(pl, list, ntraj, layer);
+        return aux ? launch_pass<6, 64, 1>(pl, list, ntraj, layer) : launch_pass<6, 64, 0>(pl, list, ntraj, layer);
     }
 
     // Tiled evolution of a group of same-size trajectories (nq > 14).
@@ -1290,7 +1302,7 @@ struct Batch
                 PassPlan pl = P.passes[p];
                 pl.collapse = (p == 0) ? 1 : 0;
                 pl.init = (p == 0 && t == 0) ? 1 : 0;
-                pl.hist = (p + 1 == P.passes.size() && chunks > 0) ? 1 : 0;
+                pl.hist = (p + 1 == P.passes.size()) ? 1 : 0;  // chunk 0 always: it also records m == 0
                 pl.chunk = 0;
                 pl.cond = 0;
                 pl.store = 1;
@@ -1304,7 +1316,8 @@ struct Batch
                 MIPT_HIP_TRY(timed_launch_end(slot));
             }
             const PassPlan &last = P.passes.back();
-            FinalizeArgs fa{a, list, t, 0, last.rest_mask, last.ntiles, last.k, last.c, last.s};
+            const Geom geo{last.rest_mask, last.ntiles, last.k, last.c, last.s};
+            FinalizeArgs fa{a, list, t, 0, geo};
             hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, stream, fa);
             MIPT_HIP_CHECK(hipGetLastError());
             for (int ch = 1; ch < chunks; ++ch)
@@ -1318,7 +1331,7 @@ struct Batch
                 pl.cond = 1;
                 pl.store = 0;
                 MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
-                FinalizeArgs fc{a, list, t, ch, last.rest_mask, last.ntiles, last.k, last.c, last.s};
+                FinalizeArgs fc{a, list, t, ch, geo};
                 hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, stream, fc);
                 MIPT_HIP_CHECK(hipGetLastError());
             }
