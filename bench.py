@@ -85,6 +85,8 @@ def main(argv=None):
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--profile-every", type=int, default=7, help="odd, so both sweeps of a layer are sampled")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB plain copy (traffic-counter calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -94,6 +96,9 @@ def main(argv=None):
 
     import mipt_amd as M
 
+    if args.copy_calibration:
+        nbytes = M.stream_copy_calibration(25)
+        log(f"calibration copy of {nbytes} bytes done")
     L, layers = args.L, args.layers
     ps = linspace(0.0, 0.3, args.np)
     specs = []

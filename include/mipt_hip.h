@@ -36,6 +36,11 @@ const char *mipt_hip_last_error(void);
 /* Library version string. */
 const char *mipt_hip_version(void);
 
+/* Measurement helper (no counterpart in the reference): copies `bytes`
+ * (a multiple of 16) device to device with a plain streaming kernel, so that
+ * hardware traffic counters can be calibrated against a known byte count. */
+int mipt_hip_stream_copy(void *dst, const void *src, uint64_t bytes, void *hip_stream);
+
 /* ------------------------------------------------------------------ drop-ins */
 
 /* Replaces mipt_cuda_measure_collapse_f32/_f64

@@ -92,6 +92,7 @@ def lib():
     L.mipt_hip_batch_export_layers.argtypes = [vp, i, dp, u8p, dp]
     L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
+    L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
     _LIB = L
     return L
 
@@ -245,6 +246,21 @@ class PlainEngine:
         out = ctypes.c_double()
         _check(lib().mipt_hip_engine_squared_norm(self._h, ctypes.byref(out)), "squared_norm")
         return out.value
+
+
+def stream_copy_calibration(log2_elems: int = 25) -> int:
+    """Copies a 2^log2_elems complex64 buffer (256 MiB at 25) once with a plain
+    streaming kernel; returns the bytes read (= bytes written)."""
+    src = PlainEngine(False)
+    dst = PlainEngine(False)
+    src.reset(log2_elems)
+    dst.reset(log2_elems)
+    nbytes = (1 << log2_elems) * 8
+    _check(lib().mipt_hip_stream_copy(ctypes.c_void_p(dst.data), ctypes.c_void_p(src.data), nbytes, None),
+           "stream_copy")
+    src.close()
+    dst.close()
+    return nbytes
 
 
 def layer_bonds(n: int, layer: int, closed: bool = True):

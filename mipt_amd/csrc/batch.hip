@@ -264,13 +264,14 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
 // consecutive amplitudes.
 struct StagePlan
 {
-    std::int8_t reg_u[6];
-    std::int8_t lane_u[6];
-    std::int8_t wave_u[4];
-    std::int8_t nb;        // bonds in this stage
-    std::int8_t bond[3];   // bond number (within the layer) on register pair i: (0,1), (2,3), (4,5); -1 none
-    std::int8_t use_lds;   // 0: M0 registers (no exchange)
-    std::int8_t pad[3];
+    // 32-bit fields: a uniform runtime index into the kernel arguments then
+    // compiles to scalar loads (sub-dword fields force vector loads).
+    int reg_u[6];
+    int lane_u[6];
+    int wave_u[4];
+    int nb;        // bonds in this stage
+    int bond[3];   // bond number (within the layer) on register pair i: (0,1), (2,3), (4,5); -1 none
+    int use_lds;   // 0: M0 registers (no exchange)
 };
 
 struct PassPlan
@@ -365,8 +366,7 @@ __device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePl
 // LDS address pieces of a mapping: thread part T (lane and wave bits) and the
 // per-register-bit XOR terms Q[r] = swz(1 << reg_u[r]).
 template <int WB>
-__device__ __forceinline__ std::uint32_t thread_term(const std::int8_t *lane_u, const std::int8_t *wave_u, int lane,
-                                                     int wave)
+__device__ __forceinline__ std::uint32_t thread_term(const int *lane_u, const int *wave_u, int lane, int wave)
 {
     std::uint32_t o = 0;
 #pragma unroll
@@ -790,14 +790,14 @@ int ilog2i(int x)
 
 void set_m0(StagePlan &sp, int rb, int wb)
 {
-    for (int r = 0; r < 6; ++r) sp.reg_u[r] = static_cast<std::int8_t>(r < rb ? 6 + r : 0);
-    for (int b = 0; b < 6; ++b) sp.lane_u[b] = static_cast<std::int8_t>(b);
-    for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<std::int8_t>(b < wb ? 6 + rb + b : 0);
+    for (int r = 0; r < 6; ++r) sp.reg_u[r] = static_cast<int>(r < rb ? 6 + r : 0);
+    for (int b = 0; b < 6; ++b) sp.lane_u[b] = static_cast<int>(b);
+    for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<int>(b < wb ? 6 + rb + b : 0);
 }
 
 // Choose 6 lane bits among `free` so that their low-nibble contributions
 // under swz() span all 16 banks-pairs where possible.
-void pick_lanes(std::vector<int> &freeb, std::int8_t lane_u[6])
+void pick_lanes(std::vector<int> &freeb, int lane_u[6])
 {
     std::vector<int> chosen;
     int covered = 0;  // residues mod 4 seen
@@ -812,7 +812,7 @@ void pick_lanes(std::vector<int> &freeb, std::int8_t lane_u[6])
             }
     for (std::size_t i = 0; i < freeb.size() && chosen.size() < 6; ++i)
         if (std::find(chosen.begin(), chosen.end(), freeb[i]) == chosen.end()) chosen.push_back(freeb[i]);
-    for (int b = 0; b < 6; ++b) lane_u[b] = static_cast<std::int8_t>(chosen[b]);
+    for (int b = 0; b < 6; ++b) lane_u[b] = static_cast<int>(chosen[b]);
     std::vector<int> rest;
     for (int x : freeb)
         if (std::find(chosen.begin(), chosen.end(), x) == chosen.end()) rest.push_back(x);
@@ -903,8 +903,8 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
             sp.use_lds = 0;
             sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
             for (std::size_t i = 0; i < m0_bonds.size(); ++i)
-                sp.bond[(u_of(b0[m0_bonds[i]]) - 6) / 2] = static_cast<std::int8_t>(m0_bonds[i]);
-            sp.nb = static_cast<std::int8_t>(m0_bonds.size());
+                sp.bond[(u_of(b0[m0_bonds[i]]) - 6) / 2] = static_cast<int>(m0_bonds[i]);
+            sp.nb = static_cast<int>(m0_bonds.size());
             stages.push_back(sp);
         }
         for (std::size_t i = 0; i < other.size(); i += per_stage)
@@ -917,13 +917,13 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
             for (std::size_t j = i; j < other.size() && j < i + per_stage; ++j, ++nbs)
             {
                 const int b = other[j];
-                sp.reg_u[2 * nbs] = static_cast<std::int8_t>(u_of(b0[b]));
-                sp.reg_u[2 * nbs + 1] = static_cast<std::int8_t>(u_of(b1[b]));
+                sp.reg_u[2 * nbs] = static_cast<int>(u_of(b0[b]));
+                sp.reg_u[2 * nbs + 1] = static_cast<int>(u_of(b1[b]));
                 used.push_back(u_of(b0[b]));
                 used.push_back(u_of(b1[b]));
-                sp.bond[nbs] = static_cast<std::int8_t>(b);
+                sp.bond[nbs] = static_cast<int>(b);
             }
-            sp.nb = static_cast<std::int8_t>(nbs);
+            sp.nb = static_cast<int>(nbs);
             std::vector<int> freeb;
             for (int u = 0; u < k; ++u)
                 if (std::find(used.begin(), used.end(), u) == used.end()) freeb.push_back(u);
@@ -935,11 +935,11 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
                 // take the largest free bit
                 const int x = freeb.back();
                 freeb.pop_back();
-                sp.reg_u[r++] = static_cast<std::int8_t>(x);
+                sp.reg_u[r++] = static_cast<int>(x);
             }
             for (int rr = cf.rb; rr < 6; ++rr) sp.reg_u[rr] = 0;
             pick_lanes(freeb, sp.lane_u);
-            for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<std::int8_t>(b < wb ? freeb[b] : 0);
+            for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<int>(b < wb ? freeb[b] : 0);
             stages.push_back(sp);
         }
         if (stages.size() > 6)

@@ -364,6 +364,15 @@ struct PlainEngine
     }
 };
 
+// Plain device copy used to calibrate hardware traffic counters
+// (FETCH_SIZE / WRITE_SIZE) against a known byte count.
+__global__ void stream_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, std::size_t n)
+{
+    const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 } // namespace mipt_hip
 
 using namespace mipt_hip;
@@ -375,6 +384,17 @@ struct mipt_hip_engine : PlainEngine
 extern "C" {
 
 const char *mipt_hip_last_error(void) { return mipt_hip::last_error(); }
+
+int mipt_hip_stream_copy(void *dst, const void *src, uint64_t bytes, void *stream)
+{
+    if (!dst || !src || (bytes % 16) != 0) { set_error("stream_copy: bad arguments"); return E_ARG; }
+    const std::size_t n = bytes / 16;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(grid_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float4 *>(src), static_cast<float4 *>(dst), n);
+    MIPT_HIP_CHECK(hipGetLastError());
+    MIPT_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return OK;
+}
 const char *mipt_hip_version(void) { return "mipt_hip 0.1 gfx950"; }
 
 int mipt_hip_measure_collapse_f32(void *s, int n, int q, double u, int *o, double *p)
