@@ -455,18 +455,30 @@ __device__ void finalize_chunk(const DevArena &a, int traj, int tl, int t, int c
         }
     }
     const std::uint32_t bins = 1u << m;
-    const std::uint32_t lbins = 1u << __popc(gloc);
     double *M = lds_d;  // level j at M[2^j .. 2^(j+1)), level m = h
     double *h = M + bins;
-    for (std::uint32_t x = threadIdx.x; x < bins; x += blockDim.x) h[x] = 0.0;
-    __syncthreads();
+    // tiles contributing to a global bin are those whose tile-index bits at the
+    // chunk's tile sites equal the bin's tile part: enumerate them in
+    // increasing tile order (deterministic), each partial read exactly once
+    const std::uint32_t ntile_bits = static_cast<std::uint32_t>(__popcll(g.rest_mask));
+    const std::uint32_t tmask_all = (ntile_bits >= 32) ? 0xffffffffu : ((1u << ntile_bits) - 1u);
+    const std::uint32_t nmask = static_cast<std::uint32_t>(pext64(non_phys, g.rest_mask));  // tile-index bits
+    const std::uint32_t fmask = tmask_all & ~nmask;
+    const std::uint32_t per_bin = 1u << __popc(fmask);
     const double *part = a.partials + static_cast<std::size_t>(tl) * g.ntiles * HSTRIDE;
-    for (int tile = 0; tile < g.ntiles; ++tile)
+    for (std::uint32_t x = threadIdx.x; x < bins; x += blockDim.x)
     {
-        const std::uint64_t Bt = pdep64(static_cast<std::uint64_t>(tile), g.rest_mask);
-        const std::uint32_t pat = static_cast<std::uint32_t>(pdep64(pext64(Bt, non_phys), gnon));
-        for (std::uint32_t b = threadIdx.x; b < lbins; b += blockDim.x)
-            h[static_cast<std::uint32_t>(pdep64(b, gloc)) | pat] += part[static_cast<std::size_t>(tile) * HSTRIDE + b];
+        const std::uint32_t bloc = static_cast<std::uint32_t>(pext64(x, gloc));
+        const std::uint32_t tsel = static_cast<std::uint32_t>(pdep64(pext64(x, gnon), nmask));
+        double acc = 0.0;
+        std::uint32_t r = 0;
+#pragma unroll 8
+        for (std::uint32_t e = 0; e < per_bin; ++e)
+        {
+            acc += part[static_cast<std::size_t>(tsel | r) * HSTRIDE + bloc];
+            r = next_submask(r, fmask);
+        }
+        h[x] = acc;
     }
     __syncthreads();
     for (int j = m - 1; j >= 0; --j)
@@ -972,6 +984,15 @@ struct Batch
     bool flags_on_host = false;
     std::vector<char> initialized;  // per trajectory: state already set by caller
 
+    // Tiled groups alternate between two streams so that one group's small
+    // finalize kernels and launch gaps overlap the other group's sweeps.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t ls = nullptr;  // stream of the group being issued
+    DevArena al{};             // arena view of that group (its own partials buffer)
+    double *partials2 = nullptr;
+    std::size_t partial2_cap = 0;
+
     // Sampled HIP-event timing of the gate sweeps (pass kernels).
     int prof_every = 0;
     long prof_counter = 0;
@@ -984,6 +1005,10 @@ struct Batch
     {
         release();
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (partials2) (void)hipFree(partials2);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
     }
 
     int timed_launch_begin(double bytes, int &slot)
@@ -991,7 +1016,7 @@ struct Batch
         slot = -1;
         ++gate_launches;
         gate_bytes += bytes;
-        if (prof_every <= 0 || (prof_counter++ % prof_every) != 0) return OK;
+        if (prof_every <= 0 || ls != stream || (prof_counter++ % prof_every) != 0) return OK;
         const int pair = static_cast<int>(ev_used.size());
         while (ev_pool.size() < static_cast<std::size_t>(2 * pair + 2))
         {
@@ -1238,20 +1263,22 @@ struct Batch
         return OK;
     }
 
-    int ensure_partials(std::size_t doubles)
+    int ensure_partials(std::size_t doubles, int lane)
     {
-        if (doubles <= a.partial_cap) return OK;
-        if (a.partials) MIPT_HIP_CHECK(hipFree(a.partials));
-        a.partials = nullptr;
-        MIPT_HIP_CHECK(hipMalloc(&a.partials, doubles * 8));
-        a.partial_cap = doubles;
+        double *&p = lane ? partials2 : a.partials;
+        std::size_t &cap = lane ? partial2_cap : a.partial_cap;
+        if (doubles <= cap) return OK;
+        if (p) MIPT_HIP_CHECK(hipFree(p));
+        p = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&p, doubles * 8));
+        cap = doubles;
         return OK;
     }
 
     template <int RB, int NT, int AUX>
     int launch_pass(const PassPlan &pl, const int *list, int ntraj, int layer)
     {
-        PassArgs pa{a, list, layer, pl};
+        PassArgs pa{al, list, layer, pl};
         const std::size_t lds = (std::size_t{1} << pl.k) * sizeof(float2) + NT * sizeof(double);
         static bool attr_set = false;
         if (!attr_set)
@@ -1262,7 +1289,7 @@ struct Batch
             attr_set = true;
         }
         hipLaunchKernelGGL((pass_kernel<RB, NT, AUX>), dim3(static_cast<unsigned>(ntraj * pl.ntiles)), dim3(NT), lds,
-                           stream, pa);
+                           ls, pa);
         MIPT_HIP_CHECK(hipGetLastError());
         return OK;
     }
@@ -1280,7 +1307,7 @@ struct Batch
     }
 
     // Tiled evolution of a group of same-size trajectories (nq > 14).
-    int evolve_group_tiled(const std::vector<int> &grp, int grp_pos, int t0, int t1)
+    int evolve_group_tiled(const std::vector<int> &grp, int grp_pos, int t0, int t1, int lane)
     {
         const TrajDesc &d0 = desc[grp[0]];
         const Config cf = config_for(d0.nq);
@@ -1289,7 +1316,11 @@ struct Batch
         MIPT_HIP_TRY(build_layer_plan(d0.L, d0.nq, d0.closed, 1, cf, lp[1]));
         const int ntraj = static_cast<int>(grp.size());
         const int *list = d_traj_list + grp_pos;
-        MIPT_HIP_TRY(ensure_partials(static_cast<std::size_t>(ntraj) * (std::size_t{1} << (d0.nq - cf.k())) * HSTRIDE));
+        MIPT_HIP_TRY(ensure_partials(static_cast<std::size_t>(ntraj) * (std::size_t{1} << (d0.nq - cf.k())) * HSTRIDE,
+                                     lane));
+        ls = lane ? side : stream;
+        al = a;
+        al.partials = lane ? partials2 : a.partials;
         // maximum measured count per layer over the group (host flags)
         for (int t = t0; t < t1; ++t)
         {
@@ -1317,8 +1348,8 @@ struct Batch
             }
             const PassPlan &last = P.passes.back();
             const Geom geo{last.rest_mask, last.ntiles, last.k, last.c, last.s};
-            FinalizeArgs fa{a, list, t, 0, geo};
-            hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, stream, fa);
+            FinalizeArgs fa{al, list, t, 0, geo};
+            hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, ls, fa);
             MIPT_HIP_CHECK(hipGetLastError());
             for (int ch = 1; ch < chunks; ++ch)
             {
@@ -1331,8 +1362,8 @@ struct Batch
                 pl.cond = 1;
                 pl.store = 0;
                 MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
-                FinalizeArgs fc{a, list, t, ch, geo};
-                hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, stream, fc);
+                FinalizeArgs fc{al, list, t, ch, geo};
+                hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, ls, fc);
                 MIPT_HIP_CHECK(hipGetLastError());
             }
         }
@@ -1346,7 +1377,7 @@ struct Batch
             pl.hist = 0;
             pl.store = 1;
             MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t1));
-            hipLaunchKernelGGL(reset_collapse_kernel, dim3((ntraj + 255) / 256), dim3(256), 0, stream, a, list, ntraj,
+            hipLaunchKernelGGL(reset_collapse_kernel, dim3((ntraj + 255) / 256), dim3(256), 0, ls, al, list, ntraj,
                                t1 - 1);
             MIPT_HIP_CHECK(hipGetLastError());
         }
@@ -1362,6 +1393,15 @@ struct Batch
             return E_ARG;
         }
         const int count = static_cast<int>(order.size());
+        if (!side)
+        {
+            MIPT_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+            MIPT_HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+            MIPT_HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        }
+        MIPT_HIP_CHECK(hipEventRecord(ev_fork, stream));
+        MIPT_HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+        int tiled_groups = 0;
         int pos = 0;
         while (pos < count)
         {
@@ -1394,19 +1434,24 @@ struct Batch
             }
             else
             {
-                // MALL-sized groups
+                // MALL-sized groups: two in flight (one per stream) within the
+                // 256 MiB Infinity Cache
                 const std::size_t bytes = (std::size_t{1} << nq) * 8;
-                const std::size_t budget = std::size_t{160} << 20;
+                const std::size_t budget = std::size_t{112} << 20;
                 int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
                 for (int q = pos; q < end; q += g)
                 {
                     const int qe = std::min(end, q + g);
                     std::vector<int> grp(order.begin() + q, order.begin() + qe);
-                    MIPT_HIP_TRY(evolve_group_tiled(grp, q, t0, t1));
+                    MIPT_HIP_TRY(evolve_group_tiled(grp, q, t0, t1, tiled_groups & 1));
+                    ++tiled_groups;
                 }
             }
             pos = end;
         }
+        ls = stream;
+        MIPT_HIP_CHECK(hipEventRecord(ev_join, side));
+        MIPT_HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
         if (t1_req < 0 || t1_req > t0)
         {
             hipLaunchKernelGGL(fill_int_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, a.init, count, 0);
