@@ -114,10 +114,14 @@ def main(argv=None):
     hip.hipEventCreate(ctypes.byref(e0))
     hip.hipEventCreate(ctypes.byref(e1))
 
+    host_issue = []
+
     def step():
         b.reset()
         b.generate()
+        t = time.perf_counter()
         b.evolve()
+        host_issue.append(time.perf_counter() - t)
 
     for w in range(args.warmup):
         step()
@@ -142,7 +146,8 @@ def main(argv=None):
     st = b.pass_stats()
     achieved = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
 
-    log(f"timed steps done: {ms_per_step:.1f} ms/step, {value:.0f} layers/s")
+    log(f"timed steps done: {ms_per_step:.1f} ms/step, {value:.0f} layers/s; host issue of evolve "
+        f"{1e3 * sum(host_issue[-args.steps:]) / args.steps:.1f} ms/step")
     # plain one-trajectory-per-launch path on the same trajectories' device draws
     plain_lps = None
     if args.plain > 0:

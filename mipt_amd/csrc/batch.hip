@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -272,6 +273,7 @@ struct StagePlan
     int nb;        // bonds in this stage
     int bond[3];   // bond number (within the layer) on register pair i: (0,1), (2,3), (4,5); -1 none
     int use_lds;   // 0: M0 registers (no exchange)
+    int mbond;     // bond on lane bits (4,5) applied on the matrix cores; -1 none
 };
 
 struct PassPlan
@@ -306,7 +308,8 @@ __device__ __forceinline__ std::uint32_t swz(std::uint32_t u)
 // nibbles so that lanes walking any 6 u bits with distinct residues mod 4 hit
 // distinct banks.  swz is GF(2)-linear: swz(a ^ b) = swz(a) ^ swz(b).
 // 4x4 complex gate (row-major, index 2*b(q1)+b(q0)) on register bits (RA <- q0, RC <- q1).
-template <int NV, int RA, int RC>
+// HALF = -1: all register groups; 0/1: only groups whose top register bit is HALF.
+template <int NV, int RA, int RC, int HALF = -1>
 __device__ __forceinline__ void gate_on_regs(float2 (&v)[NV], const float2 *__restrict__ g)
 {
     float ur[16], ui[16];
@@ -321,6 +324,7 @@ __device__ __forceinline__ void gate_on_regs(float2 (&v)[NV], const float2 *__re
     for (int base = 0; base < NV; ++base)
     {
         if (base & ((1 << RA) | (1 << RC))) continue;
+        if (HALF >= 0 && ((base & (NV / 2)) != 0) != (HALF == 1)) continue;
         const int i1 = base | (1 << RA), i2 = base | (1 << RC), i3 = i1 | i2;
         const float2 x0 = v[base], x1 = v[i1], x2 = v[i2], x3 = v[i3];
         float2 y[4];
@@ -352,15 +356,94 @@ __device__ __forceinline__ void gate_on_regs(float2 (&v)[NV], const float2 *__re
     }
 }
 
-// Bonds of a stage sit on register pairs (0,1), (2,3), (4,5); -1 = none.
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// The bond on lane bits (4,5) on the matrix cores.  Lane l holds basis state
+// q = l >> 4 of group l & 15, so one v_mfma_f32_16x16x4f32 with B = Re (then
+// Im) of a register applies a 16x4 real matrix to the four amplitudes of
+// every group: rows 4q and 4q+1 carry Re and Im of output q (so each lane
+// receives its own outputs in D[0], D[1]); rows 4q+2, 4q+3 are zero.
+template <int NV, int HALF>
+__device__ __forceinline__ void gate_on_lanes45(float2 (&v)[NV], float a1, float a2)
+{
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+    {
+        if (((j & (NV / 2)) != 0) != (HALF == 1)) continue;
+        floatx4_t d = {0.f, 0.f, 0.f, 0.f};
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, v[j].x, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, v[j].y, d, 0, 0, 0);
+        v[j] = make_float2(d[0], d[1]);
+    }
+}
+
+// Register-pair bonds of a stage: (0,1), (2,3), (4,5); -1 = none.
+template <int NV, int HALF>
+__device__ __forceinline__ void valu_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
+    __attribute__((always_inline));
+template <int NV, int HALF>
+__device__ __forceinline__ void valu_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
+{
+    if (sp.bond[0] >= 0) gate_on_regs<NV, 0, 1, HALF>(v, gl + 16 * sp.bond[0]);
+    if (sp.bond[1] >= 0) gate_on_regs<NV, 2, 3, HALF>(v, gl + 16 * sp.bond[1]);
+    if constexpr (NV == 64 && HALF < 0)
+        if (sp.bond[2] >= 0) gate_on_regs<NV, 4, 5, -1>(v, gl + 16 * sp.bond[2]);
+}
+
+// A stage's bonds.  With a matrix-core bond the registers are split on their
+// top bit and the stage runs as straight-line code (no branches between the
+// halves, so the scheduler can interleave): register-pair bonds on the first
+// half, then the MFMAs of the first half interleaved with the register-pair
+// bonds of the second half, then the MFMAs of the second half.  The planner
+// guarantees that such a stage uses pairs (0,1) [and (2,3)] only.
 template <int NV>
-__device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
+__device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl, int lane)
 {
     static_assert(NV == 32 || NV == 64, "register tiles hold 32 or 64 amplitudes");
-    if (sp.bond[0] >= 0) gate_on_regs<NV, 0, 1>(v, gl + 16 * sp.bond[0]);
-    if (sp.bond[1] >= 0) gate_on_regs<NV, 2, 3>(v, gl + 16 * sp.bond[1]);
-    if constexpr (NV == 64)
-        if (sp.bond[2] >= 0) gate_on_regs<NV, 4, 5>(v, gl + 16 * sp.bond[2]);
+    if (sp.mbond < 0)
+    {
+        valu_stage_gates<NV, -1>(v, sp, gl);
+        return;
+    }
+    const float2 um = gl[16 * sp.mbond + ((lane & 15) >> 2) * 4 + (lane >> 4)];
+    const int e = lane & 3;
+    const float a1 = e == 0 ? um.x : (e == 1 ? um.y : 0.f);
+    const float a2 = e == 0 ? -um.y : (e == 1 ? um.x : 0.f);
+    if (sp.bond[0] >= 0 && sp.bond[1] >= 0)
+    {
+        const float2 *g0 = gl + 16 * sp.bond[0], *g1 = gl + 16 * sp.bond[1];
+        gate_on_regs<NV, 0, 1, 0>(v, g0);
+        gate_on_regs<NV, 2, 3, 0>(v, g1);
+        gate_on_lanes45<NV, 0>(v, a1, a2);
+        gate_on_regs<NV, 0, 1, 1>(v, g0);
+        gate_on_regs<NV, 2, 3, 1>(v, g1);
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+        {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 16, 0); // 16 VALU
+        }
+        gate_on_lanes45<NV, 1>(v, a1, a2);
+    }
+    else if (sp.bond[0] >= 0)
+    {
+        const float2 *g0 = gl + 16 * sp.bond[0];
+        gate_on_regs<NV, 0, 1, 0>(v, g0);
+        gate_on_lanes45<NV, 0>(v, a1, a2);
+        gate_on_regs<NV, 0, 1, 1>(v, g0);
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+        {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
+        }
+        gate_on_lanes45<NV, 1>(v, a1, a2);
+    }
+    else
+    {
+        gate_on_lanes45<NV, 0>(v, a1, a2);
+        gate_on_lanes45<NV, 1>(v, a1, a2);
+    }
 }
 
 // LDS address pieces of a mapping: thread part T (lane and wave bits) and the
@@ -376,31 +459,33 @@ __device__ __forceinline__ std::uint32_t thread_term(const int *lane_u, const in
     return swz(o);
 }
 
+// LDS byte address of register j: T3 ^ (XOR of Q3[r] over the set bits r of j),
+// with T3 = 8 * swz(thread bits) and Q3[r] = 8 * swz(1 << reg_u[r]) (uniform).
 template <int NV>
-__device__ __forceinline__ void lds_write(float2 *lds, const float2 (&v)[NV], std::uint32_t T, const std::uint32_t (&Q)[6])
+__device__ __forceinline__ void lds_write(char *lds, const float2 (&v)[NV], std::uint32_t T3, const std::uint32_t (&Q3)[6])
 {
 #pragma unroll
     for (int j = 0; j < NV; ++j)
     {
-        std::uint32_t a = T;
+        std::uint32_t s = 0;
 #pragma unroll
         for (int r = 0; r < 6; ++r)
-            if ((j >> r) & 1) a ^= Q[r];
-        lds[a] = v[j];
+            if ((j >> r) & 1) s ^= Q3[r];
+        *reinterpret_cast<float2 *>(lds + (T3 ^ s)) = v[j];
     }
 }
 
 template <int NV>
-__device__ __forceinline__ void lds_read(const float2 *lds, float2 (&v)[NV], std::uint32_t T, const std::uint32_t (&Q)[6])
+__device__ __forceinline__ void lds_read(const char *lds, float2 (&v)[NV], std::uint32_t T3, const std::uint32_t (&Q3)[6])
 {
 #pragma unroll
     for (int j = 0; j < NV; ++j)
     {
-        std::uint32_t a = T;
+        std::uint32_t s = 0;
 #pragma unroll
         for (int r = 0; r < 6; ++r)
-            if ((j >> r) & 1) a ^= Q[r];
-        v[j] = lds[a];
+            if ((j >> r) & 1) s ^= Q3[r];
+        v[j] = *reinterpret_cast<const float2 *>(lds + (T3 ^ s));
     }
 }
 
@@ -537,7 +622,7 @@ __device__ void finalize_chunk(const DevArena &a, int traj, int tl, int t, int c
 // the histogram of an extra chunk of a layer with more than MAX_HIST_BITS
 // measured sites), kept as its own kernel so profiles separate the two.
 template <int RB, int NT, int AUX>
-__global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
+__global__ void __launch_bounds__(NT, (RB == 5 ? 2 : 1)) pass_kernel(PassArgs args)
 {
     constexpr int NV = 1 << RB;
     constexpr int WB = (NT == 256) ? 2 : (NT == 128 ? 1 : 0);
@@ -557,7 +642,8 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
     const int t = args.layer;
     const int c = pl.c, s = pl.s;
     const std::uint64_t B = pdep64(static_cast<std::uint64_t>(tile), pl.rest_mask);
-    float2 *tp = a.states + d.state_off + B;  // tile base (uniform)
+    char *tp = reinterpret_cast<char *>(a.states + d.state_off + B);  // tile base (uniform)
+    char *ldsb = reinterpret_cast<char *>(lds);
     const std::uint32_t lowmask = (1u << c) - 1u;
     // M0 element offset of (lane, wave, register j): thread part + j * jstride
     const std::uint32_t jshift = static_cast<std::uint32_t>(s + 6 - c);
@@ -590,13 +676,20 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
 #pragma unroll
         for (int j = 0; j < NV; ++j) v[j] = make_float2(0.f, 0.f);
     }
+    else if (cm == 0)
+    {
+        // 32-bit byte offsets keep the SGPR-base + VGPR-offset addressing form
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            v[j] = *reinterpret_cast<const float2 *>(tp + ((toff + (static_cast<std::uint32_t>(j) << jshift)) << 3));
+    }
     else
     {
 #pragma unroll
         for (int j = 0; j < NV; ++j)
         {
             const std::uint32_t o = toff + (static_cast<std::uint32_t>(j) << jshift);
-            float2 x = tp[o];
+            float2 x = *reinterpret_cast<const float2 *>(tp + (o << 3));
             const float f = ((o & cml) == cvl) ? scale : 0.f;
             x.x *= f;
             x.y *= f;
@@ -607,10 +700,14 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
     // ---- stages
     const float2 *gl = a.gates + (d.gate_off + gate_prefix(d, t)) * 16;
     // current mapping: M0 (registers = u bits 6.., lanes = u bits 0..5)
-    std::uint32_t Tcur = swz(static_cast<std::uint32_t>(lane) | (static_cast<std::uint32_t>(wave) << (6 + RB)));
-    std::uint32_t Qcur[6];
+    const std::uint32_t T3m0 = swz(static_cast<std::uint32_t>(lane) | (static_cast<std::uint32_t>(wave) << (6 + RB))) << 3;
+    std::uint32_t Q3m0[6];
 #pragma unroll
-    for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << (6 + r));
+    for (int r = 0; r < 6; ++r) Q3m0[r] = swz(1u << (6 + r)) << 3;
+    std::uint32_t T3 = T3m0;
+    std::uint32_t Q3[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) Q3[r] = Q3m0[r];
     bool in_m0 = true;
     bool lds_dirty = false;  // a previous LDS read must finish before the next write
     for (int si = 0; si < (AUX ? 0 : pl.nstages); ++si)
@@ -619,30 +716,24 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
         if (sp.use_lds)
         {
             if (lds_dirty) __syncthreads();
-            lds_write<NV>(lds, v, Tcur, Qcur);
+            lds_write<NV>(ldsb, v, T3, Q3);
             __syncthreads();
-            Tcur = thread_term<WB>(sp.lane_u, sp.wave_u, lane, wave);
+            T3 = thread_term<WB>(sp.lane_u, sp.wave_u, lane, wave) << 3;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << sp.reg_u[r]);
-            lds_read<NV>(lds, v, Tcur, Qcur);
+            for (int r = 0; r < 6; ++r) Q3[r] = swz(1u << sp.reg_u[r]) << 3;
+            lds_read<NV>(ldsb, v, T3, Q3);
             lds_dirty = true;
             in_m0 = false;
         }
-        apply_stage_gates<NV>(v, sp, gl);
+        apply_stage_gates<NV>(v, sp, gl, lane);
     }
     // ---- back to M0 through LDS (also leaves the tile in LDS for the histogram)
     if (!in_m0 || pl.hist)
     {
         if (lds_dirty) __syncthreads();
-        lds_write<NV>(lds, v, Tcur, Qcur);
+        lds_write<NV>(ldsb, v, T3, Q3);
         __syncthreads();
-        if (!in_m0)
-        {
-            Tcur = swz(static_cast<std::uint32_t>(lane) | (static_cast<std::uint32_t>(wave) << (6 + RB)));
-#pragma unroll
-            for (int r = 0; r < 6; ++r) Qcur[r] = swz(1u << (6 + r));
-            lds_read<NV>(lds, v, Tcur, Qcur);
-        }
+        if (!in_m0) lds_read<NV>(ldsb, v, T3m0, Q3m0);
     }
 
     // ---- FP64 histogram partial of this tile over the chunk's local sites
@@ -667,7 +758,7 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
             else if (q >= s && q < s + (k - c)) loc_u |= 1u << (q - s + c);
         }
         std::uint64_t condm = 0, condv = 0;
-        if (pl.cond)
+        if (AUX && pl.cond)
         {
             const Collapse cc = a.coll[d.layer_off + t];
             condm = cc.mask;
@@ -694,7 +785,7 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
                     for (std::uint32_t e = 0; e < R; ++e)
                     {
                         const std::uint32_t u = bb | x;
-                        if ((local_phys(u) & cdm) == cdv)
+                        if (!AUX || (local_phys(u) & cdm) == cdv)
                         {
                             const float2 z = lds[swz(u)];
                             acc = fma(static_cast<double>(z.x), static_cast<double>(z.x), acc);
@@ -719,7 +810,7 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
                 for (std::uint32_t e = 0; e < per; ++e)
                 {
                     const std::uint32_t u = bb | x;
-                    if ((local_phys(u) & cdm) == cdv)
+                    if (!AUX || (local_phys(u) & cdm) == cdv)
                     {
                         const float2 z = lds[swz(u)];
                         acc = fma(static_cast<double>(z.x), static_cast<double>(z.x), acc);
@@ -743,7 +834,8 @@ __global__ void __launch_bounds__(NT) pass_kernel(PassArgs args)
     if (pl.store)
     {
 #pragma unroll
-        for (int j = 0; j < NV; ++j) tp[toff + (static_cast<std::uint32_t>(j) << jshift)] = v[j];
+        for (int j = 0; j < NV; ++j)
+            *reinterpret_cast<float2 *>(tp + ((toff + (static_cast<std::uint32_t>(j) << jshift)) << 3)) = v[j];
     }
 }
 
@@ -784,6 +876,7 @@ namespace
 struct Config
 {
     int rb, nt, c;
+    bool use_mfma = true;  // one bond per stage on the matrix cores
     int k() const { return 6 + rb + (nt == 256 ? 2 : nt == 128 ? 1 : 0); }
 };
 
@@ -802,34 +895,32 @@ int ilog2i(int x)
 
 void set_m0(StagePlan &sp, int rb, int wb)
 {
+    sp.mbond = -1;
     for (int r = 0; r < 6; ++r) sp.reg_u[r] = static_cast<int>(r < rb ? 6 + r : 0);
     for (int b = 0; b < 6; ++b) sp.lane_u[b] = static_cast<int>(b);
     for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<int>(b < wb ? 6 + rb + b : 0);
 }
 
-// Choose 6 lane bits among `free` so that their low-nibble contributions
-// under swz() span all 16 banks-pairs where possible.
-void pick_lanes(std::vector<int> &freeb, int lane_u[6])
+// Choose `count` lane bits (lane_u[0..count)) among `free` so that their
+// low-nibble contributions under swz() cover distinct banks where possible
+// (one bit per residue class mod 4 first); the chosen bits leave `free`.
+void pick_lanes(std::vector<int> &freeb, int lane_u[6], int count)
 {
     std::vector<int> chosen;
-    int covered = 0;  // residues mod 4 seen
-    // first pass: one bit per residue class
-    for (int res = 0; res < 4 && chosen.size() < 6; ++res)
+    for (int res = 0; res < 4 && static_cast<int>(chosen.size()) < count; ++res)
         for (std::size_t i = 0; i < freeb.size(); ++i)
             if ((freeb[i] & 3) == res && std::find(chosen.begin(), chosen.end(), freeb[i]) == chosen.end())
             {
                 chosen.push_back(freeb[i]);
-                covered |= 1 << res;
                 break;
             }
-    for (std::size_t i = 0; i < freeb.size() && chosen.size() < 6; ++i)
+    for (std::size_t i = 0; i < freeb.size() && static_cast<int>(chosen.size()) < count; ++i)
         if (std::find(chosen.begin(), chosen.end(), freeb[i]) == chosen.end()) chosen.push_back(freeb[i]);
-    for (int b = 0; b < 6; ++b) lane_u[b] = static_cast<int>(chosen[b]);
+    for (int b = 0; b < count; ++b) lane_u[b] = chosen[b];
     std::vector<int> rest;
     for (int x : freeb)
         if (std::find(chosen.begin(), chosen.end(), x) == chosen.end()) rest.push_back(x);
     freeb.swap(rest);
-    (void)covered;
 }
 
 int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, LayerPlan &out)
@@ -896,46 +987,82 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
         pp.rest_mask = ((std::uint64_t{1} << nq) - 1) & ~local_mask;
         pp.ntiles = 1 << (nq - k);
         auto u_of = [&](int q) { return q < c ? q : q - s + c; };
-        // bonds that already sit on M0 register pairs
+        // M0 stage: bonds already on M0 register pairs, plus (matrix cores)
+        // the bond on u bits (4,5), which M0 puts on lane bits (4,5).  With a
+        // matrix-core bond the VALU pairs must leave the top register bit free.
+        const bool mfma_ok = cf.use_mfma;
         std::vector<int> m0_bonds, other;
+        int m0_m = -1;
         for (int b : assigned[p])
         {
             const int ua = u_of(b0[b]), ub = u_of(b1[b]);
+            if (mfma_ok && m0_m < 0 && ua == 4 && ub == 5)
+            {
+                m0_m = b;
+                continue;
+            }
             const int ra = ua - 6, rc = ub - 6;
             if (ra >= 0 && rc == ra + 1 && (ra % 2) == 0 && rc < cf.rb && static_cast<int>(m0_bonds.size()) < per_stage)
                 m0_bonds.push_back(b);
             else
                 other.push_back(b);
         }
+        if (m0_m >= 0)
+        {
+            // pairs on the top register bit cannot run beside the MFMAs
+            std::vector<int> keep;
+            for (int b : m0_bonds)
+                (u_of(b1[b]) - 6 >= cf.rb - 1 ? other : keep).push_back(b);
+            m0_bonds.swap(keep);
+        }
         std::vector<StagePlan> stages;
-        if (!m0_bonds.empty())
+        if (!m0_bonds.empty() || m0_m >= 0)
         {
             StagePlan sp{};
             set_m0(sp, cf.rb, wb);
             sp.use_lds = 0;
             sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
+            sp.mbond = m0_m;
             for (std::size_t i = 0; i < m0_bonds.size(); ++i)
                 sp.bond[(u_of(b0[m0_bonds[i]]) - 6) / 2] = static_cast<int>(m0_bonds[i]);
-            sp.nb = static_cast<int>(m0_bonds.size());
+            sp.nb = static_cast<int>(m0_bonds.size()) + (m0_m >= 0 ? 1 : 0);
             stages.push_back(sp);
         }
-        for (std::size_t i = 0; i < other.size(); i += per_stage)
+        // LDS stages: up to two register-pair bonds plus one matrix-core bond
+        // (or RB/2 register-pair bonds without matrix cores)
+        std::size_t i = 0;
+        while (i < other.size())
         {
             StagePlan sp{};
             sp.use_lds = 1;
             sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
+            sp.mbond = -1;
             std::vector<int> used;
+            const std::size_t left = other.size() - i;
+            const bool with_m = mfma_ok && left >= 2;
+            const int nvalu = with_m ? static_cast<int>(std::min<std::size_t>(2, left - 1))
+                                     : static_cast<int>(std::min<std::size_t>(per_stage, left));
             int nbs = 0;
-            for (std::size_t j = i; j < other.size() && j < i + per_stage; ++j, ++nbs)
+            for (; nbs < nvalu; ++nbs, ++i)
             {
-                const int b = other[j];
+                const int b = other[i];
                 sp.reg_u[2 * nbs] = static_cast<int>(u_of(b0[b]));
                 sp.reg_u[2 * nbs + 1] = static_cast<int>(u_of(b1[b]));
                 used.push_back(u_of(b0[b]));
                 used.push_back(u_of(b1[b]));
                 sp.bond[nbs] = static_cast<int>(b);
             }
-            sp.nb = static_cast<int>(nbs);
+            int lane45[2] = {-1, -1};
+            if (with_m)
+            {
+                const int b = other[i++];
+                sp.mbond = b;
+                lane45[0] = u_of(b0[b]);
+                lane45[1] = u_of(b1[b]);
+                used.push_back(lane45[0]);
+                used.push_back(lane45[1]);
+            }
+            sp.nb = nbs + (with_m ? 1 : 0);
             std::vector<int> freeb;
             for (int u = 0; u < k; ++u)
                 if (std::find(used.begin(), used.end(), u) == used.end()) freeb.push_back(u);
@@ -944,13 +1071,17 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
             int r = 2 * nbs;
             while (r < cf.rb)
             {
-                // take the largest free bit
                 const int x = freeb.back();
                 freeb.pop_back();
                 sp.reg_u[r++] = static_cast<int>(x);
             }
             for (int rr = cf.rb; rr < 6; ++rr) sp.reg_u[rr] = 0;
-            pick_lanes(freeb, sp.lane_u);
+            pick_lanes(freeb, sp.lane_u, with_m ? 4 : 6);
+            if (with_m)
+            {
+                sp.lane_u[4] = lane45[0];
+                sp.lane_u[5] = lane45[1];
+            }
             for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<int>(b < wb ? freeb[b] : 0);
             stages.push_back(sp);
         }
@@ -1055,10 +1186,17 @@ struct Batch
     Config config_for(int nq) const
     {
         (void)nq;
-        if (cfg.tile_bits == 14) return Config{6, 256, 6};
-        if (cfg.tile_bits == 13) return cfg.threads == 128 ? Config{6, 128, 5} : Config{5, 256, 5};
-        if (cfg.tile_bits == 12) return Config{6, 64, 4};
-        return Config{5, 256, 5};
+        Config cf{5, 256, 5};
+        if (cfg.tile_bits == 14) cf = Config{6, 256, 6};
+        if (cfg.tile_bits == 13) cf = cfg.threads == 128 ? Config{6, 128, 5} : Config{5, 256, 5};
+        if (cfg.tile_bits == 12) cf = Config{6, 64, 4};
+        // MIPT_HIP_MFMA_GATES=0 keeps every bond on the VALU
+        static const bool mfma = [] {
+            const char *e = std::getenv("MIPT_HIP_MFMA_GATES");
+            return !(e && e[0] == '0');
+        }();
+        cf.use_mfma = mfma && cf.rb == 5;  // 64-register tiles leave no room for the MFMA operands
+        return cf;
     }
 
     int setup(const mipt_hip_traj_spec *sp, int count)
@@ -1437,7 +1575,11 @@ struct Batch
                 // MALL-sized groups: two in flight (one per stream) within the
                 // 256 MiB Infinity Cache
                 const std::size_t bytes = (std::size_t{1} << nq) * 8;
-                const std::size_t budget = std::size_t{112} << 20;
+                static const std::size_t budget = [] {
+                    const char *e = std::getenv("MIPT_HIP_GROUP_MIB");  // per-stream group budget
+                    const long v = e ? std::atol(e) : 0;
+                    return static_cast<std::size_t>(v > 0 ? v : 112) << 20;
+                }();
                 int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
                 for (int q = pos; q < end; q += g)
                 {
