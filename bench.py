@@ -27,6 +27,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "circuit layers/s at L=20 x 4096 traj, batched vs plain; S/TMI err vs FP64"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pass_kernel_L20x112_traffic.md")
+
+
+def traffic_ratio():
+    """HBM-side bytes per algorithmic byte of the gate sweep, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_per_algorithmic_byte"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def hip_runtime():
@@ -145,6 +159,9 @@ def main(argv=None):
     value = layer_trajs / (ms_per_step / 1e3)
     st = b.pass_stats()
     achieved = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
+    # all sweeps of a step together: algorithmic bytes of the step over its wall time
+    aggregate = 16.0 * (1 << L) * layer_trajs / (ms_per_step * 1e-3) / 1e9
+    ratio = traffic_ratio() if L == 20 else None
 
     log(f"timed steps done: {ms_per_step:.1f} ms/step, {value:.0f} layers/s; host issue of evolve "
         f"{1e3 * sum(host_issue[-args.steps:]) / args.steps:.1f} ms/step")
@@ -180,9 +197,13 @@ def main(argv=None):
                                f"= {ntraj} trajectories, Haar U(4), complex64 state",
                    "L": L, "layers": layers, "trajectories": ntraj, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if ratio is None else round(ratio * st["bytes_avg"]),
                      "kernel": "pass_kernel (gate sweep)", "timed_launches": st["timed"],
-                     "avg_launch_ms": round(st["avg_ms"], 4), "algorithmic_bytes_per_launch": st["bytes_avg"]},
+                     "avg_launch_ms": round(st["avg_ms"], 4), "algorithmic_bytes_per_launch": round(st["bytes_avg"]),
+                     "streams_in_flight": 2,
+                     "aggregate_sweep_GBps": round(aggregate, 1),
+                     "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4)},
         "plain_layers_per_s": None if plain_lps is None else round(plain_lps, 1),
         "batched_vs_plain": None if plain_lps is None else round(value / plain_lps, 2),
     }

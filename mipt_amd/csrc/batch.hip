@@ -20,8 +20,10 @@
 //     in the last sweep (per-tile partials, fixed order), a finalize kernel
 //     turns them into sequential outcomes, and the collapse with FP64
 //     renormalisation is folded into the next layer's first sweep;
-//   * trajectories are processed in groups whose states fit the 256 MiB
-//     Infinity Cache so the second sweep of a layer re-reads from the MALL.
+//   * trajectories are processed in groups of about 896 MiB of state, two
+//     groups in flight on two streams; measured on MI355X at L=20, groups from
+//     448 MiB to 1.8 GiB run within 1% of each other, 112 MiB groups (which
+//     fit the 256 MiB Infinity Cache) 2% slower, as launches get 8x more.
 #include "batch.hpp"
 #include "state_ops.hpp"
 
@@ -289,6 +291,9 @@ struct PassPlan
     int store;         // write the tile back
     int chunk;         // histogram chunk index (sites chunk*MAX_HIST_BITS ...)
     int cond;          // restrict the histogram to the decided outcomes of the layer
+    int zskip;         // leave tiles on the collapsed-out branch untouched (the next sweep zeroes them)
+    std::uint64_t bonded;  // sites whose bonds this sweep applies
+    std::uint64_t pmask;   // previous layer's collapsed sites to zero again at load (no rescale)
 };
 
 struct PassArgs
@@ -661,7 +666,16 @@ __global__ void __launch_bounds__(NT, (RB == 5 ? 2 : 1)) pass_kernel(PassArgs ar
         cv = cc.val;
         scale = static_cast<float>(cc.scale);
     }
+    else if (pl.pmask && t > 0)
+    {
+        // the layer's first sweep skipped tiles of the zero branch; their
+        // sites are untouched since, so the branch is still zero by definition
+        const Collapse cc = a.coll[d.layer_off + t - 1];
+        cm = cc.mask & pl.pmask;
+        cv = cc.val & cm;
+    }
     const bool tile_zero = (B & cm) != (cv & cm & pl.rest_mask);
+    if (pl.zskip && tile_zero) return;  // uniform over the workgroup, before any barrier
     const std::uint32_t cml = static_cast<std::uint32_t>(cm & ~pl.rest_mask);
     const std::uint32_t cvl = static_cast<std::uint32_t>(cv & ~pl.rest_mask);
     const bool gen_init = pl.init && a.init[traj];
@@ -685,14 +699,20 @@ __global__ void __launch_bounds__(NT, (RB == 5 ? 2 : 1)) pass_kernel(PassArgs ar
     }
     else
     {
+        // the zero branch is not read: its lanes take a buffer offset past the
+        // resource's range, which the hardware answers with 0 and no fetch
+        // (states up to 2^27 amplitudes, so in-range offsets stay below 2^31)
+        const std::uint32_t range = static_cast<std::uint32_t>(((std::uint64_t{1} << d.nq) - B) << 3);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(tp, static_cast<short>(0), static_cast<int>(range), 0x00020000);
 #pragma unroll
         for (int j = 0; j < NV; ++j)
         {
             const std::uint32_t o = toff + (static_cast<std::uint32_t>(j) << jshift);
-            float2 x = *reinterpret_cast<const float2 *>(tp + (o << 3));
-            const float f = ((o & cml) == cvl) ? scale : 0.f;
-            x.x *= f;
-            x.y *= f;
+            const std::uint32_t off = ((o & cml) == cvl) ? (o << 3) : 0x80000000u;
+            float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+            x.x *= scale;
+            x.y *= scale;
             v[j] = x;
         }
     }
@@ -986,6 +1006,8 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
         local_mask |= ((std::uint64_t{1} << wbits) - 1) << s;
         pp.rest_mask = ((std::uint64_t{1} << nq) - 1) & ~local_mask;
         pp.ntiles = 1 << (nq - k);
+        pp.bonded = 0;
+        for (int b : assigned[p]) pp.bonded |= (std::uint64_t{1} << b0[b]) | (std::uint64_t{1} << b1[b]);
         auto u_of = [&](int q) { return q < c ? q : q - s + c; };
         // M0 stage: bonds already on M0 register pairs, plus (matrix cores)
         // the bond on u bits (4,5), which M0 puts on lane bits (4,5).  With a
@@ -1214,8 +1236,9 @@ struct Batch
         for (int i = 0; i < count; ++i)
         {
             const auto &s = specs[i];
-            if (s.L < 2 || s.L > 30 || (s.L % 2) != 0 || s.layers < 0 || s.gate_family != 1 ||
-                (s.probe_site >= s.L) || s.L + (s.probe_site >= 0 ? 1 : 0) > 30 ||
+            // up to 2^27 amplitudes per state (1 GiB): tile byte offsets stay below 2^31
+            if (s.L < 2 || s.L > 26 || (s.L % 2) != 0 || s.layers < 0 || s.gate_family != 1 ||
+                (s.probe_site >= s.L) ||
                 (s.shared_gates >= 0 && (s.shared_gates >= count || s.shared_gates == i)))
             {
                 set_error("batch_setup: unsupported spec %d (L=%d layers=%d family=%d)", i, s.L, s.layers, s.gate_family);
@@ -1475,6 +1498,10 @@ struct Batch
                 pl.chunk = 0;
                 pl.cond = 0;
                 pl.store = 1;
+                // the collapse sweep skips zero-branch tiles when a later sweep
+                // of the layer re-applies the branch on the sites it leaves alone
+                pl.zskip = (p == 0 && P.passes.size() > 1) ? 1 : 0;
+                pl.pmask = (p == 1) ? (((std::uint64_t{1} << d0.nq) - 1) & ~P.passes[0].bonded) : 0;
                 // algorithmic bytes: one read + one write of the states per
                 // layer, shared among the layer's sweeps
                 const double bytes = 16.0 * static_cast<double>(std::size_t{1} << d0.nq) * ntraj /
@@ -1572,13 +1599,12 @@ struct Batch
             }
             else
             {
-                // MALL-sized groups: two in flight (one per stream) within the
-                // 256 MiB Infinity Cache
+                // groups of trajectories, two in flight (one per stream)
                 const std::size_t bytes = (std::size_t{1} << nq) * 8;
                 static const std::size_t budget = [] {
                     const char *e = std::getenv("MIPT_HIP_GROUP_MIB");  // per-stream group budget
                     const long v = e ? std::atol(e) : 0;
-                    return static_cast<std::size_t>(v > 0 ? v : 112) << 20;
+                    return static_cast<std::size_t>(v > 0 ? v : 896) << 20;
                 }();
                 int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
                 for (int q = pos; q < end; q += g)
