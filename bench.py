@@ -101,6 +101,9 @@ def main(argv=None):
     ap.add_argument("--profile-every", type=int, default=7, help="odd, so both sweeps of a layer are sampled")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB plain copy (traffic-counter calibration)")
+    ap.add_argument("--rdm-L", type=int, default=0,
+                    help="also time the half-chain RDM entropy at this L (Bell-pair state, S = L/2 exactly)")
+    ap.add_argument("--rdm-reps", type=int, default=3)
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -245,8 +248,38 @@ def main(argv=None):
         result["batched_vs_cpu"] = round(value / cpu_lps, 1)
     else:
         result["cpu_baseline"] = None
+    if args.rdm_L:
+        result["rdm_gemm"] = rdm_timing(M, args.rdm_L, args.rdm_reps)
     print(json.dumps(result))
     return 0
+
+
+def rdm_timing(M, n, reps):
+    """Half-chain entropy of a Bell-pair state (pairs (i, i + n/2)), whose RDM
+    is 2^(n/2) square with k = 2^(n/2): gather + Gram (8 m^2 k FLOPs) + FP64
+    eigensolve.  The Gram kernel's own duration comes from rocprofv3."""
+    h = n // 2
+    idx = np.zeros(1, np.int64)
+    for i in range(h):
+        idx = np.concatenate([idx, idx | (1 << i) | (1 << (i + h))])
+    psi = np.zeros(1 << n, np.complex64)
+    psi[idx] = 1.0 / math.sqrt(len(idx))
+    eng = M.PlainEngine(False)
+    eng.reset(n)
+    eng.set_from_host(psi)
+    del psi
+    kept = list(range(h))
+    s = M.entropy(eng.data, n, kept)  # warm: workspaces, solver handle
+    times = []
+    for r in range(reps):
+        t = time.perf_counter()
+        s = M.entropy(eng.data, n, kept)
+        times.append(time.perf_counter() - t)
+        log(f"rdm L={n} rep {r + 1}/{reps}: {times[-1] * 1e3:.1f} ms, S={s:.9f}")
+    eng.close()
+    m = 1 << h
+    return {"L": n, "m": m, "k": m, "gram_flop": 8.0 * m * m * m, "entropy_call_ms": round(1e3 * min(times), 2),
+            "s_half": s, "s_exact": float(h), "s_err": abs(s - h)}
 
 
 if __name__ == "__main__":

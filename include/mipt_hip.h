@@ -178,6 +178,11 @@ int mipt_hip_batch_outcomes(mipt_hip_batch *batch, int traj, int *outcomes, uint
  * the trajectory has no probe). */
 int mipt_hip_batch_observables(mipt_hip_batch *batch, int traj, int tmi_block, double *s_half, double *tmi,
                                double *probe_entropy, double *bloch3);
+/* TMI (bits) of cyclic blocks of `block` sites (0: L/4) starting at site
+ * `cycle`, from von Neumann (order 1) or Renyi-2 (order 2) entropies; the
+ * all_cycles scan of cpp/include/mipt/tmi/compute.hpp:115-118 calls it for
+ * cycle = 0..L/4-1. */
+int mipt_hip_batch_tmi(mipt_hip_batch *batch, int traj, int block, int cycle, int order, double *tmi);
 /* The stream the batch launches on. */
 void *mipt_hip_batch_stream(mipt_hip_batch *batch);
 /* Re-arms the in-kernel initial states and clears per-run records, so the
@@ -195,6 +200,32 @@ int mipt_hip_batch_set_profiling(mipt_hip_batch *batch, int sample_every);
  * sweeps), all sweeps launched and their total algorithmic bytes. */
 int mipt_hip_batch_pass_stats(mipt_hip_batch *batch, double *avg_ms, int *timed, double *bytes_avg,
                               long *launches, double *bytes_total);
+
+/* ------------------------------------------------------------ whole runs */
+typedef struct mipt_hip_traj_result
+{
+    double s_half;        /* S(L/2), bits */
+    double probe_entropy; /* probe entropy, nats; NaN without a probe */
+    double bloch[3];      /* probe Bloch vector; NaN without a probe */
+    double norm_error;    /* | <psi|psi> - 1 | of the final complex64 state */
+    int measured;         /* measured draws of the trajectory */
+    int flagged_draws;    /* draws within 1e-6 of their Born threshold */
+} mipt_hip_traj_result;
+
+/* One call per batch of trajectories (the batched entry point a scan driver
+ * needs in place of cpp/include/mipt/circuit.hpp:409-443 + tmi/compute.hpp:
+ * 704-839 per trajectory): device draws, evolution, then per trajectory S(L/2),
+ * TMI for cycles 0..tmi_cycles-1 (block 0 = L/4; entropy_order 1 = von
+ * Neumann, 2 = Renyi-2; tmi_out[traj * tmi_cycles + cycle]), probe readout and
+ * draw statistics into caller-owned host arrays. */
+int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block, int tmi_cycles, int entropy_order,
+                       double *tmi_out, mipt_hip_traj_result *results, void *hip_stream);
+
+/* The online TMI scan of cpp/apps/sim_tmi.cpp:11-90 and
+ * cpp/include/mipt/tmi/runner.hpp:88-409 on this engine: the same positional
+ * arguments, CSV `p,tmi` and default output path; returns the exit code
+ * (0, 1 on an error, 2 on too many arguments). */
+int mipt_hip_sim_tmi_main(int argc, char **argv);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,9 @@ def lib():
     L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
+    L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
+    L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
+    L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     _LIB = L
     return L
 
@@ -437,6 +440,38 @@ class Batch:
                "batch_observables")
         return {"s_half": sh.value, "tmi": tm.value, "probe_entropy": pe.value, "bloch": bl}
 
+    def tmi(self, traj: int, cycle: int = 0, block: int = 0, order: int = 1) -> float:
+        out = ctypes.c_double()
+        _check(lib().mipt_hip_batch_tmi(self._h, traj, block, cycle, order, ctypes.byref(out)), "batch_tmi")
+        return out.value
+
+
+class TrajResult(ctypes.Structure):
+    _fields_ = [("s_half", ctypes.c_double), ("probe_entropy", ctypes.c_double), ("bloch", ctypes.c_double * 3),
+                ("norm_error", ctypes.c_double), ("measured", ctypes.c_int), ("flagged_draws", ctypes.c_int)]
+
+
+def run_batch(specs, tmi_cycles: int = 1, tmi_block: int = 0, entropy_order: int = 1):
+    """Draw, evolve and read out `specs` in one library call (mipt_hip_run_batch).
+    Returns (list of per-trajectory dicts, tmi array [count, tmi_cycles])."""
+    specs = list(specs)
+    arr = (TrajSpec * len(specs))(*[s.to_c() for s in specs])
+    res = (TrajResult * len(specs))()
+    tmi = np.zeros((len(specs), max(tmi_cycles, 1)))
+    _check(lib().mipt_hip_run_batch(ctypes.cast(arr, ctypes.c_void_p), len(specs), tmi_block, tmi_cycles,
+                                    entropy_order, _dp(tmi), ctypes.cast(res, ctypes.c_void_p), None), "run_batch")
+    out = [{"s_half": r.s_half, "probe_entropy": r.probe_entropy, "bloch": np.array(r.bloch[:]),
+            "norm_error": r.norm_error, "measured": r.measured, "flagged_draws": r.flagged_draws} for r in res]
+    return out, tmi[:, :tmi_cycles]
+
+
+def sim_tmi_main(argv) -> int:
+    """The sim_tmi scan (same positional arguments as the reference's
+    sim_tmi.exe), run in this process; returns its exit code."""
+    args = [b"sim_tmi"] + [str(a).encode() for a in argv]
+    arr = (ctypes.c_char_p * len(args))(*args)
+    return lib().mipt_hip_sim_tmi_main(len(args), arr)
+
 
 __all__ = ["lib", "last_error", "MiptHipError", "measure_collapse", "entropy", "rdm_subsystem", "rho1", "tmi",
-           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch"]
+           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch", "run_batch", "sim_tmi_main"]

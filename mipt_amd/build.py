@@ -9,7 +9,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmipt_hip.so")
-SOURCES = ["plain.hip", "batch.hip", "entropy.hip"]
+SOURCES = ["plain.hip", "batch.hip", "entropy.hip", "driver.cpp"]
+BIN = os.path.join(HERE, "bin")
+TOOLS = {"sim_tmi": os.path.join(HERE, "host", "sim_tmi_main.cpp")}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -26,7 +28,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(build_dir, exist_ok=True)
     procs = []
     for src in SOURCES:
-        obj = os.path.join(build_dir, src.replace(".hip", ".o"))
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
@@ -44,6 +46,11 @@ def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", *objs, *LIBS, "-o", tmp], check=True)
     os.replace(tmp, OUT)
+    # host executables over the C ABI (plain C++, found next to the library)
+    os.makedirs(BIN, exist_ok=True)
+    for name, src in TOOLS.items():
+        subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE,
+                        "-lmipt_hip", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BIN, name)], check=True)
     return OUT
 
 

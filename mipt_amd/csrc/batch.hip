@@ -1916,4 +1916,16 @@ int mipt_hip_batch_observables(mipt_hip_batch *b, int traj, int tmi_block, doubl
     return OK;
 }
 
+int mipt_hip_batch_tmi(mipt_hip_batch *b, int traj, int block, int cycle, int order, double *tmi)
+{
+    if (!b || traj < 0 || traj >= static_cast<int>(b->desc.size()) || !tmi || (order != 1 && order != 2))
+    {
+        set_error("batch_tmi: bad arguments");
+        return E_ARG;
+    }
+    const TrajDesc &d = b->desc[traj];
+    const int blk = block > 0 ? block : d.L / 4;
+    return tmi_bits_f32(b->a.states + d.state_off, d.nq, d.L, blk, cycle, b->stream, tmi, nullptr, order);
+}
+
 } // extern "C"

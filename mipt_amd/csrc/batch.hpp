@@ -57,7 +57,9 @@ struct DevArena
 // Entropy/observable helpers implemented in entropy.hip, used by the batch.
 int entropy_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
 int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream, double *out,
-                 double *terms);
+                 double *terms, int order = 1);
+// Renyi-2 entropy -log2 Tr(rho^2) of the kept set, bits
+int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
 int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4);
 
 } // namespace mipt_hip

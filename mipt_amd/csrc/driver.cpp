@@ -1,0 +1,353 @@
+// Whole runs on the batched engine: mipt_hip_run_batch (draws, evolution and
+// per-trajectory observables in one call) and the sim_tmi scan driver.
+//
+// The scan keeps the contract of the reference's online TMI simulator
+// (cpp/apps/sim_tmi.cpp:11-90, cpp/include/mipt/tmi/runner.hpp:52-409,
+// 429-491, 556-590): positional arguments
+//   n periods realizations resolution p_min p_max circ_type [all_cycles] [output.csv]
+// (output.csv may also come before all_cycles), a linspace p grid
+// (cpp/include/mipt/types.hpp:23-38), closed brickwork rings of 2*periods
+// layers, one TMI row per (p, realization, cycle) in p-major order, CSV
+// header `p,tmi`, numbers in 17 significant digits
+// (cpp/include/mipt/tmi/linalg.hpp:223-234), the default path
+// csv/tmi/<tag>/tmi_<tag>_n_<n>_real_<R>_<p_min>_<p_max>_res_<res>.csv, and
+// SIM_TMI_ENTROPY_ORDER=2 for Renyi-2 TMI (tag suffix _s2).  What differs:
+// every trajectory of the scan runs in one batched launch sequence instead of
+// serially; only circ_type 1 (Haar U(4)) is built, and it is the default; the
+// seeds are trajectory_seed(MIPT_SEED, index) when MIPT_SEED is set (random
+// otherwise, as the reference's random_device); an existing output is
+// overwritten (no resume).
+#include "mipt_hip.h"
+#include "common.hpp"
+
+#include <cerrno>
+#include <charconv>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <chrono>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <sys/stat.h>
+#include <vector>
+
+namespace
+{
+
+struct HipError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+void check(int status, const char *what)
+{
+    if (status != 0)
+    {
+        const char *detail = mipt_hip_last_error();
+        throw HipError(std::string(what) + " failed (status " + std::to_string(status) +
+                       "): " + ((detail && *detail) ? detail : "no detail"));
+    }
+}
+
+// cpp/include/mipt/types.hpp:262-273
+std::uint64_t splitmix64(std::uint64_t v)
+{
+    v += 0x9e3779b97f4a7c15ull;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+    return v ^ (v >> 31);
+}
+
+std::uint64_t trajectory_seed(std::uint64_t master, std::uint64_t index)
+{
+    return splitmix64(master + 0x2545f4914f6cdd1dull * (index + 1ull));
+}
+
+// cpp/include/mipt/types.hpp:23-38: the last point is p_max exactly
+std::vector<double> linspace(double first, double last, int count)
+{
+    std::vector<double> v;
+    if (count == 1)
+    {
+        v.push_back(first);
+        return v;
+    }
+    const double step = (last - first) / (count - 1);
+    for (int i = 0; i < count; ++i) v.push_back(first + step * i);
+    v.back() = last;
+    return v;
+}
+
+// 17 significant digits, shortest "general" form
+void append_double(std::string &s, double value)
+{
+    char buf[64];
+    const auto r = std::to_chars(buf, buf + sizeof(buf), value, std::chars_format::general, 17);
+    if (r.ec == std::errc{}) s.append(buf, r.ptr);
+    else s += std::to_string(value);
+}
+
+// fixed 12 decimals with trailing zeros (and a bare point) removed
+std::string number_for_filename(double value)
+{
+    if (std::abs(value) < 1.0e-15) value = 0.0;
+    std::ostringstream out;
+    out << std::fixed << std::setprecision(12) << value;
+    std::string text = out.str();
+    if (text.find('.') != std::string::npos)
+    {
+        while (!text.empty() && text.back() == '0') text.pop_back();
+        if (!text.empty() && text.back() == '.') text.pop_back();
+    }
+    if (text == "-0") text = "0";
+    return text;
+}
+
+bool is_bool01(std::string_view s) { return s == "0" || s == "1"; }
+
+bool parse_bool01(std::string_view s, const char *name)
+{
+    if (!is_bool01(s)) throw std::invalid_argument(std::string(name) + " must be 0 or 1.");
+    return s == "1";
+}
+
+void mkdirs_for(const std::string &path)
+{
+    for (std::size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/')
+        {
+            const std::string dir = path.substr(0, i);
+            if (::mkdir(dir.c_str(), 0755) != 0 && errno != EEXIST)
+                throw std::runtime_error("Could not create directory: " + dir);
+        }
+}
+
+void usage(const char *argv0)
+{
+    std::cerr << "Usage:\n  " << argv0
+              << " [n = 10] [periods = 10] [realizations = 10] [resolution = 5] [p_min = 0.0] [p_max = 1.0]"
+                 " [circ_type = 1] [all_cycles = 0] [output.csv = auto]\n\n"
+              << "Arguments:\n  circ_type:\n    1 = random Haar U(4) brickwork circuit (the only one built here)\n"
+              << "  all_cycles = 0 computes the original offset only.\n"
+              << "  all_cycles = 1 computes N/4 cyclic offsets.\n"
+              << "  For backward compatibility, output.csv may also be supplied before all_cycles.\n"
+              << "  If output.csv is omitted, the default path is:\n"
+              << "    csv/tmi/<tag>/tmi_<tag>_n_<n>_real_<realizations>_<p_min>_<p_max>_res_<resolution>.csv\n"
+              << "  The tag is haar; all_cycles=1 appends c, SIM_TMI_ENTROPY_ORDER=2 appends _s2.\n\n"
+              << "Output CSV columns:\n  p,tmi\n\n"
+              << "Environment:\n  SIM_TMI_ENTROPY_ORDER=1|2  von Neumann (default) or Renyi-2 TMI\n"
+              << "  MIPT_SEED=<u64>            reproducible trajectory seeds\n"
+              << "  MIPT_HIP_STATE_GIB=<g>     device memory for states per batch (default 64)\n";
+}
+
+int entropy_order_from_env()
+{
+    const char *e = std::getenv("SIM_TMI_ENTROPY_ORDER");
+    if (!e || !*e || std::string_view(e) == "1") return 1;
+    if (std::string_view(e) == "2") return 2;
+    throw std::invalid_argument("SIM_TMI_ENTROPY_ORDER must be 1 or 2.");
+}
+
+int run_sim_tmi(int argc, char **argv)
+{
+    if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0))
+    {
+        usage(argv[0]);
+        return 0;
+    }
+    if (argc > 10)
+    {
+        std::cerr << "Too many arguments. Use --help for usage.\n";
+        return 2;
+    }
+    const int n = argc > 1 ? std::stoi(argv[1]) : 10;
+    const int periods = argc > 2 ? std::stoi(argv[2]) : 10;
+    const int realizations = argc > 3 ? std::stoi(argv[3]) : 10;
+    const int res = argc > 4 ? std::stoi(argv[4]) : 5;
+    const double p_min = argc > 5 ? std::stod(argv[5]) : 0.0;
+    const double p_max = argc > 6 ? std::stod(argv[6]) : 1.0;
+    const int circ = argc > 7 ? std::stoi(argv[7]) : 1;
+    bool all_cycles = false;
+    std::string output;
+    if (argc > 8)
+    {
+        if (is_bool01(argv[8]))
+        {
+            all_cycles = parse_bool01(argv[8], "all_cycles");
+            if (argc > 9) output = argv[9];
+        }
+        else
+        {
+            output = argv[8];
+            if (argc > 9) all_cycles = parse_bool01(argv[9], "all_cycles");
+        }
+    }
+    // cpp/include/mipt/tmi/runner.hpp:52-76
+    if (n < 4 || (n % 4) != 0) throw std::invalid_argument("Online TMI requires n >= 4 and n divisible by 4.");
+    if (periods <= 0) throw std::invalid_argument("periods must be positive.");
+    if (realizations <= 0) throw std::invalid_argument("realizations must be positive.");
+    if (res <= 0) throw std::invalid_argument("resolution must be positive.");
+    if (!(0.0 <= p_min && p_min <= 1.0 && 0.0 <= p_max && p_max <= 1.0))
+        throw std::invalid_argument("p_min and p_max must lie in [0, 1].");
+    if (circ != 1)
+        throw std::invalid_argument("circ_type " + std::to_string(circ) +
+                                    " is not built on this engine; use 1 (random Haar U(4)).");
+    if (n > 26) throw std::invalid_argument("this engine holds states up to n = 26.");
+    const int order = entropy_order_from_env();
+    std::string tag = "haar";
+    if (all_cycles) tag += 'c';
+    if (order == 2) tag += "_s2";
+    if (output.empty())
+        output = "csv/tmi/" + tag + "/tmi_" + tag + "_n_" + std::to_string(n) + "_real_" +
+                 std::to_string(realizations) + "_" + number_for_filename(p_min) + "_" +
+                 number_for_filename(p_max) + "_res_" + std::to_string(res) + ".csv";
+
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
+    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+
+    const int cycles = all_cycles ? n / 4 : 1;
+    const std::vector<double> ps = linspace(p_min, p_max, res);
+    const std::uint64_t total_traj = static_cast<std::uint64_t>(res) * realizations;
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const double state_bytes = std::ldexp(8.0, n);
+    const std::uint64_t per_batch =
+        std::max<std::uint64_t>(1, static_cast<std::uint64_t>(budget_gib * 1073741824.0 / state_bytes));
+
+    mkdirs_for(output);
+    std::ofstream csv(output, std::ios::binary | std::ios::trunc);
+    if (!csv) throw std::runtime_error("Could not create output CSV: " + output);
+    csv << "p,tmi\n";
+    std::cerr << "Online TMI simulation (batched HIP engine)\n"
+              << "n=" << n << ", periods=" << periods << ", realizations=" << realizations << ", resolution=" << res
+              << ", p_min=" << p_min << ", p_max=" << p_max << ", block_qubits=" << n / 4
+              << ", all_cycles=" << (all_cycles ? 1 : 0) << ", tmi_cycles_per_circuit=" << cycles
+              << ", mode=Random Haar Unitary, entropy_order=" << order << ", output=" << output
+              << ", master_seed=" << master << ", trajectories_per_batch=" << per_batch << '\n';
+
+    const auto t0 = std::chrono::steady_clock::now();
+    std::string buf;
+    std::uint64_t done = 0;
+    for (std::uint64_t first = 0; first < total_traj; first += per_batch)
+    {
+        const int count = static_cast<int>(std::min<std::uint64_t>(per_batch, total_traj - first));
+        std::vector<mipt_hip_traj_spec> specs(count);
+        for (int i = 0; i < count; ++i)
+        {
+            const std::uint64_t idx = first + i;
+            mipt_hip_traj_spec &s = specs[i];
+            s.L = n;
+            s.layers = 2 * periods;
+            s.p = ps[idx / realizations];
+            s.seed = trajectory_seed(master, idx);
+            s.closed = 1;
+            s.gate_family = 1;
+            s.probe_site = -1;
+            s.p_zero_layers = 0;
+            s.shared_gates = -1;
+        }
+        std::vector<double> tmi(static_cast<std::size_t>(count) * cycles);
+        std::vector<mipt_hip_traj_result> r(count);
+        check(mipt_hip_run_batch(specs.data(), count, 0, cycles, order, tmi.data(), r.data(), nullptr),
+              "run_batch");
+        buf.clear();
+        for (int i = 0; i < count; ++i)
+            for (int c = 0; c < cycles; ++c)
+            {
+                append_double(buf, specs[i].p);
+                buf.push_back(',');
+                append_double(buf, tmi[static_cast<std::size_t>(i) * cycles + c]);
+                buf.push_back('\n');
+            }
+        csv.write(buf.data(), static_cast<std::streamsize>(buf.size()));
+        csv.flush();
+        if (!csv) throw std::runtime_error("Failed while writing output CSV.");
+        done += count;
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cerr << "processed TMI " << done * cycles << '/' << total_traj * cycles << " | avg " << std::fixed
+                  << std::setprecision(2) << (done * cycles) / std::max(1e-9, el) << "/s" << std::defaultfloat
+                  << '\n';
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block, int tmi_cycles, int entropy_order,
+                       double *tmi_out, mipt_hip_traj_result *results, void *hip_stream)
+{
+    if (!specs || count <= 0 || !results || tmi_cycles < 0 || (tmi_cycles > 0 && !tmi_out) ||
+        (entropy_order != 1 && entropy_order != 2))
+    {
+        mipt_hip::set_error("run_batch: bad arguments");
+        return mipt_hip::E_ARG;
+    }
+    mipt_hip_batch *b = nullptr;
+    mipt_hip_engine *eng = nullptr;
+    int st = 0;
+    const mipt_hip_batch_config cfg{0, 0, 0};
+    if ((st = mipt_hip_batch_create(&cfg, hip_stream, &b)) != 0) return st;
+    do
+    {
+        if ((st = mipt_hip_batch_setup(b, specs, count)) != 0) break;
+        if ((st = mipt_hip_batch_generate(b)) != 0) break;
+        if ((st = mipt_hip_batch_evolve(b)) != 0) break;
+        if ((st = mipt_hip_batch_synchronize(b)) != 0) break;
+        if ((st = mipt_hip_engine_create(0, mipt_hip_batch_stream(b), &eng)) != 0) break;
+        for (int i = 0; i < count && st == 0; ++i)
+        {
+            const mipt_hip_traj_spec &s = specs[i];
+            mipt_hip_traj_result &r = results[i];
+            const std::size_t draws = static_cast<std::size_t>(s.layers) * s.L;
+            std::vector<int> out(draws);
+            std::vector<std::uint8_t> flg(draws);
+            if ((st = mipt_hip_batch_outcomes(b, i, out.data(), flg.data(), nullptr, nullptr)) != 0) break;
+            r.measured = 0;
+            r.flagged_draws = 0;
+            for (std::size_t k = 0; k < draws; ++k)
+            {
+                r.measured += out[k] >= 0;
+                r.flagged_draws += flg[k] != 0;
+            }
+            if ((st = mipt_hip_batch_observables(b, i, tmi_block, &r.s_half, nullptr, &r.probe_entropy, r.bloch)) != 0)
+                break;
+            for (int c = 0; c < tmi_cycles && st == 0; ++c)
+                st = mipt_hip_batch_tmi(b, i, tmi_block, c, entropy_order, &tmi_out[static_cast<std::size_t>(i) * tmi_cycles + c]);
+            if (st != 0) break;
+            const int nq = s.L + (s.probe_site >= 0 ? 1 : 0);
+            double sq = 0.0;
+            if ((st = mipt_hip_engine_adopt(eng, mipt_hip_batch_state_ptr(b, i), nq)) != 0) break;
+            if ((st = mipt_hip_engine_squared_norm(eng, &sq)) != 0) break;
+            r.norm_error = std::abs(sq - 1.0);
+        }
+    } while (false);
+    if (eng) (void)mipt_hip_engine_destroy(eng);
+    (void)mipt_hip_batch_destroy(b);
+    return st;
+}
+
+int mipt_hip_sim_tmi_main(int argc, char **argv)
+{
+    try
+    {
+        return run_sim_tmi(argc, argv);
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "\nerror: " << e.what() << '\n';
+        return 1;
+    }
+}
+
+} // extern "C"

@@ -570,7 +570,7 @@ int rho1_impl(const void *psi, int n, int q, hipStream_t stream, double *out4)
 }
 
 int tmi_impl(const void *psi, bool f64, int n, int L, int block, int cycle, hipStream_t stream, double *out,
-             double *terms)
+             double *terms, int order = 1)
 {
     if (!psi || !out || L < 4 || L > n || block < 1 || 3 * block > L || cycle < 0 || cycle >= L)
     {
@@ -591,7 +591,7 @@ int tmi_impl(const void *psi, bool f64, int n, int L, int block, int cycle, hipS
     const std::vector<int> sets[7] = {A, B, Cc, D, cat(A, B), cat(A, Cc), cat(B, Cc)};
     double s[7];
     for (int i = 0; i < 7; ++i)
-        MIPT_HIP_TRY(entropy_impl(psi, f64, n, sets[i].data(), static_cast<int>(sets[i].size()), stream, 1, &s[i]));
+        MIPT_HIP_TRY(entropy_impl(psi, f64, n, sets[i].data(), static_cast<int>(sets[i].size()), stream, order, &s[i]));
     *out = s[0] + s[1] + s[2] + s[3] - s[4] - s[5] - s[6];
     if (terms)
         for (int i = 0; i < 7; ++i) terms[i] = s[i];
@@ -625,9 +625,14 @@ int entropy_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, 
 }
 
 int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream, double *out,
-                 double *terms)
+                 double *terms, int order)
 {
-    return tmi_impl(psi, false, n, chain_sites, block, cycle, stream, out, terms);
+    return tmi_impl(psi, false, n, chain_sites, block, cycle, stream, out, terms, order);
+}
+
+int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out)
+{
+    return entropy_impl(psi, false, n, kept, kept_count, stream, 2, out);
 }
 
 int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4)

@@ -1,0 +1,112 @@
+"""The batched run entry point and the sim_tmi scan driver.
+
+CPU tests: the argument contract of sim_tmi (cpp/apps/sim_tmi.cpp:11-90,
+cpp/include/mipt/tmi/runner.hpp:52-76), which fails before any GPU call.
+GPU tests: mipt_hip_run_batch and the scan's CSV against the FP64 oracle on
+the same device-drawn streams (S, TMI within 1e-5 bits; probe within 1e-5).
+"""
+import csv
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mipt_amd
+    mipt_amd.lib()
+    return mipt_amd
+
+
+@pytest.mark.parametrize("argv,code", [
+    (["--help"], 0),
+    (["8"] * 10, 2),                                   # too many arguments
+    (["10"], 1),                                       # n % 4 != 0
+    (["8", "0"], 1),                                   # periods <= 0
+    (["8", "2", "0"], 1),                              # realizations <= 0
+    (["8", "2", "2", "0"], 1),                         # resolution <= 0
+    (["8", "2", "2", "2", "0", "1.5"], 1),             # p outside [0, 1]
+    (["8", "2", "2", "2", "0", "1", "0"], 1),          # MMS is not built here
+    (["8", "2", "2", "2", "0", "1", "1", "x.csv", "7"], 1),  # all_cycles not 0/1
+    (["28"], 1),                                       # beyond 2^26 amplitudes
+])
+def test_sim_tmi_argument_contract(M, argv, code, capfd):
+    assert M.sim_tmi_main(argv) == code
+    err = capfd.readouterr().err
+    if code == 1:
+        assert "error:" in err
+    if code == 2:
+        assert "Too many arguments" in err
+
+
+def test_sim_tmi_entropy_order_env_is_checked(M, monkeypatch, capfd):
+    monkeypatch.setenv("SIM_TMI_ENTROPY_ORDER", "3")
+    assert M.sim_tmi_main(["8", "2", "1", "1"]) == 1
+    assert "SIM_TMI_ENTROPY_ORDER" in capfd.readouterr().err
+
+
+def _oracle_run(n, layers, p, seed):
+    psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed)
+    return psi, out, p1, f, u
+
+
+@pytest.mark.gpu
+def test_run_batch_matches_oracle(M):
+    specs = [M.Spec(8, 8, 0.2, 501), M.Spec(12, 10, 0.1, 502), M.Spec(16, 6, 0.25, 503), M.Spec(8, 8, 0.0, 504)]
+    print("run_batch L=8,12,16", flush=True)
+    res, tmi = M.run_batch(specs, tmi_cycles=2, entropy_order=1)
+    for i, s in enumerate(specs):
+        psi, out, p1, f, u = _oracle_run(s.L, s.layers, s.p, s.seed)
+        assert res[i]["flagged_draws"] == 0 or np.any(np.abs(u - p1)[f == 1] < 1e-6)
+        assert res[i]["measured"] == int(f.sum())
+        assert abs(res[i]["s_half"] - O.half_chain_entropy(psi, s.L)) < 1e-5
+        for c in range(2):
+            assert abs(tmi[i, c] - O.tmi(psi, s.L, s.L // 4, c)) < 1e-5
+        assert res[i]["norm_error"] < 1e-5
+        assert math.isnan(res[i]["probe_entropy"])
+    # Renyi-2 TMI
+    _, tmi2 = M.run_batch(specs[:1], tmi_cycles=1, entropy_order=2)
+    psi, *_ = _oracle_run(8, 8, 0.2, 501)
+    assert abs(tmi2[0, 0] - O.tmi(psi, 8, 2, 0, order=2)) < 1e-5
+
+
+@pytest.mark.gpu
+def test_sim_tmi_csv_matches_oracle(M, tmp_path, monkeypatch):
+    n, periods, R, res, pmin, pmax = 8, 3, 2, 2, 0.1, 0.3
+    master = 12345
+    monkeypatch.setenv("MIPT_SEED", str(master))
+    path = tmp_path / "scan.csv"
+    print("sim_tmi n=8 scan", flush=True)
+    assert M.sim_tmi_main([n, periods, R, res, pmin, pmax, 1, 1, str(path)]) == 0
+    rows = list(csv.reader(open(path)))
+    assert rows[0] == ["p", "tmi"]
+    rows = rows[1:]
+    cycles = n // 4
+    assert len(rows) == res * R * cycles
+    ps = [pmin, pmax]
+    k = 0
+    for pi, p in enumerate(ps):
+        for r in range(R):
+            seed = O.lib().orc_trajectory_seed(master, pi * R + r)
+            psi, *_ = _oracle_run(n, 2 * periods, p, seed)
+            for c in range(cycles):
+                assert float(rows[k][0]) == p
+                assert abs(float(rows[k][1]) - O.tmi(psi, n, n // 4, c)) < 1e-5
+                k += 1
+
+
+@pytest.mark.gpu
+def test_sim_tmi_default_output_path(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "7")
+    print("sim_tmi default path", flush=True)
+    assert M.sim_tmi_main(["8", "1", "1", "2", "0", "0.25"]) == 0
+    want = tmp_path / "csv/tmi/haar/tmi_haar_n_8_real_1_0_0.25_res_2.csv"
+    assert want.exists(), os.listdir(tmp_path)
+    lines = want.read_text().splitlines()
+    assert lines[0] == "p,tmi" and len(lines) == 3
+    assert lines[1].split(",")[0] == "0" and lines[2].split(",")[0] == "0.25"
