@@ -135,8 +135,10 @@ typedef struct mipt_hip_traj_spec
     uint64_t seed;      /* counter-based stream key */
     int closed;         /* 1: ring (wrap bond on odd layers), 0: open chain */
     int gate_family;    /* 1 = Haar U(4) (cpp/include/mipt/types.hpp:155-186) */
-    int probe_site;     /* -1: none; else a probe qubit at index L starts Bell-paired with this site */
-    int p_zero_layers;  /* the first p_zero_layers layers are unmeasured (Gullans-Huse scrambling) */
+    int probe_site;     /* -1: none; else a probe qubit at index L is attached to this site: Bell-paired
+                           at t=0 when p_zero_layers = 0, else reset(site), h(site), cx(site, probe) at
+                           the start of layer p_zero_layers (cpp/include/mipt/probed/kernels.hpp:68-77) */
+    int p_zero_layers;  /* the first p_zero_layers layers are unmeasured (scrambling) */
     int shared_gates;   /* -1: own gates; else use trajectory shared_gates's gate stream */
 } mipt_hip_traj_spec;
 
@@ -183,6 +185,14 @@ int mipt_hip_batch_observables(mipt_hip_batch *batch, int traj, int tmi_block, d
  * all_cycles scan of cpp/include/mipt/tmi/compute.hpp:115-118 calls it for
  * cycle = 0..L/4-1. */
 int mipt_hip_batch_tmi(mipt_hip_batch *batch, int traj, int block, int cycle, int order, double *tmi);
+/* Bulk readout for learning from the simulator: the measurement records of
+ * every trajectory for layers [layer_begin, layer_end) as int8
+ * [count][layers][L] (-1 unmeasured, else the outcome; every trajectory must
+ * have the same L), and the probe Bloch vectors [count][3]
+ * ((2 Re rho01, -2 Im rho01, rho00 - rho11), cpp/include/mipt/ancilla_mi.hpp:
+ * 24-52; NaN without a probe). */
+int mipt_hip_batch_records(mipt_hip_batch *batch, int layer_begin, int layer_end, int8_t *host_records);
+int mipt_hip_batch_probe_bloch(mipt_hip_batch *batch, double *host_bloch);
 /* The stream the batch launches on. */
 void *mipt_hip_batch_stream(mipt_hip_batch *batch);
 /* Re-arms the in-kernel initial states and clears per-run records, so the
@@ -200,6 +210,40 @@ int mipt_hip_batch_set_profiling(mipt_hip_batch *batch, int sample_every);
  * sweeps), all sweeps launched and their total algorithmic bytes. */
 int mipt_hip_batch_pass_stats(mipt_hip_batch *batch, double *avg_ms, int *timed, double *bytes_avg,
                               long *launches, double *bytes_total);
+
+/* ------------------------------------------------------- learned decoder */
+/* A small space-time convolutional network (no counterpart in the
+ * reference) that reads a measurement record int8 [height=time][width=sites]
+ * (-1 unmeasured, 0, 1) and predicts the probe qubit's conditional Bloch
+ * vector.  Input channels: record == 0, record == 1, the probe's column
+ * (x == width/2), time fraction (y+1)/height; four 3x3 convolutions to 64
+ * channels (zero padding in time, periodic in sites) with ReLU, mean
+ * pooling, 64 -> 64 (ReLU) -> 3.  BF16 weights/activations on the matrix
+ * cores with FP32 accumulation, FP32 master weights and Adam (betas
+ * 0.9/0.999, eps 1e-8).  Loss: batch mean of (|out| - |y|)^2 +
+ * (|y| - out.y/|out|), |v| = sqrt(sum v^2 + 1e-12).
+ * Parameters are one FP32 vector: per conv layer w[9 taps][CI][64] then
+ * b[64] (CI = 4, 64, 64, 64; tap = 3(dy+1) + (dx+1)), then fc1 w[64][64],
+ * b[64], fc2 w[64][3], b[3]. */
+typedef struct mipt_hip_decoder mipt_hip_decoder;
+int mipt_hip_decoder_create(int height, int width, int batch_max, uint64_t seed, mipt_hip_decoder **out);
+int mipt_hip_decoder_destroy(mipt_hip_decoder *decoder);
+int mipt_hip_decoder_param_count(mipt_hip_decoder *decoder);
+int mipt_hip_decoder_get_params(mipt_hip_decoder *decoder, float *host_params);
+/* Also resets the optimiser state. */
+int mipt_hip_decoder_set_params(mipt_hip_decoder *decoder, const float *host_params);
+/* Uploads a dataset: records int8 [count][height][width], labels FP32 [count][3] (may be NULL). */
+int mipt_hip_decoder_set_dataset(mipt_hip_decoder *decoder, const int8_t *host_records, const float *host_labels,
+                                 int count);
+/* Loss and gradient (FP32, parameter layout) of the batch of dataset rows host_idx[0..batch). */
+int mipt_hip_decoder_gradients(mipt_hip_decoder *decoder, const int *host_idx, int batch, float *host_grad,
+                               double *loss);
+/* `steps` Adam steps, step s on rows host_idx[s*batch .. (s+1)*batch); mean batch loss (may be NULL). */
+int mipt_hip_decoder_train(mipt_hip_decoder *decoder, const int *host_idx, int batch, int steps, float lr,
+                           double *mean_loss);
+/* Predicted Bloch vectors [count][3] of dataset rows [first, first + count). */
+int mipt_hip_decoder_predict(mipt_hip_decoder *decoder, int first, int count, float *host_out);
+int mipt_hip_decoder_synchronize(mipt_hip_decoder *decoder);
 
 /* ------------------------------------------------------------ whole runs */
 typedef struct mipt_hip_traj_result

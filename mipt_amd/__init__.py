@@ -94,6 +94,8 @@ def lib():
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
+    L.mipt_hip_batch_records.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_int8)]
+    L.mipt_hip_batch_probe_bloch.argtypes = [vp, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
     L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     _LIB = L
@@ -439,6 +441,20 @@ class Batch:
                                                 ctypes.byref(tm) if want_tmi else None, ctypes.byref(pe), _dp(bl)),
                "batch_observables")
         return {"s_half": sh.value, "tmi": tm.value, "probe_entropy": pe.value, "bloch": bl}
+
+    def records(self, layer_begin: int, layer_end: int) -> np.ndarray:
+        """int8 [count, layers, L]: -1 unmeasured, else the outcome (same L for all)."""
+        L = self.specs[0].L
+        out = np.zeros((len(self.specs), layer_end - layer_begin, L), np.int8)
+        _check(lib().mipt_hip_batch_records(self._h, layer_begin, layer_end,
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))), "batch_records")
+        return out
+
+    def probe_bloch(self) -> np.ndarray:
+        """[count, 3] probe Bloch vectors (NaN rows without a probe)."""
+        out = np.zeros((len(self.specs), 3))
+        _check(lib().mipt_hip_batch_probe_bloch(self._h, _dp(out)), "batch_probe_bloch")
+        return out
 
     def tmi(self, traj: int, cycle: int = 0, block: int = 0, order: int = 1) -> float:
         out = ctypes.c_double()

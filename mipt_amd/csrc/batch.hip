@@ -126,9 +126,66 @@ __device__ inline double block_sum(double v, double *red)
     return r;
 }
 
+constexpr std::uint32_t STREAM_ATCH = 0x41544348u;  // probe-attach reset uniforms
+
+// Probe attach at the start of layer p_zero_layers (> 0), as
+// cpp/include/mipt/probed/kernels.hpp:68-77: reset(site), h(site),
+// cx(site, probe), the probe (qubit L) being |0> until then.  The reset is a
+// Born draw on the site (outcome 1 iff u < P1, u from the counter stream at
+// (layer, site)), renormalised by 1/sqrt(branch weight) and flipped to 0;
+// FP64 weights in a fixed reduction order.  256 threads, any buffer.
+__device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, double *red, int &status)
+{
+    const int lo = d.probe_site, hi = d.L;
+    const std::uint32_t S = 1u << lo, P = 1u << hi;
+    double w = 0.0, w1 = 0.0;
+    for (std::uint32_t i = threadIdx.x; i < dim; i += SMALL_THREADS)
+    {
+        const float2 x = v[i];
+        const double m = static_cast<double>(x.x) * x.x + static_cast<double>(x.y) * x.y;
+        w += m;
+        if (i & S) w1 += m;
+    }
+    w = block_sum(w, red);
+    w1 = block_sum(w1, red);
+    double p = (w > 0.0) ? w1 / w : 0.0;
+    p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
+    double u, unused;
+    philox_uniform_pair(d.seed, static_cast<std::uint32_t>(d.p_zero_layers), static_cast<std::uint32_t>(lo), 0,
+                        STREAM_ATCH, u, unused);
+    const int o = u < p ? 1 : 0;
+    const double branch = o ? w1 : w - w1;
+    if (!(branch > 0.0)) status = E_NORM;
+    const float scale = static_cast<float>(0.70710678118654752440 / sqrt(branch > 0.0 ? branch : 1.0));
+    for (std::uint32_t grp = threadIdx.x; grp < (dim >> 2); grp += SMALL_THREADS)
+    {
+        std::uint32_t base = ((grp >> lo) << (lo + 1)) | (grp & (S - 1u));
+        base = ((base >> hi) << (hi + 1)) | (base & (P - 1u));
+        float2 x = v[base | (o ? S : 0u)];
+        x.x *= scale;
+        x.y *= scale;
+        v[base] = x;
+        v[base | S | P] = x;
+        v[base | S] = make_float2(0.f, 0.f);
+        v[base | P] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const int *__restrict__ traj_list, int t)
+{
+    __shared__ double red[SMALL_THREADS];
+    const int traj = traj_list[blockIdx.x];
+    const TrajDesc d = a.desc[traj];
+    if (d.probe_site < 0 || d.p_zero_layers != t || t == 0) return;  // uniform over the block
+    int status = 0;
+    attach_probe(a.states + d.state_off, 1u << d.nq, d, red, status);
+    if (threadIdx.x == 0 && status) a.status[traj] = status;
+}
+
 __device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
 {
-    if (d.probe_site >= 0)
+    if (d.probe_site >= 0 && d.p_zero_layers == 0)
     {
         const std::uint64_t bell = (std::uint64_t{1} << d.probe_site) | (std::uint64_t{1} << d.L);
         const float h = 0.70710678118654752440f;
@@ -161,6 +218,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     int status = 0;
     for (int t = layer_begin; t < layer_end; ++t)
     {
+        if (d.probe_site >= 0 && d.p_zero_layers > 0 && t == d.p_zero_layers) attach_probe(st, dim, d, red, status);
         const int nb = layer_bond_count(d.L, t, d.closed);
         const std::uint64_t gbase = d.gate_off + gate_prefix(d, t);
         for (int b = 0; b < nb; ++b)
@@ -254,6 +312,60 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     {
         a.surprisal[traj] += surpr;
         if (status) a.status[traj] = status;
+    }
+}
+
+// ============================================================== bulk readout
+// Measurement records of every trajectory, int8 [count][nt][L]: -1 where the
+// site was not measured, else the outcome.
+__global__ void records_kernel(DevArena a, int t0, int nt, int L, std::size_t total, std::int8_t *__restrict__ out)
+{
+    const std::size_t id = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    if (id >= total) return;
+    const std::size_t per = static_cast<std::size_t>(nt) * L;
+    const int traj = static_cast<int>(id / per);
+    const int rem = static_cast<int>(id - static_cast<std::size_t>(traj) * per);
+    const int t = t0 + rem / L, s = rem % L;
+    const TrajDesc d = a.desc[traj];
+    const std::uint32_t f = a.flags[d.layer_off + t], o = a.outcomes[d.layer_off + t];
+    out[id] = ((f >> s) & 1u) ? static_cast<std::int8_t>((o >> s) & 1u) : static_cast<std::int8_t>(-1);
+}
+
+// Probe Bloch vectors [count][3] ((2 Re rho01, -2 Im rho01, rho00 - rho11) / tr,
+// rho01 = sum psi[..0..] conj(psi[..1..]) over the probe qubit L; NaN without a
+// probe), FP64 sums in a fixed order, one block per trajectory.
+__global__ void __launch_bounds__(SMALL_THREADS) probe_bloch_kernel(DevArena a, double *__restrict__ out)
+{
+    __shared__ double red[SMALL_THREADS];
+    const int traj = blockIdx.x;
+    const TrajDesc d = a.desc[traj];
+    if (d.probe_site < 0)
+    {
+        if (threadIdx.x < 3) out[traj * 3 + threadIdx.x] = NAN;
+        return;
+    }
+    const float2 *psi = a.states + d.state_off;
+    const std::uint32_t half = 1u << (d.nq - 1), P = 1u << d.L;
+    double a00 = 0, a11 = 0, re = 0, im = 0;
+    for (std::uint32_t e = threadIdx.x; e < half; e += SMALL_THREADS)
+    {
+        const std::uint32_t i0 = (e & (P - 1u)) | ((e & ~(P - 1u)) << 1);
+        const float2 x = psi[i0], y = psi[i0 | P];
+        a00 += static_cast<double>(x.x) * x.x + static_cast<double>(x.y) * x.y;
+        a11 += static_cast<double>(y.x) * y.x + static_cast<double>(y.y) * y.y;
+        re += static_cast<double>(x.x) * y.x + static_cast<double>(x.y) * y.y;
+        im += static_cast<double>(x.y) * y.x - static_cast<double>(x.x) * y.y;
+    }
+    a00 = block_sum(a00, red);
+    a11 = block_sum(a11, red);
+    re = block_sum(re, red);
+    im = block_sum(im, red);
+    if (threadIdx.x == 0)
+    {
+        const double tr = a00 + a11;
+        out[traj * 3 + 0] = 2.0 * re / tr;
+        out[traj * 3 + 1] = -2.0 * im / tr;
+        out[traj * 3 + 2] = (a00 - a11) / tr;
     }
 }
 
@@ -1483,9 +1595,21 @@ struct Batch
         al = a;
         al.partials = lane ? partials2 : a.partials;
         // maximum measured count per layer over the group (host flags)
+        bool attach_any = false;
+        for (int i : grp) attach_any |= desc[i].probe_site >= 0 && desc[i].p_zero_layers > 0;
         for (int t = t0; t < t1; ++t)
         {
             const LayerPlan &P = lp[t & 1];
+            if (attach_any && t > 0)
+            {
+                bool at = false;
+                for (int i : grp) at |= desc[i].probe_site >= 0 && desc[i].p_zero_layers == t;
+                if (at)
+                {
+                    hipLaunchKernelGGL(attach_kernel, dim3(ntraj), dim3(SMALL_THREADS), 0, ls, al, list, t);
+                    MIPT_HIP_CHECK(hipGetLastError());
+                }
+            }
             int maxm = 0;
             for (int i : grp) maxm = std::max(maxm, __builtin_popcount(h_flags[desc[i].layer_off + t]));
             const int chunks = (maxm + MAX_HIST_BITS - 1) / MAX_HIST_BITS;
@@ -1827,6 +1951,69 @@ int mipt_hip_batch_synchronize(mipt_hip_batch *b)
 {
     if (!b) { set_error("batch_synchronize: null batch"); return E_ARG; }
     MIPT_HIP_CHECK(hipStreamSynchronize(b->stream));
+    return OK;
+}
+
+int mipt_hip_batch_records(mipt_hip_batch *b, int layer_begin, int layer_end, int8_t *host)
+{
+    if (!b || !host || b->desc.empty() || layer_begin < 0 || layer_end <= layer_begin)
+    {
+        set_error("batch_records: bad arguments");
+        return E_ARG;
+    }
+    const int L = b->desc[0].L;
+    for (const TrajDesc &d : b->desc)
+        if (d.L != L || d.layers < layer_end)
+        {
+            set_error("batch_records: every trajectory needs L=%d and >= %d layers", L, layer_end);
+            return E_ARG;
+        }
+    const int count = static_cast<int>(b->desc.size());
+    std::vector<int> st(count);
+    MIPT_HIP_CHECK(hipMemcpyAsync(st.data(), b->a.status, count * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    const std::size_t total = static_cast<std::size_t>(count) * (layer_end - layer_begin) * L;
+    std::int8_t *dev = nullptr;
+    MIPT_HIP_CHECK(hipMalloc(&dev, total));
+    hipLaunchKernelGGL(records_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, b->stream, b->a,
+                       layer_begin, layer_end - layer_begin, L, total, dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, total, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess)
+    {
+        set_error("batch_records: %s", hipGetErrorString(e));
+        return static_cast<int>(e);
+    }
+    for (int i = 0; i < count; ++i)
+        if (st[i] != 0)
+        {
+            set_error("trajectory %d hit a zero-weight measurement branch", i);
+            return st[i];
+        }
+    return OK;
+}
+
+int mipt_hip_batch_probe_bloch(mipt_hip_batch *b, double *host)
+{
+    if (!b || !host || b->desc.empty())
+    {
+        set_error("batch_probe_bloch: bad arguments");
+        return E_ARG;
+    }
+    const int count = static_cast<int>(b->desc.size());
+    double *dev = nullptr;
+    MIPT_HIP_CHECK(hipMalloc(&dev, count * 3 * sizeof(double)));
+    hipLaunchKernelGGL(probe_bloch_kernel, dim3(count), dim3(SMALL_THREADS), 0, b->stream, b->a, dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, count * 3 * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess)
+    {
+        set_error("batch_probe_bloch: %s", hipGetErrorString(e));
+        return static_cast<int>(e);
+    }
     return OK;
 }
 

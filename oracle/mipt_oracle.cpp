@@ -412,6 +412,15 @@ void orc_philox4x32_10(const std::uint32_t *key, const std::uint32_t *ctr, std::
     philox4x32_10(key, ctr, out);
 }
 
+// First uniform of draw `slot` of (layer, item, stream) on the counter stream.
+double orc_philox_uniform(std::uint64_t seed, std::uint32_t layer, std::uint32_t item, std::uint32_t slot,
+                          std::uint32_t stream)
+{
+    double u, v;
+    philox_uniform_pair(seed, layer, item, slot, stream, u, v);
+    return u;
+}
+
 int orc_layer_bonds(int n, int layer, int closed, int *q0, int *q1)
 {
     return layer_bonds(n, layer & 1, closed, q0, q1);

@@ -50,6 +50,9 @@ def lib():
         L = ctypes.CDLL(path)
         L.orc_splitmix64.restype = ctypes.c_uint64
         L.orc_splitmix64.argtypes = [ctypes.c_uint64]
+        L.orc_philox_uniform.restype = ctypes.c_double
+        L.orc_philox_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32]
         L.orc_trajectory_seed.restype = ctypes.c_uint64
         L.orc_trajectory_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.orc_linspace.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
@@ -297,3 +300,66 @@ def jacobi_eigvalsh(a: np.ndarray) -> np.ndarray:
     if st != 0:
         raise RuntimeError("oracle Jacobi did not converge")
     return w
+
+
+# ------------------------------------------------------------- probe protocol
+STREAM_ATCH = 0x41544348  # probe-attach reset uniforms
+
+
+def philox_uniform(seed: int, layer: int, item: int, slot: int, stream: int) -> float:
+    return lib().orc_philox_uniform(seed & (2**64 - 1), layer, item, slot, stream)
+
+
+def attach_probe(psi: np.ndarray, n: int, site: int, u: float) -> int:
+    """reset(site); h(site); cx(site, probe) with the probe at qubit n in |0>
+    (cpp/include/mipt/probed/kernels.hpp:68-77).  The reset is a sequential
+    Born draw on the site (outcome 1 iff u < P1, renormalised), then the site
+    is flipped to 0.  In place on the (n+1)-qubit state; returns the outcome."""
+    nq = n + 1
+    _, o, _ = measure_collapse(psi, nq, site, u)
+    S, P = 1 << site, 1 << n
+    idx = np.arange(1 << nq)
+    base = idx[((idx & S) == 0) & ((idx & P) == 0)]
+    a = psi[base | (S if o else 0)].copy() / np.sqrt(2.0)
+    psi[base | S] = 0.0
+    psi[base | P] = 0.0
+    psi[base] = a
+    psi[base | S | P] = a
+    return o
+
+
+def probe_trajectory(n: int, layers: int, p: float, seed: int, probe_site: int, p_zero_layers: int,
+                     gate_seed: int | None = None, closed: bool = True):
+    """The probe protocol the batched engine runs for a spec with a probe:
+    unmeasured layers 0..p_zero_layers-1, the probe attached (reset + Bell
+    pair) at the start of layer p_zero_layers (or a Bell pair at t=0 when
+    p_zero_layers = 0), measured layers after it.  Gates from gate_seed's
+    stream when given (a shared realisation), flags and uniforms from seed.
+    Returns (psi over n+1 qubits, outcomes [layers,n], p1, attach outcome)."""
+    g, f, u = build_layers(n, layers, p, seed, closed)
+    if gate_seed is not None:
+        g = build_layers(n, layers, 0.0, gate_seed, closed)[0]
+    f[:p_zero_layers] = 0
+    u[:p_zero_layers] = 0.0
+    nq = n + 1
+    psi = np.zeros(1 << nq, np.complex128)
+    att = -1
+    if p_zero_layers == 0:
+        psi[0] = psi[(1 << probe_site) | (1 << n)] = 1 / np.sqrt(2.0)
+    else:
+        psi[0] = 1.0
+    outs = np.full((layers, n), -1, np.int32)
+    p1 = np.zeros((layers, n))
+    gi = 0
+    for t in range(layers):
+        if p_zero_layers > 0 and t == p_zero_layers:
+            att = attach_probe(psi, n, probe_site, philox_uniform(seed, t, probe_site, 0, STREAM_ATCH))
+        for (q0, q1) in layer_bonds(n, t, closed):
+            apply_dense(psi, nq, [q0, q1], g[gi])
+            gi += 1
+        for s in range(n):
+            if f[t, s]:
+                st, o, pr = measure_collapse(psi, nq, s, u[t, s])
+                outs[t, s] = o
+                p1[t, s] = pr if o == 1 else 1.0 - pr
+    return psi, outs, p1, att, (g, f, u)

@@ -301,3 +301,42 @@ def test_batch_probe_bell_pair(M):
         ob = b.observables(2, want_tmi=False)
         assert abs(ob["probe_entropy"] - O.rho1_entropy_nats(r)) < 1e-5
         assert np.abs(ob["bloch"] - O.bloch_from_rho1(r)).max() < 1e-5
+
+
+def test_batch_probe_attach_after_scrambling(M):
+    # the decoder workload's protocol: one shared gate realisation (donor
+    # trajectory), unmeasured scrambling layers, then reset(site) + Bell pair
+    # with the probe (probed/kernels.hpp:68-77), then measured layers.
+    # L=8 runs in the small kernel, L=16 (+ probe = 17 qubits) in tile sweeps.
+    specs = [M.Spec(8, 12, 0.2, 7, True, probe_site=4, p_zero_layers=4),
+             M.Spec(8, 12, 0.3, 1001, True, probe_site=4, p_zero_layers=4, shared_gates=0),
+             M.Spec(16, 10, 0.2, 7, True, probe_site=8, p_zero_layers=4),
+             M.Spec(16, 10, 0.25, 1002, True, probe_site=8, p_zero_layers=4, shared_gates=2)]
+    donors = [None, 7, None, 7]
+    print("batch probe attach after scrambling", flush=True)
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    bloch = b.probe_bloch()
+    for i, s in enumerate(specs):
+        psi, outs, p1, att, (g, f, u) = O.probe_trajectory(s.L, s.layers, s.p, s.seed, s.probe_site,
+                                                           s.p_zero_layers, donors[i])
+        got, flagged, _, _ = b.outcomes(i)
+        check_outcomes(got, outs, p1, u, f, f"probe traj {i}")
+        if not flagged.any():
+            assert np.abs(b.get_state(i) - psi).max() < 1e-5, f"traj {i}"
+            want = O.bloch_from_rho1(O.rho1(psi, s.L + 1, s.L))
+            assert np.abs(bloch[i] - want).max() < 1e-5
+    with pytest.raises(M.MiptHipError):
+        b.records(4, 10)  # mixed chain lengths
+    b2 = M.Batch([M.Spec(16, 10, 0.2, 7, True, probe_site=8, p_zero_layers=4),
+                  M.Spec(16, 10, 0.25, 1002, True, probe_site=8, p_zero_layers=4, shared_gates=0)])
+    b2.generate()
+    b2.evolve()
+    b2.synchronize()
+    rec = b2.records(4, 10)
+    for i in range(2):
+        got, _, _, _ = b2.outcomes(i)
+        assert np.array_equal(rec[i], got[4:10].astype(np.int8))
+        assert np.array_equal(got, b.outcomes(i + 2)[0])

@@ -406,3 +406,41 @@ def test_measured_chain_state_stays_normalised_and_outcomes_recorded():
         if f[0, s]:
             w1 = np.sum(np.abs(psi2[((idx >> s) & 1) == 1]) ** 2)
             assert min(w1, 1 - w1) < 1e-14
+
+
+def test_probe_attach_closed_forms():
+    # reset(site) + h(site) + cx(site, probe) on a scrambled state: the site is
+    # disentangled by the reset, so {site, probe} is a pure Bell pair:
+    # S(probe) = ln 2 (nats), S({site, probe}) = 0, and the reset outcome
+    # follows Born's rule for the site.
+    rng = np.random.default_rng(5)
+    n, site = 6, 2
+    for u in (0.05, 0.5, 0.95):
+        phi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        phi /= np.linalg.norm(phi)
+        psi = np.zeros(1 << (n + 1), np.complex128)
+        psi[: 1 << n] = phi
+        p1 = float(np.sum(np.abs(phi[(np.arange(1 << n) >> site) & 1 == 1]) ** 2))
+        o = O.attach_probe(psi, n, site, u)
+        assert o == (1 if u < p1 else 0)
+        assert abs(np.linalg.norm(psi) - 1.0) < 1e-12
+        r = O.rho1(psi, n + 1, n)
+        assert abs(O.rho1_entropy_nats(r) - np.log(2)) < 1e-12
+        assert abs(O.entropy(psi, n + 1, [site, n])) < 1e-10
+        # the other qubits hold phi collapsed on site = o
+        keep = np.arange(1 << n)
+        col = phi[(keep >> site) & 1 == o]
+        rest = psi[(np.arange(1 << (n + 1)) & ((1 << site) | (1 << n))) == 0] * np.sqrt(2.0)
+        assert np.allclose(np.abs(rest), np.abs(col) / np.linalg.norm(col), atol=1e-12)
+
+
+def test_probe_protocol_oracle_runs_and_purifies():
+    # p = 1 after the attach measures every site each layer: the probe purifies
+    psi, outs, p1, att, _ = O.probe_trajectory(8, 12, 1.0, 77, 4, 4)
+    assert att in (0, 1)
+    assert np.all(outs[:4] == -1) and np.all(outs[4:] >= 0)
+    r = O.rho1(psi, 9, 8)
+    assert O.rho1_entropy_nats(r) < 1e-9
+    # p = 0: the probe stays maximally entangled
+    psi, *_ = O.probe_trajectory(8, 12, 0.0, 78, 4, 4)
+    assert abs(O.rho1_entropy_nats(O.rho1(psi, 9, 8)) - np.log(2)) < 1e-10
