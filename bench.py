@@ -104,6 +104,10 @@ def main(argv=None):
     ap.add_argument("--rdm-L", type=int, default=0,
                     help="also time the half-chain RDM entropy at this L (Bell-pair state, S = L/2 exactly)")
     ap.add_argument("--rdm-reps", type=int, default=3)
+    ap.add_argument("--no-decoder", action="store_true", help="skip the learned probe decoder leg")
+    ap.add_argument("--decoder-train", type=int, default=200_000)
+    ap.add_argument("--decoder-test", type=int, default=20_000)
+    ap.add_argument("--decoder-epochs", type=int, default=16)
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -250,8 +254,57 @@ def main(argv=None):
         result["cpu_baseline"] = None
     if args.rdm_L:
         result["rdm_gemm"] = rdm_timing(M, args.rdm_L, args.rdm_reps)
+    if not args.no_decoder:
+        try:
+            result["decoder"] = decoder_leg(args.decoder_train, args.decoder_test, args.decoder_epochs)
+        except Exception as e:  # the headline line must still print
+            result["decoder"] = {"error": repr(e)}
+            log(f"decoder leg failed: {e!r}")
     print(json.dumps(result))
     return 0
+
+
+def decoder_leg(n_train, n_test, epochs, batch=1024, lr=1e-3):
+    """Learned probe decoder workload: L=16, one shared Haar realisation (seed
+    7), 32 unmeasured scrambling layers, probe attached to site 8, 32 measured
+    layers at p ~ U[0.05, 0.30]; n_train training and n_test held-out
+    trajectories (disjoint seeds); BF16-MFMA training on the GPU; mean
+    |S_pred - S_exact| per p bin against the p-bin mean entropy."""
+    from mipt_amd import decoder as dec
+    log(f"decoder leg: simulating {n_train} + {n_test} trajectories")
+    train = dec.generate_probe_dataset(np.arange(1, n_train + 1))
+    test = dec.generate_probe_dataset(np.arange(10_000_001, 10_000_001 + n_test))
+    sim_tps = (n_train + n_test) / (train.seconds + test.seconds)
+    log(f"decoder leg: simulation {sim_tps:.0f} trajectories/s; training {epochs} epochs")
+    d = dec.Decoder(32, 16, batch, seed=5)
+    d.set_dataset(np.concatenate([train.records, test.records]), np.concatenate([train.bloch, test.bloch]))
+    rng = np.random.default_rng(0)
+    steps = n_train // batch
+    t = time.perf_counter()
+    loss = None
+    for e in range(epochs):
+        loss = d.train(rng.permutation(n_train)[: steps * batch], batch, lr=lr)
+        log(f"decoder epoch {e + 1}/{epochs}: mean batch loss {loss:.4f}")
+    train_s = time.perf_counter() - t
+    d.predict(n_train, min(n_test, batch))  # warm
+    t = time.perf_counter()
+    pred = d.predict(n_train, n_test)
+    inf_s = time.perf_counter() - t
+    s_exact = dec.entropy_from_bloch(test.bloch)
+    err = np.abs(dec.entropy_from_bloch(pred) - s_exact)
+    base, bins, edges = dec.p_bin_baseline(train, test)
+    err_b = np.abs(base - s_exact)
+    per_bin = [{"p": [round(float(edges[k]), 3), round(float(edges[k + 1]), 3)], "n": int((bins == k).sum()),
+                "decoder": round(float(err[bins == k].mean()), 4), "p_bin_mean": round(float(err_b[bins == k].mean()), 4)}
+               for k in range(len(edges) - 1)]
+    d.close()
+    return {"workload": "L=16 ring, shared Haar gates (seed 7), 32 scrambling layers, probe at site 8, "
+                        "32 measured layers, p ~ U[0.05,0.30]",
+            "train": n_train, "held_out": n_test, "epochs": epochs, "batch": batch, "lr": lr,
+            "final_train_loss": round(loss, 4), "train_seconds": round(train_s, 2),
+            "mean_abs_dS_nats": round(float(err.mean()), 4), "baseline_mean_abs_dS_nats": round(float(err_b.mean()), 4),
+            "per_p_bin": per_bin, "inferences_per_s": round(n_test / inf_s, 1),
+            "simulation_trajectories_per_s": round(sim_tps, 1)}
 
 
 def rdm_timing(M, n, reps):

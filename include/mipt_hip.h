@@ -220,8 +220,10 @@ int mipt_hip_batch_pass_stats(mipt_hip_batch *batch, double *avg_ms, int *timed,
  * channels (zero padding in time, periodic in sites) with ReLU, mean
  * pooling, 64 -> 64 (ReLU) -> 3.  BF16 weights/activations on the matrix
  * cores with FP32 accumulation, FP32 master weights and Adam (betas
- * 0.9/0.999, eps 1e-8).  Loss: batch mean of (|out| - |y|)^2 +
- * (|y| - out.y/|out|), |v| = sqrt(sum v^2 + 1e-12).
+ * 0.9/0.999, eps 1e-8).  Loss: batch mean of h(S(|out|) - S(|y|)) +
+ * (|y| - out.y/|out|), |v| = sqrt(sum v^2 + 1e-12), S the qubit entropy
+ * (nats) of Bloch length r, continued linearly past r = 1 - 1e-6, h the
+ * smooth-L1 function of width delta (default 0.05; 0 = squared error).
  * Parameters are one FP32 vector: per conv layer w[9 taps][CI][64] then
  * b[64] (CI = 4, 64, 64, 64; tap = 3(dy+1) + (dx+1)), then fc1 w[64][64],
  * b[64], fc2 w[64][3], b[3]. */
@@ -243,6 +245,8 @@ int mipt_hip_decoder_train(mipt_hip_decoder *decoder, const int *host_idx, int b
                            double *mean_loss);
 /* Predicted Bloch vectors [count][3] of dataset rows [first, first + count). */
 int mipt_hip_decoder_predict(mipt_hip_decoder *decoder, int first, int count, float *host_out);
+/* Width delta of the smooth-L1 entropy term (>= 0; 0 = squared error). */
+int mipt_hip_decoder_set_loss(mipt_hip_decoder *decoder, float huber_delta);
 int mipt_hip_decoder_synchronize(mipt_hip_decoder *decoder);
 
 /* ------------------------------------------------------------ whole runs */

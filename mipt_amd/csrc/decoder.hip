@@ -8,10 +8,13 @@
 // site), (y + 1) / H (time fraction)); four 3x3 convolutions
 // (zero padding in time, periodic in sites: the chain is a ring) to 64
 // channels with bias and ReLU; mean pooling over the H*W positions; 64 -> 64
-// (ReLU) -> 3.  Loss: batch mean of (|out| - |y|)^2 + (|y| - out.y/|out|)
-// with |v| = sqrt(sum v^2 + 1e-12): norm and direction are fitted separately,
-// so a direction the record leaves open does not shrink the predicted norm
-// (which fixes the entropy).  Adam (lr given per call, betas 0.9/0.999,
+// (ReLU) -> 3.  Loss: batch mean of h(S(|out|) - S(|y|)) + (|y| - out.y/|out|),
+// h the smooth-L1 function of width delta (d^2/(2 delta) inside, |d| - delta/2
+// outside; d^2 when delta = 0; default 0.05),
+// with |v| = sqrt(sum v^2 + 1e-12) and S(r) the qubit entropy (nats) at
+// Bloch length r, continued linearly past 1 - 1e-6 (no flat region above
+// 1): entropy and direction are fitted separately, so a direction the record
+// leaves open does not shrink the predicted length.  Adam (lr given per call, betas 0.9/0.999,
 // eps 1e-8) on FP32 master weights.
 //
 // Precision: weights and activations are bfloat16 where they feed the
@@ -54,7 +57,9 @@ constexpr int LP = TK + 8;    // LDS row stride of a K step (bf16), 80 B
 constexpr int CP = CH + 8;    // LDS row stride of the epilogue tile (bf16), 144 B
 constexpr int CHUNK = 2048;   // positions per weight-gradient partial
 constexpr float NORM_EPS = 1e-12f;
-constexpr float MU = 1.0f;  // weight of the direction term
+constexpr float MU = 1.0f;            // weight of the direction term
+constexpr float R_MAX = 1.0f - 1e-6f;  // Bloch length cap of the entropy term
+constexpr int BROWS = 256;             // positions per bias-gradient partial
 
 __device__ inline void position(int m, int HW, int W, int &b, int &y, int &x)
 {
@@ -116,6 +121,13 @@ __device__ inline void for_tile_outputs(const floatx4 acc[2][2], int lane, int w
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) f(wr + 16 * i + 4 * (lane >> 4) + e, wc + 16 * j + (lane & 15), acc[i][j][e]);
+}
+
+__device__ inline float qubit_entropy(float r)
+{
+    const float c = fminf(r, R_MAX);
+    const float a = 0.5f * (1.f + c), b = 0.5f * (1.f - c);
+    return -a * logf(a) - b * logf(b) - atanhf(R_MAX) * fmaxf(r - R_MAX, 0.f);
 }
 
 // ---------------------------------------------------------------- forward
@@ -327,29 +339,42 @@ __global__ void sum_partials_kernel(const float *__restrict__ partial, int nchun
     grad[e] = s;
 }
 
-// bias partials: bpart[chunk][co] = sum over the chunk's positions of dact * (act > 0)
-__global__ void bias_partial_kernel(const float *__restrict__ dact, const bf16 *__restrict__ act, int M,
-                                    float *__restrict__ bpart)
+// bias partials: bpart[block][co] = sum over the block's BROWS positions of
+// dact * (act > 0); 256 threads = 4 interleaved row sets per channel, added
+// in a fixed order
+__global__ void __launch_bounds__(256) bias_partial_kernel(const float *__restrict__ dact,
+                                                           const bf16 *__restrict__ act, int M,
+                                                           float *__restrict__ bpart)
 {
-    const int co = threadIdx.x, chunk = blockIdx.x;
-    const int mbeg = chunk * CHUNK, mend = std::min(M, mbeg + CHUNK);
+    __shared__ float red[4][CH];
+    const int co = threadIdx.x & (CH - 1), q = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * BROWS;
     float s = 0.f;
-    for (int m = mbeg; m < mend; ++m)
+    for (int r = q; r < BROWS; r += 4)
     {
+        const int m = r0 + r;
+        if (m >= M) break;
         const std::size_t o = static_cast<std::size_t>(m) * CH + co;
         if (static_cast<float>(act[o]) > 0.f) s += dact[o];
     }
-    bpart[chunk * CH + co] = s;
+    red[q][co] = s;
+    __syncthreads();
+    if (q == 0) bpart[blockIdx.x * CH + co] = (red[0][co] + red[1][co]) + (red[2][co] + red[3][co]);
 }
 
 // ------------------------------------------------------ pooling, MLP, loss
-__global__ void pool_fwd_kernel(const bf16 *__restrict__ act, int HW, float *__restrict__ pooled)
+// pooled[b][c] = mean over positions; 256 threads = 4 interleaved position sets
+__global__ void __launch_bounds__(256) pool_fwd_kernel(const bf16 *__restrict__ act, int HW,
+                                                       float *__restrict__ pooled)
 {
-    const int b = blockIdx.x, c = threadIdx.x;
+    __shared__ float red[4][CH];
+    const int b = blockIdx.x, c = threadIdx.x & (CH - 1), q = threadIdx.x >> 6;
     const bf16 *p = act + static_cast<std::size_t>(b) * HW * CH + c;
     float s = 0.f;
-    for (int i = 0; i < HW; ++i) s += static_cast<float>(p[static_cast<std::size_t>(i) * CH]);
-    pooled[b * CH + c] = s / static_cast<float>(HW);
+    for (int i = q; i < HW; i += 4) s += static_cast<float>(p[static_cast<std::size_t>(i) * CH]);
+    red[q][c] = s;
+    __syncthreads();
+    if (q == 0) pooled[b * CH + c] = ((red[0][c] + red[1][c]) + (red[2][c] + red[3][c])) / static_cast<float>(HW);
 }
 
 // h = relu(pooled @ w1 + b1); out = h @ w2 + b2 (bf16 weights, FP32 math)
@@ -378,8 +403,8 @@ __global__ void mlp_fwd_kernel(const float *__restrict__ pooled, const bf16 *__r
 // per sample: loss term, dout, dh, dpooled (batch size nb for the 2/B factor)
 __global__ void loss_bwd_kernel(const float *__restrict__ out, const float *__restrict__ labels,
                                 const int *__restrict__ idx, const float *__restrict__ h, const bf16 *__restrict__ w1,
-                                const bf16 *__restrict__ w2, int nb, float *__restrict__ loss, float *__restrict__ dout,
-                                float *__restrict__ dh, float *__restrict__ dpooled)
+                                const bf16 *__restrict__ w2, int nb, float huber, float *__restrict__ loss,
+                                float *__restrict__ dout, float *__restrict__ dh, float *__restrict__ dpooled)
 {
     __shared__ float so[3], sdh[CH];
     const int b = blockIdx.x, j = threadIdx.x;
@@ -397,11 +422,30 @@ __global__ void loss_bwd_kernel(const float *__restrict__ out, const float *__re
         }
         no = sqrtf(no + NORM_EPS);
         ny = sqrtf(ny + NORM_EPS);
-        loss[b] = (no - ny) * (no - ny) + MU * (ny - dot / no);
+        const float d = qubit_entropy(no) - qubit_entropy(ny);
+        float hl, hg;  // smooth-L1 value and slope in d
+        if (huber <= 0.f)
+        {
+            hl = d * d;
+            hg = 2.f * d;
+        }
+        else if (fabsf(d) < huber)
+        {
+            hl = d * d / (2.f * huber);
+            hg = d / huber;
+        }
+        else
+        {
+            hl = fabsf(d) - 0.5f * huber;
+            hg = d > 0.f ? 1.f : -1.f;
+        }
+        loss[b] = hl + MU * (ny - dot / no);
+        // dS/dr = -atanh(min(r, cap))
+        const float gs = -hg * atanhf(fminf(no, R_MAX));
         const float f = 1.f / static_cast<float>(nb);
         for (int q = 0; q < 3; ++q)
         {
-            so[q] = f * (2.f * (no - ny) / no * o[q] - MU * (yv[q] / no - dot / (no * no * no) * o[q]));
+            so[q] = f * (gs / no * o[q] - MU * (yv[q] / no - dot / (no * no * no) * o[q]));
             dout[b * 3 + q] = so[q];
         }
     }
@@ -523,6 +567,7 @@ struct Decoder
     float *data_lab = nullptr;
     int data_count = 0;
     long step = 0;
+    float huber = 0.05f;  // smooth-L1 width of the entropy term (0: squared error)
 
     ~Decoder()
     {
@@ -575,7 +620,7 @@ struct Decoder
         for (int i = 0; i < 2; ++i) MIPT_HIP_CHECK(hipMalloc(&dact[i], M * CH * 4));
         const std::size_t nch = (M + CHUNK - 1) / CHUNK;
         MIPT_HIP_CHECK(hipMalloc(&partial, nch * 9 * CH * CH * 4));
-        MIPT_HIP_CHECK(hipMalloc(&bpart, nch * CH * 4));
+        MIPT_HIP_CHECK(hipMalloc(&bpart, ((M + BROWS - 1) / BROWS) * CH * 4));
         MIPT_HIP_CHECK(hipMalloc(&pooled, bmax * CH * 4));
         MIPT_HIP_CHECK(hipMalloc(&h, bmax * CH * 4));
         MIPT_HIP_CHECK(hipMalloc(&dh, bmax * CH * 4));
@@ -630,7 +675,7 @@ struct Decoder
         for (int l = 1; l < 4; ++l)
             hipLaunchKernelGGL(conv_fwd_kernel<false>, dim3(grid), dim3(NT), 0, stream, act[l - 1], nullptr, nullptr,
                                wf[l], w + off_b[l], act[l], M, H, W, kp[l]);
-        hipLaunchKernelGGL(pool_fwd_kernel, dim3(nb), dim3(CH), 0, stream, act[3], HW, pooled);
+        hipLaunchKernelGGL(pool_fwd_kernel, dim3(nb), dim3(256), 0, stream, act[3], HW, pooled);
         hipLaunchKernelGGL(mlp_fwd_kernel, dim3(nb), dim3(CH), 0, stream, pooled, w1b, w + off_fc + CH * CH, w2b,
                            w + off_fc + CH * CH + CH + CH * 3, h, out);
         MIPT_HIP_CHECK(hipGetLastError());
@@ -643,8 +688,8 @@ struct Decoder
         const int M = nb * HW;
         const int grid = blocks(M, TM);
         const int nch = blocks(M, CHUNK);
-        hipLaunchKernelGGL(loss_bwd_kernel, dim3(nb), dim3(CH), 0, stream, out, labels, idx, h, w1b, w2b, nb, loss,
-                           dout, dh, dpooled);
+        hipLaunchKernelGGL(loss_bwd_kernel, dim3(nb), dim3(CH), 0, stream, out, labels, idx, h, w1b, w2b, nb, huber,
+                           loss, dout, dh, dpooled);
         hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks(CH * CH + CH + CH * 3 + 3, 256)), dim3(256), 0, stream, pooled,
                            h, dh, dout, nb, g + off_fc);
         hipLaunchKernelGGL(pool_bwd_kernel, dim3(blocks(static_cast<std::size_t>(M) * CH, 256)), dim3(256), 0, stream,
@@ -663,8 +708,9 @@ struct Decoder
                                    nullptr, nullptr, dact[cur], act[l], partial, M, H, W, krows);
             hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks(krows * CH, 256)), dim3(256), 0, stream, partial, nch,
                                krows * CH, g + off_w[l]);
-            hipLaunchKernelGGL(bias_partial_kernel, dim3(nch), dim3(CH), 0, stream, dact[cur], act[l], M, bpart);
-            hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(CH), 0, stream, bpart, nch, CH, g + off_b[l]);
+            const int nbp = blocks(M, BROWS);
+            hipLaunchKernelGGL(bias_partial_kernel, dim3(nbp), dim3(256), 0, stream, dact[cur], act[l], M, bpart);
+            hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(CH), 0, stream, bpart, nbp, CH, g + off_b[l]);
             if (l > 0)
                 hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid), dim3(NT), 0, stream, dact[cur], act[l], wd[l],
                                    dact[cur ^ 1], M, H, W);
@@ -845,6 +891,13 @@ int mipt_hip_decoder_predict(mipt_hip_decoder *d, int first, int count, float *h
                                       d->stream));
     }
     MIPT_HIP_CHECK(hipStreamSynchronize(d->stream));
+    return OK;
+}
+
+int mipt_hip_decoder_set_loss(mipt_hip_decoder *d, float huber_delta)
+{
+    if (!d || !(huber_delta >= 0.f)) { set_error("decoder_set_loss: bad arguments"); return E_ARG; }
+    d->huber = huber_delta;
     return OK;
 }
 

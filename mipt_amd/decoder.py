@@ -39,6 +39,7 @@ def _bind():
     L.mipt_hip_decoder_train.argtypes = [vp, _ip, i, i, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
     L.mipt_hip_decoder_predict.argtypes = [vp, i, i, _fp]
     L.mipt_hip_decoder_synchronize.argtypes = [vp]
+    L.mipt_hip_decoder_set_loss.argtypes = [vp, ctypes.c_float]
     L._decoder_bound = True
     return L
 
@@ -85,6 +86,10 @@ class Decoder:
                                                   None if lab is None else lab.ctypes.data_as(_fp), len(r)),
                "decoder_set_dataset")
         self.count = len(r)
+
+    def set_loss(self, huber_delta: float):
+        """Smooth-L1 width of the entropy term (0: squared error)."""
+        _check(lib().mipt_hip_decoder_set_loss(self._h, huber_delta), "decoder_set_loss")
 
     def gradients(self, idx):
         idx = np.ascontiguousarray(idx, np.int32)

@@ -15,10 +15,15 @@ GPU holds bfloat16 values, so that GPU gradients can be compared with it:
              3 * (dy + 1) + (dx + 1)), activations stored as bf16
   pool       mean over the H*W positions
   fc1        64 -> 64 + bias, ReLU;  fc2 64 -> 3 + bias  (bf16 weights)
-  loss       mean over the batch of (|out| - |y|)^2 + MU (|y| - out.y/|out|),
-             |v| = sqrt(sum v^2 + 1e-12): the norm (which fixes the entropy)
-             and the direction are fitted separately, so a direction that the
-             record leaves open does not shrink the predicted norm (a squared
+  loss       mean over the batch of h(S(|out|) - S(|y|)) + MU (|y| - out.y/|out|),
+             h the smooth-L1 (Huber) function of width HUBER (d^2 / (2 HUBER)
+             inside, |d| - HUBER/2 outside; d^2 when HUBER = 0),
+             with |v| = sqrt(sum v^2 + 1e-12) and S(r) the qubit entropy
+             (nats) at Bloch length r, continued linearly past R_MAX (so a
+             length above 1 is pulled back instead of sitting in a flat
+             region): the entropy and the
+             direction are fitted separately, so a direction that the record
+             leaves open does not shrink the predicted length (a squared
              error on the components would, biasing the entropy upwards)
 
 Pinned by finite differences of its own loss and by torch autograd on the
@@ -30,6 +35,8 @@ import numpy as np
 
 C = 64
 MU = 1.0
+R_MAX = float(np.float32(1.0 - 1e-6))  # the GPU's FP32 cap
+HUBER = 0.05  # default smooth-L1 width of the entropy term (0: squared error)
 NORM_EPS = 1e-12
 CONV_CI = (4, 64, 64, 64)
 
@@ -149,22 +156,39 @@ def forward(p, rec, emulate=True):
     return out, {"acts": acts, "pooled": pooled, "h": h}
 
 
-def loss_fn(out, y):
+def _qubit_entropy(r):
+    c = np.minimum(r, R_MAX)
+    a, b = 0.5 * (1 + c), 0.5 * (1 - c)
+    return -a * np.log(a) - b * np.log(b) - np.arctanh(R_MAX) * np.maximum(r - R_MAX, 0.0)
+
+
+def _huber(d, delta):
+    if delta <= 0:
+        return d * d, 2.0 * d
+    inside = np.abs(d) < delta
+    return (np.where(inside, d * d / (2 * delta), np.abs(d) - delta / 2),
+            np.where(inside, d / delta, np.sign(d)))
+
+
+def loss_fn(out, y, huber=HUBER):
     no, ny = _norm(out), _norm(y)
-    return float(np.mean((no - ny) ** 2 + MU * (ny - (out * y).sum(-1) / no)))
+    h, _ = _huber(_qubit_entropy(no) - _qubit_entropy(ny), huber)
+    return float(np.mean(h + MU * (ny - (out * y).sum(-1) / no)))
 
 
-def gradients(p, rec, y, emulate=True):
+def gradients(p, rec, y, emulate=True, huber=HUBER):
     """(loss, grads dict) with the GPU's rounding points when emulate=True."""
     q = lambda v: bf16(v, emulate)
     out, cache = forward(p, rec, emulate)
     B = out.shape[0]
     y = np.asarray(y, np.float64)
-    loss = loss_fn(out, y)
+    loss = loss_fn(out, y, huber)
     no, ny = _norm(out), _norm(y)
     dot = (out * y).sum(-1)
-    dout = (1.0 / B) * ((2.0 * (no - ny) / no)[:, None] * out
-                        - MU * (y / no[:, None] - (dot / no ** 3)[:, None] * out))
+    dsdr = -np.arctanh(np.minimum(no, R_MAX))
+    _, dh_dd = _huber(_qubit_entropy(no) - _qubit_entropy(ny), huber)
+    gs = dh_dd * dsdr
+    dout = (1.0 / B) * ((gs / no)[:, None] * out - MU * (y / no[:, None] - (dot / no ** 3)[:, None] * out))
     g = {}
     h, pooled, acts = cache["h"], cache["pooled"], cache["acts"]
     g["fc2.w"] = h.T @ dout

@@ -37,9 +37,10 @@ def test_bf16_rounding_ties_and_torch():
     assert np.array_equal(D.bf16(x), want)
 
 
-def test_gradients_match_finite_differences():
+@pytest.mark.parametrize("huber", [0.0, 0.05])
+def test_gradients_match_finite_differences(huber):
     p, rec, y = _case()
-    loss, g = D.gradients(p, rec, y, emulate=False)
+    loss, g = D.gradients(p, rec, y, emulate=False, huber=huber)
     flat = D.flatten(p)
     gflat = D.flatten(g)
     rng = np.random.default_rng(9)
@@ -57,13 +58,13 @@ def test_gradients_match_finite_differences():
         fp[i] += h
         fm = flat.copy()
         fm[i] -= h
-        lp = D.loss_fn(D.forward(D.unflatten(fp), rec, False)[0], y)
-        lm = D.loss_fn(D.forward(D.unflatten(fm), rec, False)[0], y)
+        lp = D.loss_fn(D.forward(D.unflatten(fp), rec, False)[0], y, huber)
+        lm = D.loss_fn(D.forward(D.unflatten(fm), rec, False)[0], y, huber)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - gflat[i]) < 1e-6 * max(1.0, abs(fd)), (i, fd, gflat[i])
 
 
-def _torch_net(p, rec, y):
+def _torch_net(p, rec, y, huber):
     import torch
     import torch.nn.functional as F
     t = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in p.items()}
@@ -78,15 +79,25 @@ def _torch_net(p, rec, y):
     out = h @ t["fc2.w"] + t["fc2.b"]
     yt = torch.tensor(y)
     n = lambda v: torch.sqrt((v * v).sum(-1) + D.NORM_EPS)
-    loss = ((n(out) - n(yt)) ** 2 + D.MU * (n(yt) - (out * yt).sum(-1) / n(out))).mean()
+    def S(r):
+        c = torch.clamp(r, max=D.R_MAX)
+        a, b = 0.5 * (1 + c), 0.5 * (1 - c)
+        return -a * torch.log(a) - b * torch.log(b) - float(np.arctanh(D.R_MAX)) * torch.clamp(r - D.R_MAX, min=0.0)
+    d = S(n(out)) - S(n(yt))
+    h = d * d if huber <= 0 else F.smooth_l1_loss(d, torch.zeros_like(d), reduction="none", beta=huber)
+    loss = (h + D.MU * (n(yt) - (out * yt).sum(-1) / n(out))).mean()
     loss.backward()
     return float(loss.detach()), {k: v.grad.numpy() for k, v in t.items()}, out.detach().numpy()
 
 
-def test_gradients_match_torch_autograd():
+@pytest.mark.parametrize("fc2_scale", [1.0, 0.3])  # |out| above and below R_MAX
+@pytest.mark.parametrize("huber", [0.0, 0.05])
+def test_gradients_match_torch_autograd(fc2_scale, huber):
     p, rec, y = _case(B=4, H=6, W=5, seed=2)
-    loss, g = D.gradients(p, rec, y, emulate=False)
-    tl, tg, tout = _torch_net(p, rec, y)
+    p["fc2.w"] = p["fc2.w"] * fc2_scale
+    p["fc2.b"] = p["fc2.b"] * fc2_scale
+    loss, g = D.gradients(p, rec, y, emulate=False, huber=huber)
+    tl, tg, tout = _torch_net(p, rec, y, huber)
     out, _ = D.forward(p, rec, False)
     assert np.allclose(out, tout, atol=1e-12)
     assert abs(loss - tl) < 1e-12

@@ -54,17 +54,19 @@ def test_decoder_forward_and_gradients_match_oracle(dec):
     want, _ = D.forward(P, rec, emulate=True)
     assert _rel(out, want) < 1e-3
     idx = np.array([3, 1, 4, 0, 6, 2, 7, 5])
-    loss, g = d.gradients(idx)
-    lo, go = D.gradients(P, rec[idx], lab[idx].astype(np.float64), emulate=True)
-    assert abs(loss - lo) < 1e-3 * abs(lo) + 1e-6
-    gdev = D.unflatten(g.astype(np.float64))
-    errs = {k: _rel(gdev[k], go[k]) for k in go}
-    print("relative gradient errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
-    for k, e in errs.items():
-        assert e < 2e-2, (k, e)
+    for huber in (0.05, 0.0):
+        d.set_loss(huber)
+        loss, g = d.gradients(idx)
+        lo, go = D.gradients(P, rec[idx], lab[idx].astype(np.float64), emulate=True, huber=huber)
+        assert abs(loss - lo) < 1e-3 * abs(lo) + 1e-6, (huber, loss, lo)
+        gdev = D.unflatten(g.astype(np.float64))
+        errs = {k: _rel(gdev[k], go[k]) for k in go}
+        print(f"huber {huber}: relative gradient errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+        for k, e in errs.items():
+            assert e < 2e-2, (huber, k, e)
     # a transposed or shifted layout would be O(1) off: the unrounded oracle
     # stays close too, the rounding points only move the last digits
-    _, gn = D.gradients(P, rec[idx], lab[idx].astype(np.float64), emulate=False)
+    _, gn = D.gradients(P, rec[idx], lab[idx].astype(np.float64), emulate=False, huber=0.0)
     assert _rel(gdev["conv1.w"], gn["conv1.w"]) < 0.1
 
 
