@@ -140,6 +140,10 @@ typedef struct mipt_hip_traj_spec
                            the start of layer p_zero_layers (cpp/include/mipt/probed/kernels.hpp:68-77) */
     int p_zero_layers;  /* the first p_zero_layers layers are unmeasured (scrambling) */
     int shared_gates;   /* -1: own gates; else use trajectory shared_gates's gate stream */
+    int grid_x;         /* 0: chain; else an L = grid_x * (L / grid_x) grid, site r*grid_x + c, with four
+                           bond colours per time step (layer mod 4): horizontal from even / odd columns
+                           (odd colour closes each row when closed and grid_x is even), vertical from
+                           staggered even / odd rows (cpp/include/mipt/circuits/mms.hpp:159-330) */
 } mipt_hip_traj_spec;
 
 typedef struct mipt_hip_batch_config

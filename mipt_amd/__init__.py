@@ -268,15 +268,33 @@ def stream_copy_calibration(log2_elems: int = 25) -> int:
     return nbytes
 
 
-def layer_bonds(n: int, layer: int, closed: bool = True):
-    """Brickwork bonds of a layer (cpp/include/mipt/circuits/haar.hpp:226-243)."""
-    start = layer & 1
-    stop = n if (closed and start == 1 and n > 2) else n - 1
-    return [(i, (i + 1) % n) for i in range(start, stop, 2)]
+def layer_bonds(n: int, layer: int, closed: bool = True, grid_x: int = 0):
+    """Bonds (q0, q1) of a layer in gate-stream order: the brickwork chain
+    (cpp/include/mipt/circuits/haar.hpp:226-243), or with grid_x > 0 the four
+    colours of an n = grid_x * (n / grid_x) grid (site r*grid_x + c): 0/1
+    horizontal from even/odd columns row by row (1 closes each row when
+    closed and grid_x is even), 2/3 vertical in each column from rows
+    ((c % 2) + colour - 2) % 2 in steps of 2."""
+    if grid_x <= 0:
+        start = layer & 1
+        stop = n if (closed and start == 1 and n > 2) else n - 1
+        return [(i, (i + 1) % n) for i in range(start, stop, 2)]
+    gx, gy, colour = grid_x, n // grid_x, layer % 4
+    out = []
+    if colour < 2:
+        for r in range(gy):
+            out += [(r * gx + c, r * gx + c + 1) for c in range(colour, gx - 1, 2)]
+            if colour == 1 and closed and gx % 2 == 0 and gx > 2:
+                out.append((r * gx + gx - 1, r * gx))
+        return out
+    for c in range(gx):
+        for r in range(((c % 2) + colour - 2) % 2, gy - 1, 2):
+            out.append((r * gx + c, (r + 1) * gx + c))
+    return out
 
 
 def run_plain_trajectory(engine: PlainEngine, n: int, gates, flags, uniforms, closed: bool = True,
-                         block_targets: int = 4, mode: int = 0, chunk_bits: int = 10):
+                         block_targets: int = 4, mode: int = 0, chunk_bits: int = 10, grid_x: int = 0):
     """The try_cusv loop (cpp/include/mipt/circuit.hpp:240-282) on the plain
     engine: per layer, blocked bond gates, then the measured sites.
     gates: complex128 [bonds,4,4]; flags/uniforms: [layers, n].  Returns outcomes [layers, n] (-1 unmeasured)."""
@@ -285,7 +303,7 @@ def run_plain_trajectory(engine: PlainEngine, n: int, gates, flags, uniforms, cl
     outcomes = np.full((layers, n), -1, np.int32)
     g = 0
     for t in range(layers):
-        bonds = layer_bonds(n, t, closed)
+        bonds = layer_bonds(n, t, closed, grid_x)
         engine.apply_bonds(bonds, gates[g:g + len(bonds)], block_targets)
         g += len(bonds)
         sites = np.nonzero(flags[t])[0]
@@ -300,7 +318,7 @@ def run_plain_trajectory(engine: PlainEngine, n: int, gates, flags, uniforms, cl
 class TrajSpec(ctypes.Structure):
     _fields_ = [("L", ctypes.c_int), ("layers", ctypes.c_int), ("p", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("closed", ctypes.c_int), ("gate_family", ctypes.c_int), ("probe_site", ctypes.c_int),
-                ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int)]
+                ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int), ("grid_x", ctypes.c_int)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -317,10 +335,11 @@ class Spec:
     probe_site: int = -1
     p_zero_layers: int = 0
     shared_gates: int = -1
+    grid_x: int = 0
 
     def to_c(self) -> TrajSpec:
         return TrajSpec(self.L, self.layers, float(self.p), self.seed & (2**64 - 1), int(self.closed), 1,
-                        self.probe_site, self.p_zero_layers, self.shared_gates)
+                        self.probe_site, self.p_zero_layers, self.shared_gates, self.grid_x)
 
 
 class Batch:
@@ -398,7 +417,7 @@ class Batch:
         s = self.specs[traj]
         nb = 0
         for t in range(s.layers):
-            nb += len(layer_bonds(s.L, t, s.closed))
+            nb += len(layer_bonds(s.L, t, s.closed, s.grid_x))
         g = np.zeros((nb, 4, 4), np.complex128)
         f = np.zeros((s.layers, s.L), np.uint8)
         u = np.zeros((s.layers, s.L), np.float64)

@@ -58,16 +58,17 @@ __global__ void gen_gates_kernel(const TrajDesc *__restrict__ desc, const int *_
     const TrajDesc d = desc[gen_list[lo]];
     const std::uint64_t local = id - item_prefix[lo];  // gate index within the trajectory
     // locate (layer, bond)
-    const int nb_even = layer_bond_count(d.L, 0, d.closed);
-    const int nb_odd = layer_bond_count(d.L, 1, d.closed);
-    const std::uint64_t per_period = static_cast<std::uint64_t>(nb_even + nb_odd);
+    const int P = period_layers(d.gx);
+    const std::uint64_t per_period = geo_gate_prefix(d.L, d.gx, d.closed, P);
     const std::uint64_t period = local / per_period;
     std::uint64_t rem = local % per_period;
-    int layer = static_cast<int>(2 * period);
-    if (rem >= static_cast<std::uint64_t>(nb_even))
+    int layer = static_cast<int>(P * period);
+    for (int j = 0; j < P; ++j)
     {
-        rem -= nb_even;
-        layer += 1;
+        const std::uint64_t c = static_cast<std::uint64_t>(geo_bond_count(d.L, d.gx, j, d.closed));
+        if (rem < c) break;
+        rem -= c;
+        ++layer;
     }
     double qr[16], qi[16];
     haar_u4_philox(d.seed, static_cast<std::uint32_t>(layer), static_cast<std::uint32_t>(rem), qr, qi);
@@ -194,12 +195,7 @@ __device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
     return i == 0 ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
 }
 
-__device__ inline std::uint64_t gate_prefix(const TrajDesc &d, int t)
-{
-    const int nb_even = layer_bond_count(d.L, 0, d.closed);
-    const int nb_odd = layer_bond_count(d.L, 1, d.closed);
-    return static_cast<std::uint64_t>((t + 1) / 2) * nb_even + static_cast<std::uint64_t>(t / 2) * nb_odd;
-}
+__device__ inline std::uint64_t gate_prefix(const TrajDesc &d, int t) { return geo_gate_prefix(d.L, d.gx, d.closed, t); }
 
 __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     DevArena a, const int *__restrict__ traj_list, int layer_begin, int layer_end, int init)
@@ -219,12 +215,12 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     for (int t = layer_begin; t < layer_end; ++t)
     {
         if (d.probe_site >= 0 && d.p_zero_layers > 0 && t == d.p_zero_layers) attach_probe(st, dim, d, red, status);
-        const int nb = layer_bond_count(d.L, t, d.closed);
+        const int nb = geo_bond_count(d.L, d.gx, t, d.closed);
         const std::uint64_t gbase = d.gate_off + gate_prefix(d, t);
         for (int b = 0; b < nb; ++b)
         {
             int q0, q1;
-            layer_bond(d.L, t, b, q0, q1);
+            geo_bond(d.L, d.gx, t, b, d.closed, q0, q1);
             const float2 *g = a.gates + (gbase + b) * 16;
             float ur[16], ui[16];
             for (int e = 0; e < 16; ++e)
@@ -1055,7 +1051,7 @@ void pick_lanes(std::vector<int> &freeb, int lane_u[6], int count)
     freeb.swap(rest);
 }
 
-int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, LayerPlan &out)
+int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config &cf, LayerPlan &out)
 {
     out.passes.clear();
     const int k = cf.k();
@@ -1063,8 +1059,15 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
     const int wbits = k - c;
     const int wb = cf.nt == 256 ? 2 : (cf.nt == 128 ? 1 : 0);
     if (nq <= k) return E_UNSUPPORTED;
-    // windows over the physical bits c..nq-1, consecutive windows overlapping
-    // by one bit so that every bond (i, i+1) lies in some window
+    const int nb = geo_bond_count(L, gx, layer, closed);
+    std::vector<int> b0(nb), b1(nb);
+    for (int b = 0; b < nb; ++b) geo_bond(L, gx, layer, b, closed, b0[b], b1[b]);
+    auto in_window = [&](int s, int q) { return q < c || (q >= s && q < s + wbits); };
+    // windows over the physical bits c..nq-1: for a chain, consecutive
+    // windows overlapping by one bit, so that every bond (i, i+1) lies in
+    // some window; when a layout leaves a bond outside all of them (grid
+    // columns), a greedy cover: the window start holding most bonds not yet
+    // covered, until every bond is covered
     std::vector<int> starts;
     int st = c;
     while (true)
@@ -1077,10 +1080,46 @@ int build_layer_plan(int L, int nq, int closed, int parity, const Config &cf, La
         starts.push_back(st);
         st = st + wbits - 1;
     }
-    const int nb = layer_bond_count(L, parity, closed);
-    std::vector<int> b0(nb), b1(nb);
-    for (int b = 0; b < nb; ++b) layer_bond(L, parity, b, b0[b], b1[b]);
-    auto local = [&](int p, int q) { return q < c || (q >= starts[p] && q < starts[p] + wbits); };
+    auto covered_by_any = [&](int b) {
+        for (int s0 : starts)
+            if (in_window(s0, b0[b]) && in_window(s0, b1[b])) return true;
+        return false;
+    };
+    bool all = true;
+    for (int b = 0; b < nb && all; ++b) all = covered_by_any(b);
+    if (!all)
+    {
+        starts.clear();
+        std::vector<char> done(nb, 0);
+        int left = nb;
+        while (left > 0)
+        {
+            int best_s = -1, best_n = 0;
+            for (int s0 = c; s0 + wbits <= nq; ++s0)
+            {
+                int n_here = 0;
+                for (int b = 0; b < nb; ++b) n_here += !done[b] && in_window(s0, b0[b]) && in_window(s0, b1[b]);
+                if (n_here > best_n)
+                {
+                    best_n = n_here;
+                    best_s = s0;
+                }
+            }
+            if (best_s < 0)
+            {
+                set_error("tile plan: a bond of layer %d is local to no window (L=%d k=%d)", layer, L, k);
+                return E_UNSUPPORTED;
+            }
+            starts.push_back(best_s);
+            for (int b = 0; b < nb; ++b)
+                if (!done[b] && in_window(best_s, b0[b]) && in_window(best_s, b1[b]))
+                {
+                    done[b] = 1;
+                    --left;
+                }
+        }
+    }
+    auto local = [&](int p, int q) { return in_window(starts[p], q); };
     std::vector<std::vector<int>> assigned(starts.size());
     // bonds local to exactly one window first, then balance the rest
     std::vector<int> flexible;
@@ -1349,8 +1388,10 @@ struct Batch
         {
             const auto &s = specs[i];
             // up to 2^27 amplitudes per state (1 GiB): tile byte offsets stay below 2^31
-            if (s.L < 2 || s.L > 26 || (s.L % 2) != 0 || s.layers < 0 || s.gate_family != 1 ||
-                (s.probe_site >= s.L) ||
+            const bool grid = s.grid_x > 0;
+            if (s.L < 2 || s.L > 26 || (!grid && (s.L % 2) != 0) || s.layers < 0 || s.gate_family != 1 ||
+                (s.probe_site >= s.L) || s.grid_x < 0 ||
+                (grid && (s.grid_x < 2 || (s.L % s.grid_x) != 0 || s.L / s.grid_x < 2)) ||
                 (s.shared_gates >= 0 && (s.shared_gates >= count || s.shared_gates == i)))
             {
                 set_error("batch_setup: unsupported spec %d (L=%d layers=%d family=%d)", i, s.L, s.layers, s.gate_family);
@@ -1361,6 +1402,8 @@ struct Batch
             d.nq = s.L + (s.probe_site >= 0 ? 1 : 0);
             d.layers = s.layers;
             d.closed = s.closed ? 1 : 0;
+            d.gx = s.grid_x;
+            d.pad = 0;
             d.probe_site = s.probe_site;
             d.p_zero_layers = s.p_zero_layers;
             d.p = s.p;
@@ -1375,7 +1418,7 @@ struct Batch
             {
                 d.gate_off = go;
                 std::size_t nb = 0;
-                for (int t = 0; t < s.layers; ++t) nb += layer_bond_count(s.L, t, d.closed);
+                for (int t = 0; t < s.layers; ++t) nb += geo_bond_count(s.L, d.gx, t, d.closed);
                 go += nb;
                 gen_list.push_back(i);
             }
@@ -1384,7 +1427,8 @@ struct Batch
             if (specs[i].shared_gates >= 0)
             {
                 const auto &o = specs[specs[i].shared_gates];
-                if (o.shared_gates >= 0 || o.L != specs[i].L || o.layers < specs[i].layers || o.closed != specs[i].closed)
+                if (o.shared_gates >= 0 || o.L != specs[i].L || o.layers < specs[i].layers || o.closed != specs[i].closed ||
+                    o.grid_x != specs[i].grid_x)
                 {
                     set_error("batch_setup: shared_gates of %d must name an owning trajectory of the same shape", i);
                     return E_ARG;
@@ -1423,6 +1467,7 @@ struct Batch
             if (a0.nq != b0.nq) return a0.nq < b0.nq;
             if (a0.L != b0.L) return a0.L < b0.L;
             if (a0.closed != b0.closed) return a0.closed < b0.closed;
+            if (a0.gx != b0.gx) return a0.gx < b0.gx;
             return a0.layers < b0.layers;
         });
         MIPT_HIP_CHECK(hipMalloc(&d_traj_list, count * sizeof(int)));
@@ -1444,7 +1489,7 @@ struct Batch
         {
             const TrajDesc &d = desc[gen_list[i]];
             std::size_t nb = 0;
-            for (int t = 0; t < d.layers; ++t) nb += layer_bond_count(d.L, t, d.closed);
+            for (int t = 0; t < d.layers; ++t) nb += geo_bond_count(d.L, d.gx, t, d.closed);
             item_prefix[i + 1] = item_prefix[i] + nb;
         }
         std::vector<std::uint64_t> layer_prefix(count + 1, 0);
@@ -1490,7 +1535,7 @@ struct Batch
         }
         const TrajDesc &d = desc[traj];
         std::size_t nb = 0;
-        for (int t = 0; t < d.layers; ++t) nb += layer_bond_count(d.L, t, d.closed);
+        for (int t = 0; t < d.layers; ++t) nb += geo_bond_count(d.L, d.gx, t, d.closed);
         std::vector<float2> gf(nb * 16);
         for (std::size_t i = 0; i < nb * 16; ++i)
             gf[i] = make_float2(static_cast<float>(g[2 * i]), static_cast<float>(g[2 * i + 1]));
@@ -1584,9 +1629,9 @@ struct Batch
     {
         const TrajDesc &d0 = desc[grp[0]];
         const Config cf = config_for(d0.nq);
-        LayerPlan lp[2];
-        MIPT_HIP_TRY(build_layer_plan(d0.L, d0.nq, d0.closed, 0, cf, lp[0]));
-        MIPT_HIP_TRY(build_layer_plan(d0.L, d0.nq, d0.closed, 1, cf, lp[1]));
+        const int P = period_layers(d0.gx);
+        LayerPlan lp[4];
+        for (int j = 0; j < P; ++j) MIPT_HIP_TRY(build_layer_plan(d0.L, d0.nq, d0.closed, d0.gx, j, cf, lp[j]));
         const int ntraj = static_cast<int>(grp.size());
         const int *list = d_traj_list + grp_pos;
         MIPT_HIP_TRY(ensure_partials(static_cast<std::size_t>(ntraj) * (std::size_t{1} << (d0.nq - cf.k())) * HSTRIDE,
@@ -1599,7 +1644,7 @@ struct Batch
         for (int i : grp) attach_any |= desc[i].probe_site >= 0 && desc[i].p_zero_layers > 0;
         for (int t = t0; t < t1; ++t)
         {
-            const LayerPlan &P = lp[t & 1];
+            const LayerPlan &LP = lp[t % P];
             if (attach_any && t > 0)
             {
                 bool at = false;
@@ -1613,29 +1658,29 @@ struct Batch
             int maxm = 0;
             for (int i : grp) maxm = std::max(maxm, __builtin_popcount(h_flags[desc[i].layer_off + t]));
             const int chunks = (maxm + MAX_HIST_BITS - 1) / MAX_HIST_BITS;
-            for (std::size_t p = 0; p < P.passes.size(); ++p)
+            for (std::size_t p = 0; p < LP.passes.size(); ++p)
             {
-                PassPlan pl = P.passes[p];
+                PassPlan pl = LP.passes[p];
                 pl.collapse = (p == 0) ? 1 : 0;
                 pl.init = (p == 0 && t == 0) ? 1 : 0;
-                pl.hist = (p + 1 == P.passes.size()) ? 1 : 0;  // chunk 0 always: it also records m == 0
+                pl.hist = (p + 1 == LP.passes.size()) ? 1 : 0;  // chunk 0 always: it also records m == 0
                 pl.chunk = 0;
                 pl.cond = 0;
                 pl.store = 1;
                 // the collapse sweep skips zero-branch tiles when a later sweep
                 // of the layer re-applies the branch on the sites it leaves alone
-                pl.zskip = (p == 0 && P.passes.size() > 1) ? 1 : 0;
-                pl.pmask = (p == 1) ? (((std::uint64_t{1} << d0.nq) - 1) & ~P.passes[0].bonded) : 0;
+                pl.zskip = (p == 0 && LP.passes.size() > 1) ? 1 : 0;
+                pl.pmask = (p == 1) ? (((std::uint64_t{1} << d0.nq) - 1) & ~LP.passes[0].bonded) : 0;
                 // algorithmic bytes: one read + one write of the states per
                 // layer, shared among the layer's sweeps
                 const double bytes = 16.0 * static_cast<double>(std::size_t{1} << d0.nq) * ntraj /
-                                     static_cast<double>(P.passes.size());
+                                     static_cast<double>(LP.passes.size());
                 int slot;
                 MIPT_HIP_TRY(timed_launch_begin(bytes, slot));
                 MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
                 MIPT_HIP_TRY(timed_launch_end(slot));
             }
-            const PassPlan &last = P.passes.back();
+            const PassPlan &last = LP.passes.back();
             const Geom geo{last.rest_mask, last.ntiles, last.k, last.c, last.s};
             FinalizeArgs fa{al, list, t, 0, geo};
             hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, ls, fa);
@@ -1659,7 +1704,7 @@ struct Batch
         // apply the last layer's pending collapse
         if (t1 > t0)
         {
-            PassPlan pl = lp[(t1 - 1) & 1].passes[0];
+            PassPlan pl = lp[(t1 - 1) % P].passes[0];
             pl.nstages = 0;
             pl.collapse = 1;
             pl.init = 0;
@@ -1697,7 +1742,7 @@ struct Batch
             const int nq = desc[order[pos]].nq;
             int end = pos;
             while (end < count && desc[order[end]].nq == nq && desc[order[end]].L == desc[order[pos]].L &&
-                   desc[order[end]].closed == desc[order[pos]].closed &&
+                   desc[order[end]].closed == desc[order[pos]].closed && desc[order[end]].gx == desc[order[pos]].gx &&
                    (t1_req >= 0 || desc[order[end]].layers == desc[order[pos]].layers))
                 ++end;
             const int t1 = t1_req >= 0 ? t1_req : desc[order[pos]].layers;
@@ -1772,7 +1817,7 @@ struct Batch
         if (traj < 0 || traj >= static_cast<int>(desc.size())) { set_error("export_layers: bad trajectory"); return E_ARG; }
         const TrajDesc &d = desc[traj];
         std::size_t nb = 0;
-        for (int t = 0; t < d.layers; ++t) nb += layer_bond_count(d.L, t, d.closed);
+        for (int t = 0; t < d.layers; ++t) nb += geo_bond_count(d.L, d.gx, t, d.closed);
         if (g)
         {
             std::vector<float2> gf(nb * 16);

@@ -22,6 +22,7 @@ struct TrajDesc
     std::uint32_t unif_off;   // offset into per-(layer, site) arrays (uniforms, p1)
     int L, nq, layers, closed;
     int probe_site, p_zero_layers;
+    int gx, pad;              // grid columns (0: chain)
     double p;
     std::uint64_t seed;
 };

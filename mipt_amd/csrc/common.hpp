@@ -193,4 +193,90 @@ __host__ __device__ inline void layer_bond(int n, int layer, int b, int &q0, int
     q1 = (q0 + 1) % n;
 }
 
+// Bond layouts by geometry.  gx = 0: the brickwork chain above (two layers
+// per period).  gx > 0: an n = gx * gy grid, site r*gx + c, four bond colours
+// per time step (layer mod 4), the layering of the reference's 2D circuit
+// (cpp/include/mipt/circuits/mms.hpp:159-330) with Haar bonds:
+//   0: horizontal bonds (c, c+1) from even columns, row by row;
+//   1: from odd columns, plus the row closure (r*gx + gx-1, r*gx) when closed
+//      and gx is even (an odd gx would give the last site two bonds);
+//   2, 3: vertical bonds (r, r+1) in column c from rows
+//      ((c mod 2) + colour - 2) mod 2 in steps of 2, column by column.
+__host__ __device__ inline int period_layers(int gx) { return gx > 0 ? 4 : 2; }
+
+__host__ __device__ inline int vertical_start(int c, int colour) { return ((c & 1) + colour - 2) & 1; }
+
+__host__ __device__ inline int geo_bond_count(int n, int gx, int layer, int closed)
+{
+    if (gx <= 0) return layer_bond_count(n, layer, closed);
+    const int gy = n / gx, colour = layer & 3;
+    if (colour < 2)
+    {
+        const int per_row = (gx - colour) / 2 + ((colour == 1 && closed && (gx % 2) == 0 && gx > 2) ? 1 : 0);
+        return per_row * gy;
+    }
+    int total = 0;
+    for (int c = 0; c < gx; ++c)
+    {
+        const int s = vertical_start(c, colour);
+        if (gy - 2 >= s) total += (gy - 2 - s) / 2 + 1;
+    }
+    return total;
+}
+
+__host__ __device__ inline void geo_bond(int n, int gx, int layer, int b, int closed, int &q0, int &q1)
+{
+    if (gx <= 0)
+    {
+        layer_bond(n, layer, b, q0, q1);
+        return;
+    }
+    const int gy = n / gx, colour = layer & 3;
+    if (colour < 2)
+    {
+        const int regular = (gx - colour) / 2;
+        const int per_row = regular + ((colour == 1 && closed && (gx % 2) == 0 && gx > 2) ? 1 : 0);
+        const int r = b / per_row, k = b % per_row;
+        if (k < regular)
+        {
+            q0 = r * gx + colour + 2 * k;
+            q1 = q0 + 1;
+        }
+        else
+        {
+            q0 = r * gx + gx - 1;
+            q1 = r * gx;
+        }
+        return;
+    }
+    for (int c = 0; c < gx; ++c)
+    {
+        const int s = vertical_start(c, colour);
+        const int cnt = (gy - 2 >= s) ? (gy - 2 - s) / 2 + 1 : 0;
+        if (b < cnt)
+        {
+            const int r = s + 2 * b;
+            q0 = r * gx + c;
+            q1 = (r + 1) * gx + c;
+            return;
+        }
+        b -= cnt;
+    }
+    q0 = q1 = -1;  // unreachable for b < geo_bond_count
+}
+
+// Gates of layers [0, t) of a trajectory (the gate stream's layer offset).
+__host__ __device__ inline std::uint64_t geo_gate_prefix(int n, int gx, int closed, int t)
+{
+    const int P = period_layers(gx);
+    std::uint64_t per = 0, part = 0;
+    for (int j = 0; j < P; ++j)
+    {
+        const int c = geo_bond_count(n, gx, j, closed);
+        per += static_cast<std::uint64_t>(c);
+        if (j < t % P) part += static_cast<std::uint64_t>(c);
+    }
+    return static_cast<std::uint64_t>(t / P) * per + part;
+}
+
 } // namespace mipt_hip

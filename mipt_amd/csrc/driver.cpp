@@ -254,6 +254,7 @@ int run_sim_tmi(int argc, char **argv)
             s.probe_site = -1;
             s.p_zero_layers = 0;
             s.shared_gates = -1;
+            s.grid_x = 0;
         }
         std::vector<double> tmi(static_cast<std::size_t>(count) * cycles);
         std::vector<mipt_hip_traj_result> r(count);

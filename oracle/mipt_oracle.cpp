@@ -387,7 +387,39 @@ void rdm(const cd *psi, int n, const int *kept, int count, cd *rho)
 } // namespace
 
 // =================================================================== C ABI
+// Haar U(4) of (layer, bond) on the counter stream: Box-Muller normals (cos
+// then sin of each uniform pair, Re before Im), Gram-Schmidt as above.
+void philox_haar(std::uint64_t seed, std::uint32_t t, std::uint32_t b, cd q[16])
+{
+    std::uint32_t slot = 0;
+    int have = 0;
+    double saved = 0.0;
+    auto normal = [&]() {
+        if (have) { have = 0; return saved; }
+        double u, v;
+        philox_uniform_pair(seed, t, b, slot++, STREAM_GATE, u, v);
+        const double r = std::sqrt(-2.0 * std::log(1.0 - u));
+        const double a = 6.283185307179586476925286766559 * v;
+        saved = r * std::sin(a);
+        have = 1;
+        return r * std::cos(a);
+    };
+    haar_u4_gram_schmidt(normal, q);
+}
+
 extern "C" {
+
+// One gate of the counter stream, complex128 row-major interleaved (32 doubles).
+void orc_philox_haar(std::uint64_t seed, std::uint32_t layer, std::uint32_t bond, double *out)
+{
+    cd q[16];
+    philox_haar(seed, layer, bond, q);
+    for (int e = 0; e < 16; ++e)
+    {
+        out[2 * e] = q[e].real();
+        out[2 * e + 1] = q[e].imag();
+    }
+}
 
 std::uint64_t orc_splitmix64(std::uint64_t v) { return splitmix64(v); }
 
@@ -493,21 +525,8 @@ void orc_philox_build_layers(int n, int layers, double p, int closed, std::uint6
         const int nb = layer_bonds(n, t & 1, closed, nullptr, nullptr);
         for (int b = 0; b < nb; ++b, ++g)
         {
-            std::uint32_t slot = 0;
-            int have = 0;
-            double saved = 0.0;
-            auto normal = [&]() {
-                if (have) { have = 0; return saved; }
-                double u, v;
-                philox_uniform_pair(seed, static_cast<std::uint32_t>(t), static_cast<std::uint32_t>(b), slot++, STREAM_GATE, u, v);
-                const double r = std::sqrt(-2.0 * std::log(1.0 - u));
-                const double a = 6.283185307179586476925286766559 * v;
-                saved = r * std::sin(a);
-                have = 1;
-                return r * std::cos(a);
-            };
             cd q[16];
-            haar_u4_gram_schmidt(normal, q);
+            philox_haar(seed, static_cast<std::uint32_t>(t), static_cast<std::uint32_t>(b), q);
             for (int e = 0; e < 16; ++e)
             {
                 gates[g * 32 + 2 * e] = q[e].real();

@@ -50,6 +50,8 @@ def lib():
         L = ctypes.CDLL(path)
         L.orc_splitmix64.restype = ctypes.c_uint64
         L.orc_splitmix64.argtypes = [ctypes.c_uint64]
+        L.orc_philox_haar.restype = None
+        L.orc_philox_haar.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _dp]
         L.orc_philox_uniform.restype = ctypes.c_double
         L.orc_philox_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32]
@@ -363,3 +365,63 @@ def probe_trajectory(n: int, layers: int, p: float, seed: int, probe_site: int, 
                 outs[t, s] = o
                 p1[t, s] = pr if o == 1 else 1.0 - pr
     return psi, outs, p1, att, (g, f, u)
+
+
+# ------------------------------------------------------------------ 2D grid
+STREAM_FLAG = 0x464C4147
+STREAM_MEAS = 0x4D454153
+
+
+def grid_layer_bonds(gx: int, gy: int, layer: int, closed: bool = False):
+    """Bonds of one colour layer of a gx x gy grid (site r*gx + c), in
+    gate-stream order: colour 0/1 = horizontal bonds from even/odd columns,
+    row by row (colour 1 closes each row with (r*gx+gx-1, r*gx) when closed
+    and gx is even); colour 2/3 = vertical bonds, column by column, from rows
+    ((c % 2) + colour - 2) % 2 in steps of 2 (the staggered four-colour
+    layering of cpp/include/mipt/circuits/mms.hpp:159-330)."""
+    colour = layer % 4
+    bonds = []
+    if colour in (0, 1):
+        for r in range(gy):
+            c = colour
+            while c + 1 < gx:
+                bonds.append((r * gx + c, r * gx + c + 1))
+                c += 2
+            if colour == 1 and closed and gx % 2 == 0 and gx > 2:
+                bonds.append((r * gx + gx - 1, r * gx))
+    else:
+        for c in range(gx):
+            r = ((c % 2) + colour - 2) % 2
+            while r + 1 < gy:
+                bonds.append((r * gx + c, (r + 1) * gx + c))
+                r += 2
+    return bonds
+
+
+def simulate_grid(gx: int, gy: int, layers: int, p: float, seed: int, closed: bool = False):
+    """One 2D trajectory from |0...0> on the counter stream: gates
+    orc_philox_haar(seed, layer, bond), flags Bernoulli(p) and uniforms per
+    (layer, site) as for the chain; each colour layer is followed by the
+    sequential measurement of its flagged sites (ascending).
+    Returns (psi, outcomes [layers, n], p1 [layers, n], flags, uniforms)."""
+    n = gx * gy
+    psi = zero_state(n)
+    outs = np.full((layers, n), -1, np.int32)
+    p1 = np.zeros((layers, n))
+    flags = np.zeros((layers, n), np.uint8)
+    unif = np.zeros((layers, n))
+    g = np.zeros(32)
+    for t in range(layers):
+        for b, (q0, q1) in enumerate(grid_layer_bonds(gx, gy, t, closed)):
+            lib().orc_philox_haar(seed & (2**64 - 1), t, b, g.ctypes.data_as(_dp))
+            apply_dense(psi, n, [q0, q1], (g[0::2] + 1j * g[1::2]).reshape(4, 4))
+        for s in range(n):
+            a = philox_uniform(seed, t, s, 0, STREAM_FLAG)
+            f = p >= 1.0 or (0.0 < p and a < p)
+            if f:
+                flags[t, s] = 1
+                unif[t, s] = philox_uniform(seed, t, s, 0, STREAM_MEAS)
+                st, o, pr = measure_collapse(psi, n, s, unif[t, s])
+                outs[t, s] = o
+                p1[t, s] = pr if o == 1 else 1.0 - pr
+    return psi, outs, p1, flags, unif

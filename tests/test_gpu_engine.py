@@ -340,3 +340,32 @@ def test_batch_probe_attach_after_scrambling(M):
         got, _, _, _ = b2.outcomes(i)
         assert np.array_equal(rec[i], got[4:10].astype(np.int8))
         assert np.array_equal(got, b.outcomes(i + 2)[0])
+
+
+def test_batch_grid_matches_oracle(M):
+    # 2D grids with the four-colour layering: 4 rows x 5 columns (20 qubits,
+    # tile sweeps with windows chosen to cover the vertical bonds), 3 x 4
+    # (small kernel) and 5 x 4 with closed rows
+    cases = [(5, 4, 8, 0.12, 2001, False), (4, 3, 8, 0.2, 2002, False), (4, 5, 6, 0.3, 2003, True)]
+    specs = [M.Spec(gx * gy, layers, p, seed, closed, grid_x=gx) for (gx, gy, layers, p, seed, closed) in cases]
+    print("batch 2D grids", flush=True)
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    for i, (gx, gy, layers, p, seed, closed) in enumerate(cases):
+        n = gx * gy
+        psi, outs, p1, f, u = O.simulate_grid(gx, gy, layers, p, seed, closed)
+        got, flagged, _, _ = b.outcomes(i)
+        check_outcomes(got, outs, p1, u, f, f"grid {gx}x{gy}")
+        if not flagged.any():
+            assert np.abs(b.get_state(i) - psi).max() < 1e-5, f"grid {i}"
+            ob = b.observables(i)
+            assert abs(ob["s_half"] - O.half_chain_entropy(psi, n)) < 1e-5
+            assert abs(ob["tmi"] - O.tmi(psi, n)) < 1e-5
+        # the exported draws replay on the plain engine too
+        g, fl, un = b.export_layers(i)
+        eng = M.PlainEngine(False)
+        want = M.run_plain_trajectory(eng, n, g, fl, un, closed, 4, 0, 10, grid_x=gx)
+        assert np.array_equal(want[fl == 1], outs[fl == 1]) or flagged.any()
+        eng.close()

@@ -444,3 +444,37 @@ def test_probe_protocol_oracle_runs_and_purifies():
     # p = 0: the probe stays maximally entangled
     psi, *_ = O.probe_trajectory(8, 12, 0.0, 78, 4, 4)
     assert abs(O.rho1_entropy_nats(O.rho1(psi, 9, 8)) - np.log(2)) < 1e-10
+
+
+@pytest.mark.parametrize("gx,gy,closed", [(5, 4, False), (4, 5, True), (4, 3, False), (2, 3, True)])
+def test_grid_bond_layout(gx, gy, closed):
+    # four colours: each site in at most one bond per layer, the union is
+    # every nearest-neighbour pair of the grid (rows closed when asked and gx
+    # is even), and the engine's host restatement lists the same bonds in the
+    # same (gate-stream) order
+    import mipt_amd
+    n = gx * gy
+    seen = set()
+    for t in range(4):
+        bonds = O.grid_layer_bonds(gx, gy, t, closed)
+        sites = [q for b in bonds for q in b]
+        assert len(sites) == len(set(sites))
+        assert bonds == mipt_amd.layer_bonds(n, t, closed, gx)
+        assert bonds == O.grid_layer_bonds(gx, gy, t + 4, closed)
+        seen |= {tuple(sorted(b)) for b in bonds}
+    want = {(r * gx + c, r * gx + c + 1) for r in range(gy) for c in range(gx - 1)}
+    want |= {(r * gx + c, (r + 1) * gx + c) for r in range(gy - 1) for c in range(gx)}
+    if closed and gx % 2 == 0 and gx > 2:
+        want |= {(r * gx, r * gx + gx - 1) for r in range(gy)}
+    assert seen == want
+
+
+def test_single_gate_stream_matches_layer_builder():
+    g, f, u = O.build_layers(8, 3, 0.2, 5)
+    i = 0
+    x = np.zeros(32)
+    for t in range(3):
+        for b, _ in enumerate(O.layer_bonds(8, t, True)):
+            O.lib().orc_philox_haar(5, t, b, x.ctypes.data_as(O._dp))
+            assert np.array_equal(x[0::2] + 1j * x[1::2], g[i].ravel())
+            i += 1
