@@ -105,6 +105,10 @@ def main(argv=None):
                     help="also time the half-chain RDM entropy at this L (Bell-pair state, S = L/2 exactly)")
     ap.add_argument("--rdm-reps", type=int, default=3)
     ap.add_argument("--no-decoder", action="store_true", help="skip the learned probe decoder leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-length / 2D-grid leg")
+    ap.add_argument("--no-observables", action="store_true", help="skip the S/TMI readout timing")
+    ap.add_argument("--obs-traj", type=int, default=256)
+    ap.add_argument("--uneven-per", type=int, default=512, help="trajectories per (L, p) in the uneven leg")
     ap.add_argument("--decoder-train", type=int, default=200_000)
     ap.add_argument("--decoder-test", type=int, default=20_000)
     ap.add_argument("--decoder-epochs", type=int, default=16)
@@ -254,6 +258,18 @@ def main(argv=None):
         result["cpu_baseline"] = None
     if args.rdm_L:
         result["rdm_gemm"] = rdm_timing(M, args.rdm_L, args.rdm_reps)
+    if not args.no_observables:
+        try:
+            result["observables"] = observables_leg(M, L, layers, ps, args.obs_traj)
+        except Exception as e:
+            result["observables"] = {"error": repr(e)}
+            log(f"observables leg failed: {e!r}")
+    if not args.no_uneven:
+        try:
+            result["uneven"] = uneven_leg(M, hip, args.uneven_per)
+        except Exception as e:
+            result["uneven"] = {"error": repr(e)}
+            log(f"uneven leg failed: {e!r}")
     if not args.no_decoder:
         try:
             result["decoder"] = decoder_leg(args.decoder_train, args.decoder_test, args.decoder_epochs)
@@ -262,6 +278,81 @@ def main(argv=None):
             log(f"decoder leg failed: {e!r}")
     print(json.dumps(result))
     return 0
+
+
+def observables_leg(M, L, layers, ps, n):
+    """S(L/2) and TMI (cycle 0) of n trajectories of the benchmark's shape,
+    RDMs built on the GPU and their eigenvalues solved in batches of one size
+    (mipt_hip_batch_observables_all), against one entropy call at a time."""
+    specs = [M.Spec(L, layers, ps[i % len(ps)], 70_000_000 + i, True) for i in range(n)]
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    b.observables_all(0, 1)  # warm: workspaces, solver handle
+    t = time.perf_counter()
+    sh, tm = b.observables_all(0, 1)
+    dt = time.perf_counter() - t
+    k = min(8, n)
+    t = time.perf_counter()
+    for i in range(k):
+        b.observables(i)
+    dt1 = (time.perf_counter() - t) / k
+    b.close()
+    log(f"observables: batched {1e3 * dt / n:.2f} ms/trajectory, one at a time {1e3 * dt1:.2f} ms/trajectory")
+    return {"trajectories": n, "batched_ms_per_trajectory": round(1e3 * dt / n, 3),
+            "single_ms_per_trajectory": round(1e3 * dt1, 3), "s_half_mean": round(float(np.mean(sh)), 6),
+            "tmi_mean": round(float(np.mean(tm)), 6)}
+
+
+def _timed_evolve(M, hip, specs):
+    """Draw and evolve one batch; returns (GPU ms from events on the batch
+    stream, layer-trajectories)."""
+    b = M.Batch(specs)
+    stream = ctypes.c_void_p(b.stream)
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    hip.hipEventRecord(e0, stream)
+    b.generate()
+    b.evolve()
+    hip.hipEventRecord(e1, stream)
+    hip.hipEventSynchronize(e1)
+    b.synchronize()
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    b.outcomes(0)  # raises if a trajectory hit a zero-weight branch
+    work = sum(s.layers for s in specs)
+    b.close()
+    return ms.value, work
+
+
+def uneven_leg(M, hip, per):
+    """Uneven-size workload: chains L in {14,...,24} with T = 2L and p in
+    {0.10, 0.16, 0.22}, `per` trajectories per (L, p), one batch per L (the
+    engine groups by length inside a batch; one batch per L bounds memory at
+    L=24); plus the 4 x 5 grid (L=20, four bond colours, open) at p = 0.12,
+    1024 trajectories, T = 40.  Throughput in layer-trajectories/s."""
+    out = {"per_L": [], "workload": f"L in 14..24, T=2L, p in (0.10,0.16,0.22) x {per}; 4x5 grid p=0.12 x 1024, T=40"}
+    tot_ms = tot_work = 0.0
+    seed = 50_000_000
+    for L in range(14, 26, 2):
+        specs = []
+        for p in (0.10, 0.16, 0.22):
+            for r in range(per):
+                seed += 1
+                specs.append(M.Spec(L, 2 * L, p, seed, True))
+        ms, work = _timed_evolve(M, hip, specs)
+        tot_ms += ms
+        tot_work += work
+        out["per_L"].append({"L": L, "trajectories": len(specs), "layers_per_s": round(work / (ms * 1e-3), 1)})
+        log(f"uneven leg: L={L} {len(specs)} trajectories, {work / (ms * 1e-3):.0f} layers/s")
+    out["chains_layers_per_s"] = round(tot_work / (tot_ms * 1e-3), 1)
+    specs = [M.Spec(20, 40, 0.12, 60_000_000 + i, False, grid_x=5) for i in range(1024)]
+    ms, work = _timed_evolve(M, hip, specs)
+    out["grid_4x5_layers_per_s"] = round(work / (ms * 1e-3), 1)
+    log(f"uneven leg: 4x5 grid {work / (ms * 1e-3):.0f} layers/s")
+    return out
 
 
 def decoder_leg(n_train, n_test, epochs, batch=1024, lr=1e-3):

@@ -95,6 +95,7 @@ def lib():
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
     L.mipt_hip_batch_records.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_int8)]
+    L.mipt_hip_batch_observables_all.argtypes = [vp, i, i, dp, dp]
     L.mipt_hip_batch_probe_bloch.argtypes = [vp, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
     L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
@@ -468,6 +469,16 @@ class Batch:
         _check(lib().mipt_hip_batch_records(self._h, layer_begin, layer_end,
                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))), "batch_records")
         return out
+
+    def observables_all(self, tmi_block: int = 0, tmi_cycles: int = 1):
+        """(S(L/2) [count], TMI [count, tmi_cycles]) of every trajectory, bits,
+        with the eigensolves of one RDM size batched."""
+        n = len(self.specs)
+        sh = np.zeros(n)
+        tm = np.zeros((n, max(tmi_cycles, 1)))
+        _check(lib().mipt_hip_batch_observables_all(self._h, tmi_block, tmi_cycles, _dp(sh),
+                                                    _dp(tm) if tmi_cycles > 0 else None), "batch_observables_all")
+        return sh, tm[:, :tmi_cycles]
 
     def probe_bloch(self) -> np.ndarray:
         """[count, 3] probe Bloch vectors (NaN rows without a probe)."""

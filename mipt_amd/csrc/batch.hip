@@ -2148,6 +2148,105 @@ int mipt_hip_batch_observables(mipt_hip_batch *b, int traj, int tmi_block, doubl
     return OK;
 }
 
+int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cycles, double *s_half, double *tmi)
+{
+    if (!b || tmi_cycles < 0 || (tmi_cycles > 0 && !tmi))
+    {
+        set_error("batch_observables_all: bad arguments");
+        return E_ARG;
+    }
+    const int count = static_cast<int>(b->desc.size());
+    // groups of one (qubits, chain length): every RDM size is shared inside
+    std::vector<int> idx(count);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+        return b->desc[x].nq != b->desc[y].nq ? b->desc[x].nq < b->desc[y].nq : b->desc[x].L < b->desc[y].L;
+    });
+    for (int g0 = 0; g0 < count;)
+    {
+        int g1 = g0;
+        while (g1 < count && b->desc[idx[g1]].nq == b->desc[idx[g0]].nq && b->desc[idx[g1]].L == b->desc[idx[g0]].L) ++g1;
+        const int nq = b->desc[idx[g0]].nq, L = b->desc[idx[g0]].L, cnt = g1 - g0;
+        std::vector<const float2 *> psis(cnt);
+        for (int j = 0; j < cnt; ++j) psis[j] = b->a.states + b->desc[idx[g0 + j]].state_off;
+        if (s_half)
+        {
+            std::vector<int> kept(static_cast<std::size_t>(cnt) * (L / 2));
+            for (int j = 0; j < cnt; ++j)
+                for (int q = 0; q < L / 2; ++q) kept[static_cast<std::size_t>(j) * (L / 2) + q] = q;
+            std::vector<double> sv(cnt);
+            MIPT_HIP_TRY(entropy_many_bits_f32(psis.data(), nq, kept.data(), L / 2, cnt, b->stream, sv.data()));
+            for (int j = 0; j < cnt; ++j) s_half[idx[g0 + j]] = sv[j];
+        }
+        if (tmi_cycles > 0)
+        {
+            const int bk = tmi_block > 0 ? tmi_block : L / 4;
+            if (L < 4 || bk < 1 || 3 * bk > L || tmi_cycles > L)
+            {
+                set_error("batch_observables_all: bad TMI block %d for L=%d", bk, L);
+                return E_ARG;
+            }
+            // the seven regions of cpp/include/mipt/tmi/compute.hpp:89-113 per
+            // (trajectory, cycle), grouped by size for the batched solves
+            struct Item
+            {
+                int traj_j, cycle, term;
+            };
+            std::vector<std::pair<int, std::vector<int>>> by_size;  // size -> flattened kept sets
+            std::vector<std::vector<Item>> items;
+            auto add = [&](int sz, const std::vector<int> &set, Item it) {
+                std::size_t k = 0;
+                while (k < by_size.size() && by_size[k].first != sz) ++k;
+                if (k == by_size.size())
+                {
+                    by_size.push_back({sz, {}});
+                    items.emplace_back();
+                }
+                by_size[k].second.insert(by_size[k].second.end(), set.begin(), set.end());
+                items[k].push_back(it);
+            };
+            auto range = [&](int begin, int n_) {
+                std::vector<int> v;
+                for (int i = 0; i < n_; ++i) v.push_back((begin + i) % L);
+                return v;
+            };
+            for (int j = 0; j < cnt; ++j)
+                for (int c = 0; c < tmi_cycles; ++c)
+                {
+                    const auto A = range(c, bk), B = range(c + bk, bk), Cc = range(c + 2 * bk, bk);
+                    const auto D = range(c + 3 * bk, L - 3 * bk);
+                    auto cat = [](std::vector<int> x, const std::vector<int> &y) {
+                        x.insert(x.end(), y.begin(), y.end());
+                        return x;
+                    };
+                    const std::vector<int> sets[7] = {A, B, Cc, D, cat(A, B), cat(A, Cc), cat(B, Cc)};
+                    for (int term = 0; term < 7; ++term)
+                        add(static_cast<int>(sets[term].size()), sets[term], Item{j, c, term});
+                }
+            std::vector<double> acc(static_cast<std::size_t>(cnt) * tmi_cycles, 0.0);
+            for (std::size_t k = 0; k < by_size.size(); ++k)
+            {
+                const int sz = by_size[k].first;
+                const int ni = static_cast<int>(items[k].size());
+                std::vector<const float2 *> ps(ni);
+                for (int i = 0; i < ni; ++i) ps[i] = psis[items[k][i].traj_j];
+                std::vector<double> sv(ni);
+                MIPT_HIP_TRY(entropy_many_bits_f32(ps.data(), nq, by_size[k].second.data(), sz, ni, b->stream, sv.data()));
+                for (int i = 0; i < ni; ++i)
+                {
+                    const Item &it = items[k][i];
+                    acc[static_cast<std::size_t>(it.traj_j) * tmi_cycles + it.cycle] += (it.term < 4 ? 1.0 : -1.0) * sv[i];
+                }
+            }
+            for (int j = 0; j < cnt; ++j)
+                for (int c = 0; c < tmi_cycles; ++c)
+                    tmi[static_cast<std::size_t>(idx[g0 + j]) * tmi_cycles + c] = acc[static_cast<std::size_t>(j) * tmi_cycles + c];
+        }
+        g0 = g1;
+    }
+    return OK;
+}
+
 int mipt_hip_batch_tmi(mipt_hip_batch *b, int traj, int block, int cycle, int order, double *tmi)
 {
     if (!b || traj < 0 || traj >= static_cast<int>(b->desc.size()) || !tmi || (order != 1 && order != 2))

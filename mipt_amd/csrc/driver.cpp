@@ -20,6 +20,7 @@
 #include "mipt_hip.h"
 #include "common.hpp"
 
+#include <algorithm>
 #include <cerrno>
 #include <charconv>
 #include <cmath>
@@ -306,6 +307,12 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
         if ((st = mipt_hip_batch_evolve(b)) != 0) break;
         if ((st = mipt_hip_batch_synchronize(b)) != 0) break;
         if ((st = mipt_hip_engine_create(0, mipt_hip_batch_stream(b), &eng)) != 0) break;
+        // S(L/2) and (von Neumann) TMI of all trajectories with batched eigensolves
+        std::vector<double> sh(count), bloch(static_cast<std::size_t>(count) * 3);
+        if ((st = mipt_hip_batch_observables_all(b, tmi_block, entropy_order == 1 ? tmi_cycles : 0, sh.data(),
+                                                 entropy_order == 1 ? tmi_out : nullptr)) != 0)
+            break;
+        if ((st = mipt_hip_batch_probe_bloch(b, bloch.data())) != 0) break;
         for (int i = 0; i < count && st == 0; ++i)
         {
             const mipt_hip_traj_spec &s = specs[i];
@@ -321,10 +328,21 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
                 r.measured += out[k] >= 0;
                 r.flagged_draws += flg[k] != 0;
             }
-            if ((st = mipt_hip_batch_observables(b, i, tmi_block, &r.s_half, nullptr, &r.probe_entropy, r.bloch)) != 0)
-                break;
-            for (int c = 0; c < tmi_cycles && st == 0; ++c)
-                st = mipt_hip_batch_tmi(b, i, tmi_block, c, entropy_order, &tmi_out[static_cast<std::size_t>(i) * tmi_cycles + c]);
+            r.s_half = sh[i];
+            for (int k = 0; k < 3; ++k) r.bloch[k] = bloch[static_cast<std::size_t>(i) * 3 + k];
+            if (s.probe_site < 0)
+                r.probe_entropy = NAN;
+            else
+            {
+                // cpp/include/mipt/ancilla_mi.hpp:24-52 (nats)
+                const double len = std::min(1.0, std::sqrt(r.bloch[0] * r.bloch[0] + r.bloch[1] * r.bloch[1] +
+                                                           r.bloch[2] * r.bloch[2]));
+                const double l0 = 0.5 * (1 + len), l1 = 0.5 * (1 - len);
+                r.probe_entropy = (l0 > 0 ? -l0 * std::log(l0) : 0.0) + (l1 > 0 ? -l1 * std::log(l1) : 0.0);
+            }
+            if (entropy_order == 2)
+                for (int c = 0; c < tmi_cycles && st == 0; ++c)
+                    st = mipt_hip_batch_tmi(b, i, tmi_block, c, 2, &tmi_out[static_cast<std::size_t>(i) * tmi_cycles + c]);
             if (st != 0) break;
             const int nq = s.L + (s.probe_site >= 0 ? 1 : 0);
             double sq = 0.0;

