@@ -88,7 +88,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps (the second captures the HIP graph)")
     ap.add_argument("--L", type=int, default=20)
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--per-p", type=int, default=256)
@@ -97,6 +97,8 @@ def main(argv=None):
     ap.add_argument("--replay", type=int, default=8, help="trajectories checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg (profiling runs)")
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile configuration")
+    ap.add_argument("--no-graph", action="store_true", help="launch every kernel eagerly (no HIP graph replay)")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--profile-every", type=int, default=7, help="odd, so both sweeps of a layer are sampled")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -133,6 +135,15 @@ def main(argv=None):
     ntraj = len(specs)
     log(f"setting up {ntraj} trajectories (L={L}, T={layers})")
     b = M.Batch(specs, tile_bits=args.tile_bits, threads=args.threads)
+    tune = None
+    if not args.no_autotune and args.tile_bits == 0:
+        b.generate()
+        best, ms = b.autotune(4)
+        names = ["13b/256t+mfma", "13b/256t", "14b/256t", "13b/128t", "12b/64t"]
+        tune = {"choice": names[best] if best >= 0 else None, "probe_ms": {n: round(float(m), 2) for n, m in zip(names, ms)}}
+        log(f"autotune: {tune}")
+    if not args.no_graph:
+        b.set_graph(True)
     hip = hip_runtime()
     stream = ctypes.c_void_p(b.stream)
     e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
@@ -152,7 +163,7 @@ def main(argv=None):
         step()
         b.synchronize()
         log(f"warmup step {w + 1}/{args.warmup} done")
-    b.set_profiling(args.profile_every)
+    b.set_profiling(0 if not args.no_graph else args.profile_every)
     b.synchronize()
     hip.hipEventRecord(e0, stream)
     t_wall = time.perf_counter()
@@ -168,6 +179,14 @@ def main(argv=None):
     ms_per_step = ms.value / args.steps
     layer_trajs = ntraj * layers
     value = layer_trajs / (ms_per_step / 1e3)
+    if not args.no_graph:
+        # graph replays cannot be sampled per launch: one more eager step,
+        # outside the timed region, times the sweep kernels
+        b.set_graph(False)
+        b.set_profiling(args.profile_every)
+        step()
+        b.synchronize()
+        log("profiled eager step done")
     st = b.pass_stats()
     achieved = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
     # all sweeps of a step together: algorithmic bytes of the step over its wall time
@@ -215,6 +234,8 @@ def main(argv=None):
                      "streams_in_flight": 2,
                      "aggregate_sweep_GBps": round(aggregate, 1),
                      "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4)},
+        "autotune": tune,
+        "graph": not args.no_graph,
         "plain_layers_per_s": None if plain_lps is None else round(plain_lps, 1),
         "batched_vs_plain": None if plain_lps is None else round(value / plain_lps, 2),
     }

@@ -148,7 +148,7 @@ typedef struct mipt_hip_traj_spec
 
 typedef struct mipt_hip_batch_config
 {
-    int tile_bits;      /* 0 = autotuned choice; else 12..14 */
+    int tile_bits;      /* 0 = the autotuner's choice (mipt_hip_batch_autotune; 13 before it runs); else 12..14 */
     int max_hist_bits;  /* measured sites per fused histogram chunk (<= 10); 0 = default */
     int threads;        /* reserved: 0 */
 } mipt_hip_batch_config;
@@ -175,6 +175,19 @@ int mipt_hip_batch_evolve(mipt_hip_batch *batch);
 /* Evolves layers [layer_begin, layer_end) of all trajectories (asynchronous). */
 int mipt_hip_batch_evolve_range(mipt_hip_batch *batch, int layer_begin, int layer_end);
 int mipt_hip_batch_synchronize(mipt_hip_batch *batch);
+/* Autotuner: times probe_layers layers of this batch (its own draws and
+ * shapes) under each tile configuration -- 0: 2^13-amplitude tiles, 256
+ * threads, one bond per stage on the matrix cores; 1: the same on the VALU
+ * only; 2: 2^14 tiles, 256 threads; 3: 2^13 tiles, 128 threads; 4: 2^12 tiles,
+ * 64 threads -- and keeps the fastest for the tile-swept trajectories of a
+ * batch created with tile_bits = 0.  The batch is re-armed before and after
+ * (draws kept).  best_index (may be NULL) receives the choice (-1: nothing is
+ * tile-swept), ms (may be NULL, 5 doubles) each candidate's time. */
+int mipt_hip_batch_autotune(mipt_hip_batch *batch, int probe_layers, int *best_index, double *ms);
+/* 1: full evolutions run as one captured HIP graph (both streams), captured
+ * on the second full evolve after the draws or configuration last changed
+ * and replayed afterwards; 0 (default): eager launches. */
+int mipt_hip_batch_set_graph(mipt_hip_batch *batch, int enable);
 /* outcomes [layers][L]: -1 unmeasured, else 0/1; flagged [layers][L]: 1 where
  * |u - P(1)| < 1e-6; p1 [layers][L] conditional P(1) (may be NULL). */
 int mipt_hip_batch_outcomes(mipt_hip_batch *batch, int traj, int *outcomes, uint8_t *flagged, double *p1,

@@ -96,6 +96,8 @@ def lib():
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
     L.mipt_hip_batch_records.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_int8)]
     L.mipt_hip_batch_observables_all.argtypes = [vp, i, i, dp, dp]
+    L.mipt_hip_batch_autotune.argtypes = [vp, i, ip, dp]
+    L.mipt_hip_batch_set_graph.argtypes = [vp, i]
     L.mipt_hip_batch_probe_bloch.argtypes = [vp, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
     L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
@@ -479,6 +481,16 @@ class Batch:
         _check(lib().mipt_hip_batch_observables_all(self._h, tmi_block, tmi_cycles, _dp(sh),
                                                     _dp(tm) if tmi_cycles > 0 else None), "batch_observables_all")
         return sh, tm[:, :tmi_cycles]
+
+    def autotune(self, probe_layers: int = 4):
+        """Pick the fastest tile configuration on this batch; returns (index, ms per candidate)."""
+        best = ctypes.c_int()
+        ms = np.zeros(5)
+        _check(lib().mipt_hip_batch_autotune(self._h, probe_layers, ctypes.byref(best), _dp(ms)), "batch_autotune")
+        return best.value, ms
+
+    def set_graph(self, enable: bool = True):
+        _check(lib().mipt_hip_batch_set_graph(self._h, int(enable)), "batch_set_graph")
 
     def probe_bloch(self) -> np.ndarray:
         """[count, 3] probe Bloch vectors (NaN rows without a probe)."""

@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <numeric>
 #include <vector>
 
@@ -1307,6 +1308,7 @@ struct Batch
 
     ~Batch()
     {
+        drop_graph();
         release();
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (partials2) (void)hipFree(partials2);
@@ -1356,9 +1358,30 @@ struct Batch
     // Tile variants: 14 -> 2^14-amplitude tiles, 256 threads x 64 registers
     // (one workgroup per CU); 13 -> 2^13 tiles, 256 x 32 (two per CU), or
     // 128 x 64 with threads = 128; 12 -> 64 x 64.
+    std::map<int, Config> tuned;  // per qubit count, picked by autotune()
+    // graph capture of the full evolution (set_graph): captured on the
+    // second full evolve after the draws last changed, replayed afterwards
+    bool graph_on = false;
+    int graph_warm = 0;  // full eager evolves since the draws/config changed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    void drop_graph()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_warm = 0;
+    }
+
     Config config_for(int nq) const
     {
-        (void)nq;
+        if (cfg.tile_bits == 0)
+        {
+            const auto it = tuned.find(nq);
+            if (it != tuned.end()) return it->second;
+        }
         Config cf{5, 256, 5};
         if (cfg.tile_bits == 14) cf = Config{6, 256, 6};
         if (cfg.tile_bits == 13) cf = cfg.threads == 128 ? Config{6, 128, 5} : Config{5, 256, 5};
@@ -1374,6 +1397,8 @@ struct Batch
 
     int setup(const mipt_hip_traj_spec *sp, int count)
     {
+        drop_graph();
+        tuned.clear();
         void *ptrs[] = {a.states, a.gates, a.desc, a.flags, a.outcomes, a.flagged, a.uniforms,
                         a.p1, a.coll, a.surprisal, a.status, a.init, a.partials, d_traj_list};
         for (void *p : ptrs)
@@ -1528,6 +1553,7 @@ struct Batch
 
     int set_layers(int traj, const double *g, const std::uint8_t *flags, const double *unif)
     {
+        drop_graph();
         if (traj < 0 || traj >= static_cast<int>(desc.size()) || specs[traj].shared_gates >= 0)
         {
             set_error("set_layers: bad trajectory (or it shares another's gates)");
@@ -1719,6 +1745,9 @@ struct Batch
     }
 
     // t1 < 0: every trajectory through its own depth.
+    // Full evolutions may run as a replayed graph (see set_graph); the flags
+    // and draws the graph was captured with are those on the host now, which
+    // only set_layers()/setup() change (generate() redraws the same streams).
     int evolve(int t0, int t1_req)
     {
         if (!flags_on_host)
@@ -1726,6 +1755,42 @@ struct Batch
             set_error("evolve: call generate() or set_layers() first");
             return E_ARG;
         }
+        if (graph_on && t0 == 0 && t1_req < 0)
+        {
+            if (graph_exec)
+            {
+                MIPT_HIP_CHECK(hipGraphLaunch(graph_exec, stream));
+                for (auto &x : initialized) x = 1;
+                return OK;
+            }
+            if (graph_warm > 0)
+            {
+                // every buffer exists after an eager run: capture one
+                const int saved_prof = prof_every;
+                prof_every = 0;
+                MIPT_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+                const int st = evolve_body(t0, t1_req);
+                hipGraph_t gph = nullptr;
+                const hipError_t ec = hipStreamEndCapture(stream, &gph);
+                prof_every = saved_prof;
+                if (st != OK)
+                {
+                    if (gph) (void)hipGraphDestroy(gph);
+                    return st;
+                }
+                MIPT_HIP_CHECK(ec);
+                graph = gph;
+                MIPT_HIP_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+                MIPT_HIP_CHECK(hipGraphLaunch(graph_exec, stream));
+                return OK;
+            }
+            ++graph_warm;
+        }
+        return evolve_body(t0, t1_req);
+    }
+
+    int evolve_body(int t0, int t1_req)
+    {
         const int count = static_cast<int>(order.size());
         if (!side)
         {
@@ -1800,6 +1865,79 @@ struct Batch
 
     // Re-arm in-kernel initial states and clear per-run records (a new run of
     // the same trajectories).
+    // Autotuner: times `probe_layers` layers of this batch (its own draws and
+    // shapes) under each tile configuration and keeps the fastest for the
+    // tile-swept qubit counts; the batch is re-armed before and after, so the
+    // draws stay and the next evolve starts from the initial states.
+    int autotune(int probe_layers, int *best_index, double *ms_out)
+    {
+        if (!flags_on_host)
+        {
+            set_error("autotune: call generate() or set_layers() first");
+            return E_ARG;
+        }
+        if (cfg.tile_bits != 0)
+        {
+            set_error("autotune: the batch was created with a fixed tile configuration");
+            return E_ARG;
+        }
+        int depth = probe_layers;
+        std::vector<int> nqs;
+        for (const TrajDesc &d : desc)
+            if (d.nq > SMALL_MAX_QUBITS)
+            {
+                depth = std::min(depth, d.layers);
+                if (std::find(nqs.begin(), nqs.end(), d.nq) == nqs.end()) nqs.push_back(d.nq);
+            }
+        if (best_index) *best_index = -1;
+        if (nqs.empty() || depth < 1) return OK;  // nothing tile-swept
+        const Config cand[] = {Config{5, 256, 5, true}, Config{5, 256, 5, false}, Config{6, 256, 6, false},
+                               Config{6, 128, 5, false}, Config{6, 64, 4, false}};
+        const int ncand = static_cast<int>(sizeof(cand) / sizeof(cand[0]));
+        drop_graph();
+        hipEvent_t e0, e1;
+        MIPT_HIP_CHECK(hipEventCreate(&e0));
+        MIPT_HIP_CHECK(hipEventCreate(&e1));
+        int best = -1;
+        double best_ms = 0.0;
+        int st = OK;
+        for (int c = 0; c < ncand && st == OK; ++c)
+        {
+            for (int nq : nqs) tuned[nq] = cand[c];
+            // every plan must exist for every shape (a candidate may not fit)
+            bool ok = true;
+            for (const TrajDesc &d : desc)
+                if (d.nq > SMALL_MAX_QUBITS)
+                    for (int j = 0; j < period_layers(d.gx) && ok; ++j)
+                    {
+                        LayerPlan lp;
+                        ok = build_layer_plan(d.L, d.nq, d.closed, d.gx, j, cand[c], lp) == OK;
+                    }
+            if (!ok) continue;
+            for (int rep = 0; rep < 2 && st == OK; ++rep)  // first run warms buffers and code
+            {
+                if ((st = reset()) != OK) break;
+                if (hipEventRecord(e0, stream) != hipSuccess) { st = E_ARG; break; }
+                if ((st = evolve_body(0, depth)) != OK) break;
+                if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { st = E_ARG; break; }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms_out) ms_out[c] = ms;
+                if (rep == 1 && (best < 0 || ms < best_ms))
+                {
+                    best = c;
+                    best_ms = ms;
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (st != OK) return st;
+        for (int nq : nqs) tuned[nq] = cand[best < 0 ? 0 : best];
+        if (best_index) *best_index = best;
+        return reset();
+    }
+
     int reset()
     {
         const int count = static_cast<int>(desc.size());
@@ -1990,6 +2128,20 @@ int mipt_hip_batch_evolve_range(mipt_hip_batch *b, int t0, int t1)
     for (const auto &d : b->desc)
         if (t1 > d.layers) { set_error("batch_evolve_range: layer beyond a trajectory's depth"); return E_ARG; }
     return b->evolve(t0, t1);
+}
+
+int mipt_hip_batch_autotune(mipt_hip_batch *b, int probe_layers, int *best_index, double *ms)
+{
+    if (!b || probe_layers < 1) { set_error("batch_autotune: bad arguments"); return E_ARG; }
+    return b->autotune(probe_layers, best_index, ms);
+}
+
+int mipt_hip_batch_set_graph(mipt_hip_batch *b, int enable)
+{
+    if (!b) { set_error("batch_set_graph: null batch"); return E_ARG; }
+    b->drop_graph();
+    b->graph_on = enable != 0;
+    return OK;
 }
 
 int mipt_hip_batch_synchronize(mipt_hip_batch *b)

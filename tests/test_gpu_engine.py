@@ -393,3 +393,46 @@ def test_batch_observables_all_matches_single_calls_and_oracle(M):
             assert abs(sh[i] - O.half_chain_entropy(psi, n)) < 1e-5
             for c in range(3):
                 assert abs(tm[i, c] - O.tmi(psi, n, n // 4, c)) < 1e-5
+
+
+def test_batch_autotune_then_evolve_matches_oracle(M):
+    cases = [(16, 6, 0.2, 81), (18, 6, 0.25, 82), (20, 5, 0.15, 83), (10, 6, 0.3, 84)]
+    specs = [M.Spec(n, layers, p, seed) for (n, layers, p, seed) in cases]
+    print("batch autotune", flush=True)
+    b = M.Batch(specs)
+    b.generate()
+    best, ms = b.autotune(3)
+    print("autotune choice", best, "ms", ms, flush=True)
+    assert 0 <= best < 5 and ms[best] > 0
+    b.evolve()
+    b.synchronize()
+    for i, (n, layers, p, seed) in enumerate(cases):
+        psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed)
+        got, flagged, _, _ = b.outcomes(i)
+        check_outcomes(got, out, p1, u, f, f"tuned {i}")
+        if not flagged.any():
+            assert np.abs(b.get_state(i) - psi).max() < 1e-5
+
+
+def test_batch_graph_replays_match_eager(M):
+    # graph replays (with eager work and synchronisation in between) give the
+    # eager run's outcomes and states exactly
+    cases = [(16, 6, 0.2, 91), (20, 4, 0.3, 92), (10, 6, 0.25, 93), (18, 5, 0.5, 94)]
+    specs = [M.Spec(n, layers, p, seed) for (n, layers, p, seed) in cases]
+    print("batch graph replays", flush=True)
+    b = M.Batch(specs)
+    b.set_graph(True)
+    ref = None
+    for rep in range(4):  # eager, capture + launch, replay, replay
+        b.reset()
+        b.generate()
+        b.evolve()
+        b.synchronize()
+        got = [(b.outcomes(i)[0], b.get_state(i)) for i in range(len(cases))]
+        b.observables(0)  # eager work between replays
+        if ref is None:
+            ref = got
+            continue
+        for i in range(len(cases)):
+            assert np.array_equal(got[i][0], ref[i][0]), (rep, i)
+            assert np.array_equal(got[i][1], ref[i][1]), (rep, i)
