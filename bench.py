@@ -109,6 +109,7 @@ def main(argv=None):
     ap.add_argument("--no-decoder", action="store_true", help="skip the learned probe decoder leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-length / 2D-grid leg")
     ap.add_argument("--no-observables", action="store_true", help="skip the S/TMI readout timing")
+    ap.add_argument("--no-c1", action="store_true", help="skip the L=12 open-chain (CPU-path) workload")
     ap.add_argument("--obs-traj", type=int, default=256)
     ap.add_argument("--uneven-per", type=int, default=512, help="trajectories per (L, p) in the uneven leg")
     ap.add_argument("--decoder-train", type=int, default=200_000)
@@ -279,6 +280,12 @@ def main(argv=None):
         result["cpu_baseline"] = None
     if args.rdm_L:
         result["rdm_gemm"] = rdm_timing(M, args.rdm_L, args.rdm_reps)
+    if not args.no_cpu and not args.no_c1:
+        try:
+            result["c1"] = c1_leg(M)
+        except Exception as e:
+            result["c1"] = {"error": repr(e)}
+            log(f"C1 leg failed: {e!r}")
     if not args.no_observables:
         try:
             result["observables"] = observables_leg(M, L, layers, ps, args.obs_traj)
@@ -299,6 +306,68 @@ def main(argv=None):
             log(f"decoder leg failed: {e!r}")
     print(json.dumps(result))
     return 0
+
+
+def _oracle_c1(args):
+    n, layers, p, seed = args
+    from oracle import oracle as O
+    psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed, False, "philox", 0)
+    return psi, out, p1, f, u
+
+
+def c1_leg(M):
+    """The reference's CPU-path workload: open chain L=12 from |0...0>, T=24,
+    p=0.16, 200 trajectories with seeds trajectory_seed(0, i).  The FP64
+    oracle runs all 200 on the host's cores (the CPU baseline on this
+    workload); the GPU runs them in the batched engine (complex64, one launch)
+    and on the plain engine in complex128; outcomes, S(6) and the TMI of
+    3-site regions are checked against the oracle for every trajectory."""
+    from oracle import oracle as O
+    n, layers, p, count = 12, 24, 0.16, 200
+    seeds = [int(O.lib().orc_trajectory_seed(0, i)) for i in range(count)]
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    t = time.perf_counter()
+    with ThreadPoolExecutor(cores) as pool:
+        ref = list(pool.map(_oracle_c1, [(n, layers, p, sd) for sd in seeds]))
+    cpu_s = time.perf_counter() - t
+    specs = [M.Spec(n, layers, p, sd, False) for sd in seeds]
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()  # warm
+    b.reset()
+    t = time.perf_counter()
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    gpu_s = time.perf_counter() - t
+    eng = M.PlainEngine(True)
+    draws = [b.export_layers(i) for i in range(16)]
+    t = time.perf_counter()
+    for g, f, u in draws:
+        M.run_plain_trajectory(eng, n, g, f, u, False, 4, 0, 10)
+    plain_s = time.perf_counter() - t
+    eng.close()
+    sh, tm = b.observables_all(3, 1)
+    mism = flagged = 0
+    s_err = tmi_err = 0.0
+    for i, (psi, out, p1, f, u) in enumerate(ref):
+        got, flg, _, _ = b.outcomes(i)
+        near = np.abs(u - p1) < 1e-6
+        mism += int(np.sum((got != out) & (f == 1) & ~near))
+        flagged += int(flg.sum())
+        if not flg.any():
+            s_err = max(s_err, abs(sh[i] - O.half_chain_entropy(psi, n)))
+            tmi_err = max(tmi_err, abs(tm[i, 0] - O.tmi(psi, n, 3, 0)))
+    b.close()
+    log(f"C1: oracle {count * layers / cpu_s:.0f} layers/s on {cores} threads, batched GPU "
+        f"{count * layers / gpu_s:.0f} layers/s, plain complex128 {16 * layers / plain_s:.0f} layers/s")
+    return {"workload": "L=12 open chain, T=24, p=0.16, 200 trajectories (seeds trajectory_seed(0, i))",
+            "cpu_oracle_layers_per_s": round(count * layers / cpu_s, 1), "cpu_threads": cores,
+            "gpu_batched_layers_per_s": round(count * layers / gpu_s, 1),
+            "gpu_plain_complex128_layers_per_s": round(16 * layers / plain_s, 1),
+            "parity": {"trajectories": count, "outcome_mismatches": mism, "flagged_draws": flagged,
+                       "s_half_err_max": s_err, "tmi_err_max": tmi_err}}
 
 
 def observables_leg(M, L, layers, ps, n):

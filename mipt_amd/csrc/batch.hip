@@ -754,6 +754,11 @@ __global__ void __launch_bounds__(NT, (RB == 5 ? 2 : 1)) pass_kernel(PassArgs ar
     const int traj = args.traj_list[tl];
     const TrajDesc d = a.desc[traj];
     const int t = args.layer;
+    // an overflow-chunk histogram sweep (cond: layer t is one of the
+    // trajectory's layers) has nothing to add for a trajectory with too few
+    // measured sites (its finalize returns at once); uniform per block.  The
+    // closing collapse sweep also runs stage-free but has cond = 0.
+    if (AUX && pl.cond && __popc(a.flags[d.layer_off + t]) <= pl.chunk * MAX_HIST_BITS) return;
     const int c = pl.c, s = pl.s;
     const std::uint64_t B = pdep64(static_cast<std::uint64_t>(tile), pl.rest_mask);
     char *tp = reinterpret_cast<char *>(a.states + d.state_off + B);  // tile base (uniform)

@@ -648,6 +648,19 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
     MIPT_HIP_TRY(ensure_handle(stream));
     const int m = 1 << kept_count;
     const std::size_t mm = static_cast<std::size_t>(m) * m;
+    if (m < 32)
+    {
+        // tiny RDMs: one solve each (cheap at this size)
+        for (int i = 0; i < count; ++i)
+        {
+            int mo = 0;
+            MIPT_HIP_TRY(build_rho(psis[i], false, n, kept + static_cast<std::size_t>(i) * kept_count, kept_count, stream, mo));
+            std::vector<double> lam1;
+            MIPT_HIP_TRY(eigvalsh(m, stream, lam1));
+            out[i] = entropy_from_eigs(lam1);
+        }
+        return OK;
+    }
     const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, (std::size_t{1} << 30) / (mm * 16))));
     MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, chunk * mm));
     MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk) * m));

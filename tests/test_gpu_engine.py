@@ -374,7 +374,8 @@ def test_batch_grid_matches_oracle(M):
 def test_batch_observables_all_matches_single_calls_and_oracle(M):
     # batched eigensolves give the same S(L/2) and per-cycle TMI as the
     # one-at-a-time entry points, and the oracle's within 1e-5
-    cases = [(12, 10, 0.1, 41), (12, 8, 0.2, 42), (16, 8, 0.15, 43), (20, 4, 0.1, 44), (16, 6, 0.0, 45)]
+    cases = [(12, 10, 0.1, 41), (12, 8, 0.2, 42), (16, 8, 0.15, 43), (20, 4, 0.1, 44), (16, 6, 0.0, 45),
+             (8, 8, 0.2, 46), (8, 6, 0.1, 47), (10, 6, 0.2, 48)]
     specs = [M.Spec(n, layers, p, seed) for (n, layers, p, seed) in cases]
     print("batch observables_all", flush=True)
     b = M.Batch(specs)
