@@ -15,8 +15,9 @@
 // every trajectory of the scan runs in one batched launch sequence instead of
 // serially; only circ_type 1 (Haar U(4)) is built, and it is the default; the
 // seeds are trajectory_seed(MIPT_SEED, index) when MIPT_SEED is set (random
-// otherwise, as the reference's random_device); an existing output is
-// overwritten (no resume).
+// otherwise, as the reference's random_device).  An existing output is
+// resumed as the reference does (SIM_TMI_RESUME, default on: whole circuits
+// kept, a trailing partial circuit cut off, the p grid checked row by row).
 #include "mipt_hip.h"
 #include "common.hpp"
 
@@ -38,6 +39,9 @@
 #include <string>
 #include <string_view>
 #include <sys/stat.h>
+#include <unistd.h>
+#include <cctype>
+#include <iterator>
 #include <vector>
 
 namespace
@@ -149,6 +153,140 @@ void usage(const char *argv0)
               << "  MIPT_HIP_STATE_GIB=<g>     device memory for states per batch (default 64)\n";
 }
 
+bool env_flag(const char *name, bool dflt)
+{
+    const char *e = std::getenv(name);
+    if (!e || !*e) return dflt;
+    std::string v(e);
+    for (char &ch : v) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+    if (v == "0" || v == "false" || v == "no" || v == "off") return false;
+    if (v == "1" || v == "true" || v == "yes" || v == "on") return true;
+    throw std::invalid_argument(std::string(name) + " must be 0/1, true/false, yes/no or on/off.");
+}
+
+// Fields of a generated output name tmi_<tag>_n_<n>_real_<R>_<p_min>_<p_max>_res_<res>.csv
+struct NameIdentity
+{
+    std::string tag, n, real, pmin, pmax, res;
+};
+
+bool parse_output_name(std::string path, NameIdentity &id)
+{
+    const std::size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos) path = path.substr(slash + 1);
+    if (path.size() < 8 || path.compare(0, 4, "tmi_") != 0 || path.compare(path.size() - 4, 4, ".csv") != 0)
+        return false;
+    const std::string body = path.substr(4, path.size() - 8);
+    const std::size_t r = body.rfind("_res_"), q = body.find("_real_"), nn = body.rfind("_n_", q);
+    if (r == std::string::npos || q == std::string::npos || nn == std::string::npos || !(nn < q && q < r)) return false;
+    id.tag = body.substr(0, nn);
+    id.n = body.substr(nn + 3, q - nn - 3);
+    id.res = body.substr(r + 5);
+    const std::string mid = body.substr(q + 6, r - q - 6);  // <R>_<p_min>_<p_max>
+    const std::size_t a = mid.find('_'), b = mid.find('_', a == std::string::npos ? a : a + 1);
+    if (a == std::string::npos || b == std::string::npos || mid.find('_', b + 1) != std::string::npos) return false;
+    id.real = mid.substr(0, a);
+    id.pmin = mid.substr(a + 1, b - a - 1);
+    id.pmax = mid.substr(b + 1);
+    return !id.tag.empty() && !id.n.empty() && !id.real.empty() && !id.res.empty();
+}
+
+// Where an interrupted scan stopped (the contract of the reference's
+// cpp/include/mipt/tmi/resume.hpp:43-50, 205-333): header `p,tmi`; rows grouped
+// by p in grid order, the p field compared as text against the grid rendered
+// in 17 significant digits; a group switches only when full
+// (realizations * cycles rows); only whole circuits (cycles rows) are kept,
+// the file is cut back to the last one.
+struct Checkpoint
+{
+    bool name_verified = false;
+    std::uint64_t rows = 0, dropped_rows = 0;
+    std::uintmax_t kept_bytes = 0;
+    std::size_t start_p = 0;
+    int start_real = 0;
+};
+
+[[noreturn]] void refuse(const std::string &why) { throw std::runtime_error(why); }
+
+Checkpoint load_checkpoint(const std::string &path, const std::string &expected_name, int realizations, int cycles,
+                           const std::vector<std::string> &p_texts)
+{
+    Checkpoint cp;
+    NameIdentity want, got;
+    if (parse_output_name(expected_name, want) && parse_output_name(path, got))
+    {
+        std::string mism;
+        auto cmp = [&](const char *what, const std::string &a, const std::string &b) {
+            if (a != b) mism += std::string(mism.empty() ? "" : ", ") + what + "=" + b + " (this run: " + a + ")";
+        };
+        cmp("tag", want.tag, got.tag);
+        cmp("n", want.n, got.n);
+        cmp("realizations", want.real, got.real);
+        cmp("p_min", want.pmin, got.pmin);
+        cmp("p_max", want.pmax, got.pmax);
+        cmp("resolution", want.res, got.res);
+        if (!mism.empty())
+            refuse(path + " was produced with " + mism + ". Move the existing file aside, or set SIM_TMI_RESUME=0 to "
+                                                        "overwrite it.");
+        cp.name_verified = true;
+    }
+    std::ifstream in(path, std::ios::binary);
+    std::string content((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::vector<std::string> lines;
+    std::size_t pos = 0;
+    while (true)
+    {
+        const std::size_t e = content.find('\n', pos);
+        if (e == std::string::npos) break;  // an unterminated last line is not kept
+        lines.push_back(content.substr(pos, e - pos));
+        pos = e + 1;
+    }
+    if (lines.empty()) return cp;
+    if (lines.front() != "p,tmi") refuse(path + " does not start with the expected header \"p,tmi\"; found \"" + lines.front() + "\".");
+    const std::uint64_t per_p = static_cast<std::uint64_t>(realizations) * cycles;
+    std::size_t group = 0;
+    std::uint64_t in_group = 0, total = 0;
+    std::uintmax_t offset = lines.front().size() + 1, boundary = offset;
+    for (std::size_t i = 1; i < lines.size(); ++i)
+    {
+        const std::string &row = lines[i];
+        const std::size_t comma = row.find(',');
+        if (comma == std::string::npos) refuse(path + " has a malformed row at line " + std::to_string(i + 1) + ".");
+        const std::string pt = row.substr(0, comma);
+        if (pt != p_texts[group])
+        {
+            if (in_group != per_p)
+                refuse(path + " switches away from p=" + p_texts[group] + " after " + std::to_string(in_group) +
+                       " rows, but a complete p group holds realizations*cycles = " + std::to_string(per_p) + ".");
+            ++group;
+            if (group >= p_texts.size() || pt != p_texts[group])
+                refuse(path + " carries p=" + pt + " where this run's grid expects " +
+                       (group < p_texts.size() ? p_texts[group] : std::string("no further values")) + ".");
+            in_group = 0;
+        }
+        ++in_group;
+        ++total;
+        offset += row.size() + 1;
+        if (in_group % static_cast<std::uint64_t>(cycles) == 0) boundary = offset;
+    }
+    if (in_group > per_p) refuse(path + " holds more rows for p=" + p_texts[group] + " than a complete group.");
+    const std::uint64_t whole = in_group / cycles;
+    cp.dropped_rows = in_group - whole * cycles;
+    cp.rows = total - cp.dropped_rows;
+    cp.kept_bytes = boundary;
+    if (whole >= static_cast<std::uint64_t>(realizations))
+    {
+        cp.start_p = group + 1;
+        cp.start_real = 0;
+    }
+    else
+    {
+        cp.start_p = group;
+        cp.start_real = static_cast<int>(whole);
+    }
+    return cp;
+}
+
 int entropy_order_from_env()
 {
     const char *e = std::getenv("SIM_TMI_ENTROPY_ORDER");
@@ -206,10 +344,11 @@ int run_sim_tmi(int argc, char **argv)
     std::string tag = "haar";
     if (all_cycles) tag += 'c';
     if (order == 2) tag += "_s2";
-    if (output.empty())
-        output = "csv/tmi/" + tag + "/tmi_" + tag + "_n_" + std::to_string(n) + "_real_" +
-                 std::to_string(realizations) + "_" + number_for_filename(p_min) + "_" +
-                 number_for_filename(p_max) + "_res_" + std::to_string(res) + ".csv";
+    const std::string default_path = "csv/tmi/" + tag + "/tmi_" + tag + "_n_" + std::to_string(n) + "_real_" +
+                                     std::to_string(realizations) + "_" + number_for_filename(p_min) + "_" +
+                                     number_for_filename(p_max) + "_res_" + std::to_string(res) + ".csv";
+    if (output.empty()) output = default_path;
+    const bool resume = env_flag("SIM_TMI_RESUME", true);
 
     std::uint64_t master;
     if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
@@ -224,10 +363,42 @@ int run_sim_tmi(int argc, char **argv)
     const std::uint64_t per_batch =
         std::max<std::uint64_t>(1, static_cast<std::uint64_t>(budget_gib * 1073741824.0 / state_bytes));
 
+    // resume: keep the whole circuits an earlier run of the same scan wrote
+    std::vector<std::string> p_texts;
+    for (double pv : ps)
+    {
+        std::string t;
+        append_double(t, pv);
+        p_texts.push_back(t);
+    }
+    Checkpoint cp;
+    struct stat sb;
+    const bool exists = ::stat(output.c_str(), &sb) == 0;
+    if (resume && exists) cp = load_checkpoint(output, default_path, realizations, cycles, p_texts);
+    const std::uint64_t first_traj = static_cast<std::uint64_t>(cp.start_p) * realizations + cp.start_real;
+    if (cp.rows >= total_traj * cycles)
+    {
+        std::cerr << output << " already holds all " << total_traj * cycles
+                  << " TMI row(s) for this scan; nothing to do.\nSet SIM_TMI_RESUME=0 to overwrite it instead.\n";
+        return 0;
+    }
+    const bool resuming = cp.rows > 0;
     mkdirs_for(output);
-    std::ofstream csv(output, std::ios::binary | std::ios::trunc);
+    if (resuming && cp.kept_bytes < static_cast<std::uintmax_t>(sb.st_size) &&
+        ::truncate(output.c_str(), static_cast<off_t>(cp.kept_bytes)) != 0)
+        throw std::runtime_error("Could not trim the partial circuit off " + output);
+    std::ofstream csv(output, std::ios::binary | (resuming ? std::ios::app : std::ios::trunc));
     if (!csv) throw std::runtime_error("Could not create output CSV: " + output);
-    csv << "p,tmi\n";
+    if (!resuming) csv << "p,tmi\n";
+    if (resuming)
+        std::cerr << "RESUMED from " << output << ": " << cp.rows << " TMI row(s) already recorded"
+                  << (cp.dropped_rows ? "; dropped " + std::to_string(cp.dropped_rows) + " row(s) of a partial circuit" : "")
+                  << ". Continuing at p[" << cp.start_p << "]=" << ps[cp.start_p] << ", realization " << cp.start_real
+                  << '/' << realizations << ".\n"
+                  << (cp.name_verified ? "" : "Warning: the output name does not follow the generated convention; only "
+                                              "the p grid and the per-p row counts were verified.\n");
+    else if (!resume)
+        std::cerr << "Resume disabled by SIM_TMI_RESUME=0; any existing output was overwritten.\n";
     std::cerr << "Online TMI simulation (batched HIP engine)\n"
               << "n=" << n << ", periods=" << periods << ", realizations=" << realizations << ", resolution=" << res
               << ", p_min=" << p_min << ", p_max=" << p_max << ", block_qubits=" << n / 4
@@ -238,7 +409,7 @@ int run_sim_tmi(int argc, char **argv)
     const auto t0 = std::chrono::steady_clock::now();
     std::string buf;
     std::uint64_t done = 0;
-    for (std::uint64_t first = 0; first < total_traj; first += per_batch)
+    for (std::uint64_t first = first_traj; first < total_traj; first += per_batch)
     {
         const int count = static_cast<int>(std::min<std::uint64_t>(per_batch, total_traj - first));
         std::vector<mipt_hip_traj_spec> specs(count);
@@ -275,7 +446,7 @@ int run_sim_tmi(int argc, char **argv)
         if (!csv) throw std::runtime_error("Failed while writing output CSV.");
         done += count;
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        std::cerr << "processed TMI " << done * cycles << '/' << total_traj * cycles << " | avg " << std::fixed
+        std::cerr << "processed TMI " << (first_traj + done) * cycles << '/' << total_traj * cycles << " | avg " << std::fixed
                   << std::setprecision(2) << (done * cycles) / std::max(1e-9, el) << "/s" << std::defaultfloat
                   << '\n';
     }

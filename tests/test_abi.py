@@ -45,3 +45,18 @@ def test_argument_errors_need_no_gpu():
     assert L.mipt_hip_engine_create(0, None, None) == -1
     assert L.mipt_hip_batch_setup(None, None, 0) == -1
     assert mipt_amd.lib().mipt_hip_version().startswith(b"mipt_hip")
+
+
+def test_header_is_plain_c(tmp_path):
+    # the boundary is a C ABI: the header compiles as C99 (and C++17)
+    import shutil
+    import subprocess
+    src = tmp_path / "t.c"
+    src.write_text('#include "mipt_hip.h"\nint main(void) { return sizeof(mipt_hip_traj_spec) > 0 ? 0 : 1; }\n')
+    inc = os.path.join(ROOT, "include")
+    for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
+        if shutil.which(cc) is None:
+            pytest.skip(f"{cc} not available")
+        r = subprocess.run([cc, std, "-Wall", "-Werror", "-fsyntax-only", "-I" + inc] +
+                           (["-x", "c++"] if cc == "g++" else []) + [str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

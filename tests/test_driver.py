@@ -49,6 +49,42 @@ def test_sim_tmi_entropy_order_env_is_checked(M, monkeypatch, capfd):
     assert "SIM_TMI_ENTROPY_ORDER" in capfd.readouterr().err
 
 
+def _write(path, text):
+    path.parent.mkdir(parents=True, exist_ok=True)
+    path.write_text(text)
+    return str(path)
+
+
+def test_sim_tmi_resume_refuses_a_foreign_file(M, tmp_path, capfd):
+    # the resume contract of cpp/include/mipt/tmi/resume.hpp checks the
+    # generated name, the header and the p grid before any simulation
+    args = ["8", "2", "3", "2", "0", "1", "1", "0"]
+    # a generated name for other realizations
+    f = _write(tmp_path / "tmi_haar_n_8_real_5_0_1_res_2.csv", "p,tmi\n")
+    assert M.sim_tmi_main(args + [f]) == 1
+    assert "was produced with realizations=5" in capfd.readouterr().err
+    f = _write(tmp_path / "a.csv", "x,y\n")
+    assert M.sim_tmi_main(args + [f]) == 1
+    assert "expected header" in capfd.readouterr().err
+    f = _write(tmp_path / "b.csv", "p,tmi\n0.5,0.1\n")
+    assert M.sim_tmi_main(args + [f]) == 1
+    assert "switches away from p=0" in capfd.readouterr().err
+    f = _write(tmp_path / "c.csv", "p,tmi\n0,0.1\n0,0.2\n0,0.3\n0,0.4\n")  # 4 rows for a 3-row group
+    assert M.sim_tmi_main(args + [f]) == 1
+    assert "more rows" in capfd.readouterr().err
+
+
+def test_sim_tmi_resume_complete_file_is_left_alone(M, tmp_path, monkeypatch, capfd):
+    # 2 p values x 3 realizations x 1 cycle, already complete: nothing to do
+    text = "p,tmi\n" + "".join(f"{p},{0.1 * i}\n" for p in ("0", "1") for i in range(3))
+    f = _write(tmp_path / "done.csv", text)
+    assert M.sim_tmi_main(["8", "2", "3", "2", "0", "1", "1", "0", f]) == 0
+    assert "nothing to do" in capfd.readouterr().err
+    assert (tmp_path / "done.csv").read_text() == text
+    monkeypatch.setenv("SIM_TMI_RESUME", "maybe")
+    assert M.sim_tmi_main(["8", "2", "3", "2", "0", "1", "1", "0", f]) == 1
+
+
 def _oracle_run(n, layers, p, seed):
     psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed)
     return psi, out, p1, f, u
@@ -110,3 +146,25 @@ def test_sim_tmi_default_output_path(M, tmp_path, monkeypatch):
     lines = want.read_text().splitlines()
     assert lines[0] == "p,tmi" and len(lines) == 3
     assert lines[1].split(",")[0] == "0" and lines[2].split(",")[0] == "0.25"
+
+
+@pytest.mark.gpu
+def test_sim_tmi_resume_after_interruption_reproduces_the_scan(M, tmp_path, monkeypatch):
+    # a complete scan, then the same file cut inside a circuit (and mid-line):
+    # the re-issued command keeps the whole circuits, trims the rest and
+    # recomputes exactly the same rows (fixed MIPT_SEED)
+    monkeypatch.setenv("MIPT_SEED", "99")
+    args = ["8", "2", "3", "2", "0.1", "0.2", "1", "1"]
+    full = tmp_path / "full.csv"
+    print("sim_tmi resume", flush=True)
+    assert M.sim_tmi_main(args + [str(full)]) == 0
+    text = full.read_text()
+    lines = text.splitlines(keepends=True)
+    assert len(lines) == 1 + 2 * 3 * 2
+    cut = tmp_path / "cut.csv"
+    cut.write_text("".join(lines[:6]) + lines[6][:5])  # header + 5 rows + a partial line
+    assert M.sim_tmi_main(args + [str(cut)]) == 0
+    assert cut.read_text() == text
+    monkeypatch.setenv("SIM_TMI_RESUME", "0")
+    assert M.sim_tmi_main(args + [str(cut)]) == 0
+    assert cut.read_text() == text
