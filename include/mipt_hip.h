@@ -216,6 +216,14 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *batch, int tmi_block, int tmi
  * 24-52; NaN without a probe). */
 int mipt_hip_batch_records(mipt_hip_batch *batch, int layer_begin, int layer_end, int8_t *host_records);
 int mipt_hip_batch_probe_bloch(mipt_hip_batch *batch, double *host_bloch);
+/* Three-site reduced density matrices of trajectories [first, first+count):
+ * subsystems [nsub][3] (qubit k of a subsystem = reduced-basis bit k; the
+ * cyclic (i, i+1, i+2) list of cpp/include/mipt/rdm.hpp:92-106, or the
+ * horizontal-then-vertical grid list of rdm.hpp:108-125), output
+ * [count][nsub][8][8][re, im] FP64, trace-normalised -- the record payload of
+ * the MIPTRHO2 file (cpp/include/mipt/density.hpp:10-48). */
+int mipt_hip_batch_rho3(mipt_hip_batch *batch, int first, int count, const int *subsystems, int nsub,
+                        double *host_rho_ri);
 /* The stream the batch launches on. */
 void *mipt_hip_batch_stream(mipt_hip_batch *batch);
 /* Re-arms the in-kernel initial states and clears per-run records, so the
@@ -297,6 +305,13 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
  * arguments, CSV `p,tmi` and default output path; returns the exit code
  * (0, 1 on an error, 2 on too many arguments). */
 int mipt_hip_sim_tmi_main(int argc, char **argv);
+
+/* The three-site RDM record of cpp/apps/mipt.cpp:79-197, 309-400 on this
+ * engine: [d] [n] [periods] [realizations] [resolution] [p_min] [p_max]
+ * [circ_type]; writes info.csv and rho3.bin (d = 1) or rho3_2d.bin (d = 2) in
+ * the MIPTRHO2 version-2 format of cpp/include/mipt/density.hpp:10-48; returns
+ * the exit code. */
+int mipt_hip_mipt_main(int argc, char **argv);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,8 @@ def lib():
     L.mipt_hip_batch_records.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_int8)]
     L.mipt_hip_batch_observables_all.argtypes = [vp, i, i, dp, dp]
     L.mipt_hip_batch_autotune.argtypes = [vp, i, ip, dp]
+    L.mipt_hip_batch_rho3.argtypes = [vp, i, i, ip, i, dp]
+    L.mipt_hip_mipt_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     L.mipt_hip_batch_set_graph.argtypes = [vp, i]
     L.mipt_hip_batch_probe_bloch.argtypes = [vp, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
@@ -482,6 +484,15 @@ class Batch:
                                                     _dp(tm) if tmi_cycles > 0 else None), "batch_observables_all")
         return sh, tm[:, :tmi_cycles]
 
+    def rho3(self, subsystems, first: int = 0, count: int | None = None) -> np.ndarray:
+        """Trace-normalised three-site RDMs [count, nsub, 8, 8] (complex128);
+        subsystem qubit k = reduced-basis bit k."""
+        count = len(self.specs) - first if count is None else count
+        sub = np.ascontiguousarray(subsystems, np.int32).reshape(-1, 3)
+        out = np.zeros((count, len(sub), 8, 8, 2))
+        _check(lib().mipt_hip_batch_rho3(self._h, first, count, _ip(sub), len(sub), _dp(out)), "batch_rho3")
+        return out[..., 0] + 1j * out[..., 1]
+
     def autotune(self, probe_layers: int = 4):
         """Pick the fastest tile configuration on this batch; returns (index, ms per candidate)."""
         best = ctypes.c_int()
@@ -523,6 +534,14 @@ def run_batch(specs, tmi_cycles: int = 1, tmi_block: int = 0, entropy_order: int
     return out, tmi[:, :tmi_cycles]
 
 
+def mipt_main(argv) -> int:
+    """The three-site RDM record (same positional arguments as the
+    reference's mipt.exe), run in this process; returns its exit code."""
+    args = [b"mipt"] + [str(a).encode() for a in argv]
+    arr = (ctypes.c_char_p * len(args))(*args)
+    return lib().mipt_hip_mipt_main(len(args), arr)
+
+
 def sim_tmi_main(argv) -> int:
     """The sim_tmi scan (same positional arguments as the reference's
     sim_tmi.exe), run in this process; returns its exit code."""
@@ -532,4 +551,4 @@ def sim_tmi_main(argv) -> int:
 
 
 __all__ = ["lib", "last_error", "MiptHipError", "measure_collapse", "entropy", "rdm_subsystem", "rho1", "tmi",
-           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch", "run_batch", "sim_tmi_main"]
+           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch", "run_batch", "sim_tmi_main", "mipt_main"]

@@ -366,6 +366,101 @@ __global__ void __launch_bounds__(SMALL_THREADS) probe_bloch_kernel(DevArena a, 
     }
 }
 
+// Three-site reduced density matrices (8 x 8) of listed subsystems, one block
+// per (trajectory, subsystem): rho[r][c] = sum over the environment of
+// psi[e|r] conj(psi[e|c]), reduced-basis bit k = subsystem qubit k, FP64
+// sums of the complex64 amplitudes in a fixed order, then divided by the
+// trace.  Output [traj][subsystem][8][8][re, im] (the record payload of
+// cpp/include/mipt/density.hpp:40-48).
+__global__ void __launch_bounds__(SMALL_THREADS) rho3_kernel(DevArena a, int first, int nsub,
+                                                            const int *__restrict__ subs, double *__restrict__ out)
+{
+    __shared__ double acc_s[72];
+    const int traj = first + blockIdx.x / nsub, sub = blockIdx.x % nsub;
+    const TrajDesc d = a.desc[traj];
+    const float2 *psi = a.states + d.state_off;
+    const int q[3] = {subs[3 * sub], subs[3 * sub + 1], subs[3 * sub + 2]};
+    int srt[3] = {q[0], q[1], q[2]};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2 - i; ++j)
+            if (srt[j] > srt[j + 1])
+            {
+                const int t = srt[j];
+                srt[j] = srt[j + 1];
+                srt[j + 1] = t;
+            }
+    std::uint32_t off[8];
+    for (int k = 0; k < 8; ++k)
+        off[k] = ((k & 1) ? (1u << q[0]) : 0u) | ((k & 2) ? (1u << q[1]) : 0u) | ((k & 4) ? (1u << q[2]) : 0u);
+    // lower triangle (36 complex entries) per thread
+    double re[36], im[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) re[i] = im[i] = 0.0;
+    const std::uint32_t nenv = 1u << (d.nq - 3);
+    for (std::uint32_t e = threadIdx.x; e < nenv; e += SMALL_THREADS)
+    {
+        std::uint32_t base = e;
+        for (int j = 0; j < 3; ++j)
+            base = ((base >> srt[j]) << (srt[j] + 1)) | (base & ((1u << srt[j]) - 1u));
+        double xr[8], xi[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            const float2 v = psi[base | off[k]];
+            xr[k] = v.x;
+            xi[k] = v.y;
+        }
+        int t = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c <= r; ++c, ++t)
+            {
+                re[t] += xr[r] * xr[c] + xi[r] * xi[c];
+                im[t] += xi[r] * xr[c] - xr[r] * xi[c];
+            }
+    }
+    // wave sums by shuffles (fixed tree), then the four waves in order
+    __shared__ double wsum[SMALL_THREADS / 64][72];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 36; ++i)
+    {
+        double vr = re[i], vi = im[i];
+#pragma unroll
+        for (int sh = 32; sh > 0; sh >>= 1)
+        {
+            vr += __shfl_down(vr, sh, 64);
+            vi += __shfl_down(vi, sh, 64);
+        }
+        if (lane == 0)
+        {
+            wsum[wave][2 * i] = vr;
+            wsum[wave][2 * i + 1] = vi;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 72)
+    {
+        double t = 0.0;
+        for (int w = 0; w < SMALL_THREADS / 64; ++w) t += wsum[w][threadIdx.x];
+        acc_s[threadIdx.x] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64)
+    {
+        const int r = threadIdx.x >> 3, c = threadIdx.x & 7;
+        double tr = 0.0;
+        for (int k = 0; k < 8; ++k) tr += acc_s[2 * (k * (k + 1) / 2 + k)];
+        const int lo = r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r;
+        const double vr = acc_s[2 * lo], vi = (r >= c ? 1.0 : -1.0) * acc_s[2 * lo + 1];
+        const double f = tr > 0.0 ? 1.0 / tr : 1.0;
+        double *o = out + (static_cast<std::size_t>(blockIdx.x) * 64 + r * 8 + c) * 2;
+        o[0] = vr * f;
+        o[1] = vi * f;
+    }
+}
+
 // ============================================================== tiled sweeps
 // u = in-tile index (k bits).  u bits 0..c-1 <-> physical bits 0..c-1; u bits
 // c..k-1 <-> physical bits s..s+k-c-1 (the window); the other physical bits
@@ -2193,6 +2288,57 @@ int mipt_hip_batch_records(mipt_hip_batch *b, int layer_begin, int layer_end, in
             set_error("trajectory %d hit a zero-weight measurement branch", i);
             return st[i];
         }
+    return OK;
+}
+
+int mipt_hip_batch_rho3(mipt_hip_batch *b, int first, int count, const int *subsystems, int nsub, double *host)
+{
+    if (!b || !host || !subsystems || nsub < 1 || first < 0 || count < 1 ||
+        first + count > static_cast<int>(b->desc.size()))
+    {
+        set_error("batch_rho3: bad arguments");
+        return E_ARG;
+    }
+    for (int i = first; i < first + count; ++i)
+    {
+        const TrajDesc &d = b->desc[i];
+        for (int k = 0; k < 3 * nsub; ++k)
+            if (subsystems[k] < 0 || subsystems[k] >= d.L || d.nq < 4)
+            {
+                set_error("batch_rho3: subsystem qubit %d outside trajectory %d's %d sites", subsystems[k], i, d.L);
+                return E_ARG;
+            }
+        for (int s = 0; s < nsub; ++s)
+        {
+            const int *q = subsystems + 3 * s;
+            if (q[0] == q[1] || q[0] == q[2] || q[1] == q[2])
+            {
+                set_error("batch_rho3: subsystem %d repeats a qubit", s);
+                return E_ARG;
+            }
+        }
+    }
+    int *dsub = nullptr;
+    double *dev = nullptr;
+    const std::size_t nout = static_cast<std::size_t>(count) * nsub * 128;
+    MIPT_HIP_CHECK(hipMalloc(&dsub, 3 * nsub * sizeof(int)));
+    hipError_t e = hipMalloc(&dev, nout * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(dsub, subsystems, 3 * nsub * sizeof(int), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+    {
+        hipLaunchKernelGGL(rho3_kernel, dim3(count * nsub), dim3(SMALL_THREADS), 0, b->stream, b->a, first, nsub, dsub,
+                           dev);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, nout * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dsub);
+    if (dev) (void)hipFree(dev);
+    if (e != hipSuccess)
+    {
+        set_error("batch_rho3: %s", hipGetErrorString(e));
+        return static_cast<int>(e);
+    }
     return OK;
 }
 

@@ -453,6 +453,157 @@ int run_sim_tmi(int argc, char **argv)
     return 0;
 }
 
+// ------------------------------------------------------------------ mipt
+// The three-site RDM record of the reference's mipt.exe
+// (cpp/apps/mipt.cpp:79-197, 309-400; cpp/include/mipt/rdm.hpp:92-125;
+// cpp/include/mipt/density.hpp:10-48): positional arguments
+//   [d = 1] [n] [periods] [realizations] [resolution] [p_min] [p_max] [circ_type]
+// info.csv plus rho3.bin (d = 1: the n cyclic triples (i, i+1, i+2) of a ring)
+// or rho3_2d.bin (d = 2: an n x n grid, row triples then column triples,
+// periodic), MIPTRHO2 version 2, one record per trajectory in (p, realization)
+// order with the trace-normalised 8 x 8 matrices.  What differs: circ_type 1
+// (Haar U(4)) only, also for d = 2 (the four-colour grid layering with Haar
+// bonds, rows closed when n is even); 2 * periods layers; seeds as sim_tmi.
+template <typename T>
+void put(std::ofstream &f, const T &v)
+{
+    f.write(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+
+int run_mipt(int argc, char **argv)
+{
+    if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0))
+    {
+        std::cout << "Usage: " << argv[0]
+                  << " [d = 1] [n = 10] [periods = 10] [realizations = 10] [resolution = 5] [p_min = 0.0] [p_max = 1.0]"
+                     " [circ_type = 1]\n"
+                  << "  d = 1: an n-site ring, the n cyclic three-site RDMs per trajectory into rho3.bin\n"
+                  << "  d = 2: an n x n grid, 2 n^2 three-site RDMs (rows, then columns) into rho3_2d.bin\n"
+                  << "  circ_type: 1 = random Haar U(4) (the only one built here)\n";
+        return 0;
+    }
+    if (argc > 9)
+    {
+        std::cerr << "Too many arguments. Use --help for usage.\n";
+        return 2;
+    }
+    const int d = argc > 1 ? std::stoi(argv[1]) : 1;
+    const int n = argc > 2 ? std::stoi(argv[2]) : (d == 1 ? 10 : 4);
+    const int periods = argc > 3 ? std::stoi(argv[3]) : (d == 1 ? 10 : 32);
+    const int realizations = argc > 4 ? std::stoi(argv[4]) : 10;
+    const int res = argc > 5 ? std::stoi(argv[5]) : 5;
+    const double p_min = argc > 6 ? std::stod(argv[6]) : 0.0;
+    const double p_max = argc > 7 ? std::stod(argv[7]) : 1.0;
+    const int circ = argc > 8 ? std::stoi(argv[8]) : 1;
+    // cpp/apps/mipt.cpp:342-366
+    if (n <= 0) throw std::invalid_argument("n must be positive.");
+    if (d != 1 && d != 2) throw std::invalid_argument("d must be 1 or 2.");
+    if (res <= 0) throw std::invalid_argument("resolution must be positive.");
+    if (realizations <= 0) throw std::invalid_argument("realizations must be positive.");
+    if (periods < 0) throw std::invalid_argument("periods must be non-negative.");
+    if (p_min < 0.0 || p_max > 1.0 || p_min > p_max) throw std::invalid_argument("Require 0 <= p_min <= p_max <= 1.");
+    if (circ != 1)
+        throw std::invalid_argument("circ_type " + std::to_string(circ) +
+                                    " is not built on this engine; use 1 (random Haar U(4)).");
+    const int L = d == 1 ? n : n * n;
+    if (n < 3) throw std::invalid_argument("Saving three-qubit reduced density matrices requires n >= 3.");
+    if (d == 1 && (n % 2) != 0) throw std::invalid_argument("Random Haar Unitary requires an even n for periodic brickwork geometry.");
+    if (L > 26) throw std::invalid_argument("this engine holds states up to 26 qubits.");
+    std::vector<int> subs;
+    if (d == 1)
+        for (int i = 0; i < n; ++i) subs.insert(subs.end(), {i, (i + 1) % n, (i + 2) % n});
+    else
+    {
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) subs.insert(subs.end(), {i * n + j, i * n + (j + 1) % n, i * n + (j + 2) % n});
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                subs.insert(subs.end(), {j * n + i, ((j + 1) % n) * n + i, ((j + 2) % n) * n + i});
+    }
+    const int nsub = static_cast<int>(subs.size() / 3);
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
+    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    const std::vector<double> ps = linspace(p_min, p_max, res);
+    const std::uint64_t total = static_cast<std::uint64_t>(res) * realizations;
+    {
+        std::ofstream info("info.csv");
+        info << "n,periods,realizations,resolution,p_min,p_max,circ_type,circuit_name,fermion_trace,cudaq_backend\n";
+        info << n << "," << periods << "," << realizations << "," << res << "," << p_min << "," << p_max << "," << circ
+             << ",Random Haar Unitary,0,hip-gfx950\n";
+        if (!info) throw std::runtime_error("Could not write info.csv");
+    }
+    const std::string path = d == 1 ? "rho3.bin" : "rho3_2d.bin";
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) throw std::runtime_error("Could not create density-matrix file: " + path);
+    const char magic[8] = {'M', 'I', 'P', 'T', 'R', 'H', 'O', '2'};
+    f.write(magic, 8);
+    put<std::uint32_t>(f, 0x01020304u);                 // endian marker
+    put<std::uint32_t>(f, 2u);                          // version
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(d));
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(L));  // total qubits
+    put<std::uint32_t>(f, 3u);                          // kept qubits
+    put<std::uint32_t>(f, 8u);                          // matrix dimension
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(periods));
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(realizations));
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(res));
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(n));  // grid_x
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(d == 1 ? 1 : n));
+    put<std::uint32_t>(f, static_cast<std::uint32_t>(nsub));
+    put<std::uint64_t>(f, total);                       // records = trajectories
+    put<double>(f, p_min);
+    put<double>(f, p_max);
+    for (int q : subs) put<std::uint32_t>(f, static_cast<std::uint32_t>(q));
+    std::cout << "Running " << (d == 1 ? "1D " : "2D ") << L << "-qubit Haar simulation of " << total << " circuits.\n";
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const std::uint64_t per_batch =
+        std::max<std::uint64_t>(1, static_cast<std::uint64_t>(budget_gib * 1073741824.0 / std::ldexp(8.0, L)));
+    std::vector<double> rho;
+    for (std::uint64_t first = 0; first < total; first += per_batch)
+    {
+        const int count = static_cast<int>(std::min<std::uint64_t>(per_batch, total - first));
+        std::vector<mipt_hip_traj_spec> specs(count);
+        for (int i = 0; i < count; ++i)
+        {
+            const std::uint64_t idx = first + i;
+            specs[i] = mipt_hip_traj_spec{L, 2 * periods, ps[idx / realizations], trajectory_seed(master, idx), 1, 1, -1,
+                                          0, -1, d == 1 ? 0 : n};
+        }
+        mipt_hip_batch *b = nullptr;
+        const mipt_hip_batch_config cfg{0, 0, 0};
+        check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
+        int st = mipt_hip_batch_setup(b, specs.data(), count);
+        if (!st) st = mipt_hip_batch_generate(b);
+        if (!st) st = mipt_hip_batch_evolve(b);
+        if (!st) st = mipt_hip_batch_synchronize(b);
+        rho.assign(static_cast<std::size_t>(count) * nsub * 128, 0.0);
+        if (!st) st = mipt_hip_batch_rho3(b, 0, count, subs.data(), nsub, rho.data());
+        if (!st)
+        {
+            // a zero-weight branch anywhere in the batch is an error, as in the engine's readouts
+            std::vector<int> o(static_cast<std::size_t>(2 * periods) * L);
+            std::vector<std::uint8_t> fl(o.size());
+            for (int i = 0; i < count && !st; ++i) st = mipt_hip_batch_outcomes(b, i, o.data(), fl.data(), nullptr, nullptr);
+        }
+        (void)mipt_hip_batch_destroy(b);
+        check(st, "mipt batch");
+        for (int i = 0; i < count; ++i)
+        {
+            const std::uint64_t idx = first + i;
+            put<std::uint32_t>(f, static_cast<std::uint32_t>(idx / realizations));
+            put<std::uint32_t>(f, static_cast<std::uint32_t>(idx % realizations));
+            put<double>(f, specs[i].p);
+            f.write(reinterpret_cast<const char *>(rho.data() + static_cast<std::size_t>(i) * nsub * 128),
+                    static_cast<std::streamsize>(nsub * 128 * sizeof(double)));
+        }
+        if (!f) throw std::runtime_error("Failed while writing density-matrix payload.");
+    }
+    f.close();
+    std::cout << "Saved " << nsub << " three-qubit reduced density matrices per trajectory to " << path << ".\n";
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -525,6 +676,19 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
     if (eng) (void)mipt_hip_engine_destroy(eng);
     (void)mipt_hip_batch_destroy(b);
     return st;
+}
+
+int mipt_hip_mipt_main(int argc, char **argv)
+{
+    try
+    {
+        return run_mipt(argc, argv);
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "\nerror: " << e.what() << '\n';
+        return 1;
+    }
 }
 
 int mipt_hip_sim_tmi_main(int argc, char **argv)
