@@ -208,6 +208,14 @@ int mipt_hip_batch_tmi(mipt_hip_batch *batch, int traj, int block, int cycle, in
  * which at L=20 is about 8x faster than one entropy call at a time. */
 int mipt_hip_batch_observables_all(mipt_hip_batch *batch, int tmi_block, int tmi_cycles, double *s_half,
                                    double *tmi);
+/* Translation-averaged entropies of cyclic blocks, the per-trajectory
+ * quantity of cpp/include/mipt/entropy.hpp:521-577 (entropy.exe): for block
+ * sizes l = min_kept..max_kept, the mean over the L starts i of the von
+ * Neumann S1 (only for l <= max_s1_kept, else NaN) and Renyi-2 S2 =
+ * -ln Tr(rho^2) of sites {i, ..., i+l-1 mod L}, in nats; outputs
+ * [count][max_kept - min_kept + 1] (s1_nats may be NULL). */
+int mipt_hip_batch_translation_entropies(mipt_hip_batch *batch, int min_kept, int max_kept, int max_s1_kept,
+                                         double *s1_nats, double *s2_nats);
 /* Bulk readout for learning from the simulator: the measurement records of
  * every trajectory for layers [layer_begin, layer_end) as int8
  * [count][layers][L] (-1 unmeasured, else the outcome; every trajectory must
@@ -312,6 +320,12 @@ int mipt_hip_sim_tmi_main(int argc, char **argv);
  * the MIPTRHO2 version-2 format of cpp/include/mipt/density.hpp:10-48; returns
  * the exit code. */
 int mipt_hip_mipt_main(int argc, char **argv);
+
+/* entropy.exe on this engine (cpp/apps/entropy.cpp:16-80): `N periods p
+ * realizations circ_type output.csv`, translation-averaged S1/S2 (nats) per
+ * block size with Welford statistics over realizations, the CSV schema of
+ * cpp/include/mipt/entropy_resume.hpp:52-56; returns the exit code. */
+int mipt_hip_entropy_main(int argc, char **argv);
 
 #ifdef __cplusplus
 }

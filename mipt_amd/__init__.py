@@ -99,6 +99,8 @@ def lib():
     L.mipt_hip_batch_autotune.argtypes = [vp, i, ip, dp]
     L.mipt_hip_batch_rho3.argtypes = [vp, i, i, ip, i, dp]
     L.mipt_hip_mipt_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
+    L.mipt_hip_entropy_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
+    L.mipt_hip_batch_translation_entropies.argtypes = [vp, i, i, i, dp, dp]
     L.mipt_hip_batch_set_graph.argtypes = [vp, i]
     L.mipt_hip_batch_probe_bloch.argtypes = [vp, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
@@ -493,6 +495,16 @@ class Batch:
         _check(lib().mipt_hip_batch_rho3(self._h, first, count, _ip(sub), len(sub), _dp(out)), "batch_rho3")
         return out[..., 0] + 1j * out[..., 1]
 
+    def translation_entropies(self, min_kept: int, max_kept: int, max_s1_kept: int | None = None):
+        """(S1, S2) [count, max_kept-min_kept+1] in nats, averaged over the L cyclic starts."""
+        max_s1_kept = max_kept if max_s1_kept is None else max_s1_kept
+        n = len(self.specs)
+        s1 = np.zeros((n, max_kept - min_kept + 1))
+        s2 = np.zeros_like(s1)
+        _check(lib().mipt_hip_batch_translation_entropies(self._h, min_kept, max_kept, max_s1_kept, _dp(s1), _dp(s2)),
+               "batch_translation_entropies")
+        return s1, s2
+
     def autotune(self, probe_layers: int = 4):
         """Pick the fastest tile configuration on this batch; returns (index, ms per candidate)."""
         best = ctypes.c_int()
@@ -542,6 +554,13 @@ def mipt_main(argv) -> int:
     return lib().mipt_hip_mipt_main(len(args), arr)
 
 
+def entropy_main(argv) -> int:
+    """entropy.exe's scan (N periods p realizations circ_type output.csv) in this process."""
+    args = [b"entropy"] + [str(a).encode() for a in argv]
+    arr = (ctypes.c_char_p * len(args))(*args)
+    return lib().mipt_hip_entropy_main(len(args), arr)
+
+
 def sim_tmi_main(argv) -> int:
     """The sim_tmi scan (same positional arguments as the reference's
     sim_tmi.exe), run in this process; returns its exit code."""
@@ -551,4 +570,4 @@ def sim_tmi_main(argv) -> int:
 
 
 __all__ = ["lib", "last_error", "MiptHipError", "measure_collapse", "entropy", "rdm_subsystem", "rho1", "tmi",
-           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch", "run_batch", "sim_tmi_main", "mipt_main"]
+           "PlainEngine", "layer_bonds", "run_plain_trajectory", "Spec", "Batch", "run_batch", "sim_tmi_main", "mipt_main", "entropy_main"]

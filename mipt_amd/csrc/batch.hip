@@ -2550,6 +2550,78 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cyc
     return OK;
 }
 
+int mipt_hip_batch_translation_entropies(mipt_hip_batch *b, int min_kept, int max_kept, int max_s1_kept,
+                                         double *s1_nats, double *s2_nats)
+{
+    if (!b || !s2_nats || min_kept < 1 || max_kept < min_kept || (max_s1_kept >= min_kept && !s1_nats))
+    {
+        set_error("batch_translation_entropies: bad arguments");
+        return E_ARG;
+    }
+    const int count = static_cast<int>(b->desc.size());
+    const int sizes = max_kept - min_kept + 1;
+    for (const TrajDesc &d : b->desc)
+        if (max_kept >= d.L)
+        {
+            set_error("batch_translation_entropies: blocks of %d sites in a %d-site chain", max_kept, d.L);
+            return E_ARG;
+        }
+    std::vector<int> idx(count);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+        return b->desc[x].nq != b->desc[y].nq ? b->desc[x].nq < b->desc[y].nq : b->desc[x].L < b->desc[y].L;
+    });
+    const double ln2 = 0.69314718055994530942;
+    for (int g0 = 0; g0 < count;)
+    {
+        int g1 = g0;
+        while (g1 < count && b->desc[idx[g1]].nq == b->desc[idx[g0]].nq && b->desc[idx[g1]].L == b->desc[idx[g0]].L) ++g1;
+        const int nq = b->desc[idx[g0]].nq, L = b->desc[idx[g0]].L, cnt = g1 - g0;
+        // items (trajectory j, start i) for i = 0..L-1: the cyclic block of
+        // cpp/include/mipt/entropy.hpp:521-577, averaged over the L starts
+        std::vector<const float2 *> ps(static_cast<std::size_t>(cnt) * L);
+        for (int j = 0; j < cnt; ++j)
+            for (int i = 0; i < L; ++i) ps[static_cast<std::size_t>(j) * L + i] = b->a.states + b->desc[idx[g0 + j]].state_off;
+        for (int l = min_kept; l <= max_kept; ++l)
+        {
+            std::vector<int> kept(static_cast<std::size_t>(cnt) * L * l);
+            for (int j = 0; j < cnt; ++j)
+                for (int i = 0; i < L; ++i)
+                    for (int k = 0; k < l; ++k) kept[(static_cast<std::size_t>(j) * L + i) * l + k] = (i + k) % L;
+            std::vector<double> v(static_cast<std::size_t>(cnt) * L);
+            MIPT_HIP_TRY(purity_many_f32(ps.data(), nq, kept.data(), l, cnt * L, b->stream, v.data()));
+            for (int j = 0; j < cnt; ++j)
+            {
+                double acc = 0.0;
+                for (int i = 0; i < L; ++i)
+                {
+                    const double pu = v[static_cast<std::size_t>(j) * L + i];
+                    acc += (pu > 0.0 && pu < 1.0) ? -std::log(pu) : 0.0;
+                }
+                s2_nats[static_cast<std::size_t>(idx[g0 + j]) * sizes + (l - min_kept)] = acc / L;
+            }
+            if (s1_nats)
+            {
+                if (l <= max_s1_kept)
+                {
+                    MIPT_HIP_TRY(entropy_many_bits_f32(ps.data(), nq, kept.data(), l, cnt * L, b->stream, v.data()));
+                    for (int j = 0; j < cnt; ++j)
+                    {
+                        double acc = 0.0;
+                        for (int i = 0; i < L; ++i) acc += v[static_cast<std::size_t>(j) * L + i];
+                        s1_nats[static_cast<std::size_t>(idx[g0 + j]) * sizes + (l - min_kept)] = acc * ln2 / L;
+                    }
+                }
+                else
+                    for (int j = 0; j < cnt; ++j)
+                        s1_nats[static_cast<std::size_t>(idx[g0 + j]) * sizes + (l - min_kept)] = NAN;
+            }
+        }
+        g0 = g1;
+    }
+    return OK;
+}
+
 int mipt_hip_batch_tmi(mipt_hip_batch *b, int traj, int block, int cycle, int order, double *tmi)
 {
     if (!b || traj < 0 || traj >= static_cast<int>(b->desc.size()) || !tmi || (order != 1 && order != 2))

@@ -63,6 +63,9 @@ int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle
 // with batched eigensolves; kept is [count][kept_count]
 int entropy_many_bits_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count,
                           hipStream_t stream, double *out);
+// Tr(rho^2) of many (state, kept set) items of one shape
+int purity_many_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+                    double *out);
 // Renyi-2 entropy -log2 Tr(rho^2) of the kept set, bits
 int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
 int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4);

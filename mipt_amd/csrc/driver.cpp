@@ -42,6 +42,7 @@
 #include <unistd.h>
 #include <cctype>
 #include <iterator>
+#include <limits>
 #include <vector>
 
 namespace
@@ -604,6 +605,124 @@ int run_mipt(int argc, char **argv)
     return 0;
 }
 
+// --------------------------------------------------------------- entropy
+// entropy.exe (cpp/apps/entropy.cpp:16-80, cpp/include/mipt/entropy.hpp:400-425,
+// 521-640, cpp/include/mipt/entropy_resume.hpp:52-56): `n periods p
+// realizations circ_type output.csv`; per trajectory the translation-averaged
+// S1 (MIPT_ENTROPY_S1, MIPT_ENTROPY_S1_MAX_LA) and S2 in nats for blocks
+// L_A = 2..n/2, accumulated over realizations (Welford), one CSV row per
+// L_A.  What differs: circ_type 1 (Haar) only; no resume; seeds as sim_tmi.
+struct Welford
+{
+    std::uint64_t count = 0;
+    double mean = 0.0, m2 = 0.0;
+    void add(double v)
+    {
+        ++count;
+        const double d = v - mean;
+        mean += d / static_cast<double>(count);
+        m2 += d * (v - mean);
+    }
+    double sd() const { return count > 1 ? std::sqrt(std::max(0.0, m2 / static_cast<double>(count - 1))) : 0.0; }
+    double se() const { return count > 1 ? sd() / std::sqrt(static_cast<double>(count)) : 0.0; }
+};
+
+int run_entropy(int argc, char **argv)
+{
+    if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0))
+    {
+        std::cout << "Usage: " << argv[0] << " N periods p realizations circ_type output.csv\n"
+                  << "  translation-averaged S1 and S2 (nats) of cyclic blocks L_A = 2..N/2\n"
+                  << "  circ_type: 1 = random Haar U(4) (the only one built here)\n"
+                  << "  MIPT_ENTROPY_S1=0 skips S1; MIPT_ENTROPY_S1_MAX_LA=k caps its block size\n";
+        return 0;
+    }
+    if (argc != 7)
+    {
+        std::cout << "Usage: " << argv[0] << " N periods p realizations circ_type output.csv\n";
+        return 2;
+    }
+    const int n = std::stoi(argv[1]);
+    const int periods = std::stoi(argv[2]);
+    const double p = std::stod(argv[3]);
+    const int realizations = std::stoi(argv[4]);
+    const int circ = std::stoi(argv[5]);
+    const std::string output = argv[6];
+    if (n < 4) throw std::invalid_argument("N must be at least 4 for the requested cyclic entropy scan.");
+    if (periods < 0) throw std::invalid_argument("periods must be non-negative.");
+    if (p < 0.0 || p > 1.0) throw std::invalid_argument("p must be in [0,1].");
+    if (realizations <= 0) throw std::invalid_argument("realizations must be positive.");
+    if (circ != 1)
+        throw std::invalid_argument("circ_type " + std::to_string(circ) +
+                                    " is not built on this engine; use 1 (random Haar U(4)).");
+    if (n % 2 != 0) throw std::invalid_argument("Random Haar Unitary requires an even N for periodic brickwork geometry.");
+    if (n > 26) throw std::invalid_argument("this engine holds states up to N = 26.");
+    const int max_kept = n / 2, min_kept = 2;
+    int max_s1 = env_flag("MIPT_ENTROPY_S1", true) ? max_kept : 0;
+    if (const char *e = std::getenv("MIPT_ENTROPY_S1_MAX_LA"))
+    {
+        const long cap = std::atol(e);
+        if (cap < 0 || cap > 62) throw std::invalid_argument("MIPT_ENTROPY_S1_MAX_LA must lie in [0, 62].");
+        if (cap > 0 && max_s1 > 0) max_s1 = std::min<int>(max_s1, static_cast<int>(cap));
+    }
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
+    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    const int sizes = max_kept - min_kept + 1;
+    std::vector<Welford> s1(sizes), s2(sizes);
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const int per_batch = static_cast<int>(std::max<double>(1.0, std::min<double>(realizations, budget_gib * 1073741824.0 / std::ldexp(8.0, n))));
+    for (int first = 0; first < realizations; first += per_batch)
+    {
+        const int count = std::min(per_batch, realizations - first);
+        std::vector<mipt_hip_traj_spec> specs(count);
+        for (int i = 0; i < count; ++i)
+            specs[i] = mipt_hip_traj_spec{n, 2 * periods, p, trajectory_seed(master, static_cast<std::uint64_t>(first + i)),
+                                          1, 1, -1, 0, -1, 0};
+        mipt_hip_batch *b = nullptr;
+        const mipt_hip_batch_config cfg{0, 0, 0};
+        check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
+        std::vector<double> v1(static_cast<std::size_t>(count) * sizes), v2(v1.size());
+        int st = mipt_hip_batch_setup(b, specs.data(), count);
+        if (!st) st = mipt_hip_batch_generate(b);
+        if (!st) st = mipt_hip_batch_evolve(b);
+        if (!st) st = mipt_hip_batch_synchronize(b);
+        if (!st) st = mipt_hip_batch_translation_entropies(b, min_kept, max_kept, max_s1, v1.data(), v2.data());
+        (void)mipt_hip_batch_destroy(b);
+        check(st, "entropy batch");
+        for (int i = 0; i < count; ++i)
+            for (int k = 0; k < sizes; ++k)
+            {
+                if (min_kept + k <= max_s1) s1[k].add(v1[static_cast<std::size_t>(i) * sizes + k]);
+                s2[k].add(v2[static_cast<std::size_t>(i) * sizes + k]);
+            }
+        std::cerr << "processed " << first + count << '/' << realizations << " realizations\n";
+    }
+    mkdirs_for(output);
+    std::ofstream csv(output, std::ios::out | std::ios::trunc);
+    if (!csv) throw std::runtime_error("Could not open output CSV: " + output);
+    csv << std::setprecision(17);
+    csv << "N,periods,p,requested_realizations,completed_realizations,circ_type,circuit_name,fermionic_trace,L_A,x,ln_x,"
+           "S1_mean,S1_stddev,S1_stderr,S2_mean,S2_stddev,S2_stderr,subsystems_per_realization,total_subsystems\n";
+    const double pi = 3.141592653589793238462643383279502884;
+    for (int k = 0; k < sizes; ++k)
+    {
+        const int la = min_kept + k;
+        const double x = (static_cast<double>(n) / pi) * std::sin(pi * la / static_cast<double>(n));
+        const bool has1 = s1[k].count > 0;
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        csv << n << ',' << periods << ',' << p << ',' << realizations << ',' << s2[k].count << ',' << circ << ','
+            << "\"Random Haar Unitary\"" << ',' << 0 << ',' << la << ',' << x << ',' << std::log(x) << ','
+            << (has1 ? s1[k].mean : nan) << ',' << (has1 ? s1[k].sd() : nan) << ',' << (has1 ? s1[k].se() : nan) << ','
+            << s2[k].mean << ',' << s2[k].sd() << ',' << s2[k].se() << ',' << n << ','
+            << static_cast<std::uint64_t>(n) * s2[k].count << '\n';
+    }
+    csv.flush();
+    if (!csv) throw std::runtime_error("Failed while writing output CSV: " + output);
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -676,6 +795,19 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
     if (eng) (void)mipt_hip_engine_destroy(eng);
     (void)mipt_hip_batch_destroy(b);
     return st;
+}
+
+int mipt_hip_entropy_main(int argc, char **argv)
+{
+    try
+    {
+        return run_entropy(argc, argv);
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "\nerror: " << e.what() << '\n';
+        return 1;
+    }
 }
 
 int mipt_hip_mipt_main(int argc, char **argv)

@@ -704,6 +704,63 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
     return OK;
 }
 
+// Tr(rho^2) of `count` m x m matrices laid out every m*m entries, one block
+// each, fixed reduction order.
+__global__ void purity_many_kernel(const double2 *__restrict__ rho, int m, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const std::size_t mm = static_cast<std::size_t>(m) * m;
+    const double2 *r = rho + blockIdx.x * mm;
+    double acc = 0.0;
+    for (std::size_t i = threadIdx.x; i < mm; i += 256) acc += r[i].x * r[i].x + r[i].y * r[i].y;
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Purities Tr(rho^2) of `count` (state, kept set) items of one shape (the
+// Renyi-2 counterpart of entropy_many: one strided RDM buffer per chunk).
+int purity_many(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+                double *out)
+{
+    if (!psis || !out || count < 0 || (count > 0 && !kept))
+    {
+        set_error("purity_many: null argument");
+        return E_ARG;
+    }
+    if (kept_count == 0 || kept_count == n)
+    {
+        for (int i = 0; i < count; ++i) out[i] = 1.0;
+        return OK;
+    }
+    Workspace &w = ws();
+    const int m = 1 << kept_count;
+    const std::size_t mm = static_cast<std::size_t>(m) * m;
+    const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, (std::size_t{1} << 30) / (mm * 16))));
+    MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, chunk * mm));
+    MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk)));
+    for (int c0 = 0; c0 < count; c0 += chunk)
+    {
+        const int nb = std::min(chunk, count - c0);
+        for (int i = 0; i < nb; ++i)
+        {
+            int mo = 0;
+            MIPT_HIP_TRY(build_rho(psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
+                                   kept_count, stream, mo, w.rho_b + i * mm));
+        }
+        hipLaunchKernelGGL(purity_many_kernel, dim3(nb), dim3(256), 0, stream, w.rho_b, m, w.ev_b);
+        MIPT_HIP_CHECK(hipGetLastError());
+        MIPT_HIP_CHECK(hipMemcpyAsync(out + c0, w.ev_b, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    return OK;
+}
+
 int rdm_export(const void *psi, bool f64, int n, const int *kept, int count, double *host)
 {
     if (!host || count > 12)
@@ -745,6 +802,12 @@ int entropy_many_bits_f32(const float2 *const *psis, int n, const int *kept, int
                           hipStream_t stream, double *out)
 {
     return entropy_many(psis, n, kept, kept_count, count, stream, out);
+}
+
+int purity_many_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+                    double *out)
+{
+    return purity_many(psis, n, kept, kept_count, count, stream, out);
 }
 
 int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4)
