@@ -464,7 +464,7 @@ def decoder_leg(n_train, n_test, epochs, batch=1024, lr=1e-3):
     t = time.perf_counter()
     loss = None
     for e in range(epochs):
-        loss = d.train(rng.permutation(n_train)[: steps * batch], batch, lr=lr)
+        loss = d.train(rng.permutation(n_train)[: steps * batch], batch, lr=dec.cosine_lr(lr, e, epochs))
         log(f"decoder epoch {e + 1}/{epochs}: mean batch loss {loss:.4f}")
     train_s = time.perf_counter() - t
     d.predict(n_train, min(n_test, batch))  # warm
@@ -481,7 +481,7 @@ def decoder_leg(n_train, n_test, epochs, batch=1024, lr=1e-3):
     d.close()
     return {"workload": "L=16 ring, shared Haar gates (seed 7), 32 scrambling layers, probe at site 8, "
                         "32 measured layers, p ~ U[0.05,0.30]",
-            "train": n_train, "held_out": n_test, "epochs": epochs, "batch": batch, "lr": lr,
+            "train": n_train, "held_out": n_test, "epochs": epochs, "batch": batch, "lr": lr, "lr_schedule": "half cosine per epoch",
             "final_train_loss": round(loss, 4), "train_seconds": round(train_s, 2),
             "mean_abs_dS_nats": round(float(err.mean()), 4), "baseline_mean_abs_dS_nats": round(float(err_b.mean()), 4),
             "per_p_bin": per_bin, "inferences_per_s": round(n_test / inf_s, 1),

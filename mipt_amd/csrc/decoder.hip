@@ -55,7 +55,7 @@ constexpr int TM = 64;        // output tile rows
 constexpr int TK = 32;        // K step (one MFMA 16x16x32)
 constexpr int LP = TK + 8;    // LDS row stride of a K step (bf16), 80 B
 constexpr int CP = CH + 8;    // LDS row stride of the epilogue tile (bf16), 144 B
-constexpr int CHUNK = 2048;   // positions per weight-gradient partial
+constexpr int WG_BLOCKS = 2048;  // weight-gradient grid: 8 blocks on each of 256 CUs
 constexpr float NORM_EPS = 1e-12f;
 constexpr float MU = 1.0f;            // weight of the direction term
 constexpr float R_MAX = 1.0f - 1e-6f;  // Bloch length cap of the entropy term
@@ -201,12 +201,10 @@ __global__ void __launch_bounds__(NT) conv_fwd_kernel(const bf16 *__restrict__ i
 }
 
 // ------------------------------------------------------------- input grad
-// din[q][ci] = sum_{tap,co} wd[ci][tap*64+co] * dz[q - off(tap)][co], with
-// dz = dact * (act > 0) rounded to bf16 (zero where q - off(tap) leaves the
-// time range; periodic in sites).
-__global__ void __launch_bounds__(NT) conv_bwd_data_kernel(const float *__restrict__ dact, const bf16 *__restrict__ act,
-                                                           const bf16 *__restrict__ wd, float *__restrict__ din, int M,
-                                                           int H, int W)
+// din[q][ci] = sum_{tap,co} wd[ci][tap*64+co] * dz[q - off(tap)][co] (zero
+// where q - off(tap) leaves the time range; periodic in sites).
+__global__ void __launch_bounds__(NT) conv_bwd_data_kernel(const bf16 *__restrict__ dz, const bf16 *__restrict__ wd,
+                                                           float *__restrict__ din, int M, int H, int W)
 {
     __shared__ __attribute__((aligned(16))) bf16 sm[2 * TM * LP];
     bf16 *As = sm, *Bs = sm + TM * LP;
@@ -229,15 +227,7 @@ __global__ void __launch_bounds__(NT) conv_bwd_data_kernel(const float *__restri
         const int yy = y - (tap / 3 - 1), xx = (x - (tap % 3 - 1) + W) % W;
         bf16x8 av = zero8();
         if (m < M && yy >= 0 && yy < H)
-        {
-            const std::size_t o = ((static_cast<std::size_t>(b) * H + yy) * W + xx) * CH + co0;
-            const float4 g0 = *reinterpret_cast<const float4 *>(dact + o);
-            const float4 g1 = *reinterpret_cast<const float4 *>(dact + o + 4);
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(act + o);
-            const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-#pragma unroll
-            for (int i = 0; i < 8; ++i) av[i] = static_cast<bf16>(static_cast<float>(a[i]) > 0.f ? g[i] : 0.f);
-        }
+            av = *reinterpret_cast<const bf16x8 *>(dz + ((static_cast<std::size_t>(b) * H + yy) * W + xx) * CH + co0);
         *reinterpret_cast<bf16x8 *>(As + r * LP + ks) = av;
         *reinterpret_cast<bf16x8 *>(Bs + r * LP + ks) =
             *reinterpret_cast<const bf16x8 *>(wd + static_cast<std::size_t>(r) * K + k0 + ks);
@@ -256,9 +246,9 @@ __global__ void __launch_bounds__(NT) conv_bwd_data_kernel(const float *__restri
 template <bool FIRST>
 __global__ void __launch_bounds__(NT) conv_bwd_weight_kernel(const bf16 *__restrict__ in,
                                                              const std::int8_t *__restrict__ rec,
-                                                             const int *__restrict__ idx, const float *__restrict__ dact,
-                                                             const bf16 *__restrict__ act, float *__restrict__ partial,
-                                                             int M, int H, int W, int krows)
+                                                             const int *__restrict__ idx, const bf16 *__restrict__ dz,
+                                                             float *__restrict__ partial,
+                                                             int M, int H, int W, int krows, int chunk_len)
 {
     __shared__ __attribute__((aligned(16))) bf16 sm[2 * TM * LP];
     bf16 *As = sm, *Bs = sm + TM * LP;  // As[k][m], Bs[co][m]
@@ -271,16 +261,60 @@ __global__ void __launch_bounds__(NT) conv_bwd_weight_kernel(const bf16 *__restr
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int mbeg = chunk * CHUNK, mend = std::min(M, mbeg + CHUNK);
+    const int mbeg = chunk * chunk_len, mend = std::min(M, mbeg + chunk_len);
+    // Staging of dz and of the shifted bf16 input: thread (channel pair cp,
+    // position quad mq) loads channels 2cp, 2cp+1 of positions ms+4mq..+3
+    // and writes each channel's four values as one 8-byte LDS store.
+    const int cp = tid & 31, mq = tid >> 5;
     for (int ms = mbeg; ms < mend; ms += TK)
     {
-        const int m = ms + mm;
-        int b, y, x;
-        position(m, HW, W, b, y, x);
-        const bool valid = m < mend;
-        // A^T: rows k = kt*64 + seg + i, column m
+        std::uint32_t va[4] = {0u, 0u, 0u, 0u}, vb[4] = {0u, 0u, 0u, 0u};
+        {
+            int b, y, x;
+            position(ms + 4 * mq, HW, W, b, y, x);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int m = ms + 4 * mq + j;
+                if (m < mend)
+                {
+                    vb[j] = *reinterpret_cast<const std::uint32_t *>(dz + static_cast<std::size_t>(m) * CH + 2 * cp);
+                    if (!FIRST)
+                    {
+                        const int yy = y + kt / 3 - 1;
+                        int xx = x + kt % 3 - 1;
+                        xx += xx < 0 ? W : 0;
+                        xx -= xx >= W ? W : 0;
+                        if (yy >= 0 && yy < H)
+                            va[j] = *reinterpret_cast<const std::uint32_t *>(
+                                in + ((static_cast<std::size_t>(b) * H + yy) * W + xx) * CH + 2 * cp);
+                    }
+                }
+                if (++x == W)
+                {
+                    x = 0;
+                    if (++y == H)
+                    {
+                        y = 0;
+                        ++b;
+                    }
+                }
+            }
+        }
+        auto store_pair = [&](bf16 *S, const std::uint32_t v[4]) {
+            const uint2 lo = {(v[0] & 0xffffu) | (v[1] << 16), (v[2] & 0xffffu) | (v[3] << 16)};
+            const uint2 hi = {(v[0] >> 16) | (v[1] & 0xffff0000u), (v[2] >> 16) | (v[3] & 0xffff0000u)};
+            *reinterpret_cast<uint2 *>(S + (2 * cp) * LP + 4 * mq) = lo;
+            *reinterpret_cast<uint2 *>(S + (2 * cp + 1) * LP + 4 * mq) = hi;
+        };
+        store_pair(Bs, vb);
+        // A^T: rows k = kt*64 + ci, column m
         if (FIRST)
         {
+            const int m = ms + mm;
+            int b, y, x;
+            position(m, HW, W, b, y, x);
+            const bool valid = m < mend;
             const int s = valid ? (idx ? idx[b] : b) : 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i)
@@ -292,32 +326,7 @@ __global__ void __launch_bounds__(NT) conv_bwd_weight_kernel(const bf16 *__restr
             }
         }
         else
-        {
-            const int tap = kt;
-            const int yy = y + tap / 3 - 1, xx = (x + tap % 3 - 1 + W) % W;
-            bf16x8 av = zero8();
-            if (valid && yy >= 0 && yy < H)
-                av = *reinterpret_cast<const bf16x8 *>(in + ((static_cast<std::size_t>(b) * H + yy) * W + xx) * CH + seg);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) As[(seg + i) * LP + mm] = av[i];
-        }
-        // B^T: rows co = seg + i, column m: dz = dact * (act > 0), bf16
-        {
-            float g[8];
-            bf16x8 a = zero8();
-            if (valid)
-            {
-                const std::size_t o = static_cast<std::size_t>(m) * CH + seg;
-                const float4 g0 = *reinterpret_cast<const float4 *>(dact + o);
-                const float4 g1 = *reinterpret_cast<const float4 *>(dact + o + 4);
-                a = *reinterpret_cast<const bf16x8 *>(act + o);
-                g[0] = g0.x; g[1] = g0.y; g[2] = g0.z; g[3] = g0.w;
-                g[4] = g1.x; g[5] = g1.y; g[6] = g1.z; g[7] = g1.w;
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                Bs[(seg + i) * LP + mm] = static_cast<bf16>(valid && static_cast<float>(a[i]) > 0.f ? g[i] : 0.f);
-        }
+            store_pair(As, va);
         __syncthreads();
         tile_step(As, Bs, acc, lane, wave);
         __syncthreads();
@@ -329,37 +338,91 @@ __global__ void __launch_bounds__(NT) conv_bwd_weight_kernel(const bf16 *__restr
     });
 }
 
-// grad[e] = sum over chunks (in order) of partial[chunk][e]
-__global__ void sum_partials_kernel(const float *__restrict__ partial, int nchunks, int n, float *__restrict__ grad)
+// grad[e] = sum over chunks of partial[chunk][e].  A block covers 64
+// consecutive elements; wave w takes chunks w, w+4, w+8, ... (two running
+// sums), and the four wave sums are added in a fixed order.
+__global__ void __launch_bounds__(256) sum_partials_kernel(const float *__restrict__ partial, int nchunks, int n,
+                                                           float *__restrict__ grad)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += partial[static_cast<std::size_t>(c) * n + e];
-    grad[e] = s;
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
+    float s0 = 0.f, s1 = 0.f;
+    if (e < n)
+    {
+        int c = wave;
+#pragma unroll 4
+        for (; c + 4 < nchunks; c += 8)
+        {
+            s0 += partial[static_cast<std::size_t>(c) * n + e];
+            s1 += partial[static_cast<std::size_t>(c + 4) * n + e];
+        }
+        if (c < nchunks) s0 += partial[static_cast<std::size_t>(c) * n + e];
+    }
+    red[wave][lane] = s0 + s1;
+    __syncthreads();
+    if (wave == 0 && e < n) grad[e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
-// bias partials: bpart[block][co] = sum over the block's BROWS positions of
-// dact * (act > 0); 256 threads = 4 interleaved row sets per channel, added
-// in a fixed order
-__global__ void __launch_bounds__(256) bias_partial_kernel(const float *__restrict__ dact,
-                                                           const bf16 *__restrict__ act, int M,
-                                                           float *__restrict__ bpart)
+// out[c] = sum over rows r of part[r][c]: one block per column, rows split
+// over 256 threads (r = t, t+256, ...) and added in a fixed tree order
+__global__ void __launch_bounds__(256) sum_columns_kernel(const float *__restrict__ part, int nrows, int ncols,
+                                                          float *__restrict__ out)
 {
-    __shared__ float red[4][CH];
-    const int co = threadIdx.x & (CH - 1), q = threadIdx.x >> 6;
-    const int r0 = blockIdx.x * BROWS;
+    __shared__ float red[256];
+    const int c = blockIdx.x;
     float s = 0.f;
-    for (int r = q; r < BROWS; r += 4)
+    for (int r = threadIdx.x; r < nrows; r += 256) s += part[static_cast<std::size_t>(r) * ncols + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// dz[m][c] = bf16(dact[m][c] if act[m][c] > 0 else 0), the gradient at the
+// convolution's output that both backward GEMMs read, and bpart[block][c] =
+// the sum of the unrounded values over the block's BROWS positions.  Thread
+// (row group tid/8, channels 8*(tid%8)...+7) takes rows g, g+32, ...; the 32
+// group sums are added in order.
+__global__ void __launch_bounds__(256) dz_bias_kernel(const float *__restrict__ dact, const bf16 *__restrict__ act,
+                                                      int M, bf16 *__restrict__ dz, float *__restrict__ bpart)
+{
+    __shared__ float red[32][CH + 1];
+    const int grp = threadIdx.x >> 3, seg = (threadIdx.x & 7) * 8;
+    const int r0 = blockIdx.x * BROWS;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int r = grp; r < BROWS; r += 32)
     {
         const int m = r0 + r;
         if (m >= M) break;
-        const std::size_t o = static_cast<std::size_t>(m) * CH + co;
-        if (static_cast<float>(act[o]) > 0.f) s += dact[o];
+        const std::size_t o = static_cast<std::size_t>(m) * CH + seg;
+        const float4 g0 = *reinterpret_cast<const float4 *>(dact + o);
+        const float4 g1 = *reinterpret_cast<const float4 *>(dact + o + 4);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(act + o);
+        const float g[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+        bf16x8 z;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+        {
+            const float v = static_cast<float>(a[i]) > 0.f ? g[i] : 0.f;
+            s[i] += v;
+            z[i] = static_cast<bf16>(v);
+        }
+        *reinterpret_cast<bf16x8 *>(dz + o) = z;
     }
-    red[q][co] = s;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) red[grp][seg + i] = s[i];
     __syncthreads();
-    if (q == 0) bpart[blockIdx.x * CH + co] = (red[0][co] + red[1][co]) + (red[2][co] + red[3][co]);
+    if (threadIdx.x < CH)
+    {
+        float t = 0.f;
+        for (int k = 0; k < 32; ++k) t += red[k][threadIdx.x];
+        bpart[blockIdx.x * CH + threadIdx.x] = t;
+    }
 }
 
 // ------------------------------------------------------ pooling, MLP, loss
@@ -462,37 +525,47 @@ __global__ void loss_bwd_kernel(const float *__restrict__ out, const float *__re
     dpooled[b * CH + j] = dp;
 }
 
-// FC gradients, one thread per parameter, batch in order.  Layout of g:
-// w1[64][64], b1[64], w2[64][3], b2[3].
-__global__ void mlp_grad_kernel(const float *__restrict__ pooled, const float *__restrict__ h,
-                                const float *__restrict__ dh, const float *__restrict__ dout, int nb,
-                                float *__restrict__ g)
+// FC gradients, one block per parameter, the batch split over 256 threads
+// and added in a fixed tree order.  Layout of g: w1[64][64], b1[64],
+// w2[64][3], b2[3].
+constexpr int N_FC = CH * CH + CH + CH * 3 + 3;
+
+__global__ void __launch_bounds__(256) mlp_grad_kernel(const float *__restrict__ pooled, const float *__restrict__ h,
+                                                       const float *__restrict__ dh, const float *__restrict__ dout,
+                                                       int nb, float *__restrict__ g)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    constexpr int NW1 = CH * CH, NB1 = CH, NW2 = CH * 3, NB2 = 3;
-    if (e >= NW1 + NB1 + NW2 + NB2) return;
+    __shared__ float red[256];
+    const int e = blockIdx.x;
+    constexpr int NW1 = CH * CH, NB1 = CH, NW2 = CH * 3;
     float s = 0.f;
     if (e < NW1)
     {
         const int c = e / CH, j = e % CH;
-        for (int b = 0; b < nb; ++b) s += pooled[b * CH + c] * dh[b * CH + j];
+        for (int b = threadIdx.x; b < nb; b += 256) s += pooled[b * CH + c] * dh[b * CH + j];
     }
     else if (e < NW1 + NB1)
     {
         const int j = e - NW1;
-        for (int b = 0; b < nb; ++b) s += dh[b * CH + j];
+        for (int b = threadIdx.x; b < nb; b += 256) s += dh[b * CH + j];
     }
     else if (e < NW1 + NB1 + NW2)
     {
         const int q = e - NW1 - NB1, j = q / 3, o = q % 3;
-        for (int b = 0; b < nb; ++b) s += h[b * CH + j] * dout[b * 3 + o];
+        for (int b = threadIdx.x; b < nb; b += 256) s += h[b * CH + j] * dout[b * 3 + o];
     }
     else
     {
         const int o = e - NW1 - NB1 - NW2;
-        for (int b = 0; b < nb; ++b) s += dout[b * 3 + o];
+        for (int b = threadIdx.x; b < nb; b += 256) s += dout[b * 3 + o];
     }
-    g[e] = s;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g[e] = red[0];
 }
 
 // d act4[m][c] = dpooled[b(m)][c] / HW
@@ -558,7 +631,8 @@ struct Decoder
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
     bf16 *wf[4] = {}, *wd[4] = {}, *w1b = nullptr, *w2b = nullptr;
     bf16 *act[4] = {};
-    float *dact[2] = {};
+    float *dact = nullptr;  // gradient at a convolution's output (FP32)
+    bf16 *dz = nullptr;     // the same through the ReLU, bf16
     float *partial = nullptr, *bpart = nullptr;
     float *pooled = nullptr, *h = nullptr, *out = nullptr, *dout = nullptr, *dh = nullptr, *dpooled = nullptr,
           *loss = nullptr;
@@ -572,7 +646,7 @@ struct Decoder
     ~Decoder()
     {
         void *ptrs[] = {w, m, v, g, w1b, w2b, partial, bpart, pooled, h, out, dout, dh, dpooled, loss, d_idx,
-                        data_rec, data_lab, dact[0], dact[1]};
+                        data_rec, data_lab, dact, dz};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (int l = 0; l < 4; ++l)
@@ -617,9 +691,15 @@ struct Decoder
         MIPT_HIP_CHECK(hipMalloc(&w2b, CH * 3 * 2));
         const std::size_t M = static_cast<std::size_t>(bmax) * HW;
         for (int l = 0; l < 4; ++l) MIPT_HIP_CHECK(hipMalloc(&act[l], M * CH * 2));
-        for (int i = 0; i < 2; ++i) MIPT_HIP_CHECK(hipMalloc(&dact[i], M * CH * 4));
-        const std::size_t nch = (M + CHUNK - 1) / CHUNK;
-        MIPT_HIP_CHECK(hipMalloc(&partial, nch * 9 * CH * CH * 4));
+        MIPT_HIP_CHECK(hipMalloc(&dact, M * CH * 4));
+        MIPT_HIP_CHECK(hipMalloc(&dz, M * CH * 2));
+        std::size_t psize = 0;  // at most WG_BLOCKS / ktiles chunks of krows x 64 partials per layer
+        for (int l = 0; l < 4; ++l)
+        {
+            const int krows = 9 * ci[l];
+            psize = std::max(psize, static_cast<std::size_t>(blocks(WG_BLOCKS, blocks(krows, TM))) * krows * CH * 4);
+        }
+        MIPT_HIP_CHECK(hipMalloc(&partial, psize));
         MIPT_HIP_CHECK(hipMalloc(&bpart, ((M + BROWS - 1) / BROWS) * CH * 4));
         MIPT_HIP_CHECK(hipMalloc(&pooled, bmax * CH * 4));
         MIPT_HIP_CHECK(hipMalloc(&h, bmax * CH * 4));
@@ -687,35 +767,33 @@ struct Decoder
     {
         const int M = nb * HW;
         const int grid = blocks(M, TM);
-        const int nch = blocks(M, CHUNK);
         hipLaunchKernelGGL(loss_bwd_kernel, dim3(nb), dim3(CH), 0, stream, out, labels, idx, h, w1b, w2b, nb, huber,
                            loss, dout, dh, dpooled);
-        hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks(CH * CH + CH + CH * 3 + 3, 256)), dim3(256), 0, stream, pooled,
-                           h, dh, dout, nb, g + off_fc);
+        hipLaunchKernelGGL(mlp_grad_kernel, dim3(N_FC), dim3(256), 0, stream, pooled, h, dh, dout, nb, g + off_fc);
         hipLaunchKernelGGL(pool_bwd_kernel, dim3(blocks(static_cast<std::size_t>(M) * CH, 256)), dim3(256), 0, stream,
-                           dpooled, M, HW, dact[0]);
+                           dpooled, M, HW, dact);
         MIPT_HIP_CHECK(hipGetLastError());
-        int cur = 0;
         for (int l = 3; l >= 0; --l)
         {
             const int krows = 9 * ci[l];
             const int ktiles = blocks(krows, TM);
+            // positions per partial: about WG_BLOCKS blocks in all, at least 8 K steps each
+            const int chunk_len = std::max(8 * TK, blocks(blocks(M, blocks(WG_BLOCKS, ktiles)), TK) * TK);
+            const int nch = blocks(M, chunk_len);
+            const int nbp = blocks(M, BROWS);
+            hipLaunchKernelGGL(dz_bias_kernel, dim3(nbp), dim3(256), 0, stream, dact, act[l], M, dz, bpart);
+            hipLaunchKernelGGL(sum_columns_kernel, dim3(CH), dim3(256), 0, stream, bpart, nbp, CH, g + off_b[l]);
             if (l == 0)
                 hipLaunchKernelGGL(conv_bwd_weight_kernel<true>, dim3(ktiles, nch), dim3(NT), 0, stream, nullptr, rec,
-                                   idx, dact[cur], act[0], partial, M, H, W, krows);
+                                   idx, dz, partial, M, H, W, krows, chunk_len);
             else
                 hipLaunchKernelGGL(conv_bwd_weight_kernel<false>, dim3(ktiles, nch), dim3(NT), 0, stream, act[l - 1],
-                                   nullptr, nullptr, dact[cur], act[l], partial, M, H, W, krows);
-            hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks(krows * CH, 256)), dim3(256), 0, stream, partial, nch,
+                                   nullptr, nullptr, dz, partial, M, H, W, krows, chunk_len);
+            hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks(krows * CH, 64)), dim3(256), 0, stream, partial, nch,
                                krows * CH, g + off_w[l]);
-            const int nbp = blocks(M, BROWS);
-            hipLaunchKernelGGL(bias_partial_kernel, dim3(nbp), dim3(256), 0, stream, dact[cur], act[l], M, bpart);
-            hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(CH), 0, stream, bpart, nbp, CH, g + off_b[l]);
             if (l > 0)
-                hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid), dim3(NT), 0, stream, dact[cur], act[l], wd[l],
-                                   dact[cur ^ 1], M, H, W);
+                hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid), dim3(NT), 0, stream, dz, wd[l], dact, M, H, W);
             MIPT_HIP_CHECK(hipGetLastError());
-            cur ^= 1;
         }
         return OK;
     }

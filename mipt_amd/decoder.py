@@ -12,6 +12,7 @@ BF16-MFMA forward/backward/Adam kernels.
 from __future__ import annotations
 
 import ctypes
+import math
 import time
 from dataclasses import dataclass
 
@@ -112,6 +113,12 @@ class Decoder:
         out = np.zeros((count, 3), np.float32)
         _check(lib().mipt_hip_decoder_predict(self._h, first, count, out.ctypes.data_as(_fp)), "decoder_predict")
         return out
+
+
+def cosine_lr(lr: float, epoch: int, epochs: int) -> float:
+    """Learning rate of epoch `epoch` (0-based): a half cosine from lr down
+    towards 0 over `epochs` epochs."""
+    return lr * 0.5 * (1.0 + math.cos(math.pi * epoch / epochs))
 
 
 def entropy_from_bloch(r) -> np.ndarray:

@@ -92,16 +92,16 @@ def test_decoder_adam_step_matches_oracle(dec):
 
 def test_decoder_learns_probe_entropy_beyond_p_bin_baseline(dec):
     print("decoder training on simulator data (L=16, 32 scrambling + 32 measured layers)", flush=True)
-    n_train, n_test = 12288, 2048
-    train = dec.generate_probe_dataset(np.arange(1, n_train + 1), chunk=4096)
+    n_train, n_test = 65536, 4096
+    train = dec.generate_probe_dataset(np.arange(1, n_train + 1), chunk=16384)
     test = dec.generate_probe_dataset(np.arange(1_000_001, 1_000_001 + n_test), chunk=4096)
     print(f"simulated {n_train + n_test} trajectories in {train.seconds + test.seconds:.2f} s", flush=True)
     assert np.all(np.isfinite(train.bloch)) and np.all(np.abs(train.bloch) <= 1.0 + 1e-5)
-    d = dec.Decoder(32, 16, 256, seed=5)
+    d = dec.Decoder(32, 16, 512, seed=5)
     d.set_dataset(np.concatenate([train.records, test.records]), np.concatenate([train.bloch, test.bloch]))
     rng = np.random.default_rng(0)
     for epoch in range(12):
-        loss = d.train(rng.permutation(n_train), 256, lr=1e-3)
+        loss = d.train(rng.permutation(n_train), 512, lr=dec.cosine_lr(1e-3, epoch, 12))
         print(f"epoch {epoch}: mean batch loss {loss:.4f}", flush=True)
     t = time.perf_counter()
     pred = d.predict(n_train, n_test)
