@@ -144,6 +144,12 @@ typedef struct mipt_hip_traj_spec
                            bond colours per time step (layer mod 4): horizontal from even / odd columns
                            (odd colour closes each row when closed and grid_x is even), vertical from
                            staggered even / odd rows (cpp/include/mipt/circuits/mms.hpp:159-330) */
+    int probe_count;    /* 0: one probe iff probe_site >= 0 (above).  1..4 (probe_site then -1): reference j
+                           is qubit L + j, paired with the distinct site probe_sites[j]; all are attached in
+                           order j = 0, 1, ... at the start of layer p_zero_layers (reset, h, cx each; the
+                           reset's uniform is keyed by (p_zero_layers, site)), or Bell-paired at t = 0 when
+                           p_zero_layers = 0 (cpp/include/mipt/probed/kernels.hpp:33-66) */
+    int probe_sites[4];
 } mipt_hip_traj_spec;
 
 typedef struct mipt_hip_batch_config
@@ -224,6 +230,16 @@ int mipt_hip_batch_translation_entropies(mipt_hip_batch *batch, int min_kept, in
  * 24-52; NaN without a probe). */
 int mipt_hip_batch_records(mipt_hip_batch *batch, int layer_begin, int layer_end, int8_t *host_records);
 int mipt_hip_batch_probe_bloch(mipt_hip_batch *batch, double *host_bloch);
+/* Joint density matrix of every trajectory's references (every trajectory needs the same count k >= 1):
+ * host [count][2^k][2^k][re, im] FP64, trace 1, reduced-basis bit j = reference j (qubit L + j); the
+ * multi-reference matrices of cpp/include/mipt/ancilla_rdm.hpp (rho2 / rho3 / rho4 of ancilla_rdm.cu). */
+int mipt_hip_batch_probe_rdm(mipt_hip_batch *batch, double *host_rho);
+/* Information measures (nats) of a k-reference density matrix (k = 1..4, [2^k][2^k][re, im]), in the
+ * field order of the reference's probe sample (cpp/include/mipt/probed/sample.hpp:21-33):
+ * out[0] sq, [1] i2, [2] i3, [3] i4, [4] negativity, [5] log_negativity, [6] tmi, [7] average_mi,
+ * [8] min_bipartite_negativity, [9] joint_purity, [10] mean_single_purity; filled as
+ * cpp/include/mipt/probed/kernels.hpp:106-147 fills them for k references (host code, no GPU). */
+int mipt_hip_probe_information(const double *rho_ri, int k, double *out);
 /* Three-site reduced density matrices of trajectories [first, first+count):
  * subsystems [nsub][3] (qubit k of a subsystem = reduced-basis bit k; the
  * cyclic (i, i+1, i+2) list of cpp/include/mipt/rdm.hpp:92-106, or the

@@ -103,6 +103,8 @@ def lib():
     L.mipt_hip_batch_translation_entropies.argtypes = [vp, i, i, i, dp, dp]
     L.mipt_hip_batch_set_graph.argtypes = [vp, i]
     L.mipt_hip_batch_probe_bloch.argtypes = [vp, dp]
+    L.mipt_hip_batch_probe_rdm.argtypes = [vp, dp]
+    L.mipt_hip_probe_information.argtypes = [dp, i, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
     L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     _LIB = L
@@ -325,7 +327,8 @@ def run_plain_trajectory(engine: PlainEngine, n: int, gates, flags, uniforms, cl
 class TrajSpec(ctypes.Structure):
     _fields_ = [("L", ctypes.c_int), ("layers", ctypes.c_int), ("p", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("closed", ctypes.c_int), ("gate_family", ctypes.c_int), ("probe_site", ctypes.c_int),
-                ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int), ("grid_x", ctypes.c_int)]
+                ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int), ("grid_x", ctypes.c_int),
+                ("probe_count", ctypes.c_int), ("probe_sites", ctypes.c_int * 4)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -343,10 +346,13 @@ class Spec:
     p_zero_layers: int = 0
     shared_gates: int = -1
     grid_x: int = 0
+    probe_sites: tuple = ()  # several references (qubits L, L+1, ...); probe_site stays -1
 
     def to_c(self) -> TrajSpec:
+        sites = list(self.probe_sites) + [0] * (4 - len(self.probe_sites))
         return TrajSpec(self.L, self.layers, float(self.p), self.seed & (2**64 - 1), int(self.closed), 1,
-                        self.probe_site, self.p_zero_layers, self.shared_gates, self.grid_x)
+                        self.probe_site, self.p_zero_layers, self.shared_gates, self.grid_x,
+                        len(self.probe_sites), (ctypes.c_int * 4)(*sites[:4]))
 
 
 class Batch:
@@ -390,7 +396,7 @@ class Batch:
 
     def nq(self, traj: int) -> int:
         s = self.specs[traj]
-        return s.L + (1 if s.probe_site >= 0 else 0)
+        return s.L + (len(s.probe_sites) or (1 if s.probe_site >= 0 else 0))
 
     def set_state(self, traj: int, psi):
         a = np.ascontiguousarray(psi, np.complex64)
@@ -521,6 +527,15 @@ class Batch:
         _check(lib().mipt_hip_batch_probe_bloch(self._h, _dp(out)), "batch_probe_bloch")
         return out
 
+    def probe_rdm(self) -> np.ndarray:
+        """[count, 2^k, 2^k] joint density matrices of the k references
+        (reduced-basis bit j = reference j), trace 1."""
+        k = len(self.specs[0].probe_sites) or (1 if self.specs[0].probe_site >= 0 else 0)
+        D = 1 << k
+        out = np.zeros((len(self.specs), D, D, 2))
+        _check(lib().mipt_hip_batch_probe_rdm(self._h, _dp(out)), "batch_probe_rdm")
+        return out[..., 0] + 1j * out[..., 1]
+
     def tmi(self, traj: int, cycle: int = 0, block: int = 0, order: int = 1) -> float:
         out = ctypes.c_double()
         _check(lib().mipt_hip_batch_tmi(self._h, traj, block, cycle, order, ctypes.byref(out)), "batch_tmi")
@@ -530,6 +545,21 @@ class Batch:
 class TrajResult(ctypes.Structure):
     _fields_ = [("s_half", ctypes.c_double), ("probe_entropy", ctypes.c_double), ("bloch", ctypes.c_double * 3),
                 ("norm_error", ctypes.c_double), ("measured", ctypes.c_int), ("flagged_draws", ctypes.c_int)]
+
+
+PROBE_FIELDS = ("sq", "i2", "i3", "i4", "negativity", "log_negativity", "tmi", "average_mi",
+                "min_bipartite_negativity", "joint_purity", "mean_single_purity")
+
+
+def probe_information(rho) -> dict:
+    """Information measures (nats) of a k-reference density matrix
+    (mipt_hip_probe_information), keyed by the reference's sample fields."""
+    rho = np.asarray(rho, np.complex128)
+    k = int(round(np.log2(rho.shape[0])))
+    ri = np.ascontiguousarray(np.stack([rho.real, rho.imag], -1))
+    out = np.zeros(11)
+    _check(lib().mipt_hip_probe_information(_dp(ri), k, _dp(out)), "probe_information")
+    return dict(zip(PROBE_FIELDS, out.tolist()))
 
 
 def run_batch(specs, tmi_cycles: int = 1, tmi_block: int = 0, entropy_order: int = 1):

@@ -130,15 +130,15 @@ __device__ inline double block_sum(double v, double *red)
 
 constexpr std::uint32_t STREAM_ATCH = 0x41544348u;  // probe-attach reset uniforms
 
-// Probe attach at the start of layer p_zero_layers (> 0), as
-// cpp/include/mipt/probed/kernels.hpp:68-77: reset(site), h(site),
-// cx(site, probe), the probe (qubit L) being |0> until then.  The reset is a
-// Born draw on the site (outcome 1 iff u < P1, u from the counter stream at
-// (layer, site)), renormalised by 1/sqrt(branch weight) and flipped to 0;
-// FP64 weights in a fixed reduction order.  256 threads, any buffer.
-__device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, double *red, int &status)
+// Reference attach at the start of layer p_zero_layers (> 0), as
+// cpp/include/mipt/probed/kernels.hpp:55-77: for j = 0, 1, ... in order,
+// reset(site_j), h(site_j), cx(site_j, L + j), reference j being |0> until
+// then.  The reset is a Born draw on the site (outcome 1 iff u < P1, u from
+// the counter stream at (layer, site)), renormalised by 1/sqrt(branch weight)
+// and flipped to 0; FP64 weights in a fixed reduction order.  256 threads,
+// any buffer.
+__device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int lo, int hi, double *red, int &status)
 {
-    const int lo = d.probe_site, hi = d.L;
     const std::uint32_t S = 1u << lo, P = 1u << hi;
     double w = 0.0, w1 = 0.0;
     for (std::uint32_t i = threadIdx.x; i < dim; i += SMALL_THREADS)
@@ -174,12 +174,17 @@ __device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, do
     __syncthreads();
 }
 
+__device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, double *red, int &status)
+{
+    for (int j = 0; j < d.probes; ++j) attach_one(v, dim, d, d.psite[j], d.L + j, red, status);
+}
+
 __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const int *__restrict__ traj_list, int t)
 {
     __shared__ double red[SMALL_THREADS];
     const int traj = traj_list[blockIdx.x];
     const TrajDesc d = a.desc[traj];
-    if (d.probe_site < 0 || d.p_zero_layers != t || t == 0) return;  // uniform over the block
+    if (d.probes == 0 || d.p_zero_layers != t || t == 0) return;  // uniform over the block
     int status = 0;
     attach_probe(a.states + d.state_off, 1u << d.nq, d, red, status);
     if (threadIdx.x == 0 && status) a.status[traj] = status;
@@ -187,11 +192,23 @@ __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const
 
 __device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
 {
-    if (d.probe_site >= 0 && d.p_zero_layers == 0)
+    if (d.probes > 0 && d.p_zero_layers == 0)
     {
-        const std::uint64_t bell = (std::uint64_t{1} << d.probe_site) | (std::uint64_t{1} << d.L);
-        const float h = 0.70710678118654752440f;
-        return (i == 0 || i == bell) ? make_float2(h, 0.f) : make_float2(0.f, 0.f);
+        // product of Bell pairs (site_j, L + j): amplitude 2^(-probes/2) where
+        // every pair's two bits agree and all other bits are 0
+        std::uint64_t rest = i;
+        for (int j = 0; j < d.probes; ++j)
+        {
+            const std::uint64_t s = std::uint64_t{1} << d.psite[j], r = std::uint64_t{1} << (d.L + j);
+            if (((i & s) != 0) != ((i & r) != 0)) return make_float2(0.f, 0.f);
+            rest &= ~(s | r);
+        }
+        if (rest) return make_float2(0.f, 0.f);
+        const float h = d.probes == 1 ? 0.70710678118654752440f
+                        : d.probes == 2 ? 0.5f
+                        : d.probes == 3 ? 0.35355339059327376220f
+                                        : 0.25f;
+        return make_float2(h, 0.f);
     }
     return i == 0 ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
 }
@@ -215,7 +232,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     int status = 0;
     for (int t = layer_begin; t < layer_end; ++t)
     {
-        if (d.probe_site >= 0 && d.p_zero_layers > 0 && t == d.p_zero_layers) attach_probe(st, dim, d, red, status);
+        if (d.probes > 0 && d.p_zero_layers > 0 && t == d.p_zero_layers) attach_probe(st, dim, d, red, status);
         const int nb = geo_bond_count(d.L, d.gx, t, d.closed);
         const std::uint64_t gbase = d.gate_off + gate_prefix(d, t);
         for (int b = 0; b < nb; ++b)
@@ -363,6 +380,54 @@ __global__ void __launch_bounds__(SMALL_THREADS) probe_bloch_kernel(DevArena a, 
         out[traj * 3 + 0] = 2.0 * re / tr;
         out[traj * 3 + 1] = -2.0 * im / tr;
         out[traj * 3 + 2] = (a00 - a11) / tr;
+    }
+}
+
+// Joint density matrix of a trajectory's k references (qubits L..L+k-1,
+// reduced-basis bit j = reference j): rho[r][c] = sum over the chain's 2^L
+// basis states e of psi[e + r 2^L] conj(psi[e + c 2^L]).  Block (g, traj)
+// accumulates the lower-triangle entries q = 8g..8g+7 (row-major over r >= c)
+// in FP64 over a fixed thread order; out[traj][q][re, im], not normalised.
+__global__ void __launch_bounds__(SMALL_THREADS) probe_rdm_kernel(DevArena a, int D, double *__restrict__ out)
+{
+    __shared__ double red[SMALL_THREADS];
+    const int traj = blockIdx.y;
+    const TrajDesc d = a.desc[traj];
+    const int npair = D * (D + 1) / 2, q0 = blockIdx.x * 8;
+    const int cnt = npair - q0 < 8 ? npair - q0 : 8;
+    std::uint32_t ro[8], co[8];
+    for (int e = 0; e < 8; ++e)
+    {
+        int q = q0 + (e < cnt ? e : 0), r = 0;
+        while (q > r)
+        {
+            ++r;
+            q -= r;
+        }
+        ro[e] = static_cast<std::uint32_t>(r) << d.L;
+        co[e] = static_cast<std::uint32_t>(q) << d.L;
+    }
+    const float2 *psi = a.states + d.state_off;
+    const std::uint32_t n = 1u << d.L;
+    double re[8], im[8];
+    for (int e = 0; e < 8; ++e) re[e] = im[e] = 0.0;
+    for (std::uint32_t i = threadIdx.x; i < n; i += SMALL_THREADS)
+        for (int e = 0; e < 8; ++e)
+        {
+            if (e >= cnt) break;
+            const float2 x = psi[i + ro[e]], y = psi[i + co[e]];
+            re[e] += static_cast<double>(x.x) * y.x + static_cast<double>(x.y) * y.y;
+            im[e] += static_cast<double>(x.y) * y.x - static_cast<double>(x.x) * y.y;
+        }
+    for (int e = 0; e < cnt; ++e)
+    {
+        const double sr = block_sum(re[e], red);
+        const double si = block_sum(im[e], red);
+        if (threadIdx.x == 0)
+        {
+            out[(static_cast<std::size_t>(traj) * npair + q0 + e) * 2] = sr;
+            out[(static_cast<std::size_t>(traj) * npair + q0 + e) * 2 + 1] = si;
+        }
     }
 }
 
@@ -1371,6 +1436,22 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
     }
     return OK;
 }
+// References of a spec: count 0..4; a list needs probe_site = -1 and
+// distinct sites in [0, L); at most 27 qubits in all.
+bool probe_spec_ok(const mipt_hip_traj_spec &s)
+{
+    if (s.probe_count < 0 || s.probe_count > 4) return false;
+    if (s.probe_count == 0) return s.L + (s.probe_site >= 0 ? 1 : 0) <= 27;
+    if (s.probe_site >= 0 || s.L + s.probe_count > 27) return false;
+    for (int j = 0; j < s.probe_count; ++j)
+    {
+        if (s.probe_sites[j] < 0 || s.probe_sites[j] >= s.L) return false;
+        for (int k = 0; k < j; ++k)
+            if (s.probe_sites[k] == s.probe_sites[j]) return false;
+    }
+    return true;
+}
+
 } // namespace
 
 struct Batch
@@ -1515,7 +1596,7 @@ struct Batch
             // up to 2^27 amplitudes per state (1 GiB): tile byte offsets stay below 2^31
             const bool grid = s.grid_x > 0;
             if (s.L < 2 || s.L > 26 || (!grid && (s.L % 2) != 0) || s.layers < 0 || s.gate_family != 1 ||
-                (s.probe_site >= s.L) || s.grid_x < 0 ||
+                (s.probe_site >= s.L) || s.grid_x < 0 || !probe_spec_ok(s) ||
                 (grid && (s.grid_x < 2 || (s.L % s.grid_x) != 0 || s.L / s.grid_x < 2)) ||
                 (s.shared_gates >= 0 && (s.shared_gates >= count || s.shared_gates == i)))
             {
@@ -1524,12 +1605,17 @@ struct Batch
             }
             TrajDesc &d = desc[i];
             d.L = s.L;
-            d.nq = s.L + (s.probe_site >= 0 ? 1 : 0);
+            d.probes = s.probe_count > 0 ? s.probe_count : (s.probe_site >= 0 ? 1 : 0);
+            for (int j = 0; j < 4; ++j) d.psite[j] = -1;
+            if (s.probe_count > 0)
+                for (int j = 0; j < s.probe_count; ++j) d.psite[j] = s.probe_sites[j];
+            else if (s.probe_site >= 0)
+                d.psite[0] = s.probe_site;
+            d.nq = s.L + d.probes;
             d.layers = s.layers;
             d.closed = s.closed ? 1 : 0;
             d.gx = s.grid_x;
-            d.pad = 0;
-            d.probe_site = s.probe_site;
+            d.probe_site = d.psite[0];
             d.p_zero_layers = s.p_zero_layers;
             d.p = s.p;
             d.seed = s.seed;
@@ -1767,14 +1853,14 @@ struct Batch
         al.partials = lane ? partials2 : a.partials;
         // maximum measured count per layer over the group (host flags)
         bool attach_any = false;
-        for (int i : grp) attach_any |= desc[i].probe_site >= 0 && desc[i].p_zero_layers > 0;
+        for (int i : grp) attach_any |= desc[i].probes > 0 && desc[i].p_zero_layers > 0;
         for (int t = t0; t < t1; ++t)
         {
             const LayerPlan &LP = lp[t % P];
             if (attach_any && t > 0)
             {
                 bool at = false;
-                for (int i : grp) at |= desc[i].probe_site >= 0 && desc[i].p_zero_layers == t;
+                for (int i : grp) at |= desc[i].probes > 0 && desc[i].p_zero_layers == t;
                 if (at)
                 {
                     hipLaunchKernelGGL(attach_kernel, dim3(ntraj), dim3(SMALL_THREADS), 0, ls, al, list, t);
@@ -2361,6 +2447,58 @@ int mipt_hip_batch_probe_bloch(mipt_hip_batch *b, double *host)
     {
         set_error("batch_probe_bloch: %s", hipGetErrorString(e));
         return static_cast<int>(e);
+    }
+    return OK;
+}
+
+int mipt_hip_batch_probe_rdm(mipt_hip_batch *b, double *host)
+{
+    if (!b || !host || b->desc.empty())
+    {
+        set_error("batch_probe_rdm: bad arguments");
+        return E_ARG;
+    }
+    const int count = static_cast<int>(b->desc.size());
+    const int k = b->desc[0].probes;
+    for (const TrajDesc &d : b->desc)
+        if (d.probes != k || k < 1)
+        {
+            set_error("batch_probe_rdm: every trajectory needs the same number (>= 1) of references");
+            return E_ARG;
+        }
+    const int D = 1 << k, npair = D * (D + 1) / 2;
+    double *dev = nullptr;
+    MIPT_HIP_CHECK(hipMalloc(&dev, static_cast<std::size_t>(count) * npair * 2 * sizeof(double)));
+    hipLaunchKernelGGL(probe_rdm_kernel, dim3((npair + 7) / 8, count), dim3(SMALL_THREADS), 0, b->stream, b->a, D,
+                       dev);
+    std::vector<double> tri(static_cast<std::size_t>(count) * npair * 2);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tri.data(), dev, tri.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess)
+    {
+        set_error("batch_probe_rdm: %s", hipGetErrorString(e));
+        return static_cast<int>(e);
+    }
+    // Hermitian fill and division by the trace
+    for (int t = 0; t < count; ++t)
+    {
+        const double *src = tri.data() + static_cast<std::size_t>(t) * npair * 2;
+        double *dst = host + static_cast<std::size_t>(t) * D * D * 2;
+        double tr = 0.0;
+        for (int r = 0; r < D; ++r) tr += src[2 * (r * (r + 1) / 2 + r)];
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c <= r; ++c)
+            {
+                const int q = r * (r + 1) / 2 + c;
+                const double re = src[2 * q] / tr, im = src[2 * q + 1] / tr;
+                dst[2 * (r * D + c)] = re;
+                dst[2 * (r * D + c) + 1] = r == c ? 0.0 : im;
+                dst[2 * (c * D + r)] = re;
+                dst[2 * (c * D + r) + 1] = r == c ? 0.0 : -im;
+            }
     }
     return OK;
 }

@@ -21,8 +21,9 @@ struct TrajDesc
     std::uint32_t layer_off;  // offset into per-layer arrays (flags, outcomes, collapse)
     std::uint32_t unif_off;   // offset into per-(layer, site) arrays (uniforms, p1)
     int L, nq, layers, closed;
-    int probe_site, p_zero_layers;
-    int gx, pad;              // grid columns (0: chain)
+    int probe_site, p_zero_layers;  // probe_site: site of reference 0 (-1: no references)
+    int gx, probes;                 // grid columns (0: chain); references (qubits L..L+probes-1)
+    int psite[4];                   // site of reference j
     double p;
     std::uint64_t seed;
 };

@@ -15,6 +15,10 @@ repository root):
 * cpp/include/mipt/tmi/compute.hpp:64-113, 186-227  TMI regions and formula
 * cpp/include/mipt/free_energy.hpp:274-287  S(L/2): kept = {0 .. n/2-1}
 * cpp/include/mipt/ancilla_mi.hpp:24-52     probe entropy (nats)
+* cpp/include/mipt/probed/kernels.hpp:55-147, ancilla_info.hpp:46-139,
+  ancilla_mi.hpp:222-262, analysis/cut_negativity.hpp:139-200
+                                            several references: attach, joint
+                                            RDM and information measures
 """
 from __future__ import annotations
 
@@ -365,6 +369,153 @@ def probe_trajectory(n: int, layers: int, p: float, seed: int, probe_site: int, 
                 outs[t, s] = o
                 p1[t, s] = pr if o == 1 else 1.0 - pr
     return psi, outs, p1, att, (g, f, u)
+
+
+# --------------------------------------------------------- several references
+def attach_reference(psi: np.ndarray, nq: int, site: int, ref: int, u: float) -> int:
+    """reset(site); h(site); cx(site, ref) on an nq-qubit state whose
+    reference qubit `ref` is |0> (cpp/include/mipt/probed/kernels.hpp:55-77):
+    a sequential Born draw on the site (outcome 1 iff u < P1, renormalised),
+    the site flipped to 0, then the Bell pair.  In place; returns the outcome."""
+    _, o, _ = measure_collapse(psi, nq, site, u)
+    S, P = 1 << site, 1 << ref
+    idx = np.arange(1 << nq)
+    base = idx[((idx & S) == 0) & ((idx & P) == 0)]
+    a = psi[base | (S if o else 0)].copy() / np.sqrt(2.0)
+    psi[base | S] = 0.0
+    psi[base | P] = 0.0
+    psi[base] = a
+    psi[base | S | P] = a
+    return o
+
+
+def references_trajectory(n: int, layers: int, p: float, seed: int, sites, p_zero_layers: int,
+                          closed: bool = True):
+    """k = len(sites) references (qubits n .. n+k-1): Bell pairs (site_j, n+j)
+    at t = 0 when p_zero_layers = 0, else unmeasured layers 0..p_zero_layers-1
+    and the references attached in order j = 0, 1, ... at the start of layer
+    p_zero_layers (cpp/include/mipt/probed/kernels.hpp:55-66, each reset's
+    uniform keyed by (layer, site)); measured layers after.  Returns
+    (psi over n+k qubits, outcomes [layers, n], p1, attach outcomes, (g, f, u))."""
+    k = len(sites)
+    g, f, u = build_layers(n, layers, p, seed, closed)
+    f[:p_zero_layers] = 0
+    u[:p_zero_layers] = 0.0
+    nq = n + k
+    psi = np.zeros(1 << nq, np.complex128)
+    att = []
+    if p_zero_layers == 0:
+        for b in range(1 << k):
+            i = 0
+            for j in range(k):
+                if (b >> j) & 1:
+                    i |= (1 << sites[j]) | (1 << (n + j))
+            psi[i] = 2.0 ** (-k / 2)
+    else:
+        psi[0] = 1.0
+    outs = np.full((layers, n), -1, np.int32)
+    p1 = np.zeros((layers, n))
+    gi = 0
+    for t in range(layers):
+        if p_zero_layers > 0 and t == p_zero_layers:
+            for j, site in enumerate(sites):
+                att.append(attach_reference(psi, nq, site, n + j, philox_uniform(seed, t, site, 0, STREAM_ATCH)))
+        for (q0, q1) in layer_bonds(n, t, closed):
+            apply_dense(psi, nq, [q0, q1], g[gi])
+            gi += 1
+        for s in range(n):
+            if f[t, s]:
+                st, o, pr = measure_collapse(psi, nq, s, u[t, s])
+                outs[t, s] = o
+                p1[t, s] = pr if o == 1 else 1.0 - pr
+    return psi, outs, p1, att, (g, f, u)
+
+
+def reference_rdm(psi: np.ndarray, n: int, k: int) -> np.ndarray:
+    """Joint density matrix of qubits n..n+k-1 (reduced-basis bit j = qubit
+    n+j), trace 1: the state index is r 2^n + e with r the references' bits."""
+    m = np.asarray(psi).reshape(1 << k, 1 << n)
+    rho = m @ m.conj().T
+    return rho / np.trace(rho).real
+
+
+def partial_trace_bits(rho: np.ndarray, k: int, keep) -> np.ndarray:
+    """Reduced matrix of the bits in `keep` (bit j of the index = party j; the
+    result's bit i = keep[i] for keep ascending), trace 1."""
+    keep = sorted(keep)
+    rows = "abcd"[:k]
+    cols = ["efgh"[j] if j in keep else rows[j] for j in range(k)]
+    # tensor axis a of a 2^k index is bit k-1-a
+    src = "".join(rows[k - 1 - a] for a in range(k)) + "".join(cols[k - 1 - a] for a in range(k))
+    dst = "".join(rows[j] for j in reversed(keep)) + "".join(cols[j] for j in reversed(keep))
+    out = np.einsum(src + "->" + dst, np.asarray(rho).reshape([2] * (2 * k)))
+    d = 1 << len(keep)
+    out = out.reshape(d, d)
+    return out / np.trace(out).real
+
+
+def _entropy_nats(rho) -> float:
+    w = np.clip(np.linalg.eigvalsh(0.5 * (rho + rho.conj().T)), 0.0, None)
+    w = w[w > 0]
+    return float(-(w * np.log(w)).sum())
+
+
+def _purity(rho) -> float:
+    return float((np.abs(rho) ** 2).sum() / np.trace(rho).real ** 2)
+
+
+def cut_negativity(rho: np.ndarray, k: int, j: int) -> float:
+    """0.5 (||rho^{T_j}||_1 - 1), partial transpose of party j
+    (cpp/include/mipt/analysis/cut_negativity.hpp:139-200, qubits)."""
+    t = np.asarray(rho).reshape([2] * (2 * k))
+    a = k - 1 - j
+    pt = np.swapaxes(t, a, k + a).reshape(1 << k, 1 << k)
+    w = np.linalg.eigvalsh(0.5 * (pt + pt.conj().T))
+    v = 0.5 * (np.abs(w).sum() - 1.0)
+    return 0.0 if -1e-10 < v < 0 else float(v)
+
+
+def probe_information(rho: np.ndarray) -> dict:
+    """The k-reference sample of cpp/include/mipt/probed/kernels.hpp:106-147
+    (nats): k=1 sq; k=2 sq, i2 = I(A:B), negativity, log_negativity
+    (ancilla_mi.hpp:222-262); k=3 tmi, average_mi, min_bipartite_negativity,
+    joint_purity, mean_single_purity (ancilla_info.hpp:46-66); k=4 sq, mean
+    I2 over pairs, mean I3 over triples, I4 (ancilla_info.hpp:75-139)."""
+    rho = np.asarray(rho, np.complex128)
+    k = int(round(np.log2(rho.shape[0])))
+    S = {}
+    for mask in range(1, 1 << k):
+        S[mask] = _entropy_nats(partial_trace_bits(rho, k, [j for j in range(k) if (mask >> j) & 1]))
+    out = dict.fromkeys(("sq", "i2", "i3", "i4", "negativity", "log_negativity", "tmi", "average_mi",
+                         "min_bipartite_negativity", "joint_purity", "mean_single_purity"), 0.0)
+    full = rho / np.trace(rho).real
+    if k == 1:
+        out["sq"] = S[1]
+    elif k == 2:
+        out["sq"] = S[3]
+        out["i2"] = max(0.0, S[1] + S[2] - S[3])
+        out["negativity"] = max(0.0, cut_negativity(full, 2, 0))
+        out["log_negativity"] = float(np.log1p(2 * out["negativity"]))
+    elif k == 3:
+        out["tmi"] = S[1] + S[2] + S[4] - S[3] - S[5] - S[6] + S[7]
+        out["average_mi"] = ((S[1] + S[2] - S[3]) + (S[1] + S[4] - S[5]) + (S[2] + S[4] - S[6])) / 3
+        out["min_bipartite_negativity"] = max(0.0, min(cut_negativity(full, 3, j) for j in range(3)))
+        out["joint_purity"] = _purity(full)
+        out["mean_single_purity"] = np.mean([_purity(partial_trace_bits(full, 3, [j])) for j in range(3)])
+    else:
+        bits = lambda m: [j for j in range(4) if (m >> j) & 1]
+        i2 = [S[1 << a] + S[1 << b] - S[m] for m in range(16) if len(bits(m)) == 2 for a, b in [bits(m)]]
+        i3 = []
+        for m in range(16):
+            if len(bits(m)) == 3:
+                a, b, c = bits(m)
+                i3.append(S[1 << a] + S[1 << b] + S[1 << c] - S[(1 << a) | (1 << b)] - S[(1 << a) | (1 << c)]
+                          - S[(1 << b) | (1 << c)] + S[m])
+        out["sq"] = S[15]
+        out["i2"] = float(np.mean(i2))
+        out["i3"] = float(np.mean(i3))
+        out["i4"] = sum((1 if len(bits(m)) % 2 else -1) * S[m] for m in range(1, 16))
+    return out
 
 
 # ------------------------------------------------------------------ 2D grid

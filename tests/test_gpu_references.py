@@ -1,0 +1,94 @@
+"""GPU: several reference qubits through the batched engine.
+
+Trajectories with 2, 3 and 4 references (Bell pairs at t = 0, or attached
+after unmeasured layers: reset + h + cx per reference, in order) against the
+FP64 oracle: outcomes, states (1e-5), the references' joint density matrix
+(1e-5 per entry) and its information measures (1e-4 nats).  Closed form on
+the device: a Bell measurement of the two sites the references are paired
+with leaves the references in a Bell state (S(AB) = 0, I(A:B) = 2 ln 2,
+negativity 1/2), in the one-block kernel (L=4) and the tiled sweeps (L=16).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.test_gpu_engine import check_outcomes
+from tests.test_references import swap_layers
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mipt_amd
+    mipt_amd.lib()
+    return mipt_amd
+
+
+CASES = [  # L, layers, p, seed, reference sites, unmeasured layers before the attach
+    (8, 10, 0.15, 41, (2, 5), 0),
+    (8, 12, 0.2, 42, (1, 4, 6), 4),
+    (10, 10, 0.15, 43, (1, 3, 6, 9), 3),
+    (12, 10, 0.15, 44, (2, 5, 8, 11), 4),
+    (16, 8, 0.1, 45, (7, 15), 0),
+    (16, 10, 0.15, 46, (3, 7, 11, 15), 3),
+    (18, 8, 0.1, 47, (0, 9, 17), 2),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_batch_references_match_oracle(M, case):
+    L, layers, p, seed, sites, pz = CASES[case]
+    k = len(sites)
+    print(f"references L={L} k={k} p_zero_layers={pz}", flush=True)
+    b = M.Batch([M.Spec(L, layers, p, seed, True, probe_sites=sites, p_zero_layers=pz)])
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    psi, outs, p1, att, (g, f, u) = O.references_trajectory(L, layers, p, seed, list(sites), pz)
+    got, flagged, _, _ = b.outcomes(0)
+    check_outcomes(got, outs, p1, u, f, f"case {case}")
+    rho = b.probe_rdm()[0]
+    assert abs(np.trace(rho) - 1) < 1e-12 and np.abs(rho - rho.conj().T).max() < 1e-15
+    if flagged.any():
+        pytest.skip("a draw within the flag margin of P(1): trajectories may differ")
+    assert np.abs(b.get_state(0) - psi).max() < 1e-5
+    want = O.reference_rdm(psi, L, k)
+    assert np.abs(rho - want).max() < 1e-5
+    gi, wi = M.probe_information(rho), O.probe_information(want)
+    for key in wi:
+        assert abs(gi[key] - wi[key]) < 1e-4, (key, gi[key], wi[key])
+    b.close()
+
+
+@pytest.mark.parametrize("n,s0", [(4, 1), (16, 7)])
+def test_entanglement_swapping_on_device(M, n, s0):
+    print(f"entanglement swapping L={n}", flush=True)
+    b = M.Batch([M.Spec(n, 2, 0.0, 1, True, probe_sites=(s0, s0 + 1))])
+    g, f, u = swap_layers(n, s0)
+    b.set_layers(0, g, f, u)
+    b.evolve()
+    b.synchronize()
+    info = M.probe_information(b.probe_rdm()[0])
+    assert abs(info["sq"]) < 1e-5
+    assert abs(info["i2"] - 2 * math.log(2)) < 1e-5
+    assert abs(info["negativity"] - 0.5) < 1e-5
+    b.close()
+
+
+def test_reference_spec_validation(M):
+    bad = [M.Spec(8, 4, 0.1, 1, True, probe_sites=(2, 2)),            # repeated site
+           M.Spec(8, 4, 0.1, 1, True, probe_site=3, probe_sites=(2, 5)),  # both forms
+           M.Spec(8, 4, 0.1, 1, True, probe_sites=(2, 8)),            # site outside the chain
+           M.Spec(24, 4, 0.1, 1, True, probe_sites=(1, 2, 3, 4))]     # 28 qubits
+    for s in bad:
+        with pytest.raises(M.MiptHipError):
+            M.Batch([s])
+    b = M.Batch([M.Spec(8, 4, 0.1, 1, True, probe_sites=(1, 2)), M.Spec(8, 4, 0.1, 2, True, probe_sites=(1, 2, 3))])
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    with pytest.raises(M.MiptHipError):
+        b.probe_rdm()  # mixed reference counts
