@@ -146,10 +146,14 @@ typedef struct mipt_hip_traj_spec
                            staggered even / odd rows (cpp/include/mipt/circuits/mms.hpp:159-330) */
     int probe_count;    /* 0: one probe iff probe_site >= 0 (above).  1..4 (probe_site then -1): reference j
                            is qubit L + j, paired with the distinct site probe_sites[j]; all are attached in
-                           order j = 0, 1, ... at the start of layer p_zero_layers (reset, h, cx each; the
-                           reset's uniform is keyed by (p_zero_layers, site)), or Bell-paired at t = 0 when
-                           p_zero_layers = 0 (cpp/include/mipt/probed/kernels.hpp:33-66) */
+                           order j = 0, 1, ... at the start of layer attach_layer (reset, h, cx each; the
+                           reset's uniform is keyed by (attach_layer, site); the previous layer's measurements
+                           are applied first), or Bell-paired at t = 0 when attach_layer = 0
+                           (cpp/include/mipt/probed/kernels.hpp:33-66) */
     int probe_sites[4];
+    int attach_layer;   /* with probe_count >= 1: the layer the references attach at (-1: p_zero_layers);
+                           independent of the unmeasured prefix, e.g. two references attached at t0 = 2N after
+                           monitored layers (cpp/include/mipt/probed/runner.hpp:258-307) */
 } mipt_hip_traj_spec;
 
 typedef struct mipt_hip_batch_config

@@ -153,8 +153,8 @@ __device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int 
     double p = (w > 0.0) ? w1 / w : 0.0;
     p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
     double u, unused;
-    philox_uniform_pair(d.seed, static_cast<std::uint32_t>(d.p_zero_layers), static_cast<std::uint32_t>(lo), 0,
-                        STREAM_ATCH, u, unused);
+    philox_uniform_pair(d.seed, static_cast<std::uint32_t>(d.attach), static_cast<std::uint32_t>(lo), 0, STREAM_ATCH, u,
+                        unused);
     const int o = u < p ? 1 : 0;
     const double branch = o ? w1 : w - w1;
     if (!(branch > 0.0)) status = E_NORM;
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const
     __shared__ double red[SMALL_THREADS];
     const int traj = traj_list[blockIdx.x];
     const TrajDesc d = a.desc[traj];
-    if (d.probes == 0 || d.p_zero_layers != t || t == 0) return;  // uniform over the block
+    if (d.probes == 0 || d.attach != t || t == 0) return;  // uniform over the block
     int status = 0;
     attach_probe(a.states + d.state_off, 1u << d.nq, d, red, status);
     if (threadIdx.x == 0 && status) a.status[traj] = status;
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const
 
 __device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
 {
-    if (d.probes > 0 && d.p_zero_layers == 0)
+    if (d.probes > 0 && d.attach == 0)
     {
         // product of Bell pairs (site_j, L + j): amplitude 2^(-probes/2) where
         // every pair's two bits agree and all other bits are 0
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     int status = 0;
     for (int t = layer_begin; t < layer_end; ++t)
     {
-        if (d.probes > 0 && d.p_zero_layers > 0 && t == d.p_zero_layers) attach_probe(st, dim, d, red, status);
+        if (d.probes > 0 && d.attach > 0 && t == d.attach) attach_probe(st, dim, d, red, status);
         const int nb = geo_bond_count(d.L, d.gx, t, d.closed);
         const std::uint64_t gbase = d.gate_off + gate_prefix(d, t);
         for (int b = 0; b < nb; ++b)
@@ -1443,6 +1443,7 @@ bool probe_spec_ok(const mipt_hip_traj_spec &s)
     if (s.probe_count < 0 || s.probe_count > 4) return false;
     if (s.probe_count == 0) return s.L + (s.probe_site >= 0 ? 1 : 0) <= 27;
     if (s.probe_site >= 0 || s.L + s.probe_count > 27) return false;
+    if (s.attach_layer < -1 || (s.attach_layer > 0 && s.attach_layer >= s.layers)) return false;
     for (int j = 0; j < s.probe_count; ++j)
     {
         if (s.probe_sites[j] < 0 || s.probe_sites[j] >= s.L) return false;
@@ -1612,6 +1613,8 @@ struct Batch
             else if (s.probe_site >= 0)
                 d.psite[0] = s.probe_site;
             d.nq = s.L + d.probes;
+            d.attach = (s.probe_count > 0 && s.attach_layer >= 0) ? s.attach_layer : s.p_zero_layers;
+            d.pad = 0;
             d.layers = s.layers;
             d.closed = s.closed ? 1 : 0;
             d.gx = s.grid_x;
@@ -1853,16 +1856,17 @@ struct Batch
         al.partials = lane ? partials2 : a.partials;
         // maximum measured count per layer over the group (host flags)
         bool attach_any = false;
-        for (int i : grp) attach_any |= desc[i].probes > 0 && desc[i].p_zero_layers > 0;
+        for (int i : grp) attach_any |= desc[i].probes > 0 && desc[i].attach > 0;
         for (int t = t0; t < t1; ++t)
         {
             const LayerPlan &LP = lp[t % P];
             if (attach_any && t > 0)
             {
                 bool at = false;
-                for (int i : grp) at |= desc[i].probes > 0 && desc[i].p_zero_layers == t;
+                for (int i : grp) at |= desc[i].probes > 0 && desc[i].attach == t;
                 if (at)
                 {
+                    if (t > t0) MIPT_HIP_TRY(close_layer(cf, lp[(t - 1) % P], list, ntraj, t));
                     hipLaunchKernelGGL(attach_kernel, dim3(ntraj), dim3(SMALL_THREADS), 0, ls, al, list, t);
                     MIPT_HIP_CHECK(hipGetLastError());
                 }
@@ -1914,19 +1918,24 @@ struct Batch
             }
         }
         // apply the last layer's pending collapse
-        if (t1 > t0)
-        {
-            PassPlan pl = lp[(t1 - 1) % P].passes[0];
-            pl.nstages = 0;
-            pl.collapse = 1;
-            pl.init = 0;
-            pl.hist = 0;
-            pl.store = 1;
-            MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t1));
-            hipLaunchKernelGGL(reset_collapse_kernel, dim3((ntraj + 255) / 256), dim3(256), 0, ls, al, list, ntraj,
-                               t1 - 1);
-            MIPT_HIP_CHECK(hipGetLastError());
-        }
+        if (t1 > t0) MIPT_HIP_TRY(close_layer(cf, lp[(t1 - 1) % P], list, ntraj, t1));
+        return OK;
+    }
+
+    // Applies layer t-1's pending collapse with a stage-free sweep and marks
+    // it applied, so that the state is the post-measurement state before
+    // layer t (the end of an evolution range, or a reference attach at t).
+    int close_layer(const Config &cf, const LayerPlan &prev, const int *list, int ntraj, int t)
+    {
+        PassPlan pl = prev.passes[0];
+        pl.nstages = 0;
+        pl.collapse = 1;
+        pl.init = 0;
+        pl.hist = 0;
+        pl.store = 1;
+        MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
+        hipLaunchKernelGGL(reset_collapse_kernel, dim3((ntraj + 255) / 256), dim3(256), 0, ls, al, list, ntraj, t - 1);
+        MIPT_HIP_CHECK(hipGetLastError());
         return OK;
     }
 

@@ -390,21 +390,23 @@ def attach_reference(psi: np.ndarray, nq: int, site: int, ref: int, u: float) ->
 
 
 def references_trajectory(n: int, layers: int, p: float, seed: int, sites, p_zero_layers: int,
-                          closed: bool = True):
-    """k = len(sites) references (qubits n .. n+k-1): Bell pairs (site_j, n+j)
-    at t = 0 when p_zero_layers = 0, else unmeasured layers 0..p_zero_layers-1
-    and the references attached in order j = 0, 1, ... at the start of layer
-    p_zero_layers (cpp/include/mipt/probed/kernels.hpp:55-66, each reset's
-    uniform keyed by (layer, site)); measured layers after.  Returns
-    (psi over n+k qubits, outcomes [layers, n], p1, attach outcomes, (g, f, u))."""
+                          closed: bool = True, attach_layer: int | None = None):
+    """k = len(sites) references (qubits n .. n+k-1); layers 0..p_zero_layers-1
+    unmeasured.  Bell pairs (site_j, n+j) at t = 0 when the attach layer
+    (attach_layer, default p_zero_layers) is 0, else the references attached
+    in order j = 0, 1, ... at the start of that layer, after the previous
+    layer's measurements (cpp/include/mipt/probed/kernels.hpp:55-66, each
+    reset's uniform keyed by (layer, site)).  Returns (psi over n+k qubits,
+    outcomes [layers, n], p1, attach outcomes, (g, f, u))."""
     k = len(sites)
+    attach = p_zero_layers if attach_layer is None else attach_layer
     g, f, u = build_layers(n, layers, p, seed, closed)
     f[:p_zero_layers] = 0
     u[:p_zero_layers] = 0.0
     nq = n + k
     psi = np.zeros(1 << nq, np.complex128)
     att = []
-    if p_zero_layers == 0:
+    if attach == 0:
         for b in range(1 << k):
             i = 0
             for j in range(k):
@@ -417,7 +419,7 @@ def references_trajectory(n: int, layers: int, p: float, seed: int, sites, p_zer
     p1 = np.zeros((layers, n))
     gi = 0
     for t in range(layers):
-        if p_zero_layers > 0 and t == p_zero_layers:
+        if attach > 0 and t == attach:
             for j, site in enumerate(sites):
                 att.append(attach_reference(psi, nq, site, n + j, philox_uniform(seed, t, site, 0, STREAM_ATCH)))
         for (q0, q1) in layer_bonds(n, t, closed):

@@ -36,6 +36,15 @@ CASES = [  # L, layers, p, seed, reference sites, unmeasured layers before the a
     (16, 10, 0.15, 46, (3, 7, 11, 15), 3),
     (18, 8, 0.1, 47, (0, 9, 17), 2),
 ]
+# references attached after monitored layers (the two- and four-reference
+# protocols attach at t0 = 2N after measured evolution): attach layer > the
+# unmeasured prefix, so the previous layer's measurements are applied first
+ATTACH_CASES = [  # L, layers, p, seed, sites, unmeasured prefix, attach layer
+    (8, 12, 0.2, 51, (1, 5), 0, 4),
+    (10, 12, 0.2, 52, (0, 2, 5, 9), 1, 5),
+    (14, 10, 0.2, 53, (3, 10), 0, 4),
+    (16, 10, 0.15, 54, (3, 7, 11, 15), 2, 5),
+]
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
@@ -63,6 +72,31 @@ def test_batch_references_match_oracle(M, case):
     b.close()
 
 
+@pytest.mark.parametrize("case", range(len(ATTACH_CASES)))
+def test_batch_references_attached_after_monitored_layers(M, case):
+    L, layers, p, seed, sites, pz, att_layer = ATTACH_CASES[case]
+    k = len(sites)
+    print(f"references L={L} k={k} p_zero_layers={pz} attach_layer={att_layer}", flush=True)
+    b = M.Batch([M.Spec(L, layers, p, seed, True, probe_sites=sites, p_zero_layers=pz, attach_layer=att_layer)])
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    psi, outs, p1, att, (g, f, u) = O.references_trajectory(L, layers, p, seed, list(sites), pz,
+                                                            attach_layer=att_layer)
+    assert f[:att_layer].sum() > 0  # measurements before the attach
+    got, flagged, _, _ = b.outcomes(0)
+    check_outcomes(got, outs, p1, u, f, f"attach case {case}")
+    if flagged.any():
+        pytest.skip("a draw within the flag margin of P(1): trajectories may differ")
+    assert np.abs(b.get_state(0) - psi).max() < 1e-5
+    rho, want = b.probe_rdm()[0], O.reference_rdm(psi, L, k)
+    assert np.abs(rho - want).max() < 1e-5
+    gi, wi = M.probe_information(rho), O.probe_information(want)
+    for key in wi:
+        assert abs(gi[key] - wi[key]) < 1e-4, (key, gi[key], wi[key])
+    b.close()
+
+
 @pytest.mark.parametrize("n,s0", [(4, 1), (16, 7)])
 def test_entanglement_swapping_on_device(M, n, s0):
     print(f"entanglement swapping L={n}", flush=True)
@@ -82,7 +116,8 @@ def test_reference_spec_validation(M):
     bad = [M.Spec(8, 4, 0.1, 1, True, probe_sites=(2, 2)),            # repeated site
            M.Spec(8, 4, 0.1, 1, True, probe_site=3, probe_sites=(2, 5)),  # both forms
            M.Spec(8, 4, 0.1, 1, True, probe_sites=(2, 8)),            # site outside the chain
-           M.Spec(24, 4, 0.1, 1, True, probe_sites=(1, 2, 3, 4))]     # 28 qubits
+           M.Spec(24, 4, 0.1, 1, True, probe_sites=(1, 2, 3, 4)),     # 28 qubits
+           M.Spec(8, 4, 0.1, 1, True, probe_sites=(1, 2), attach_layer=4)]  # attach past the last layer
     for s in bad:
         with pytest.raises(M.MiptHipError):
             M.Batch([s])
