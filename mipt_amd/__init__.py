@@ -107,6 +107,7 @@ def lib():
     L.mipt_hip_probe_information.argtypes = [dp, i, dp]
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
     L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
+    L.mipt_hip_probed_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     _LIB = L
     return L
 
@@ -590,6 +591,14 @@ def entropy_main(argv) -> int:
     args = [b"entropy"] + [str(a).encode() for a in argv]
     arr = (ctypes.c_char_p * len(args))(*args)
     return lib().mipt_hip_entropy_main(len(args), arr)
+
+
+def probed_main(argv) -> int:
+    """mipt_probed.exe's modes 0 and 1 ([probes] [N] [realizations]
+    [circ_type] [mode] [args]) in this process; returns its exit code."""
+    args = [b"mipt_probed"] + [str(a).encode() for a in argv]
+    arr = (ctypes.c_char_p * len(args))(*args)
+    return lib().mipt_hip_probed_main(len(args), arr)
 
 
 def sim_tmi_main(argv) -> int:

@@ -723,6 +723,266 @@ int run_entropy(int argc, char **argv)
     return 0;
 }
 
+// ------------------------------------------------------------ mipt_probed
+// cpp/apps/mipt_probed.cpp with cpp/include/mipt/probed/config.hpp:261-372
+// (arguments, reference sites, encoder schedule), runner.hpp:258-307 (the
+// sampling trajectory) and output.hpp:36-95, 386-476 (names and CSV
+// columns): modes 0 and 1 with one, two or four references on the Haar ring.
+// One reference: Bell pair at t = 0 and t_encode = 2N unmeasured layers
+// (mode 0: readout times are elapsed after the encoder; mode 1: t = 4N
+// absolute).  Two or four: 2N monitored layers, the references attached at
+// t0 = 2N, absolute readout times.  Every trajectory is read at every
+// readout time in order; means and standard errors per (p, t).
+struct ProbeStats
+{
+    Welford sq, i2, i3, i4, neg, logneg;
+};
+
+std::string compact_count(std::uint64_t count)
+{
+    if (count >= 1000000 && count % 1000000 == 0) return std::to_string(count / 1000000) + "m";
+    if (count >= 1000 && count % 1000 == 0) return std::to_string(count / 1000) + "k";
+    return std::to_string(count);
+}
+
+int env_int(const char *name, int dflt, long lo, long hi)
+{
+    const char *e = std::getenv(name);
+    if (!e || !*e) return dflt;
+    char *end = nullptr;
+    const long v = std::strtol(e, &end, 10);
+    if (*end != '\0' || v < lo || v > hi)
+        throw std::invalid_argument(std::string(name) + " must be an integer in [" + std::to_string(lo) + ", " +
+                                    std::to_string(hi) + "].");
+    return static_cast<int>(v);
+}
+
+void probed_usage(const char *prog)
+{
+    std::cout << "Usage: " << prog << " [probes] [N] [realizations] [circ_type] [probe_mode] [args]\n\n"
+              << "probes     1, 2 or 4 reference qubits\n"
+              << "N          monitored system sites (even; N + probes <= 27)\n"
+              << "circ_type  1 = random Haar U(4) brickwork ring (the only one built here)\n"
+              << "mode 0     [p] [t_min] [t_max]: readout times t_min..t_max at one p\n"
+              << "           (one reference: elapsed after the 2N-layer encoder, MIPT_PROBED_T_ENCODE;\n"
+              << "            two/four: absolute, t_min >= t0 = 2N)\n"
+              << "mode 1     [p_min] [p_max] [p_res]: readout at t = 4N\n"
+              << "Reference sites: one at N-MIPT_PROBED_DISTANCE (mode 0, default 1) or N/2 (mode 1);\n"
+              << "two at (N-1)/2, N-1; four at N/4-1, N/2-1, 3N/4-1, N-1.\n"
+              << "Output: the generated probed_*.csv name, or MIPT_PROBED_INTERNAL_OUTPUT; seeds from MIPT_SEED.\n";
+}
+
+int run_probed(int argc, char **argv)
+{
+    if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0))
+    {
+        probed_usage(argv[0]);
+        return 0;
+    }
+    if (argc < 6) throw std::invalid_argument("Missing arguments. Use --help for usage.");
+    const int probes = std::stoi(argv[1]);
+    const int n = std::stoi(argv[2]);
+    const int realizations = std::stoi(argv[3]);
+    const int circ = std::stoi(argv[4]);
+    const int mode = std::stoi(argv[5]);
+    if (probes != 1 && probes != 2 && probes != 3 && probes != 4)
+        throw std::invalid_argument("probes must be 1, 2, 3, or 4.");
+    if (n <= 0 || realizations <= 0) throw std::invalid_argument("N and realizations must be positive.");
+    if (circ != 1)
+        throw std::invalid_argument("circ_type " + std::to_string(circ) +
+                                    " is not built on this engine; use 1 (random Haar U(4)).");
+    if (n % 2 != 0) throw std::invalid_argument("Random Haar Unitary requires an even N for periodic brickwork geometry.");
+    if (n + probes > 27) throw std::invalid_argument("this engine holds states up to N + probes = 27 qubits.");
+    if (mode < 0 || mode > 5) throw std::invalid_argument("probe_mode must be 0, 1, 2, 3, 4, or 5.");
+    if (probes == 3) throw std::invalid_argument("Three probes are currently supported only in mode 4.");
+    if (mode > 1) throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0 or 1.");
+    if (argc != 9)
+        throw std::invalid_argument(mode == 0 ? "Mode 0 requires [p] [t_min] [t_max]." : "Mode 1 requires [p_min] [p_max] [p_res].");
+    std::vector<double> ps;
+    std::vector<int> times;
+    int t_encode = 0, origin = 0;
+    const int t0 = probes == 1 ? 0 : 2 * n;
+    if (mode == 0)
+    {
+        const double p = std::stod(argv[6]);
+        const int t_min = std::stoi(argv[7]), t_max = std::stoi(argv[8]);
+        if (p < 0.0 || p > 1.0 || t_min < 0 || t_max < t_min)
+            throw std::invalid_argument("Require p in [0,1] and 0 <= t_min <= t_max.");
+        if (probes > 1 && t_min < t0) throw std::invalid_argument("Two/four-probe legacy mode requires t_min >= t0=2N.");
+        if (probes == 1) origin = t_encode = env_int("MIPT_PROBED_T_ENCODE", 2 * n, 0, 1L << 20);
+        ps = {p};
+        for (int t = t_min; t <= t_max; ++t) times.push_back(t);
+    }
+    else
+    {
+        const double p_min = std::stod(argv[6]), p_max = std::stod(argv[7]);
+        const int p_res = std::stoi(argv[8]);
+        if (p_min < 0.0 || p_max > 1.0) throw std::invalid_argument("p endpoints must lie in [0,1].");
+        if (p_res <= 0) throw std::invalid_argument("Grid resolution must be positive.");
+        if (p_min > p_max) throw std::invalid_argument("Grid minimum must not exceed its maximum.");
+        ps = linspace(p_min, p_max, p_res);
+        times = {4 * n};
+        if (probes == 1) t_encode = 2 * n;
+    }
+    std::vector<int> sites;
+    if (probes == 1)
+        sites = {mode == 0 ? n - env_int("MIPT_PROBED_DISTANCE", 1, 1, n) : n / 2};
+    else if (probes == 2)
+        sites = {(n - 1) / 2, n - 1};
+    else
+    {
+        if (n % 4 != 0) throw std::invalid_argument("Four-probe protocols require N divisible by four.");
+        sites = {n / 4 - 1, n / 2 - 1, 3 * n / 4 - 1, n - 1};
+    }
+    const int depth = origin + times.back();
+    if (probes > 1 && depth <= t0) throw std::invalid_argument("the readout must come after the attach at t0=2N.");
+
+    std::string output;
+    if (const char *e = std::getenv("MIPT_PROBED_INTERNAL_OUTPUT"); e && *e) output = e;
+    else
+    {
+        output = "probed_" + std::to_string(probes) + "_haar_n_" + std::to_string(n) + "_reals_" +
+                 compact_count(static_cast<std::uint64_t>(realizations)) + "_mode_" + std::to_string(mode);
+        output += ps.size() == 1 ? "_p_" + number_for_filename(ps.front())
+                                 : "_p_" + number_for_filename(ps.front()) + "_" + number_for_filename(ps.back()) +
+                                       "_res_" + std::to_string(ps.size());
+        output += "_t_" + std::to_string(times.front()) + "_" + std::to_string(times.back()) + ".csv";
+    }
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
+    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+
+    const int D = 1 << probes;
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const int per_batch = static_cast<int>(
+        std::max<double>(1.0, std::min<double>(realizations, budget_gib * 1073741824.0 / std::ldexp(8.0, n + probes))));
+    std::vector<ProbeStats> stats(ps.size() * times.size());
+    for (std::size_t pi = 0; pi < ps.size(); ++pi)
+    {
+        for (int first = 0; first < realizations; first += per_batch)
+        {
+            const int count = std::min(per_batch, realizations - first);
+            std::vector<mipt_hip_traj_spec> specs(count);
+            for (int i = 0; i < count; ++i)
+            {
+                mipt_hip_traj_spec &s = specs[i];
+                s = mipt_hip_traj_spec{};
+                s.L = n;
+                s.layers = depth;
+                s.p = ps[pi];
+                s.seed = trajectory_seed(master, pi * static_cast<std::uint64_t>(realizations) + first + i);
+                s.closed = 1;
+                s.gate_family = 1;
+                s.probe_site = -1;
+                s.p_zero_layers = probes == 1 ? std::min(t_encode, depth) : 0;
+                s.shared_gates = -1;
+                s.grid_x = 0;
+                s.probe_count = probes;
+                for (int j = 0; j < probes; ++j) s.probe_sites[j] = sites[j];
+                s.attach_layer = probes == 1 ? 0 : t0;
+            }
+            mipt_hip_batch *b = nullptr;
+            const mipt_hip_batch_config cfg{0, 0, 0};
+            check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
+            int st = mipt_hip_batch_setup(b, specs.data(), count);
+            if (!st) st = mipt_hip_batch_generate(b);
+            std::vector<double> rho(static_cast<std::size_t>(count) * D * D * 2);
+            int cur = 0;
+            for (std::size_t ti = 0; ti < times.size() && st == 0; ++ti)
+            {
+                const int target = origin + times[ti];
+                ProbeStats &acc = stats[pi * times.size() + ti];
+                if (probes == 1 && mode == 0 && target == 0)
+                {
+                    // before any layer a single reference is maximally entangled
+                    for (int i = 0; i < count; ++i)
+                    {
+                        acc.sq.add(std::log(2.0));
+                        acc.i2.add(0.0), acc.i3.add(0.0), acc.i4.add(0.0), acc.neg.add(0.0), acc.logneg.add(0.0);
+                    }
+                    continue;
+                }
+                if (probes > 1 && target == t0)
+                {
+                    // read right after the attach (which the engine runs at the
+                    // start of layer t0): each reset site forms a fresh Bell pair
+                    // with its reference, so the references are exactly in the
+                    // maximally mixed product state: S = k ln 2, no information
+                    for (int i = 0; i < count; ++i)
+                    {
+                        acc.sq.add(probes * std::log(2.0));
+                        acc.i2.add(0.0), acc.i3.add(0.0), acc.i4.add(0.0), acc.neg.add(0.0), acc.logneg.add(0.0);
+                    }
+                    continue;
+                }
+                if (target > cur)
+                {
+                    st = mipt_hip_batch_evolve_range(b, cur, target);
+                    cur = target;
+                }
+                if (!st) st = mipt_hip_batch_probe_rdm(b, rho.data());
+                for (int i = 0; i < count && st == 0; ++i)
+                {
+                    double v[11];
+                    st = mipt_hip_probe_information(rho.data() + static_cast<std::size_t>(i) * D * D * 2, probes, v);
+                    if (st) break;
+                    acc.sq.add(v[0]), acc.i2.add(v[1]), acc.i3.add(v[2]), acc.i4.add(v[3]);
+                    acc.neg.add(v[4]), acc.logneg.add(v[5]);
+                }
+            }
+            (void)mipt_hip_batch_destroy(b);
+            check(st, "probed batch");
+            std::cerr << "p=" << ps[pi] << ": " << first + count << '/' << realizations << " realizations\n";
+        }
+    }
+    mkdirs_for(output);
+    std::ofstream csv(output, std::ios::out | std::ios::trunc);
+    if (!csv) throw std::runtime_error("Could not open output CSV: " + output);
+    csv << std::setprecision(17);
+    const char *meta = "N,circ_type,circuit_name,realizations,p";
+    if (mode == 0 && probes == 1) csv << meta << ",t,S_mean,S_stderr\n";
+    else if (mode == 0 && probes == 2)
+        csv << meta << ",t,t_minus_t0,scaled_time,I_mean,I_stderr,negativity_mean,negativity_stderr,"
+                       "log_negativity_mean,log_negativity_stderr\n";
+    else if (mode == 0)
+        csv << meta << ",t,t_minus_t0,scaled_time,I2_mean,I2_stderr,I3_mean,I3_stderr,I4_mean,I4_stderr\n";
+    else
+    {
+        csv << meta << ",t,S_Q_mean,S_Q_stderr";
+        if (probes >= 2) csv << ",I2_mean,I2_stderr";
+        if (probes == 4) csv << ",I3_mean,I3_stderr,I4_mean,I4_stderr";
+        csv << '\n';
+    }
+    for (std::size_t pi = 0; pi < ps.size(); ++pi)
+        for (std::size_t ti = 0; ti < times.size(); ++ti)
+        {
+            const ProbeStats &a = stats[pi * times.size() + ti];
+            const int t = times[ti];
+            csv << n << ',' << circ << ",\"Random Haar Unitary\"," << realizations << ',' << ps[pi] << ',' << t;
+            if (mode == 0 && probes == 1) csv << ',' << a.sq.mean << ',' << a.sq.se();
+            else if (mode == 0)
+            {
+                csv << ',' << t - t0 << ',' << static_cast<double>(t - t0) / n << ',' << a.i2.mean << ',' << a.i2.se();
+                if (probes == 2)
+                    csv << ',' << a.neg.mean << ',' << a.neg.se() << ',' << a.logneg.mean << ',' << a.logneg.se();
+                else
+                    csv << ',' << a.i3.mean << ',' << a.i3.se() << ',' << a.i4.mean << ',' << a.i4.se();
+            }
+            else
+            {
+                csv << ',' << a.sq.mean << ',' << a.sq.se();
+                if (probes >= 2) csv << ',' << a.i2.mean << ',' << a.i2.se();
+                if (probes == 4) csv << ',' << a.i3.mean << ',' << a.i3.se() << ',' << a.i4.mean << ',' << a.i4.se();
+            }
+            csv << '\n';
+        }
+    csv.flush();
+    if (!csv) throw std::runtime_error("Failed while writing output CSV: " + output);
+    std::cout << output << '\n';
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -815,6 +1075,19 @@ int mipt_hip_mipt_main(int argc, char **argv)
     try
     {
         return run_mipt(argc, argv);
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "\nerror: " << e.what() << '\n';
+        return 1;
+    }
+}
+
+int mipt_hip_probed_main(int argc, char **argv)
+{
+    try
+    {
+        return run_probed(argc, argv);
     }
     catch (const std::exception &e)
     {

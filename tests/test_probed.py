@@ -1,0 +1,134 @@
+"""mipt_probed modes 0 and 1 (one, two or four references).
+
+CPU: the argument contract of cpp/include/mipt/probed/config.hpp:261-372
+(every case fails before any device work).  GPU: the CSV the driver writes
+(cpp/include/mipt/probed/output.hpp:386-476 columns, the generated name of
+output.hpp:46-95) against the FP64 oracle run of every trajectory (same
+seeds): per-(p, t) means within 1e-4 nats, standard errors within 1e-4; the
+readout right after the attach at t0 = 2N is the closed form S = k ln 2 with
+no information.
+"""
+import csv
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mipt_amd
+    mipt_amd.lib()
+    return mipt_amd
+
+
+@pytest.mark.parametrize("argv,code", [
+    (["--help"], 0),
+    ([], 1),
+    (["3", "8", "4", "1", "1", "0", "0.3", "2"], 1),       # three references: mode 4 only
+    (["1", "8", "4", "2", "1", "0", "0.3", "2"], 1),       # circ_type 2 not built
+    (["1", "7", "4", "1", "1", "0", "0.3", "2"], 1),       # odd ring
+    (["4", "10", "4", "1", "1", "0", "0.3", "2"], 1),      # four references need N % 4 == 0
+    (["2", "8", "4", "1", "0", "0.1", "4", "20"], 1),      # t_min < t0 = 2N
+    (["1", "8", "4", "1", "2", "0", "1", "2", "0", "1", "2"], 1),  # mode 2 not built
+    (["1", "8", "4", "1", "0", "0.1", "4"], 1),            # mode 0 needs three values
+    (["1", "8", "4", "1", "1", "0.3", "0.1", "3"], 1),     # p_min > p_max
+    (["1", "8", "4", "1", "1", "0.1", "0.3", "0"], 1),     # p_res <= 0
+    (["2", "26", "4", "1", "1", "0.1", "0.3", "2"], 1),    # 28 qubits
+])
+def test_probed_argument_contract(M, argv, code, capfd):
+    assert M.probed_main(argv) == code
+    if code == 1:
+        assert "error:" in capfd.readouterr().err
+
+
+def read_csv(path):
+    with open(path) as f:
+        rows = list(csv.reader(f))
+    return rows[0], [dict(zip(rows[0], r)) for r in rows[1:]]
+
+
+def oracle_samples(n, depth, p, seeds, sites, pz, attach):
+    k = len(sites)
+    out = []
+    for s in seeds:
+        psi, *_ = O.references_trajectory(n, depth, p, s, list(sites), pz, attach_layer=attach)
+        out.append(O.probe_information(O.reference_rdm(psi, n, k)))
+    return out
+
+
+def mean_se(v):
+    v = np.asarray(v, np.float64)
+    return v.mean(), (v.std(ddof=1) / math.sqrt(len(v)) if len(v) > 1 else 0.0)
+
+
+@pytest.mark.gpu
+def test_probed_mode1_two_references(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9001")
+    n, R = 8, 6
+    print("mipt_probed mode 1, two references", flush=True)
+    assert M.probed_main([2, n, R, 1, 1, 0.1, 0.3, 2]) == 0
+    name = f"probed_2_haar_n_{n}_reals_{R}_mode_1_p_0.1_0.3_res_2_t_32_32.csv"
+    head, rows = read_csv(tmp_path / name)
+    assert head == ["N", "circ_type", "circuit_name", "realizations", "p", "t", "S_Q_mean", "S_Q_stderr",
+                    "I2_mean", "I2_stderr"]
+    assert len(rows) == 2
+    for pi, row in enumerate(rows):
+        p = [0.1, 0.3][pi]
+        assert (int(row["N"]), row["circuit_name"], int(row["t"]), float(row["p"])) == (n, "Random Haar Unitary", 32, p)
+        seeds = [O.lib().orc_trajectory_seed(9001, pi * R + i) for i in range(R)]
+        want = oracle_samples(n, 4 * n, p, seeds, [3, 7], 0, 2 * n)
+        for col, key in (("S_Q", "sq"), ("I2", "i2")):
+            m, se = mean_se([w[key] for w in want])
+            assert abs(float(row[col + "_mean"]) - m) < 1e-4, (col, p)
+            assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (col, p)
+
+
+@pytest.mark.gpu
+def test_probed_mode0_one_reference_encoder(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9002")
+    n, R, p = 8, 5, 0.2
+    print("mipt_probed mode 0, one reference", flush=True)
+    assert M.probed_main([1, n, R, 1, 0, p, 0, 3]) == 0
+    head, rows = read_csv(tmp_path / f"probed_1_haar_n_{n}_reals_{R}_mode_0_p_0.2_t_0_3.csv")
+    assert head == ["N", "circ_type", "circuit_name", "realizations", "p", "t", "S_mean", "S_stderr"]
+    seeds = [O.lib().orc_trajectory_seed(9002, i) for i in range(R)]
+    for t, row in enumerate(rows):
+        assert int(row["t"]) == t
+        # reference at site N-1, Bell pair at t=0, 2N unmeasured encoder layers, then t measured layers
+        want = oracle_samples(n, 2 * n + t, p, seeds, [n - 1], 2 * n, 0)
+        m, se = mean_se([w["sq"] for w in want])
+        assert abs(float(row["S_mean"]) - m) < 1e-4 and abs(float(row["S_stderr"]) - se) < 1e-4
+
+
+@pytest.mark.gpu
+def test_probed_mode0_four_references(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9003")
+    out = tmp_path / "four.csv"
+    monkeypatch.setenv("MIPT_PROBED_INTERNAL_OUTPUT", str(out))
+    n, R, p = 8, 4, 0.15
+    print("mipt_probed mode 0, four references", flush=True)
+    assert M.probed_main([4, n, R, 1, 0, p, 2 * n, 2 * n + 2]) == 0
+    head, rows = read_csv(out)
+    assert head[5:] == ["t", "t_minus_t0", "scaled_time", "I2_mean", "I2_stderr", "I3_mean", "I3_stderr",
+                        "I4_mean", "I4_stderr"]
+    seeds = [O.lib().orc_trajectory_seed(9003, i) for i in range(R)]
+    sites = [n // 4 - 1, n // 2 - 1, 3 * n // 4 - 1, n - 1]
+    for j, row in enumerate(rows):
+        t = 2 * n + j
+        assert (int(row["t"]), int(row["t_minus_t0"])) == (t, j)
+        assert abs(float(row["scaled_time"]) - j / n) < 1e-15
+        if j == 0:
+            for col in ("I2", "I3", "I4"):
+                assert float(row[col + "_mean"]) == 0.0
+            continue
+        want = oracle_samples(n, t, p, seeds, sites, 0, 2 * n)
+        for col, key in (("I2", "i2"), ("I3", "i3"), ("I4", "i4")):
+            m, se = mean_se([w[key] for w in want])
+            assert abs(float(row[col + "_mean"]) - m) < 1e-4, (t, col)
+            assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (t, col)
