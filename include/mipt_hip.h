@@ -351,6 +351,16 @@ int mipt_hip_entropy_main(int argc, char **argv);
  * probes 1, 2 or 4, circ_type 1; the same reference sites, encoder schedule, readout times, output name and CSV
  * columns.  Returns 0, or 1 on an error (message on stderr). */
 int mipt_hip_probed_main(int argc, char **argv);
+/* free_energy.exe (cpp/apps/free_energy.cpp): [N] [p] [realizations] [circ_type] [init_state] with
+ * circ_type 1; 4N equilibration + 24N production layers, cumulative measurement surprisal (nats) and
+ * S(L/2) (bits, t <= MIPT_FREE_ENERGY_ENTROPY_MAX_T); the reference's samples and time-series CSVs
+ * (free_energy_resume.hpp:79-88) under its generated names.  No resume: existing CSVs are refused
+ * unless MIPT_FREE_ENERGY_RESUME=0.  Returns 0, 1 on an error, 2 on a usage error. */
+int mipt_hip_free_energy_main(int argc, char **argv);
+/* The initial state free_energy uses for a trajectory seed (host code): init_state 0 = product of
+ * single-site states uniform on the Bloch sphere, 1 = normalised complex Gaussian vector
+ * (cpp/include/mipt/free_energy.hpp:127-222); psi_ri [2^n][re, im]. */
+int mipt_hip_free_energy_initial_state(int n, int init_state, uint64_t trajectory_seed, double *psi_ri);
 
 #ifdef __cplusplus
 }

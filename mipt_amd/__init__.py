@@ -108,6 +108,8 @@ def lib():
     L.mipt_hip_run_batch.argtypes = [vp, i, i, i, i, dp, vp, vp]
     L.mipt_hip_sim_tmi_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     L.mipt_hip_probed_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
+    L.mipt_hip_free_energy_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
+    L.mipt_hip_free_energy_initial_state.argtypes = [i, i, ctypes.c_uint64, dp]
     _LIB = L
     return L
 
@@ -591,6 +593,22 @@ def entropy_main(argv) -> int:
     args = [b"entropy"] + [str(a).encode() for a in argv]
     arr = (ctypes.c_char_p * len(args))(*args)
     return lib().mipt_hip_entropy_main(len(args), arr)
+
+
+def free_energy_main(argv) -> int:
+    """free_energy.exe ([N] [p] [realizations] [circ_type] [init_state]) in
+    this process; returns its exit code."""
+    args = [b"free_energy"] + [str(a).encode() for a in argv]
+    arr = (ctypes.c_char_p * len(args))(*args)
+    return lib().mipt_hip_free_energy_main(len(args), arr)
+
+
+def free_energy_initial_state(n: int, init_state: int, trajectory_seed: int) -> np.ndarray:
+    """The initial state free_energy draws for a trajectory seed (complex128 [2^n])."""
+    out = np.zeros((1 << n, 2))
+    _check(lib().mipt_hip_free_energy_initial_state(n, init_state, trajectory_seed & (2**64 - 1), _dp(out)),
+           "free_energy_initial_state")
+    return out[:, 0] + 1j * out[:, 1]
 
 
 def probed_main(argv) -> int:

@@ -13,7 +13,8 @@ SOURCES = ["plain.hip", "batch.hip", "entropy.hip", "decoder.hip", "driver.cpp",
 BIN = os.path.join(HERE, "bin")
 TOOLS = {"sim_tmi": os.path.join(HERE, "host", "sim_tmi_main.cpp"), "mipt": os.path.join(HERE, "host", "mipt_main.cpp"),
          "entropy": os.path.join(HERE, "host", "entropy_main.cpp"),
-         "mipt_probed": os.path.join(HERE, "host", "probed_main.cpp")}
+         "mipt_probed": os.path.join(HERE, "host", "probed_main.cpp"),
+         "free_energy": os.path.join(HERE, "host", "free_energy_main.cpp")}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -25,6 +25,7 @@
 #include <cerrno>
 #include <charconv>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -983,6 +984,296 @@ int run_probed(int argc, char **argv)
     return 0;
 }
 
+// ------------------------------------------------------------ free_energy
+// cpp/apps/free_energy.cpp with cpp/include/mipt/free_energy.hpp:127-246
+// (initial states) and free_energy_resume.hpp:79-88, 152-167 (CSV headers,
+// the half-chain entropy schedule): the measurement-record free energy on
+// the Haar ring.  4N equilibration + 24N production layers per trajectory;
+// F(t) = cumulative surprisal -ln P(outcome) of the measurements (nats, from
+// the engine's conditional P(1) of every measured site), S(L/2) in bits at
+// t = 0 and every stride-th t <= entropy_max_t; per-trajectory samples and
+// per-t aggregates in the reference's two CSVs (no resume: existing files
+// are refused unless MIPT_FREE_ENERGY_RESUME=0).
+std::uint64_t fe_splitmix64(std::uint64_t v)
+{
+    v += 0x9e3779b97f4a7c15ull;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+    return v ^ (v >> 31);
+}
+
+// The initial state of trajectory `trajectory_seed` (free_energy.hpp:127-222
+// draw order, std::mt19937_64 keyed as free_energy.cpp does): init 0 = a
+// product of single-site states uniform on the Bloch sphere, 1 = a Gaussian
+// vector normalised (Haar); Haar circuits fix no parity.  psi [2^n] complex128.
+void fe_initial_state(int n, int init_state, std::uint64_t trajectory_seed, std::vector<std::complex<double>> &psi)
+{
+    std::mt19937_64 rng(fe_splitmix64(trajectory_seed ^ 0x494e495449414c53ull));
+    const std::size_t dim = std::size_t{1} << n;
+    psi.assign(dim, {0.0, 0.0});
+    if (init_state == 0)
+    {
+        std::uniform_real_distribution<double> unit(0.0, 1.0);
+        psi[0] = 1.0;
+        std::size_t active = 1;
+        const double two_pi = 6.283185307179586476925286766559;
+        for (int q = 0; q < n; ++q)
+        {
+            const double z = 1.0 - 2.0 * unit(rng);
+            const double theta = std::acos(std::clamp(z, -1.0, 1.0));
+            const double phi = two_pi * unit(rng);
+            const std::complex<double> a0(std::cos(0.5 * theta), 0.0), a1 = std::polar(std::sin(0.5 * theta), phi);
+            for (std::size_t i = 0; i < active; ++i)
+            {
+                const std::complex<double> prior = psi[i];
+                psi[i] = prior * a0;
+                psi[i | active] = prior * a1;
+            }
+            active <<= 1;
+        }
+        return;
+    }
+    std::normal_distribution<double> normal(0.0, 1.0);
+    long double norm2 = 0.0L;
+    for (std::size_t i = 0; i < dim; ++i)
+    {
+        const double re = normal(rng);
+        const double im = normal(rng);
+        psi[i] = {re, im};
+        norm2 += static_cast<long double>(re * re + im * im);
+    }
+    const double inv = static_cast<double>(1.0L / std::sqrt(norm2));
+    for (auto &x : psi) x *= inv;
+}
+
+bool file_exists(const std::string &path)
+{
+    struct stat sb;
+    return ::stat(path.c_str(), &sb) == 0;
+}
+
+bool fe_records_entropy(int t, bool enabled, int max_t, int stride)
+{
+    if (!enabled || max_t < 0) return false;
+    if (t == 0) return true;
+    if (t > max_t) return false;
+    return t % stride == 0 || t == max_t;
+}
+
+void fe_write_stat(std::ostream &out, const Welford &w)
+{
+    if (w.count == 0) out << "nan,nan,nan";
+    else out << w.mean << ',' << w.sd() << ',' << w.se();
+}
+
+void fe_usage(const char *prog)
+{
+    std::cout << "Usage: " << prog << " [N] [p] [realizations] [circ_type] [init_state]\n\n"
+              << "  N            sites of the periodic ring (even, <= 26)\n"
+              << "  p            measurement rate in [0,1]\n"
+              << "  realizations trajectories\n"
+              << "  circ_type    1 = random Haar U(4) brickwork (the only one built here)\n"
+              << "  init_state   0 = random product state, 1 = random Haar state\n\n"
+              << "4L equilibration + 24L production timesteps.  Environment: MIPT_FREE_ENERGY_SEED,\n"
+              << "MIPT_FREE_ENERGY_HALF_ENTROPY=1, MIPT_FREE_ENERGY_ENTROPY_MAX_T (default 2L),\n"
+              << "MIPT_FREE_ENERGY_ENTROPY_STRIDE=1, MIPT_FREE_ENERGY_RESUME (0: overwrite existing CSVs;\n"
+              << "resuming is not built here, so existing CSVs are otherwise refused).\n";
+}
+
+int run_free_energy(int argc, char **argv)
+{
+    if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0))
+    {
+        fe_usage(argv[0]);
+        return 0;
+    }
+    if (argc != 6)
+    {
+        fe_usage(argv[0]);
+        return 2;
+    }
+    const int n = std::stoi(argv[1]);
+    const double p = std::stod(argv[2]);
+    const long parsed = std::stol(argv[3]);
+    if (parsed < 1 || parsed > 1000000000L) throw std::invalid_argument("realizations must lie in [1,1000000000].");
+    const int realizations = static_cast<int>(parsed);
+    const int circ = std::stoi(argv[4]);
+    const int init_state = std::stoi(argv[5]);
+    if (circ != 1)
+        throw std::invalid_argument("circ_type " + std::to_string(circ) +
+                                    " is not built on this engine; use 1 (random Haar U(4)).");
+    if (n < 2 || n >= 31) throw std::invalid_argument("N must lie in [2,30] for the dense exact-state protocol.");
+    if (n % 2 != 0) throw std::invalid_argument("Random Haar Unitary requires an even N for periodic brickwork geometry.");
+    if (n > 26) throw std::invalid_argument("this engine holds states up to N = 26.");
+    if (!(p >= 0.0 && p <= 1.0)) throw std::invalid_argument("p must lie in [0,1].");
+    if (init_state != 0 && init_state != 1) throw std::invalid_argument("init_state must be 0 or 1.");
+    const int eq = 4 * n, prod = 24 * n, total = eq + prod;
+    const bool record = env_flag("MIPT_FREE_ENERGY_HALF_ENTROPY", true);
+    const int max_t = record ? env_int("MIPT_FREE_ENERGY_ENTROPY_MAX_T", 2 * n, 0, total) : -1;
+    const int stride = env_int("MIPT_FREE_ENERGY_ENTROPY_STRIDE", 1, 1, total);
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_FREE_ENERGY_SEED"); e && *e)
+    {
+        char *end = nullptr;
+        const long long v = std::strtoll(e, &end, 10);
+        if (*end != '\0' || v < -1) throw std::invalid_argument("MIPT_FREE_ENERGY_SEED must be an integer >= -1.");
+        master = v >= 0 ? static_cast<std::uint64_t>(v)
+                        : (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    }
+    else
+        master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    const std::string prefix = "free_energy_haar_n_" + std::to_string(n) + "_p_" + number_for_filename(p) + "_init_" +
+                               std::to_string(init_state) + "_reals_" +
+                               compact_count(static_cast<std::uint64_t>(realizations));
+    const std::string samples_path = prefix + "_samples.csv", series_path = prefix + "_timeseries.csv";
+    if (env_flag("MIPT_FREE_ENERGY_RESUME", true) && (file_exists(samples_path) || file_exists(series_path)))
+        throw std::runtime_error(samples_path + " or " + series_path +
+                                 " exists; resuming is not built on this engine. Move it aside, or set "
+                                 "MIPT_FREE_ENERGY_RESUME=0 to start over.");
+    std::ofstream samples(samples_path, std::ios::out | std::ios::trunc);
+    if (!samples) throw std::runtime_error("Could not open sample CSV: " + samples_path);
+    samples << std::setprecision(17);
+    samples << "N,p,circ_type,circuit_name,init_state,trajectory,parity_sector,trajectory_seed,equil_timesteps,"
+               "production_timesteps,total_timesteps,measurements_total,measurements_production,F_total,"
+               "F_production,production_rate_endpoint,production_slope,free_energy_density_tilde\n";
+    std::vector<Welford> cum(total + 1), rate(total + 1), shalf(total + 1);
+    std::vector<int> rec_t;
+    for (int t = 1; t <= total; ++t)
+        if (fe_records_entropy(t, record, max_t, stride)) rec_t.push_back(t);
+
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const int per_batch = static_cast<int>(
+        std::max<double>(1.0, std::min<double>(realizations, budget_gib * 1073741824.0 / std::ldexp(8.0, n))));
+    std::vector<std::complex<double>> psi;
+    for (int first = 0; first < realizations; first += per_batch)
+    {
+        const int count = std::min(per_batch, realizations - first);
+        std::vector<mipt_hip_traj_spec> specs(count);
+        std::vector<std::uint64_t> tseed(count);
+        for (int i = 0; i < count; ++i)
+        {
+            tseed[i] = fe_splitmix64(master ^ static_cast<std::uint64_t>(first + i));
+            specs[i] = mipt_hip_traj_spec{};
+            specs[i].L = n;
+            specs[i].layers = total;
+            specs[i].p = p;
+            specs[i].seed = tseed[i];
+            specs[i].closed = 1;
+            specs[i].gate_family = 1;
+            specs[i].probe_site = -1;
+            specs[i].shared_gates = -1;
+        }
+        mipt_hip_batch *b = nullptr;
+        const mipt_hip_batch_config cfg{0, 0, 0};
+        check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
+        std::vector<double> sh(count * (rec_t.size() + 1), 0.0);
+        int st = mipt_hip_batch_setup(b, specs.data(), count);
+        if (!st) st = mipt_hip_batch_generate(b);
+        std::vector<float> ri;
+        for (int i = 0; i < count && st == 0; ++i)
+        {
+            fe_initial_state(n, init_state, tseed[i], psi);
+            ri.resize(2 * psi.size());
+            for (std::size_t k = 0; k < psi.size(); ++k)
+            {
+                ri[2 * k] = static_cast<float>(psi[k].real());
+                ri[2 * k + 1] = static_cast<float>(psi[k].imag());
+            }
+            st = mipt_hip_batch_set_state(b, i, ri.data());
+        }
+        if (!st && record) st = mipt_hip_batch_observables_all(b, 0, 0, sh.data(), nullptr);
+        int cur = 0;
+        for (std::size_t r = 0; r < rec_t.size() && st == 0; ++r)
+        {
+            st = mipt_hip_batch_evolve_range(b, cur, rec_t[r]);
+            cur = rec_t[r];
+            if (!st) st = mipt_hip_batch_synchronize(b);
+            if (!st) st = mipt_hip_batch_observables_all(b, 0, 0, sh.data() + (r + 1) * count, nullptr);
+        }
+        if (!st && cur < total) st = mipt_hip_batch_evolve_range(b, cur, total);
+        if (!st) st = mipt_hip_batch_synchronize(b);
+        std::vector<int> out(static_cast<std::size_t>(total) * n);
+        std::vector<std::uint8_t> flg(out.size());
+        std::vector<double> p1(out.size());
+        for (int i = 0; i < count && st == 0; ++i)
+        {
+            st = mipt_hip_batch_outcomes(b, i, out.data(), flg.data(), p1.data(), nullptr);
+            if (st) break;
+            double f = 0.0, fprod = 0.0;
+            std::uint64_t meas = 0, meas_prod = 0;
+            double sx = 0, sy = 0, sxx = 0, sxy = 0;
+            std::uint64_t nfit = 0;
+            cum[0].add(0.0);
+            if (record) shalf[0].add(sh[i]);
+            std::size_t r = 0;
+            for (int t = 1; t <= total; ++t)
+            {
+                double layer = 0.0;
+                int m = 0;
+                for (int s2 = 0; s2 < n; ++s2)
+                {
+                    const std::size_t k = static_cast<std::size_t>(t - 1) * n + s2;
+                    if (out[k] < 0) continue;
+                    ++m;
+                    layer -= std::log(out[k] ? p1[k] : 1.0 - p1[k]);
+                }
+                f += layer;
+                meas += m;
+                cum[t].add(f);
+                rate[t].add(f / (static_cast<double>(t) * n));
+                if (r < rec_t.size() && rec_t[r] == t)
+                {
+                    shalf[t].add(sh[(r + 1) * count + i]);
+                    ++r;
+                }
+                if (t > eq)
+                {
+                    const double x = t - eq;
+                    fprod += layer;
+                    meas_prod += m;
+                    ++nfit;
+                    sx += x, sy += fprod, sxx += x * x, sxy += x * fprod;
+                }
+            }
+            const double den = static_cast<double>(nfit) * sxx - sx * sx;
+            if (nfit < 2 || !(den > 0.0)) throw std::runtime_error("Not enough production points for a free-energy slope.");
+            const double slope = (static_cast<double>(nfit) * sxy - sx * sy) / den;
+            samples << n << ',' << p << ',' << circ << ",\"Random Haar Unitary\"," << init_state << ',' << first + i
+                    << ",-1," << tseed[i] << ',' << eq << ',' << prod << ',' << total << ',' << meas << ','
+                    << meas_prod << ',' << f << ',' << fprod << ',' << fprod / prod << ',' << slope << ','
+                    << slope / n << '\n';
+        }
+        (void)mipt_hip_batch_destroy(b);
+        check(st, "free-energy batch");
+        samples.flush();
+        std::cerr << "realizations: " << first + count << '/' << realizations << '\n';
+    }
+    if (!samples) throw std::runtime_error("Failed while writing sample CSV: " + samples_path);
+    std::ofstream series(series_path, std::ios::out | std::ios::trunc);
+    if (!series) throw std::runtime_error("Could not open time-series CSV: " + series_path);
+    series << std::setprecision(17);
+    series << "N,p,circ_type,circuit_name,init_state,requested_realizations,completed_realizations,t,t_over_L,phase,"
+              "F_mean,F_stddev,F_stderr,F_over_tL_mean,F_over_tL_stddev,F_over_tL_stderr,S_half_mean,S_half_stddev,"
+              "S_half_stderr,S_half_count\n";
+    for (int t = 0; t <= total; ++t)
+    {
+        const char *phase = t == 0 ? "initial" : (t <= eq ? "equilibration" : "production");
+        series << n << ',' << p << ',' << circ << ",\"Random Haar Unitary\"," << init_state << ',' << realizations
+               << ',' << cum[0].count << ',' << t << ',' << static_cast<double>(t) / n << ',' << phase << ',';
+        fe_write_stat(series, cum[t]);
+        series << ',';
+        fe_write_stat(series, rate[t]);
+        series << ',';
+        fe_write_stat(series, shalf[t]);
+        series << ',' << shalf[t].count << '\n';
+    }
+    series.flush();
+    if (!series) throw std::runtime_error("Failed while writing time-series CSV: " + series_path);
+    std::cout << "Saved " << samples_path << " and " << series_path << ".\n";
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -1079,6 +1370,36 @@ int mipt_hip_mipt_main(int argc, char **argv)
     catch (const std::exception &e)
     {
         std::cerr << "\nerror: " << e.what() << '\n';
+        return 1;
+    }
+}
+
+int mipt_hip_free_energy_initial_state(int n, int init_state, uint64_t trajectory_seed, double *psi_ri)
+{
+    if (n < 1 || n > 26 || (init_state != 0 && init_state != 1) || !psi_ri)
+    {
+        mipt_hip::set_error("free_energy_initial_state: bad arguments");
+        return mipt_hip::E_ARG;
+    }
+    std::vector<std::complex<double>> psi;
+    fe_initial_state(n, init_state, trajectory_seed, psi);
+    for (std::size_t k = 0; k < psi.size(); ++k)
+    {
+        psi_ri[2 * k] = psi[k].real();
+        psi_ri[2 * k + 1] = psi[k].imag();
+    }
+    return 0;
+}
+
+int mipt_hip_free_energy_main(int argc, char **argv)
+{
+    try
+    {
+        return run_free_energy(argc, argv);
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "free_energy: " << e.what() << '\n';
         return 1;
     }
 }
