@@ -176,7 +176,12 @@ __device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int 
 
 __device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, double *red, int &status)
 {
-    for (int j = 0; j < d.probes; ++j) attach_one(v, dim, d, d.psite[j], d.L + j, red, status);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)  // constant indices keep psite in registers
+    {
+        if (j >= d.probes) break;
+        attach_one(v, dim, d, d.psite[j], d.L + j, red, status);
+    }
 }
 
 __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const int *__restrict__ traj_list, int t)
@@ -196,9 +201,12 @@ __device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
     {
         // product of Bell pairs (site_j, L + j): amplitude 2^(-probes/2) where
         // every pair's two bits agree and all other bits are 0
+        // fully unrolled: constant indices keep psite in registers
         std::uint64_t rest = i;
-        for (int j = 0; j < d.probes; ++j)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
         {
+            if (j >= d.probes) break;
             const std::uint64_t s = std::uint64_t{1} << d.psite[j], r = std::uint64_t{1} << (d.L + j);
             if (((i & s) != 0) != ((i & r) != 0)) return make_float2(0.f, 0.f);
             rest &= ~(s | r);
