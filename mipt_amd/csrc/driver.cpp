@@ -768,6 +768,8 @@ void probed_usage(const char *prog)
               << "           (one reference: elapsed after the 2N-layer encoder, MIPT_PROBED_T_ENCODE;\n"
               << "            two/four: absolute, t_min >= t0 = 2N)\n"
               << "mode 1     [p_min] [p_max] [p_res]: readout at t = 4N\n"
+              << "mode 2     [t_min] [t_max] [t_res] [p_min] [p_max] [p_res]: the (p, t) grid of S_Q, references\n"
+              << "           Bell-paired at t = 0\n"
               << "Reference sites: one at N-MIPT_PROBED_DISTANCE (mode 0, default 1) or N/2 (mode 1);\n"
               << "two at (N-1)/2, N-1; four at N/4-1, N/2-1, 3N/4-1, N-1.\n"
               << "Output: the generated probed_*.csv name, or MIPT_PROBED_INTERNAL_OUTPUT; seeds from MIPT_SEED.\n";
@@ -796,14 +798,37 @@ int run_probed(int argc, char **argv)
     if (n + probes > 27) throw std::invalid_argument("this engine holds states up to N + probes = 27 qubits.");
     if (mode < 0 || mode > 5) throw std::invalid_argument("probe_mode must be 0, 1, 2, 3, 4, or 5.");
     if (probes == 3) throw std::invalid_argument("Three probes are currently supported only in mode 4.");
-    if (mode > 1) throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0 or 1.");
-    if (argc != 9)
+    if (mode > 2)
+        throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0, 1 or 2.");
+    if (mode < 2 && argc != 9)
         throw std::invalid_argument(mode == 0 ? "Mode 0 requires [p] [t_min] [t_max]." : "Mode 1 requires [p_min] [p_max] [p_res].");
+    if (mode == 2 && argc != 12)
+        throw std::invalid_argument("Mode 2 requires [t_min] [t_max] [t_res] [p_min] [p_max] [p_res].");
     std::vector<double> ps;
     std::vector<int> times;
     int t_encode = 0, origin = 0;
-    const int t0 = probes == 1 ? 0 : 2 * n;
-    if (mode == 0)
+    // mode 2 and one reference: Bell pairs at t = 0; two/four in modes 0/1: attach at t0 = 2N
+    const int t0 = (probes == 1 || mode == 2) ? 0 : 2 * n;
+    if (mode == 2)
+    {
+        const int t_min = std::stoi(argv[6]), t_max = std::stoi(argv[7]), t_res = std::stoi(argv[8]);
+        const double p_min = std::stod(argv[9]), p_max = std::stod(argv[10]);
+        const int p_res = std::stoi(argv[11]);
+        if (t_min < 0 || t_max < t_min || p_min < 0.0 || p_max > 1.0)
+            throw std::invalid_argument("Require nonnegative ordered times and p endpoints in [0,1].");
+        if (t_res <= 0 || p_res <= 0) throw std::invalid_argument("Grid resolution must be positive.");
+        if (p_min > p_max) throw std::invalid_argument("Grid minimum must not exceed its maximum.");
+        // integer times: rounded linspace, no duplicates (util/grid.hpp:45-64)
+        for (double v : linspace(t_min, t_max, t_res))
+        {
+            const int r = static_cast<int>(std::llround(v));
+            if (times.empty() || times.back() != r) times.push_back(r);
+        }
+        if (static_cast<int>(times.size()) != t_res)
+            throw std::invalid_argument("t_res requests duplicate integer times; require t_res <= t_max-t_min+1.");
+        ps = linspace(p_min, p_max, p_res);
+    }
+    else if (mode == 0)
     {
         const double p = std::stod(argv[6]);
         const int t_min = std::stoi(argv[7]), t_max = std::stoi(argv[8]);
@@ -836,7 +861,7 @@ int run_probed(int argc, char **argv)
         sites = {n / 4 - 1, n / 2 - 1, 3 * n / 4 - 1, n - 1};
     }
     const int depth = origin + times.back();
-    if (probes > 1 && depth <= t0) throw std::invalid_argument("the readout must come after the attach at t0=2N.");
+    if (t0 > 0 && depth <= t0) throw std::invalid_argument("the readout must come after the attach at t0=2N.");
 
     std::string output;
     if (const char *e = std::getenv("MIPT_PROBED_INTERNAL_OUTPUT"); e && *e) output = e;
@@ -847,7 +872,9 @@ int run_probed(int argc, char **argv)
         output += ps.size() == 1 ? "_p_" + number_for_filename(ps.front())
                                  : "_p_" + number_for_filename(ps.front()) + "_" + number_for_filename(ps.back()) +
                                        "_res_" + std::to_string(ps.size());
-        output += "_t_" + std::to_string(times.front()) + "_" + std::to_string(times.back()) + ".csv";
+        output += "_t_" + std::to_string(times.front()) + "_" + std::to_string(times.back());
+        if (mode == 2) output += "_res_" + std::to_string(times.size());
+        output += ".csv";
     }
     std::uint64_t master;
     if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
@@ -881,7 +908,7 @@ int run_probed(int argc, char **argv)
                 s.grid_x = 0;
                 s.probe_count = probes;
                 for (int j = 0; j < probes; ++j) s.probe_sites[j] = sites[j];
-                s.attach_layer = probes == 1 ? 0 : t0;
+                s.attach_layer = t0;
             }
             mipt_hip_batch *b = nullptr;
             const mipt_hip_batch_config cfg{0, 0, 0};
@@ -894,12 +921,13 @@ int run_probed(int argc, char **argv)
             {
                 const int target = origin + times[ti];
                 ProbeStats &acc = stats[pi * times.size() + ti];
-                if (probes == 1 && mode == 0 && target == 0)
+                if (target == 0)
                 {
-                    // before any layer a single reference is maximally entangled
+                    // before any layer every reference is maximally entangled
+                    // with its site (Bell pairs at t = 0): S = k ln 2, no information
                     for (int i = 0; i < count; ++i)
                     {
-                        acc.sq.add(std::log(2.0));
+                        acc.sq.add(probes * std::log(2.0));
                         acc.i2.add(0.0), acc.i3.add(0.0), acc.i4.add(0.0), acc.neg.add(0.0), acc.logneg.add(0.0);
                     }
                     continue;
@@ -942,7 +970,8 @@ int run_probed(int argc, char **argv)
     if (!csv) throw std::runtime_error("Could not open output CSV: " + output);
     csv << std::setprecision(17);
     const char *meta = "N,circ_type,circuit_name,realizations,p";
-    if (mode == 0 && probes == 1) csv << meta << ",t,S_mean,S_stderr\n";
+    if (mode == 2) csv << meta << ",t,S_Q_mean,S_Q_stderr\n";
+    else if (mode == 0 && probes == 1) csv << meta << ",t,S_mean,S_stderr\n";
     else if (mode == 0 && probes == 2)
         csv << meta << ",t,t_minus_t0,scaled_time,I_mean,I_stderr,negativity_mean,negativity_stderr,"
                        "log_negativity_mean,log_negativity_stderr\n";
@@ -961,7 +990,7 @@ int run_probed(int argc, char **argv)
             const ProbeStats &a = stats[pi * times.size() + ti];
             const int t = times[ti];
             csv << n << ',' << circ << ",\"Random Haar Unitary\"," << realizations << ',' << ps[pi] << ',' << t;
-            if (mode == 0 && probes == 1) csv << ',' << a.sq.mean << ',' << a.sq.se();
+            if (mode == 2 || (mode == 0 && probes == 1)) csv << ',' << a.sq.mean << ',' << a.sq.se();
             else if (mode == 0)
             {
                 csv << ',' << t - t0 << ',' << static_cast<double>(t - t0) / n << ',' << a.i2.mean << ',' << a.i2.se();

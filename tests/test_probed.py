@@ -1,4 +1,4 @@
-"""mipt_probed modes 0 and 1 (one, two or four references).
+"""mipt_probed modes 0, 1 and 2 (one, two or four references).
 
 CPU: the argument contract of cpp/include/mipt/probed/config.hpp:261-372
 (every case fails before any device work).  GPU: the CSV the driver writes
@@ -32,7 +32,9 @@ def M():
     (["1", "7", "4", "1", "1", "0", "0.3", "2"], 1),       # odd ring
     (["4", "10", "4", "1", "1", "0", "0.3", "2"], 1),      # four references need N % 4 == 0
     (["2", "8", "4", "1", "0", "0.1", "4", "20"], 1),      # t_min < t0 = 2N
-    (["1", "8", "4", "1", "2", "0", "1", "2", "0", "1", "2"], 1),  # mode 2 not built
+    (["1", "8", "4", "1", "3", "0", "1", "2", "0", "1", "2"], 1),  # mode 3 not built
+    (["1", "8", "4", "1", "2", "0", "4", "2", "0", "1"], 1),        # mode 2 needs six values
+    (["1", "8", "4", "1", "2", "0", "2", "5", "0.1", "0.2", "2"], 1),  # duplicate integer times
     (["1", "8", "4", "1", "0", "0.1", "4"], 1),            # mode 0 needs three values
     (["1", "8", "4", "1", "1", "0.3", "0.1", "3"], 1),     # p_min > p_max
     (["1", "8", "4", "1", "1", "0.1", "0.3", "0"], 1),     # p_res <= 0
@@ -132,3 +134,26 @@ def test_probed_mode0_four_references(M, tmp_path, monkeypatch):
             m, se = mean_se([w[key] for w in want])
             assert abs(float(row[col + "_mean"]) - m) < 1e-4, (t, col)
             assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (t, col)
+
+
+@pytest.mark.gpu
+def test_probed_mode2_grid(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9004")
+    n, R = 8, 4
+    print("mipt_probed mode 2, two references", flush=True)
+    assert M.probed_main([2, n, R, 1, 2, 0, 6, 4, 0.1, 0.2, 2]) == 0
+    head, rows = read_csv(tmp_path / f"probed_2_haar_n_{n}_reals_{R}_mode_2_p_0.1_0.2_res_2_t_0_6_res_4.csv")
+    assert head[5:] == ["t", "S_Q_mean", "S_Q_stderr"]
+    assert [(float(r["p"]), int(r["t"])) for r in rows] == [(p, t) for p in (0.1, 0.2) for t in (0, 2, 4, 6)]
+    for k, row in enumerate(rows):
+        pi, t = divmod(k, 4)
+        t *= 2
+        p = [0.1, 0.2][pi]
+        if t == 0:  # Bell pairs with their sites: S_Q = 2 ln 2
+            assert abs(float(row["S_Q_mean"]) - 2 * math.log(2)) < 1e-12
+            continue
+        seeds = [O.lib().orc_trajectory_seed(9004, pi * R + i) for i in range(R)]
+        want = oracle_samples(n, t, p, seeds, [3, 7], 0, 0)
+        m, se = mean_se([w["sq"] for w in want])
+        assert abs(float(row["S_Q_mean"]) - m) < 1e-4 and abs(float(row["S_Q_stderr"]) - se) < 1e-4
