@@ -346,7 +346,7 @@ int mipt_hip_mipt_main(int argc, char **argv);
  * block size with Welford statistics over realizations, the CSV schema of
  * cpp/include/mipt/entropy_resume.hpp:52-56; returns the exit code. */
 int mipt_hip_entropy_main(int argc, char **argv);
-/* mipt_probed.exe, modes 0, 1 and 2 (cpp/apps/mipt_probed.cpp; cpp/include/mipt/probed/config.hpp:261-390,
+/* mipt_probed.exe, modes 0, 1, 2 and two-reference 4 (cpp/apps/mipt_probed.cpp; probed/config.hpp:261-470,
  * runner.hpp:258-307, output.hpp:36-95, 386-476): [probes] [N] [realizations] [circ_type] [mode] [args] with
  * probes 1, 2 or 4, circ_type 1; the same reference sites, encoder schedule, readout times, output name and CSV
  * columns.  Returns 0, or 1 on an error (message on stderr). */

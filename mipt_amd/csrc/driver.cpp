@@ -775,6 +775,141 @@ void probed_usage(const char *prog)
               << "Output: the generated probed_*.csv name, or MIPT_PROBED_INTERNAL_OUTPUT; seeds from MIPT_SEED.\n";
 }
 
+// Mode 4 with two references (probed/config.hpp:430-470, runner.hpp:94-148,
+// 318-344, output.hpp:188-214): for every separation d = 1..N/2,
+// `realizations` trajectories with the references at (x0, x0 + d mod N)
+// from a uniform origin x0, t_eq = 2N monitored layers (MIPT_PROBED_T_EQ),
+// both attached at t_eq, read out at tau = 0..tau_max.  The origin of
+// trajectory i is splitmix64(seed_i ^ "ORIGIN") mod N, reproducible from the
+// master seed (the reference draws it from a nondeterministic stream).
+int run_probed_mode4(int argc, char **argv, int probes, int n, int realizations, int circ)
+{
+    if (probes != 2 && probes != 3) throw std::invalid_argument("Mode 4 requires probes=2 or probes=3.");
+    if (probes == 3)
+        throw std::invalid_argument("three-reference mode 4 (GMN semidefinite programs) is not built on this engine.");
+    if (n < probes) throw std::invalid_argument("Mode 4 requires N >= probes.");
+    if (argc != 8) throw std::invalid_argument("Mode 4 requires [p] [tau_max], with optional [B_min] for probes=3.");
+    const double p = std::stod(argv[6]);
+    const int tau_max = std::stoi(argv[7]);
+    if (p < 0.0 || p > 1.0 || tau_max < 0 || tau_max > (1 << 20))
+        throw std::invalid_argument("Mode 4 requires p in [0,1] and a nonnegative tau_max "
+                                    "that leaves room for the default t_eq=2N.");
+    const int t_eq = env_int("MIPT_PROBED_T_EQ", 2 * n, 0, 1L << 20);
+    const int nd = n / 2;
+    std::string output;
+    if (const char *e = std::getenv("MIPT_PROBED_INTERNAL_OUTPUT"); e && *e) output = e;
+    else
+        output = "probed_2_haar_n_" + std::to_string(n) + "_reals_" +
+                 compact_count(static_cast<std::uint64_t>(realizations)) + "_mode_4_p_" + number_for_filename(p) +
+                 "_teq_" + std::to_string(t_eq) + "_tau_0_" + std::to_string(tau_max) + ".csv";
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
+    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    struct PairStats
+    {
+        Welford i2, neg, logneg;
+        std::uint64_t positive = 0;
+    };
+    std::vector<PairStats> stats(static_cast<std::size_t>(nd) * (tau_max + 1));
+    const std::uint64_t total = static_cast<std::uint64_t>(nd) * realizations;
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const std::uint64_t per_batch = static_cast<std::uint64_t>(
+        std::max<double>(1.0, std::min<double>(static_cast<double>(total), budget_gib * 1073741824.0 / std::ldexp(8.0, n + 2))));
+    const int depth = t_eq + tau_max;
+    std::vector<double> rho;
+    for (std::uint64_t first = 0; first < total; first += per_batch)
+    {
+        const int count = static_cast<int>(std::min<std::uint64_t>(per_batch, total - first));
+        std::vector<mipt_hip_traj_spec> specs(count);
+        std::vector<int> dist(count);
+        for (int i = 0; i < count; ++i)
+        {
+            const std::uint64_t idx = first + i;
+            dist[i] = static_cast<int>(idx / realizations) + 1;
+            mipt_hip_traj_spec &s = specs[i];
+            s = mipt_hip_traj_spec{};
+            s.L = n;
+            s.layers = depth + 1;  // the attach layer t_eq lies inside the drawn range even for tau_max = 0
+            s.p = p;
+            s.seed = trajectory_seed(master, idx);
+            s.closed = 1;
+            s.gate_family = 1;
+            s.probe_site = -1;
+            s.shared_gates = -1;
+            s.probe_count = 2;
+            const int x0 = static_cast<int>(splitmix64(s.seed ^ 0x4f524947494eull) % static_cast<std::uint64_t>(n));
+            s.probe_sites[0] = x0;
+            s.probe_sites[1] = (x0 + dist[i]) % n;
+            s.attach_layer = t_eq;
+        }
+        mipt_hip_batch *b = nullptr;
+        const mipt_hip_batch_config cfg{0, 0, 0};
+        check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
+        int st = mipt_hip_batch_setup(b, specs.data(), count);
+        if (!st) st = mipt_hip_batch_generate(b);
+        rho.resize(static_cast<std::size_t>(count) * 32);
+        int cur = 0;
+        for (int tau = 0; tau <= tau_max && st == 0; ++tau)
+        {
+            const int target = t_eq + tau;
+            if (tau == 0)
+            {
+                // right after the attach the references are maximally mixed and uncorrelated
+                for (int i = 0; i < count; ++i)
+                {
+                    PairStats &a = stats[static_cast<std::size_t>(dist[i] - 1) * (tau_max + 1)];
+                    a.i2.add(0.0), a.neg.add(0.0), a.logneg.add(0.0);
+                }
+                continue;
+            }
+            if (target > cur)
+            {
+                st = mipt_hip_batch_evolve_range(b, cur, target);
+                cur = target;
+            }
+            if (!st) st = mipt_hip_batch_probe_rdm(b, rho.data());
+            for (int i = 0; i < count && st == 0; ++i)
+            {
+                double v[11];
+                st = mipt_hip_probe_information(rho.data() + static_cast<std::size_t>(i) * 32, 2, v);
+                if (st) break;
+                PairStats &a = stats[static_cast<std::size_t>(dist[i] - 1) * (tau_max + 1) + tau];
+                a.i2.add(v[1]), a.neg.add(v[4]), a.logneg.add(v[5]);
+                a.positive += v[4] > 1.0e-12;
+            }
+        }
+        (void)mipt_hip_batch_destroy(b);
+        check(st, "probed mode 4 batch");
+        std::cerr << first + count << '/' << total << " trajectories\n";
+    }
+    mkdirs_for(output);
+    std::ofstream csv(output, std::ios::out | std::ios::trunc);
+    if (!csv) throw std::runtime_error("Could not open output CSV: " + output);
+    csv << std::setprecision(17);
+    csv << "N,circ_type,circuit_name,realizations,p,distance,chord_length,t_eq,tau,tau_over_chord,t_absolute,I_mean,"
+           "I_stderr,negativity_mean,negativity_stderr,log_negativity_mean,log_negativity_stderr,"
+           "negativity_positive_count,negativity_positive_fraction,origins_randomized\n";
+    const double pi = std::acos(-1.0);
+    for (int d = 1; d <= nd; ++d)
+    {
+        const double chord = static_cast<double>(n) / pi * std::sin(pi * d / static_cast<double>(n));
+        for (int tau = 0; tau <= tau_max; ++tau)
+        {
+            const PairStats &a = stats[static_cast<std::size_t>(d - 1) * (tau_max + 1) + tau];
+            csv << n << ',' << circ << ",\"Random Haar Unitary\"," << realizations << ',' << p << ',' << d << ','
+                << chord << ',' << t_eq << ',' << tau << ',' << tau / chord << ',' << t_eq + tau << ',' << a.i2.mean
+                << ',' << a.i2.se() << ',' << a.neg.mean << ',' << a.neg.se() << ',' << a.logneg.mean << ','
+                << a.logneg.se() << ',' << a.positive << ','
+                << static_cast<double>(a.positive) / static_cast<double>(a.neg.count) << ",1\n";
+        }
+    }
+    csv.flush();
+    if (!csv) throw std::runtime_error("Failed while writing output CSV: " + output);
+    std::cout << output << '\n';
+    return 0;
+}
+
 int run_probed(int argc, char **argv)
 {
     if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0))
@@ -797,9 +932,10 @@ int run_probed(int argc, char **argv)
     if (n % 2 != 0) throw std::invalid_argument("Random Haar Unitary requires an even N for periodic brickwork geometry.");
     if (n + probes > 27) throw std::invalid_argument("this engine holds states up to N + probes = 27 qubits.");
     if (mode < 0 || mode > 5) throw std::invalid_argument("probe_mode must be 0, 1, 2, 3, 4, or 5.");
-    if (probes == 3) throw std::invalid_argument("Three probes are currently supported only in mode 4.");
+    if (probes == 3 && mode != 4) throw std::invalid_argument("Three probes are currently supported only in mode 4.");
+    if (mode == 4) return run_probed_mode4(argc, argv, probes, n, realizations, circ);
     if (mode > 2)
-        throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0, 1 or 2.");
+        throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0, 1, 2 or 4.");
     if (mode < 2 && argc != 9)
         throw std::invalid_argument(mode == 0 ? "Mode 0 requires [p] [t_min] [t_max]." : "Mode 1 requires [p_min] [p_max] [p_res].");
     if (mode == 2 && argc != 12)
@@ -861,7 +997,6 @@ int run_probed(int argc, char **argv)
         sites = {n / 4 - 1, n / 2 - 1, 3 * n / 4 - 1, n - 1};
     }
     const int depth = origin + times.back();
-    if (t0 > 0 && depth <= t0) throw std::invalid_argument("the readout must come after the attach at t0=2N.");
 
     std::string output;
     if (const char *e = std::getenv("MIPT_PROBED_INTERNAL_OUTPUT"); e && *e) output = e;
@@ -897,7 +1032,7 @@ int run_probed(int argc, char **argv)
                 mipt_hip_traj_spec &s = specs[i];
                 s = mipt_hip_traj_spec{};
                 s.L = n;
-                s.layers = depth;
+                s.layers = depth + 1;  // the attach layer t0 lies inside the drawn range
                 s.p = ps[pi];
                 s.seed = trajectory_seed(master, pi * static_cast<std::uint64_t>(realizations) + first + i);
                 s.closed = 1;

@@ -1,4 +1,4 @@
-"""mipt_probed modes 0, 1 and 2 (one, two or four references).
+"""mipt_probed modes 0, 1, 2 (one, two or four references) and 4 (two).
 
 CPU: the argument contract of cpp/include/mipt/probed/config.hpp:261-372
 (every case fails before any device work).  GPU: the CSV the driver writes
@@ -33,6 +33,9 @@ def M():
     (["4", "10", "4", "1", "1", "0", "0.3", "2"], 1),      # four references need N % 4 == 0
     (["2", "8", "4", "1", "0", "0.1", "4", "20"], 1),      # t_min < t0 = 2N
     (["1", "8", "4", "1", "3", "0", "1", "2", "0", "1", "2"], 1),  # mode 3 not built
+    (["3", "8", "4", "1", "4", "0.1", "4"], 1),            # three-reference mode 4: GMN SDPs not built
+    (["1", "8", "4", "1", "4", "0.1", "4"], 1),            # mode 4 needs two or three references
+    (["2", "8", "4", "1", "4", "0.1"], 1),                 # mode 4 needs [p] [tau_max]
     (["1", "8", "4", "1", "2", "0", "4", "2", "0", "1"], 1),        # mode 2 needs six values
     (["1", "8", "4", "1", "2", "0", "2", "5", "0.1", "0.2", "2"], 1),  # duplicate integer times
     (["1", "8", "4", "1", "0", "0.1", "4"], 1),            # mode 0 needs three values
@@ -157,3 +160,43 @@ def test_probed_mode2_grid(M, tmp_path, monkeypatch):
         want = oracle_samples(n, t, p, seeds, [3, 7], 0, 0)
         m, se = mean_se([w["sq"] for w in want])
         assert abs(float(row["S_Q_mean"]) - m) < 1e-4 and abs(float(row["S_Q_stderr"]) - se) < 1e-4
+
+
+@pytest.mark.gpu
+def test_probed_mode4_pairs(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9005")
+    n, R, p, tau_max = 8, 3, 0.15, 3
+    print("mipt_probed mode 4, two references", flush=True)
+    assert M.probed_main([2, n, R, 1, 4, p, tau_max]) == 0
+    head, rows = read_csv(tmp_path / f"probed_2_haar_n_{n}_reals_{R}_mode_4_p_0.15_teq_16_tau_0_3.csv")
+    assert head[5:] == ["distance", "chord_length", "t_eq", "tau", "tau_over_chord", "t_absolute", "I_mean",
+                        "I_stderr", "negativity_mean", "negativity_stderr", "log_negativity_mean",
+                        "log_negativity_stderr", "negativity_positive_count", "negativity_positive_fraction",
+                        "origins_randomized"]
+    assert len(rows) == (n // 2) * (tau_max + 1)
+    M64 = 2**64 - 1
+
+    def splitmix(v):
+        v = (v + 0x9E3779B97F4A7C15) & M64
+        v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & M64
+        return v ^ (v >> 31)
+
+    for row in rows:
+        d, tau = int(row["distance"]), int(row["tau"])
+        chord = n / math.pi * math.sin(math.pi * d / n)
+        assert abs(float(row["chord_length"]) - chord) < 1e-12 and int(row["t_absolute"]) == 16 + tau
+        if tau == 0:
+            assert float(row["I_mean"]) == 0.0 and int(row["negativity_positive_count"]) == 0
+            continue
+        want = []
+        for i in range(R):
+            seed = O.lib().orc_trajectory_seed(9005, (d - 1) * R + i)
+            x0 = splitmix(seed ^ 0x4F524947494E) % n
+            want += oracle_samples(n, 16 + tau, p, [seed], [x0, (x0 + d) % n], 0, 16)
+        for col, key in (("I", "i2"), ("negativity", "negativity"), ("log_negativity", "log_negativity")):
+            m, se = mean_se([w[key] for w in want])
+            assert abs(float(row[col + "_mean"]) - m) < 1e-4, (d, tau, col)
+            assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (d, tau, col)
+        assert int(row["negativity_positive_count"]) == sum(w["negativity"] > 1e-12 for w in want)
