@@ -154,6 +154,9 @@ typedef struct mipt_hip_traj_spec
     int attach_layer;   /* with probe_count >= 1: the layer the references attach at (-1: p_zero_layers);
                            independent of the unmeasured prefix, e.g. two references attached at t0 = 2N after
                            monitored layers (cpp/include/mipt/probed/runner.hpp:258-307) */
+    int attach_stagger; /* with probe_count >= 2: reference j attaches at attach_layer + j * attach_stagger
+                           (0: all together); a site may then serve several references, as the staggered
+                           insertion of cpp/include/mipt/probed/runner.hpp:230-256 does */
 } mipt_hip_traj_spec;
 
 typedef struct mipt_hip_batch_config
@@ -346,7 +349,7 @@ int mipt_hip_mipt_main(int argc, char **argv);
  * block size with Welford statistics over realizations, the CSV schema of
  * cpp/include/mipt/entropy_resume.hpp:52-56; returns the exit code. */
 int mipt_hip_entropy_main(int argc, char **argv);
-/* mipt_probed.exe, modes 0, 1, 2 and two-reference 4 (cpp/apps/mipt_probed.cpp; probed/config.hpp:261-470,
+/* mipt_probed.exe, modes 0 to 3 and two-reference 4 (cpp/apps/mipt_probed.cpp; probed/config.hpp:261-470,
  * runner.hpp:258-307, output.hpp:36-95, 386-476): [probes] [N] [realizations] [circ_type] [mode] [args] with
  * probes 1, 2 or 4, circ_type 1; the same reference sites, encoder schedule, readout times, output name and CSV
  * columns.  Returns 0, or 1 on an error (message on stderr). */

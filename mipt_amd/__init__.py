@@ -331,7 +331,8 @@ class TrajSpec(ctypes.Structure):
     _fields_ = [("L", ctypes.c_int), ("layers", ctypes.c_int), ("p", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("closed", ctypes.c_int), ("gate_family", ctypes.c_int), ("probe_site", ctypes.c_int),
                 ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int), ("grid_x", ctypes.c_int),
-                ("probe_count", ctypes.c_int), ("probe_sites", ctypes.c_int * 4), ("attach_layer", ctypes.c_int)]
+                ("probe_count", ctypes.c_int), ("probe_sites", ctypes.c_int * 4), ("attach_layer", ctypes.c_int),
+                ("attach_stagger", ctypes.c_int)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -351,12 +352,13 @@ class Spec:
     grid_x: int = 0
     probe_sites: tuple = ()  # several references (qubits L, L+1, ...); probe_site stays -1
     attach_layer: int = -1   # with probe_sites: the attach layer (-1: p_zero_layers)
+    attach_stagger: int = 0  # reference j attaches at attach_layer + j * attach_stagger
 
     def to_c(self) -> TrajSpec:
         sites = list(self.probe_sites) + [0] * (4 - len(self.probe_sites))
         return TrajSpec(self.L, self.layers, float(self.p), self.seed & (2**64 - 1), int(self.closed), 1,
                         self.probe_site, self.p_zero_layers, self.shared_gates, self.grid_x,
-                        len(self.probe_sites), (ctypes.c_int * 4)(*sites[:4]), self.attach_layer)
+                        len(self.probe_sites), (ctypes.c_int * 4)(*sites[:4]), self.attach_layer, self.attach_stagger)
 
 
 class Batch:

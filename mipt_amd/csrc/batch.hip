@@ -137,7 +137,8 @@ constexpr std::uint32_t STREAM_ATCH = 0x41544348u;  // probe-attach reset unifor
 // the counter stream at (layer, site)), renormalised by 1/sqrt(branch weight)
 // and flipped to 0; FP64 weights in a fixed reduction order.  256 threads,
 // any buffer.
-__device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int lo, int hi, double *red, int &status)
+__device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int lo, int hi, int layer, double *red,
+                           int &status)
 {
     const std::uint32_t S = 1u << lo, P = 1u << hi;
     double w = 0.0, w1 = 0.0;
@@ -153,7 +154,7 @@ __device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int 
     double p = (w > 0.0) ? w1 / w : 0.0;
     p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
     double u, unused;
-    philox_uniform_pair(d.seed, static_cast<std::uint32_t>(d.attach), static_cast<std::uint32_t>(lo), 0, STREAM_ATCH, u,
+    philox_uniform_pair(d.seed, static_cast<std::uint32_t>(layer), static_cast<std::uint32_t>(lo), 0, STREAM_ATCH, u,
                         unused);
     const int o = u < p ? 1 : 0;
     const double branch = o ? w1 : w - w1;
@@ -174,14 +175,27 @@ __device__ void attach_one(float2 *v, std::uint32_t dim, const TrajDesc &d, int 
     __syncthreads();
 }
 
-__device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, double *red, int &status)
+// Reference j attaches at layer attach + j * stagger (0: Bell pair at t = 0).
+__device__ inline int attach_layer_of(const TrajDesc &d, int j) { return d.attach + j * d.stagger; }
+
+// Attaches, in order, the references whose layer is t.
+__device__ void attach_probe(float2 *v, std::uint32_t dim, const TrajDesc &d, int t, double *red, int &status)
 {
 #pragma unroll
     for (int j = 0; j < 4; ++j)  // constant indices keep psite in registers
     {
         if (j >= d.probes) break;
-        attach_one(v, dim, d, d.psite[j], d.L + j, red, status);
+        if (attach_layer_of(d, j) == t) attach_one(v, dim, d, d.psite[j], d.L + j, t, red, status);
     }
+}
+
+__device__ inline bool attaches_at(const TrajDesc &d, int t)
+{
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < d.probes && attach_layer_of(d, j) == t) any = true;
+    return any;
 }
 
 __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const int *__restrict__ traj_list, int t)
@@ -189,9 +203,9 @@ __global__ void __launch_bounds__(SMALL_THREADS) attach_kernel(DevArena a, const
     __shared__ double red[SMALL_THREADS];
     const int traj = traj_list[blockIdx.x];
     const TrajDesc d = a.desc[traj];
-    if (d.probes == 0 || d.attach != t || t == 0) return;  // uniform over the block
+    if (t == 0 || !attaches_at(d, t)) return;  // uniform over the block
     int status = 0;
-    attach_probe(a.states + d.state_off, 1u << d.nq, d, red, status);
+    attach_probe(a.states + d.state_off, 1u << d.nq, d, t, red, status);
     if (threadIdx.x == 0 && status) a.status[traj] = status;
 }
 
@@ -199,23 +213,26 @@ __device__ inline float2 initial_amp(const TrajDesc &d, std::uint64_t i)
 {
     if (d.probes > 0 && d.attach == 0)
     {
-        // product of Bell pairs (site_j, L + j): amplitude 2^(-probes/2) where
-        // every pair's two bits agree and all other bits are 0
-        // fully unrolled: constant indices keep psite in registers
+        // Bell pairs (site_j, L + j) of the references attached at t = 0:
+        // amplitude 2^(-pairs/2) where every pair's two bits agree and all
+        // other bits (later references included) are 0
         std::uint64_t rest = i;
+        int pairs = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j)  // fully unrolled: constant indices keep psite in registers
         {
             if (j >= d.probes) break;
+            if (attach_layer_of(d, j) != 0) continue;
             const std::uint64_t s = std::uint64_t{1} << d.psite[j], r = std::uint64_t{1} << (d.L + j);
             if (((i & s) != 0) != ((i & r) != 0)) return make_float2(0.f, 0.f);
             rest &= ~(s | r);
+            ++pairs;
         }
         if (rest) return make_float2(0.f, 0.f);
-        const float h = d.probes == 1 ? 0.70710678118654752440f
-                        : d.probes == 2 ? 0.5f
-                        : d.probes == 3 ? 0.35355339059327376220f
-                                        : 0.25f;
+        const float h = pairs == 1 ? 0.70710678118654752440f
+                        : pairs == 2 ? 0.5f
+                        : pairs == 3 ? 0.35355339059327376220f
+                                     : 0.25f;
         return make_float2(h, 0.f);
     }
     return i == 0 ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
@@ -240,7 +257,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) small_evolve_kernel(
     int status = 0;
     for (int t = layer_begin; t < layer_end; ++t)
     {
-        if (d.probes > 0 && d.attach > 0 && t == d.attach) attach_probe(st, dim, d, red, status);
+        if (t > 0 && attaches_at(d, t)) attach_probe(st, dim, d, t, red, status);
         const int nb = geo_bond_count(d.L, d.gx, t, d.closed);
         const std::uint64_t gbase = d.gate_off + gate_prefix(d, t);
         for (int b = 0; b < nb; ++b)
@@ -1450,13 +1467,18 @@ bool probe_spec_ok(const mipt_hip_traj_spec &s)
 {
     if (s.probe_count < 0 || s.probe_count > 4) return false;
     if (s.probe_count == 0) return s.L + (s.probe_site >= 0 ? 1 : 0) <= 27;
-    if (s.probe_site >= 0 || s.L + s.probe_count > 27) return false;
-    if (s.attach_layer < -1 || (s.attach_layer > 0 && s.attach_layer >= s.layers)) return false;
+    if (s.probe_site >= 0 || s.L + s.probe_count > 27 || s.attach_stagger < 0) return false;
+    const int first = s.attach_layer >= 0 ? s.attach_layer : s.p_zero_layers;
+    if (s.attach_layer < -1) return false;
+    if (s.probe_count > 1 && s.attach_stagger > 0 && first + (s.probe_count - 1) * s.attach_stagger >= s.layers)
+        return false;
+    if (first > 0 && first >= s.layers) return false;
     for (int j = 0; j < s.probe_count; ++j)
     {
         if (s.probe_sites[j] < 0 || s.probe_sites[j] >= s.L) return false;
+        // one site can serve several references only at different attach layers
         for (int k = 0; k < j; ++k)
-            if (s.probe_sites[k] == s.probe_sites[j]) return false;
+            if (s.probe_sites[k] == s.probe_sites[j] && s.attach_stagger == 0) return false;
     }
     return true;
 }
@@ -1622,7 +1644,7 @@ struct Batch
                 d.psite[0] = s.probe_site;
             d.nq = s.L + d.probes;
             d.attach = (s.probe_count > 0 && s.attach_layer >= 0) ? s.attach_layer : s.p_zero_layers;
-            d.pad = 0;
+            d.stagger = s.probe_count > 0 ? s.attach_stagger : 0;
             d.layers = s.layers;
             d.closed = s.closed ? 1 : 0;
             d.gx = s.grid_x;
@@ -1864,14 +1886,16 @@ struct Batch
         al.partials = lane ? partials2 : a.partials;
         // maximum measured count per layer over the group (host flags)
         bool attach_any = false;
-        for (int i : grp) attach_any |= desc[i].probes > 0 && desc[i].attach > 0;
+        for (int i : grp)
+            attach_any |= desc[i].probes > 0 && (desc[i].attach > 0 || (desc[i].probes > 1 && desc[i].stagger > 0));
         for (int t = t0; t < t1; ++t)
         {
             const LayerPlan &LP = lp[t % P];
             if (attach_any && t > 0)
             {
                 bool at = false;
-                for (int i : grp) at |= desc[i].probes > 0 && desc[i].attach == t;
+                for (int i : grp)
+                    for (int j = 0; j < desc[i].probes; ++j) at |= desc[i].attach + j * desc[i].stagger == t;
                 if (at)
                 {
                     if (t > t0) MIPT_HIP_TRY(close_layer(cf, lp[(t - 1) % P], list, ntraj, t));

@@ -24,7 +24,7 @@ struct TrajDesc
     int probe_site, p_zero_layers;  // probe_site: site of reference 0 (-1: no references)
     int gx, probes;                 // grid columns (0: chain); references (qubits L..L+probes-1)
     int psite[4];                   // site of reference j
-    int attach, pad;                // layer the references attach at (0: Bell pairs at t = 0)
+    int attach, stagger;            // reference j attaches at layer attach + j * stagger (0: Bell pair at t = 0)
     double p;
     std::uint64_t seed;
 };

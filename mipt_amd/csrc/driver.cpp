@@ -934,8 +934,10 @@ int run_probed(int argc, char **argv)
     if (mode < 0 || mode > 5) throw std::invalid_argument("probe_mode must be 0, 1, 2, 3, 4, or 5.");
     if (probes == 3 && mode != 4) throw std::invalid_argument("Three probes are currently supported only in mode 4.");
     if (mode == 4) return run_probed_mode4(argc, argv, probes, n, realizations, circ);
-    if (mode > 2)
-        throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0, 1, 2 or 4.");
+    if (mode > 3)
+        throw std::invalid_argument("probe_mode " + std::to_string(mode) + " is not built on this engine; use 0 to 4.");
+    if (mode == 3 && probes != 2) throw std::invalid_argument("Mode 3 requires probes=2.");
+    if (mode == 3 && argc != 9) throw std::invalid_argument("Mode 3 requires [p] [delta_x] [delta_t].");
     if (mode < 2 && argc != 9)
         throw std::invalid_argument(mode == 0 ? "Mode 0 requires [p] [t_min] [t_max]." : "Mode 1 requires [p_min] [p_max] [p_res].");
     if (mode == 2 && argc != 12)
@@ -943,9 +945,30 @@ int run_probed(int argc, char **argv)
     std::vector<double> ps;
     std::vector<int> times;
     int t_encode = 0, origin = 0;
-    // mode 2 and one reference: Bell pairs at t = 0; two/four in modes 0/1: attach at t0 = 2N
-    const int t0 = (probes == 1 || mode == 2) ? 0 : 2 * n;
-    if (mode == 2)
+    // mode 2 and one reference: Bell pairs at t = 0; two/four in modes 0/1: attach at t0 = 2N;
+    // mode 3: reference 0 at tau1 = 4N, reference 1 delta_t later (t0 = tau2)
+    int delta_x = 0, delta_t = 0;
+    if (mode == 3)
+    {
+        delta_x = std::stoi(argv[7]);
+        delta_t = std::stoi(argv[8]);
+    }
+    const int t0 = mode == 3 ? 4 * n + delta_t : (probes == 1 || mode == 2) ? 0 : 2 * n;
+    const int attach0 = mode == 3 ? 4 * n : t0;
+    if (mode == 3)
+    {
+        const double p = std::stod(argv[6]);
+        if (p < 0.0 || p > 1.0 || delta_x < 0 || delta_x > n / 2 || delta_t < 0)
+            throw std::invalid_argument("Mode 3 requires p in [0,1], 0 <= delta_x <= N/2, and delta_t >= 0.");
+        if (delta_t == 0 && delta_x != n / 2)
+            throw std::invalid_argument("The mode-3 spatial branch requires even N, delta_x=N/2, delta_t=0.");
+        if (delta_t != 0 && delta_x != 0)
+            throw std::invalid_argument("The mode-3 temporal branch requires delta_x=0 and delta_t>0.");
+        ps = {p};
+        for (int tau = 0; tau <= 8 * n; ++tau) times.push_back(tau);
+        origin = t0;
+    }
+    else if (mode == 2)
     {
         const int t_min = std::stoi(argv[6]), t_max = std::stoi(argv[7]), t_res = std::stoi(argv[8]);
         const double p_min = std::stod(argv[9]), p_max = std::stod(argv[10]);
@@ -989,6 +1012,8 @@ int run_probed(int argc, char **argv)
     std::vector<int> sites;
     if (probes == 1)
         sites = {mode == 0 ? n - env_int("MIPT_PROBED_DISTANCE", 1, 1, n) : n / 2};
+    else if (mode == 3)
+        sites = {0, delta_x % n};
     else if (probes == 2)
         sites = {(n - 1) / 2, n - 1};
     else
@@ -1007,7 +1032,11 @@ int run_probed(int argc, char **argv)
         output += ps.size() == 1 ? "_p_" + number_for_filename(ps.front())
                                  : "_p_" + number_for_filename(ps.front()) + "_" + number_for_filename(ps.back()) +
                                        "_res_" + std::to_string(ps.size());
-        output += "_t_" + std::to_string(times.front()) + "_" + std::to_string(times.back());
+        if (mode == 3)
+            output += "_dx_" + std::to_string(delta_x) + "_dt_" + std::to_string(delta_t) + "_tau_" +
+                      std::to_string(times.front()) + "_" + std::to_string(times.back());
+        else
+            output += "_t_" + std::to_string(times.front()) + "_" + std::to_string(times.back());
         if (mode == 2) output += "_res_" + std::to_string(times.size());
         output += ".csv";
     }
@@ -1043,7 +1072,8 @@ int run_probed(int argc, char **argv)
                 s.grid_x = 0;
                 s.probe_count = probes;
                 for (int j = 0; j < probes; ++j) s.probe_sites[j] = sites[j];
-                s.attach_layer = t0;
+                s.attach_layer = attach0;
+                s.attach_stagger = delta_t;
             }
             mipt_hip_batch *b = nullptr;
             const mipt_hip_batch_config cfg{0, 0, 0};
@@ -1105,7 +1135,10 @@ int run_probed(int argc, char **argv)
     if (!csv) throw std::runtime_error("Could not open output CSV: " + output);
     csv << std::setprecision(17);
     const char *meta = "N,circ_type,circuit_name,realizations,p";
-    if (mode == 2) csv << meta << ",t,S_Q_mean,S_Q_stderr\n";
+    if (mode == 3)
+        csv << meta << ",delta_x,delta_t,tau1,tau2,tau,tau_over_L,t_absolute,I_mean,I_stderr,negativity_mean,"
+                       "negativity_stderr,log_negativity_mean,log_negativity_stderr\n";
+    else if (mode == 2) csv << meta << ",t,S_Q_mean,S_Q_stderr\n";
     else if (mode == 0 && probes == 1) csv << meta << ",t,S_mean,S_stderr\n";
     else if (mode == 0 && probes == 2)
         csv << meta << ",t,t_minus_t0,scaled_time,I_mean,I_stderr,negativity_mean,negativity_stderr,"
@@ -1124,8 +1157,13 @@ int run_probed(int argc, char **argv)
         {
             const ProbeStats &a = stats[pi * times.size() + ti];
             const int t = times[ti];
-            csv << n << ',' << circ << ",\"Random Haar Unitary\"," << realizations << ',' << ps[pi] << ',' << t;
-            if (mode == 2 || (mode == 0 && probes == 1)) csv << ',' << a.sq.mean << ',' << a.sq.se();
+            csv << n << ',' << circ << ",\"Random Haar Unitary\"," << realizations << ',' << ps[pi];
+            if (mode != 3) csv << ',' << t;
+            if (mode == 3)
+                csv << ',' << delta_x << ',' << delta_t << ',' << 4 * n << ',' << t0 << ',' << t << ','
+                    << static_cast<double>(t) / n << ',' << t0 + t << ',' << a.i2.mean << ',' << a.i2.se() << ','
+                    << a.neg.mean << ',' << a.neg.se() << ',' << a.logneg.mean << ',' << a.logneg.se();
+            else if (mode == 2 || (mode == 0 && probes == 1)) csv << ',' << a.sq.mean << ',' << a.sq.se();
             else if (mode == 0)
             {
                 csv << ',' << t - t0 << ',' << static_cast<double>(t - t0) / n << ',' << a.i2.mean << ',' << a.i2.se();

@@ -390,13 +390,15 @@ def attach_reference(psi: np.ndarray, nq: int, site: int, ref: int, u: float) ->
 
 
 def references_trajectory(n: int, layers: int, p: float, seed: int, sites, p_zero_layers: int,
-                          closed: bool = True, attach_layer: int | None = None):
+                          closed: bool = True, attach_layer: int | None = None, attach_stagger: int = 0):
     """k = len(sites) references (qubits n .. n+k-1); layers 0..p_zero_layers-1
     unmeasured.  Bell pairs (site_j, n+j) at t = 0 when the attach layer
     (attach_layer, default p_zero_layers) is 0, else the references attached
     in order j = 0, 1, ... at the start of that layer, after the previous
     layer's measurements (cpp/include/mipt/probed/kernels.hpp:55-66, each
-    reset's uniform keyed by (layer, site)).  Returns (psi over n+k qubits,
+    reset's uniform keyed by (layer, site)); with attach_stagger, reference j
+    attaches at attach_layer + j * attach_stagger (runner.hpp:230-256).
+    Returns (psi over n+k qubits,
     outcomes [layers, n], p1, attach outcomes, (g, f, u))."""
     k = len(sites)
     attach = p_zero_layers if attach_layer is None else attach_layer
@@ -406,21 +408,20 @@ def references_trajectory(n: int, layers: int, p: float, seed: int, sites, p_zer
     nq = n + k
     psi = np.zeros(1 << nq, np.complex128)
     att = []
-    if attach == 0:
-        for b in range(1 << k):
-            i = 0
-            for j in range(k):
-                if (b >> j) & 1:
-                    i |= (1 << sites[j]) | (1 << (n + j))
-            psi[i] = 2.0 ** (-k / 2)
-    else:
-        psi[0] = 1.0
+    at = [attach + j * attach_stagger for j in range(k)]
+    paired = [j for j in range(k) if at[j] == 0]
+    for b in range(1 << len(paired)):
+        i = 0
+        for bit, j in enumerate(paired):
+            if (b >> bit) & 1:
+                i |= (1 << sites[j]) | (1 << (n + j))
+        psi[i] = 2.0 ** (-len(paired) / 2)
     outs = np.full((layers, n), -1, np.int32)
     p1 = np.zeros((layers, n))
     gi = 0
     for t in range(layers):
-        if attach > 0 and t == attach:
-            for j, site in enumerate(sites):
+        for j, site in enumerate(sites):
+            if t > 0 and at[j] == t:
                 att.append(attach_reference(psi, nq, site, n + j, philox_uniform(seed, t, site, 0, STREAM_ATCH)))
         for (q0, q1) in layer_bonds(n, t, closed):
             apply_dense(psi, nq, [q0, q1], g[gi])

@@ -97,6 +97,35 @@ def test_batch_references_attached_after_monitored_layers(M, case):
     b.close()
 
 
+STAGGER_CASES = [  # L, layers, p, seed, sites, attach layer, stagger (a site may serve both)
+    (8, 14, 0.15, 61, (2, 2), 4, 3),
+    (10, 12, 0.2, 62, (1, 6), 2, 5),
+    (16, 12, 0.15, 63, (5, 5), 3, 4),
+]
+
+
+@pytest.mark.parametrize("case", range(len(STAGGER_CASES)))
+def test_batch_references_staggered_attach(M, case):
+    L, layers, p, seed, sites, att_layer, stagger = STAGGER_CASES[case]
+    print(f"staggered references L={L} sites={sites} attach={att_layer}+{stagger}j", flush=True)
+    b = M.Batch([M.Spec(L, layers, p, seed, True, probe_sites=sites, attach_layer=att_layer,
+                        attach_stagger=stagger)])
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    psi, outs, p1, att, (g, f, u) = O.references_trajectory(L, layers, p, seed, list(sites), 0,
+                                                            attach_layer=att_layer, attach_stagger=stagger)
+    assert len(att) == 2
+    got, flagged, _, _ = b.outcomes(0)
+    check_outcomes(got, outs, p1, u, f, f"stagger case {case}")
+    if flagged.any():
+        pytest.skip("a draw within the flag margin of P(1): trajectories may differ")
+    assert np.abs(b.get_state(0) - psi).max() < 1e-5
+    rho, want = b.probe_rdm()[0], O.reference_rdm(psi, L, 2)
+    assert np.abs(rho - want).max() < 1e-5
+    b.close()
+
+
 @pytest.mark.parametrize("n,s0", [(4, 1), (16, 7)])
 def test_entanglement_swapping_on_device(M, n, s0):
     print(f"entanglement swapping L={n}", flush=True)

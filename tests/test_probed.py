@@ -1,4 +1,4 @@
-"""mipt_probed modes 0, 1, 2 (one, two or four references) and 4 (two).
+"""mipt_probed modes 0, 1, 2 (one, two or four references), 3 and 4 (two).
 
 CPU: the argument contract of cpp/include/mipt/probed/config.hpp:261-372
 (every case fails before any device work).  GPU: the CSV the driver writes
@@ -32,7 +32,10 @@ def M():
     (["1", "7", "4", "1", "1", "0", "0.3", "2"], 1),       # odd ring
     (["4", "10", "4", "1", "1", "0", "0.3", "2"], 1),      # four references need N % 4 == 0
     (["2", "8", "4", "1", "0", "0.1", "4", "20"], 1),      # t_min < t0 = 2N
-    (["1", "8", "4", "1", "3", "0", "1", "2", "0", "1", "2"], 1),  # mode 3 not built
+    (["1", "8", "4", "1", "5", "0.1", "4"], 1),            # mode 5 not built
+    (["1", "8", "4", "1", "3", "0.1", "0", "2"], 1),       # mode 3 needs two references
+    (["2", "8", "4", "1", "3", "0.1", "2", "2"], 1),       # temporal branch needs delta_x = 0
+    (["2", "8", "4", "1", "3", "0.1", "3", "0"], 1),       # spatial branch needs delta_x = N/2
     (["3", "8", "4", "1", "4", "0.1", "4"], 1),            # three-reference mode 4: GMN SDPs not built
     (["1", "8", "4", "1", "4", "0.1", "4"], 1),            # mode 4 needs two or three references
     (["2", "8", "4", "1", "4", "0.1"], 1),                 # mode 4 needs [p] [tau_max]
@@ -200,3 +203,35 @@ def test_probed_mode4_pairs(M, tmp_path, monkeypatch):
             assert abs(float(row[col + "_mean"]) - m) < 1e-4, (d, tau, col)
             assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (d, tau, col)
         assert int(row["negativity_positive_count"]) == sum(w["negativity"] > 1e-12 for w in want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dx,dt", [(0, 2), (3, 0)])
+def test_probed_mode3_staggered(M, tmp_path, monkeypatch, dx, dt):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9006")
+    n, R, p = 6, 2, 0.15
+    print(f"mipt_probed mode 3 dx={dx} dt={dt}", flush=True)
+    assert M.probed_main([2, n, R, 1, 3, p, dx, dt]) == 0
+    head, rows = read_csv(tmp_path / f"probed_2_haar_n_{n}_reals_{R}_mode_3_p_0.15_dx_{dx}_dt_{dt}_tau_0_{8 * n}.csv")
+    assert head[5:] == ["delta_x", "delta_t", "tau1", "tau2", "tau", "tau_over_L", "t_absolute", "I_mean",
+                        "I_stderr", "negativity_mean", "negativity_stderr", "log_negativity_mean",
+                        "log_negativity_stderr"]
+    tau1, tau2 = 4 * n, 4 * n + dt
+    assert len(rows) == 8 * n + 1
+    seeds = [O.lib().orc_trajectory_seed(9006, i) for i in range(R)]
+    for tau, row in enumerate(rows):
+        assert (int(row["tau1"]), int(row["tau2"]), int(row["tau"]), int(row["t_absolute"])) == (tau1, tau2, tau,
+                                                                                                 tau2 + tau)
+        if tau == 0:
+            assert float(row["I_mean"]) == 0.0
+            continue
+        if tau % 8 and tau != 1:
+            continue  # the oracle replays every trajectory from t = 0: check a subset of readout times
+        want = [O.probe_information(O.reference_rdm(
+            O.references_trajectory(n, tau2 + tau, p, s, [0, dx % n], 0, attach_layer=tau1, attach_stagger=dt)[0],
+            n, 2)) for s in seeds]
+        for col, key in (("I", "i2"), ("negativity", "negativity"), ("log_negativity", "log_negativity")):
+            m, se = mean_se([w[key] for w in want])
+            assert abs(float(row[col + "_mean"]) - m) < 1e-4, (tau, col)
+            assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (tau, col)
