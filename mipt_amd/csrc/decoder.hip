@@ -55,7 +55,8 @@ constexpr int TM = 64;        // output tile rows
 constexpr int TK = 32;        // K step (one MFMA 16x16x32)
 constexpr int LP = TK + 8;    // LDS row stride of a K step (bf16), 80 B
 constexpr int CP = CH + 8;    // LDS row stride of the epilogue tile (bf16), 144 B
-constexpr int WG_BLOCKS = 2048;  // weight-gradient grid: 8 blocks on each of 256 CUs
+constexpr int WG_BLOCKS = 2048;   // first-layer weight-gradient grid: 8 blocks on each of 256 CUs
+constexpr int WG3_BLOCKS = 1024;  // grid of the three-tap weight gradient (4 blocks per CU)
 constexpr float NORM_EPS = 1e-12f;
 constexpr float MU = 1.0f;            // weight of the direction term
 constexpr float R_MAX = 1.0f - 1e-6f;  // Bloch length cap of the entropy term
@@ -336,6 +337,98 @@ __global__ void __launch_bounds__(NT) conv_bwd_weight_kernel(const bf16 *__restr
         const int k = kt * TM + row;
         if (k < krows) dst[static_cast<std::size_t>(k) * CH + col] = v;
     });
+}
+
+// Layers 1..3 (64 input channels): one block per (row offset dy = g - 1,
+// chunk) computes the three taps (g, dx = -1, 0, 1), k = (3g + dx + 1)*64 + ci,
+// so the dz tile is staged once per K step for three MFMA tiles; the three
+// shifted inputs are neighbouring positions of the same row.  Staging as in
+// conv_bwd_weight_kernel (8-byte LDS stores of four positions).
+__device__ inline void store_quad(bf16 *S, const std::uint32_t v[4], int cp, int mq)
+{
+    const uint2 lo = {(v[0] & 0xffffu) | (v[1] << 16), (v[2] & 0xffffu) | (v[3] << 16)};
+    const uint2 hi = {(v[0] >> 16) | (v[1] & 0xffff0000u), (v[2] >> 16) | (v[3] & 0xffff0000u)};
+    *reinterpret_cast<uint2 *>(S + (2 * cp) * LP + 4 * mq) = lo;
+    *reinterpret_cast<uint2 *>(S + (2 * cp + 1) * LP + 4 * mq) = hi;
+}
+
+__global__ void __launch_bounds__(NT) conv_bwd_weight3_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ dz,
+                                                              float *__restrict__ partial, int M, int H, int W,
+                                                              int chunk_len)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sm[4 * TM * LP];
+    bf16 *Bs = sm + 3 * TM * LP;  // As[tt] = sm + tt*TM*LP: [ci][m]; Bs: [co][m]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int HW = H * W;
+    const int g = blockIdx.x, chunk = blockIdx.y;
+    floatx4 acc[3][2][2];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[t][i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int mbeg = chunk * chunk_len, mend = std::min(M, mbeg + chunk_len);
+    const int cp = tid & 31, mq = tid >> 5;
+    for (int ms = mbeg; ms < mend; ms += TK)
+    {
+        std::uint32_t va[3][4], vb[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            vb[j] = 0u;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) va[t][j] = 0u;
+        }
+        {
+            int b, y, x;
+            position(ms + 4 * mq, HW, W, b, y, x);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int m = ms + 4 * mq + j;
+                if (m < mend)
+                {
+                    vb[j] = *reinterpret_cast<const std::uint32_t *>(dz + static_cast<std::size_t>(m) * CH + 2 * cp);
+                    const int yy = y + g - 1;
+                    if (yy >= 0 && yy < H)
+                    {
+                        const std::size_t row = (static_cast<std::size_t>(b) * H + yy) * W;
+#pragma unroll
+                        for (int t = 0; t < 3; ++t)
+                        {
+                            int xx = x + t - 1;
+                            xx += xx < 0 ? W : 0;
+                            xx -= xx >= W ? W : 0;
+                            va[t][j] = *reinterpret_cast<const std::uint32_t *>(in + (row + xx) * CH + 2 * cp);
+                        }
+                    }
+                }
+                if (++x == W)
+                {
+                    x = 0;
+                    if (++y == H)
+                    {
+                        y = 0;
+                        ++b;
+                    }
+                }
+            }
+        }
+        store_quad(Bs, vb, cp, mq);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) store_quad(sm + t * TM * LP, va[t], cp, mq);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 3; ++t) tile_step(sm + t * TM * LP, Bs, acc[t], lane, wave);
+        __syncthreads();
+    }
+    float *dst = partial + static_cast<std::size_t>(chunk) * 9 * CH * CH;
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+        for_tile_outputs(acc[t], lane, wave, [&](int row, int col, float v) {
+            dst[static_cast<std::size_t>((3 * g + t) * TM + row) * CH + col] = v;
+        });
 }
 
 // grad[e] = sum over chunks of partial[chunk][e].  A block covers 64
@@ -697,7 +790,8 @@ struct Decoder
         for (int l = 0; l < 4; ++l)
         {
             const int krows = 9 * ci[l];
-            psize = std::max(psize, static_cast<std::size_t>(blocks(WG_BLOCKS, blocks(krows, TM))) * krows * CH * 4);
+            const int per = l == 0 ? blocks(WG_BLOCKS, blocks(krows, TM)) : blocks(WG3_BLOCKS, 3);
+            psize = std::max(psize, static_cast<std::size_t>(per) * krows * CH * 4);
         }
         MIPT_HIP_CHECK(hipMalloc(&partial, psize));
         MIPT_HIP_CHECK(hipMalloc(&bpart, ((M + BROWS - 1) / BROWS) * CH * 4));
@@ -777,8 +871,11 @@ struct Decoder
         {
             const int krows = 9 * ci[l];
             const int ktiles = blocks(krows, TM);
-            // positions per partial: about WG_BLOCKS blocks in all, at least 8 K steps each
-            const int chunk_len = std::max(8 * TK, blocks(blocks(M, blocks(WG_BLOCKS, ktiles)), TK) * TK);
+            // positions per partial: about WG_BLOCKS (first layer) or WG3_BLOCKS blocks in
+            // all, at least 8 K steps each
+            const int xblocks = l == 0 ? ktiles : 3;
+            const int chunk_len = std::max(
+                8 * TK, blocks(blocks(M, blocks(l == 0 ? WG_BLOCKS : WG3_BLOCKS, xblocks)), TK) * TK);
             const int nch = blocks(M, chunk_len);
             const int nbp = blocks(M, BROWS);
             hipLaunchKernelGGL(dz_bias_kernel, dim3(nbp), dim3(256), 0, stream, dact, act[l], M, dz, bpart);
@@ -787,8 +884,8 @@ struct Decoder
                 hipLaunchKernelGGL(conv_bwd_weight_kernel<true>, dim3(ktiles, nch), dim3(NT), 0, stream, nullptr, rec,
                                    idx, dz, partial, M, H, W, krows, chunk_len);
             else
-                hipLaunchKernelGGL(conv_bwd_weight_kernel<false>, dim3(ktiles, nch), dim3(NT), 0, stream, act[l - 1],
-                                   nullptr, nullptr, dz, partial, M, H, W, krows, chunk_len);
+                hipLaunchKernelGGL(conv_bwd_weight3_kernel, dim3(3, nch), dim3(NT), 0, stream, act[l - 1], dz, partial,
+                                   M, H, W, chunk_len);
             hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks(krows * CH, 64)), dim3(256), 0, stream, partial, nch,
                                krows * CH, g + off_w[l]);
             if (l > 0)
