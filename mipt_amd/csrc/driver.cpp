@@ -22,6 +22,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cerrno>
 #include <charconv>
 #include <cmath>
@@ -775,6 +776,182 @@ void probed_usage(const char *prog)
               << "Output: the generated probed_*.csv name, or MIPT_PROBED_INTERNAL_OUTPUT; seeds from MIPT_SEED.\n";
 }
 
+// Mode 4 with three references (probed/geometry.hpp:30-69, config.hpp:430-470,
+// runner.hpp:94-148, output.hpp:216-267): every triangle of ring sites,
+// grouped by its sorted periodic side lengths, kept when its chord balance
+// (shortest / longest chord) reaches B_min; per geometry `realizations`
+// trajectories on an embedding drawn from the trajectory seed, the three
+// references attached after t_eq = 2N monitored layers, read out at
+// tau = 0..tau_max: tripartite information, mean pair MI, minimum cut
+// negativity, joint and mean single purity.  The genuine multipartite
+// negativity needs a conic (SDP) solver and is not evaluated: its columns
+// hold NaN and zero counts.
+int run_probed_triangles(int argc, char **argv, int n, int realizations, int circ)
+{
+    const double p = std::stod(argv[6]);
+    const int tau_max = std::stoi(argv[7]);
+    if (p < 0.0 || p > 1.0 || tau_max < 0 || tau_max > (1 << 20))
+        throw std::invalid_argument("Mode 4 requires p in [0,1] and a nonnegative tau_max "
+                                    "that leaves room for the default t_eq=2N.");
+    double bmin = 0.5;
+    if (argc == 9) bmin = std::stod(argv[8]);
+    else if (const char *e = std::getenv("MIPT_PROBED_TRIANGLE_BALANCE"); e && *e) bmin = std::atof(e);
+    if (!std::isfinite(bmin) || bmin < 0.0 || bmin > 1.0)
+        throw std::invalid_argument("Three-probe mode 4 requires B_min in [0,1].");
+    const int t_eq = env_int("MIPT_PROBED_T_EQ", 2 * n, 0, 1L << 20);
+    const double pi = std::acos(-1.0);
+    auto chord = [&](int d) { return static_cast<double>(n) / pi * std::sin(pi * d / static_cast<double>(n)); };
+    auto pdist = [&](int a, int b) { return std::min(std::abs(a - b), n - std::abs(a - b)); };
+    struct Geometry
+    {
+        std::array<int, 3> d;
+        double balance;
+        std::vector<std::array<int, 3>> emb;
+    };
+    std::vector<Geometry> geo;  // sorted by side lengths, as an ordered map would hold them
+    for (int a = 0; a < n - 2; ++a)
+        for (int b = a + 1; b < n - 1; ++b)
+            for (int c = b + 1; c < n; ++c)
+            {
+                std::array<int, 3> sig{pdist(a, b), pdist(a, c), pdist(b, c)};
+                std::sort(sig.begin(), sig.end());
+                auto it = std::lower_bound(geo.begin(), geo.end(), sig,
+                                           [](const Geometry &g, const std::array<int, 3> &x) { return g.d < x; });
+                if (it == geo.end() || it->d != sig) it = geo.insert(it, Geometry{sig, 0.0, {}});
+                it->emb.push_back({a, b, c});
+            }
+    std::vector<Geometry> kept;
+    for (auto &g : geo)
+    {
+        const double c0 = chord(g.d[0]), c1 = chord(g.d[1]), c2 = chord(g.d[2]);
+        g.balance = std::min({c0, c1, c2}) / std::max({c0, c1, c2});
+        if (g.balance + 1.0e-12 >= bmin) kept.push_back(g);
+    }
+    if (kept.empty())
+        throw std::invalid_argument("No three-probe triangle geometry satisfies B >= " + std::to_string(bmin) +
+                                    "; lower the mode-4 B_min argument or MIPT_PROBED_TRIANGLE_BALANCE.");
+    std::string output;
+    if (const char *e = std::getenv("MIPT_PROBED_INTERNAL_OUTPUT"); e && *e) output = e;
+    else
+        output = "probed_3_haar_n_" + std::to_string(n) + "_reals_" +
+                 compact_count(static_cast<std::uint64_t>(realizations)) + "_mode_4_p_" + number_for_filename(p) +
+                 "_b_" + number_for_filename(bmin) + "_teq_" + std::to_string(t_eq) + "_tau_0_" +
+                 std::to_string(tau_max) + ".csv";
+    std::uint64_t master;
+    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
+    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    struct TriStats
+    {
+        Welford tmi, avg_mi, min_neg, joint, single;
+    };
+    const int ng = static_cast<int>(kept.size());
+    std::vector<TriStats> stats(static_cast<std::size_t>(ng) * (tau_max + 1));
+    const std::uint64_t total = static_cast<std::uint64_t>(ng) * realizations;
+    double budget_gib = 64.0;
+    if (const char *e = std::getenv("MIPT_HIP_STATE_GIB")) budget_gib = std::max(0.001, std::atof(e));
+    const std::uint64_t per_batch = static_cast<std::uint64_t>(std::max<double>(
+        1.0, std::min<double>(static_cast<double>(total), budget_gib * 1073741824.0 / std::ldexp(8.0, n + 3))));
+    std::vector<double> rho;
+    for (std::uint64_t first = 0; first < total; first += per_batch)
+    {
+        const int count = static_cast<int>(std::min<std::uint64_t>(per_batch, total - first));
+        std::vector<mipt_hip_traj_spec> specs(count);
+        std::vector<int> gid(count);
+        for (int i = 0; i < count; ++i)
+        {
+            const std::uint64_t idx = first + i;
+            gid[i] = static_cast<int>(idx / realizations);
+            mipt_hip_traj_spec &s = specs[i];
+            s = mipt_hip_traj_spec{};
+            s.L = n;
+            s.layers = t_eq + tau_max + 1;  // the attach layer t_eq lies inside the drawn range
+            s.p = p;
+            s.seed = trajectory_seed(master, idx);
+            s.closed = 1;
+            s.gate_family = 1;
+            s.probe_site = -1;
+            s.shared_gates = -1;
+            s.probe_count = 3;
+            const auto &emb = kept[gid[i]].emb;
+            const auto &e = emb[splitmix64(s.seed ^ 0x4f524947494eull) % emb.size()];
+            for (int j = 0; j < 3; ++j) s.probe_sites[j] = e[j];
+            s.attach_layer = t_eq;
+        }
+        mipt_hip_batch *b = nullptr;
+        const mipt_hip_batch_config cfg{0, 0, 0};
+        check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
+        int st = mipt_hip_batch_setup(b, specs.data(), count);
+        if (!st) st = mipt_hip_batch_generate(b);
+        rho.resize(static_cast<std::size_t>(count) * 128);
+        int cur = 0;
+        for (int tau = 0; tau <= tau_max && st == 0; ++tau)
+        {
+            if (tau == 0)
+            {
+                // right after the attach: three maximally mixed, uncorrelated references
+                for (int i = 0; i < count; ++i)
+                {
+                    TriStats &a = stats[static_cast<std::size_t>(gid[i]) * (tau_max + 1)];
+                    a.tmi.add(0.0), a.avg_mi.add(0.0), a.min_neg.add(0.0), a.joint.add(0.125), a.single.add(0.5);
+                }
+                continue;
+            }
+            const int target = t_eq + tau;
+            if (target > cur)
+            {
+                st = mipt_hip_batch_evolve_range(b, cur, target);
+                cur = target;
+            }
+            if (!st) st = mipt_hip_batch_probe_rdm(b, rho.data());
+            for (int i = 0; i < count && st == 0; ++i)
+            {
+                double v[11];
+                st = mipt_hip_probe_information(rho.data() + static_cast<std::size_t>(i) * 128, 3, v);
+                if (st) break;
+                TriStats &a = stats[static_cast<std::size_t>(gid[i]) * (tau_max + 1) + tau];
+                a.tmi.add(v[6]), a.avg_mi.add(v[7]), a.min_neg.add(v[8]), a.joint.add(v[9]), a.single.add(v[10]);
+            }
+        }
+        (void)mipt_hip_batch_destroy(b);
+        check(st, "probed mode 4 batch");
+        std::cerr << first + count << '/' << total << " trajectories\n";
+    }
+    int gmn_stride = env_int("MIPT_PROBED_GMN_STRIDE", 1, 1, 1L << 30);
+    int gmn_limit = env_int("MIPT_PROBED_GMN_SAMPLES_PER_BIN", 0, 0, realizations);
+    mkdirs_for(output);
+    std::ofstream csv(output, std::ios::out | std::ios::trunc);
+    if (!csv) throw std::runtime_error("Could not open output CSV: " + output);
+    csv << std::setprecision(17);
+    csv << "N,circ_type,circuit_name,realizations,p,geometry_id,distance_1,distance_2,distance_3,chord_1,chord_2,"
+           "chord_3,chord_geometric_mean,triangle_balance,triangle_balance_cutoff,embedding_count,t_eq,tau,"
+           "tau_over_chord_geometric_mean,t_absolute,TMI_mean,TMI_stderr,average_MI_mean,average_MI_stderr,"
+           "min_bipartite_negativity_mean,min_bipartite_negativity_stderr,gmn_type,GMN_mean,GMN_stderr,"
+           "GMN_sample_count,GMN_requested_count,GMN_zero_prefilter_count,GMN_solver_failure_count,GMN_stride,"
+           "GMN_samples_per_bin,joint_purity_mean,joint_purity_stderr,mean_single_purity_mean,"
+           "mean_single_purity_stderr,realizations_per_geometry\n";
+    for (int g = 0; g < ng; ++g)
+    {
+        const Geometry &G = kept[g];
+        const double c0 = chord(G.d[0]), c1 = chord(G.d[1]), c2 = chord(G.d[2]);
+        const double cg = std::cbrt(c0 * c1 * c2);
+        for (int tau = 0; tau <= tau_max; ++tau)
+        {
+            const TriStats &a = stats[static_cast<std::size_t>(g) * (tau_max + 1) + tau];
+            csv << n << ',' << circ << ",\"Random Haar Unitary\"," << realizations << ',' << p << ',' << g << ','
+                << G.d[0] << ',' << G.d[1] << ',' << G.d[2] << ',' << c0 << ',' << c1 << ',' << c2 << ',' << cg << ','
+                << G.balance << ',' << bmin << ',' << G.emb.size() << ',' << t_eq << ',' << tau << ',' << tau / cg
+                << ',' << t_eq + tau << ',' << a.tmi.mean << ',' << a.tmi.se() << ',' << a.avg_mi.mean << ','
+                << a.avg_mi.se() << ',' << a.min_neg.mean << ',' << a.min_neg.se() << ",GMN,nan,nan,0,0,0,0,"
+                << gmn_stride << ',' << gmn_limit << ',' << a.joint.mean << ',' << a.joint.se() << ','
+                << a.single.mean << ',' << a.single.se() << ',' << realizations << '\n';
+        }
+    }
+    csv.flush();
+    if (!csv) throw std::runtime_error("Failed while writing output CSV: " + output);
+    std::cout << output << '\n';
+    return 0;
+}
+
 // Mode 4 with two references (probed/config.hpp:430-470, runner.hpp:94-148,
 // 318-344, output.hpp:188-214): for every separation d = 1..N/2,
 // `realizations` trajectories with the references at (x0, x0 + d mod N)
@@ -785,10 +962,10 @@ void probed_usage(const char *prog)
 int run_probed_mode4(int argc, char **argv, int probes, int n, int realizations, int circ)
 {
     if (probes != 2 && probes != 3) throw std::invalid_argument("Mode 4 requires probes=2 or probes=3.");
-    if (probes == 3)
-        throw std::invalid_argument("three-reference mode 4 (GMN semidefinite programs) is not built on this engine.");
     if (n < probes) throw std::invalid_argument("Mode 4 requires N >= probes.");
-    if (argc != 8) throw std::invalid_argument("Mode 4 requires [p] [tau_max], with optional [B_min] for probes=3.");
+    if ((probes == 2 && argc != 8) || (probes == 3 && argc != 8 && argc != 9))
+        throw std::invalid_argument("Mode 4 requires [p] [tau_max], with optional [B_min] for probes=3.");
+    if (probes == 3) return run_probed_triangles(argc, argv, n, realizations, circ);
     const double p = std::stod(argv[6]);
     const int tau_max = std::stoi(argv[7]);
     if (p < 0.0 || p > 1.0 || tau_max < 0 || tau_max > (1 << 20))

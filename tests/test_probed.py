@@ -1,4 +1,4 @@
-"""mipt_probed modes 0, 1, 2 (one, two or four references), 3 and 4 (two).
+"""mipt_probed modes 0, 1, 2 (one, two or four references), 3 (two) and 4 (two or three).
 
 CPU: the argument contract of cpp/include/mipt/probed/config.hpp:261-372
 (every case fails before any device work).  GPU: the CSV the driver writes
@@ -36,7 +36,8 @@ def M():
     (["1", "8", "4", "1", "3", "0.1", "0", "2"], 1),       # mode 3 needs two references
     (["2", "8", "4", "1", "3", "0.1", "2", "2"], 1),       # temporal branch needs delta_x = 0
     (["2", "8", "4", "1", "3", "0.1", "3", "0"], 1),       # spatial branch needs delta_x = N/2
-    (["3", "8", "4", "1", "4", "0.1", "4"], 1),            # three-reference mode 4: GMN SDPs not built
+    (["3", "8", "4", "1", "4", "0.1", "4", "1.5"], 1),     # B_min outside [0, 1]
+    (["3", "8", "4", "1", "4", "0.1", "4", "0.99"], 1),    # no triangle of an 8-ring that balanced
     (["1", "8", "4", "1", "4", "0.1", "4"], 1),            # mode 4 needs two or three references
     (["2", "8", "4", "1", "4", "0.1"], 1),                 # mode 4 needs [p] [tau_max]
     (["1", "8", "4", "1", "2", "0", "4", "2", "0", "1"], 1),        # mode 2 needs six values
@@ -235,3 +236,48 @@ def test_probed_mode3_staggered(M, tmp_path, monkeypatch, dx, dt):
             m, se = mean_se([w[key] for w in want])
             assert abs(float(row[col + "_mean"]) - m) < 1e-4, (tau, col)
             assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (tau, col)
+
+
+@pytest.mark.gpu
+def test_probed_mode4_triangles(M, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("MIPT_SEED", "9007")
+    n, R, p, tau_max = 6, 2, 0.15, 2
+    print("mipt_probed mode 4, three references", flush=True)
+    assert M.probed_main([3, n, R, 1, 4, p, tau_max]) == 0
+    head, rows = read_csv(tmp_path / f"probed_3_haar_n_{n}_reals_{R}_mode_4_p_0.15_b_0.5_teq_12_tau_0_2.csv")
+    assert head[5:9] == ["geometry_id", "distance_1", "distance_2", "distance_3"]
+    # triangles of a 6-ring by sorted periodic sides, balance >= 0.5: (1,1,2), (1,2,3), (2,2,2)
+    from itertools import combinations
+    pd = lambda a, b: min(abs(a - b), n - abs(a - b))
+    groups = {}
+    for a, b, c in combinations(range(n), 3):
+        groups.setdefault(tuple(sorted((pd(a, b), pd(a, c), pd(b, c)))), []).append((a, b, c))
+    sig = sorted(groups)
+    assert [tuple(int(r["distance_%d" % i]) for i in (1, 2, 3)) for r in rows[::tau_max + 1]] == sig
+    M64 = 2**64 - 1
+
+    def splitmix(v):
+        v = (v + 0x9E3779B97F4A7C15) & M64
+        v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & M64
+        return v ^ (v >> 31)
+
+    for row in rows:
+        g, tau = int(row["geometry_id"]), int(row["tau"])
+        emb = groups[sig[g]]
+        assert int(row["embedding_count"]) == len(emb) and row["GMN_mean"] == "nan"
+        if tau == 0:
+            assert float(row["TMI_mean"]) == 0.0 and float(row["joint_purity_mean"]) == 0.125
+            continue
+        want = []
+        for i in range(R):
+            seed = O.lib().orc_trajectory_seed(9007, g * R + i)
+            sites = list(emb[splitmix(seed ^ 0x4F524947494E) % len(emb)])
+            want += oracle_samples(n, 12 + tau, p, [seed], sites, 0, 12)
+        for col, key in (("TMI", "tmi"), ("average_MI", "average_mi"),
+                         ("min_bipartite_negativity", "min_bipartite_negativity"),
+                         ("joint_purity", "joint_purity"), ("mean_single_purity", "mean_single_purity")):
+            m, se = mean_se([w[key] for w in want])
+            assert abs(float(row[col + "_mean"]) - m) < 1e-4, (g, tau, col)
+            assert abs(float(row[col + "_stderr"]) - se) < 1e-4, (g, tau, col)
