@@ -1,4 +1,4 @@
-"""Three-site RDM record (the reference's mipt.exe output, SURVEY next item 1).
+"""Three-site RDM record (the reference's mipt.exe output).
 
 CPU: the argument contract of cpp/apps/mipt.cpp:309-366.  GPU: the bulk
 three-site RDM kernel against the FP64 oracle's RDMs (within 1e-5, with
