@@ -156,3 +156,30 @@ def test_reference_spec_validation(M):
     b.synchronize()
     with pytest.raises(M.MiptHipError):
         b.probe_rdm()  # mixed reference counts
+
+
+@pytest.mark.parametrize("L", [8, 16])
+def test_batch_mixed_attach_layers_in_one_group(M, L):
+    # trajectories of one sweep group with different attach layers: every
+    # attach layer applies the pending measurements of the whole group first
+    specs = [M.Spec(L, 10, 0.2, 71, True, probe_sites=(3, L - 2), attach_layer=2),
+             M.Spec(L, 10, 0.2, 72, True, probe_sites=(1, 5), attach_layer=4),
+             M.Spec(L, 10, 0.2, 73, True, probe_sites=(6, 6), attach_layer=2, attach_stagger=3),
+             M.Spec(L, 10, 0.2, 74, True, probe_sites=(0, 4), p_zero_layers=3)]
+    print(f"mixed attach layers L={L}", flush=True)
+    b = M.Batch(specs)
+    b.generate()
+    b.evolve()
+    b.synchronize()
+    rho = b.probe_rdm()
+    for i, s in enumerate(specs):
+        psi, outs, p1, att, (g, f, u) = O.references_trajectory(
+            L, s.layers, s.p, s.seed, list(s.probe_sites), s.p_zero_layers,
+            attach_layer=None if s.attach_layer < 0 else s.attach_layer, attach_stagger=s.attach_stagger)
+        got, flagged, _, _ = b.outcomes(i)
+        check_outcomes(got, outs, p1, u, f, f"mixed traj {i}")
+        if flagged.any():
+            continue
+        assert np.abs(b.get_state(i) - psi).max() < 1e-5, i
+        assert np.abs(rho[i] - O.reference_rdm(psi, L, 2)).max() < 1e-5, i
+    b.close()
