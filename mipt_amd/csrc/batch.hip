@@ -410,50 +410,66 @@ __global__ void __launch_bounds__(SMALL_THREADS) probe_bloch_kernel(DevArena a, 
 
 // Joint density matrix of a trajectory's k references (qubits L..L+k-1,
 // reduced-basis bit j = reference j): rho[r][c] = sum over the chain's 2^L
-// basis states e of psi[e + r 2^L] conj(psi[e + c 2^L]).  Block (g, traj)
-// accumulates the lower-triangle entries q = 8g..8g+7 (row-major over r >= c)
-// in FP64 over a fixed thread order; out[traj][q][re, im], not normalised.
+// basis states e of psi[e + r 2^L] conj(psi[e + c 2^L]).  The lower
+// triangle is cut into T x T tiles (T = min(2^k, 4)); block (tile, traj)
+// loads its T (diagonal tile) or 2T rows once per e and forms the tile's
+// products from registers; FP64 sums over a fixed thread order.
+// out[traj][q][re, im] with q = r(r+1)/2 + c for r >= c, not normalised.
 __global__ void __launch_bounds__(SMALL_THREADS) probe_rdm_kernel(DevArena a, int D, double *__restrict__ out)
 {
     __shared__ double red[SMALL_THREADS];
     const int traj = blockIdx.y;
     const TrajDesc d = a.desc[traj];
-    const int npair = D * (D + 1) / 2, q0 = blockIdx.x * 8;
-    const int cnt = npair - q0 < 8 ? npair - q0 : 8;
-    std::uint32_t ro[8], co[8];
-    for (int e = 0; e < 8; ++e)
+    const int T = D < 4 ? D : 4;
+    int tq = blockIdx.x, I = 0;
+    while (tq > I)
     {
-        int q = q0 + (e < cnt ? e : 0), r = 0;
-        while (q > r)
-        {
-            ++r;
-            q -= r;
-        }
-        ro[e] = static_cast<std::uint32_t>(r) << d.L;
-        co[e] = static_cast<std::uint32_t>(q) << d.L;
+        ++I;
+        tq -= I;
     }
+    const int J = tq;
     const float2 *psi = a.states + d.state_off;
     const std::uint32_t n = 1u << d.L;
-    double re[8], im[8];
-    for (int e = 0; e < 8; ++e) re[e] = im[e] = 0.0;
-    for (std::uint32_t i = threadIdx.x; i < n; i += SMALL_THREADS)
-        for (int e = 0; e < 8; ++e)
-        {
-            if (e >= cnt) break;
-            const float2 x = psi[i + ro[e]], y = psi[i + co[e]];
-            re[e] += static_cast<double>(x.x) * y.x + static_cast<double>(x.y) * y.y;
-            im[e] += static_cast<double>(x.y) * y.x - static_cast<double>(x.x) * y.y;
-        }
-    for (int e = 0; e < cnt; ++e)
+    double re[4][4], im[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) re[r][c] = im[r][c] = 0.0;
+    for (std::uint32_t e = threadIdx.x; e < n; e += SMALL_THREADS)
     {
-        const double sr = block_sum(re[e], red);
-        const double si = block_sum(im[e], red);
-        if (threadIdx.x == 0)
+        float2 x[4], y[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
         {
-            out[(static_cast<std::size_t>(traj) * npair + q0 + e) * 2] = sr;
-            out[(static_cast<std::size_t>(traj) * npair + q0 + e) * 2 + 1] = si;
+            x[t] = t < T ? psi[e + (static_cast<std::uint32_t>(I * T + t) << d.L)] : make_float2(0.f, 0.f);
+            y[t] = (t < T && I != J) ? psi[e + (static_cast<std::uint32_t>(J * T + t) << d.L)] : x[t];
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                re[r][c] += static_cast<double>(x[r].x) * y[c].x + static_cast<double>(x[r].y) * y[c].y;
+                im[r][c] += static_cast<double>(x[r].y) * y[c].x - static_cast<double>(x[r].x) * y[c].y;
+            }
     }
+    const int npair = D * (D + 1) / 2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const int R = I * T + r, C = J * T + c;
+            if (r >= T || c >= T || C > R) continue;  // uniform over the block
+            const double sr = block_sum(re[r][c], red);
+            const double si = block_sum(im[r][c], red);
+            if (threadIdx.x == 0)
+            {
+                const std::size_t q = static_cast<std::size_t>(traj) * npair + R * (R + 1) / 2 + C;
+                out[2 * q] = sr;
+                out[2 * q + 1] = si;
+            }
+        }
 }
 
 // Three-site reduced density matrices (8 x 8) of listed subsystems, one block
@@ -2510,8 +2526,9 @@ int mipt_hip_batch_probe_rdm(mipt_hip_batch *b, double *host)
     const int D = 1 << k, npair = D * (D + 1) / 2;
     double *dev = nullptr;
     MIPT_HIP_CHECK(hipMalloc(&dev, static_cast<std::size_t>(count) * npair * 2 * sizeof(double)));
-    hipLaunchKernelGGL(probe_rdm_kernel, dim3((npair + 7) / 8, count), dim3(SMALL_THREADS), 0, b->stream, b->a, D,
-                       dev);
+    const int tiles = D < 4 ? 1 : D / 4;
+    hipLaunchKernelGGL(probe_rdm_kernel, dim3(tiles * (tiles + 1) / 2, count), dim3(SMALL_THREADS), 0, b->stream, b->a,
+                       D, dev);
     std::vector<double> tri(static_cast<std::size_t>(count) * npair * 2);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)

@@ -45,6 +45,7 @@
 #include <cctype>
 #include <iterator>
 #include <limits>
+#include <thread>
 #include <vector>
 
 namespace
@@ -776,6 +777,27 @@ void probed_usage(const char *prog)
               << "Output: the generated probed_*.csv name, or MIPT_PROBED_INTERNAL_OUTPUT; seeds from MIPT_SEED.\n";
 }
 
+// Information measures of `count` k-reference density matrices ([2^k][2^k]
+// complex, consecutive) into v [count][11], on up to 16 host threads (the
+// eigensolves are independent); returns the first failing status.
+int probe_information_all(const double *rho, int count, int k, std::vector<double> &v)
+{
+    const std::size_t stride = static_cast<std::size_t>(2) << (2 * k);
+    v.assign(static_cast<std::size_t>(count) * 11, 0.0);
+    const int nthreads = std::max(1, std::min({16, count, static_cast<int>(std::thread::hardware_concurrency())}));
+    std::vector<int> status(nthreads, 0);
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nthreads; ++w)
+        pool.emplace_back([&, w]() {
+            for (int i = w; i < count && status[w] == 0; i += nthreads)
+                status[w] = mipt_hip_probe_information(rho + i * stride, k, v.data() + static_cast<std::size_t>(i) * 11);
+        });
+    for (auto &t : pool) t.join();
+    for (int st : status)
+        if (st) return st;
+    return 0;
+}
+
 // Mode 4 with three references (probed/geometry.hpp:30-69, config.hpp:430-470,
 // runner.hpp:94-148, output.hpp:216-267): every triangle of ring sites,
 // grouped by its sorted periodic side lengths, kept when its chord balance
@@ -902,12 +924,12 @@ int run_probed_triangles(int argc, char **argv, int n, int realizations, int cir
                 st = mipt_hip_batch_evolve_range(b, cur, target);
                 cur = target;
             }
+            std::vector<double> info;
             if (!st) st = mipt_hip_batch_probe_rdm(b, rho.data());
+            if (!st) st = probe_information_all(rho.data(), count, 3, info);
             for (int i = 0; i < count && st == 0; ++i)
             {
-                double v[11];
-                st = mipt_hip_probe_information(rho.data() + static_cast<std::size_t>(i) * 128, 3, v);
-                if (st) break;
+                const double *v = info.data() + static_cast<std::size_t>(i) * 11;
                 TriStats &a = stats[static_cast<std::size_t>(gid[i]) * (tau_max + 1) + tau];
                 a.tmi.add(v[6]), a.avg_mi.add(v[7]), a.min_neg.add(v[8]), a.joint.add(v[9]), a.single.add(v[10]);
             }
@@ -1045,12 +1067,12 @@ int run_probed_mode4(int argc, char **argv, int probes, int n, int realizations,
                 st = mipt_hip_batch_evolve_range(b, cur, target);
                 cur = target;
             }
+            std::vector<double> info;
             if (!st) st = mipt_hip_batch_probe_rdm(b, rho.data());
+            if (!st) st = probe_information_all(rho.data(), count, 2, info);
             for (int i = 0; i < count && st == 0; ++i)
             {
-                double v[11];
-                st = mipt_hip_probe_information(rho.data() + static_cast<std::size_t>(i) * 32, 2, v);
-                if (st) break;
+                const double *v = info.data() + static_cast<std::size_t>(i) * 11;
                 PairStats &a = stats[static_cast<std::size_t>(dist[i] - 1) * (tau_max + 1) + tau];
                 a.i2.add(v[1]), a.neg.add(v[4]), a.logneg.add(v[5]);
                 a.positive += v[4] > 1.0e-12;
@@ -1292,12 +1314,12 @@ int run_probed(int argc, char **argv)
                     st = mipt_hip_batch_evolve_range(b, cur, target);
                     cur = target;
                 }
+                std::vector<double> info;
                 if (!st) st = mipt_hip_batch_probe_rdm(b, rho.data());
+                if (!st) st = probe_information_all(rho.data(), count, probes, info);
                 for (int i = 0; i < count && st == 0; ++i)
                 {
-                    double v[11];
-                    st = mipt_hip_probe_information(rho.data() + static_cast<std::size_t>(i) * D * D * 2, probes, v);
-                    if (st) break;
+                    const double *v = info.data() + static_cast<std::size_t>(i) * 11;
                     acc.sq.add(v[0]), acc.i2.add(v[1]), acc.i3.add(v[2]), acc.i4.add(v[3]);
                     acc.neg.add(v[4]), acc.logneg.add(v[5]);
                 }
