@@ -56,12 +56,20 @@ const char *mipt_hip_measure_collapse_last_error(void);
 
 /* Replace mipt_cuda_entropy_svd_f32/_f64 (cpp/include/mipt/cuda/tmi_cuda_svd.hpp:20-34):
  * von Neumann entropy (bits) of the kept modes (little-endian order) of a pure
- * device state.  RDM by an FP32-MFMA Gram for _f32, FP64 for _f64; FP64
- * Hermitian eigenvalues in both.  fermion_trace must be 0 (qubit trace). */
+ * device state.  RDM of the complex64 state by a Gram on the matrix cores
+ * (FP64 MFMA by default, FP32 MFMA with FP64 promotion when selected below),
+ * FP64 for _f64; FP64 Hermitian eigenvalues in both.  fermion_trace must be 0
+ * (qubit trace). */
 int mipt_hip_entropy_svd_f32(const void *device_state_vector, int n_qubits, const int *host_kept_modes,
                              int kept_count, int fermion_trace, double *entropy_out);
 int mipt_hip_entropy_svd_f64(const void *device_state_vector, int n_qubits, const int *host_kept_modes,
                              int kept_count, int fermion_trace, double *entropy_out);
+
+/* Gram arithmetic of every complex64 RDM (no counterpart in the reference,
+ * whose Cherk is FP32): 64 = FP64 MFMA (default), 32 = FP32 MFMA with FP64
+ * promotion every 16-deep K step.  Process-wide; overrides
+ * MIPT_HIP_GRAM_PRECISION.  -1 for any other value. */
+int mipt_hip_set_gram_precision(int bits);
 
 /* Replace mipt_cuda_renyi2_f32/_f64 (tmi_cuda_svd.hpp:36-58): S_2 = -log2 Tr rho^2 (bits). */
 int mipt_hip_renyi2_f32(const void *device_state_vector, int n_qubits, const int *host_kept_modes, int kept_count,

@@ -60,6 +60,7 @@ def lib():
     for name in ("mipt_hip_rho1_complex_f32", "mipt_hip_rho1_complex_f64"):
         getattr(L, name).argtypes = [vp, i, i, dp]
     L.mipt_hip_tmi_f32.argtypes = [vp, i, i, i, i, dp, dp]
+    L.mipt_hip_set_gram_precision.argtypes = [i]
     L.mipt_hip_engine_create.argtypes = [i, vp, ctypes.POINTER(vp)]
     L.mipt_hip_engine_destroy.argtypes = [vp]
     L.mipt_hip_engine_adopt.argtypes = [vp, vp, i]
@@ -134,6 +135,11 @@ def _dp(a):
 
 
 # ------------------------------------------------------------------ drop-ins
+def set_gram_precision(bits: int) -> None:
+    """Gram arithmetic of complex64 RDMs: 64 (FP64 MFMA, default) or 32 (FP32 MFMA)."""
+    _check(lib().mipt_hip_set_gram_precision(int(bits)), "set_gram_precision")
+
+
 def measure_collapse(state_ptr: int, n_qubits: int, qubit: int, uniform: float, fp64: bool = False):
     """mipt_cuda_measure_collapse_* (free_energy_measure.hpp:13-27): (outcome, P(outcome))."""
     o = ctypes.c_int()

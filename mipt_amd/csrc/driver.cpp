@@ -81,6 +81,27 @@ std::uint64_t trajectory_seed(std::uint64_t master, std::uint64_t index)
 }
 
 // cpp/include/mipt/types.hpp:23-38: the last point is p_max exactly
+// MIPT_SEED: a decimal (or 0x-prefixed hexadecimal) unsigned 64-bit value.
+// A malformed value is refused rather than read as a prefix or as 0, so a
+// resumed scan cannot continue on another seed stream unnoticed.
+bool seed_from_env(std::uint64_t &master)
+{
+    const char *e = std::getenv("MIPT_SEED");
+    if (!e) return false;
+    const char *s = e;
+    while (*s == ' ' || *s == '\t') ++s;
+    if (*s == '\0' || *s == '-' || *s == '+')
+        throw std::invalid_argument(std::string("MIPT_SEED must be an unsigned 64-bit integer, got '") + e + "'.");
+    errno = 0;
+    char *end = nullptr;
+    const unsigned long long v = std::strtoull(s, &end, 0);
+    while (end && (*end == ' ' || *end == '\t')) ++end;
+    if (errno == ERANGE || end == s || *end != '\0')
+        throw std::invalid_argument(std::string("MIPT_SEED must be an unsigned 64-bit integer, got '") + e + "'.");
+    master = static_cast<std::uint64_t>(v);
+    return true;
+}
+
 std::vector<double> linspace(double first, double last, int count)
 {
     std::vector<double> v;
@@ -355,8 +376,7 @@ int run_sim_tmi(int argc, char **argv)
     const bool resume = env_flag("SIM_TMI_RESUME", true);
 
     std::uint64_t master;
-    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
-    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    if (!seed_from_env(master)) master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
 
     const int cycles = all_cycles ? n / 4 : 1;
     const std::vector<double> ps = linspace(p_min, p_max, res);
@@ -526,8 +546,7 @@ int run_mipt(int argc, char **argv)
     }
     const int nsub = static_cast<int>(subs.size() / 3);
     std::uint64_t master;
-    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
-    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    if (!seed_from_env(master)) master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
     const std::vector<double> ps = linspace(p_min, p_max, res);
     const std::uint64_t total = static_cast<std::uint64_t>(res) * realizations;
     {
@@ -669,8 +688,7 @@ int run_entropy(int argc, char **argv)
         if (cap > 0 && max_s1 > 0) max_s1 = std::min<int>(max_s1, static_cast<int>(cap));
     }
     std::uint64_t master;
-    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
-    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    if (!seed_from_env(master)) master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
     const int sizes = max_kept - min_kept + 1;
     std::vector<Welford> s1(sizes), s2(sizes);
     double budget_gib = 64.0;
@@ -860,8 +878,7 @@ int run_probed_triangles(int argc, char **argv, int n, int realizations, int cir
                  "_b_" + number_for_filename(bmin) + "_teq_" + std::to_string(t_eq) + "_tau_0_" +
                  std::to_string(tau_max) + ".csv";
     std::uint64_t master;
-    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
-    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    if (!seed_from_env(master)) master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
     struct TriStats
     {
         Welford tmi, avg_mi, min_neg, joint, single;
@@ -1002,8 +1019,7 @@ int run_probed_mode4(int argc, char **argv, int probes, int n, int realizations,
                  compact_count(static_cast<std::uint64_t>(realizations)) + "_mode_4_p_" + number_for_filename(p) +
                  "_teq_" + std::to_string(t_eq) + "_tau_0_" + std::to_string(tau_max) + ".csv";
     std::uint64_t master;
-    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
-    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    if (!seed_from_env(master)) master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
     struct PairStats
     {
         Welford i2, neg, logneg;
@@ -1240,8 +1256,7 @@ int run_probed(int argc, char **argv)
         output += ".csv";
     }
     std::uint64_t master;
-    if (const char *e = std::getenv("MIPT_SEED")) master = std::strtoull(e, nullptr, 0);
-    else master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+    if (!seed_from_env(master)) master = (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
 
     const int D = 1 << probes;
     double budget_gib = 64.0;

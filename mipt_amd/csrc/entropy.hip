@@ -24,8 +24,10 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -386,18 +388,31 @@ int validate_kept(int n, const int *kept, int count, std::uint64_t &kmask)
 }
 
 // Gram arithmetic of the complex64 path: 64 (default) = FP64 MFMA with exact
-// products; 32 = FP32 MFMA promoted to FP64 every 16-deep K step
-// (MIPT_HIP_GRAM_PRECISION=32).  The FP32 form measured up to 1e-5 bits of
-// entropy error on 256x256 half-chain RDMs, so it is not the default.
-int gram_precision()
+// products; 32 = FP32 MFMA promoted to FP64 every 16-deep K step.  Chosen by
+// mipt_hip_set_gram_precision, else MIPT_HIP_GRAM_PRECISION (32 or 64; any
+// other value is refused).  The FP32 form measured up to 1e-5 bits of entropy
+// error on half-chain RDMs, so it is not the default.
+std::atomic<int> g_gram_bits{0};  // 0: not chosen yet
+
+int gram_precision(int &bits)
 {
-    static const int p = [] {
+    int b = g_gram_bits.load();
+    if (b == 0)
+    {
         const char *e = std::getenv("MIPT_HIP_GRAM_PRECISION");
-        if (!e || !*e) return 64;
-        const int v = std::atoi(e);
-        return v == 32 ? 32 : 64;
-    }();
-    return p;
+        if (!e || !*e || std::strcmp(e, "64") == 0) b = 64;
+        else if (std::strcmp(e, "32") == 0) b = 32;
+        else
+        {
+            set_error("MIPT_HIP_GRAM_PRECISION must be 32 or 64, got '%s'", e);
+            return E_ARG;
+        }
+        int expect = 0;
+        g_gram_bits.compare_exchange_strong(expect, b);
+        b = g_gram_bits.load();
+    }
+    bits = b;
+    return OK;
 }
 
 // rho (FP64, device, row-major m x m) of the kept modes of psi.
@@ -447,7 +462,9 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
         int ksplit = 1;
         while (ntri * ksplit < 1024 && static_cast<std::size_t>(ksplit) * 2 * KCHUNK <= K) ksplit *= 2;
         MIPT_HIP_TRY(grow(w.part, w.part_cap, static_cast<std::size_t>(ksplit) * ntri * GT * GT));
-        if (gram_precision() == 32)
+        int gbits = 64;
+        MIPT_HIP_TRY(gram_precision(gbits));
+        if (gbits == 32)
             hipLaunchKernelGGL(gram_mfma_kernel, dim3(static_cast<unsigned>(ntri * ksplit)), dim3(256), 0, stream,
                                w.ar, w.ai, m, static_cast<int>(K), ksplit, ntri, w.part);
         else
@@ -869,6 +886,17 @@ int mipt_hip_rho1_complex_f64(const void *s, int n, int q, double *rho)
 {
     return rho1_impl<double2>(s, n, q, nullptr, rho);
 }
+int mipt_hip_set_gram_precision(int bits)
+{
+    if (bits != 32 && bits != 64)
+    {
+        set_error("gram precision must be 32 or 64, got %d", bits);
+        return E_ARG;
+    }
+    g_gram_bits.store(bits);
+    return OK;
+}
+
 int mipt_hip_tmi_f32(const void *s, int n, int L, int block, int cycle, double *out, double *terms)
 {
     return tmi_impl(s, false, n, L, block, cycle, nullptr, out, terms);

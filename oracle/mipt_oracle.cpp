@@ -171,11 +171,15 @@ void apply_dense(cd *psi, int n, const int *targets, int k, const cd *m)
 // Two-qubit specialisation of apply_dense (same arithmetic order).
 void apply_bond(cd *psi, int n, int q0, int q1, const cd *u)
 {
-    const std::uint64_t dim = std::uint64_t{1} << n;
     const std::uint64_t m0 = std::uint64_t{1} << q0, m1 = std::uint64_t{1} << q1;
-    for (std::uint64_t base = 0; base < dim; ++base)
+    const int lo = std::min(q0, q1), hi = std::max(q0, q1);
+    const std::uint64_t groups = std::uint64_t{1} << (n - 2);
+    for (std::uint64_t j = 0; j < groups; ++j)
     {
-        if (base & (m0 | m1)) continue;
+        // j with zero bits inserted at positions lo and hi: the group bases in
+        // increasing order (each amplitude group visited once, as before)
+        std::uint64_t base = ((j >> lo) << (lo + 1)) | (j & ((std::uint64_t{1} << lo) - 1));
+        base = ((base >> hi) << (hi + 1)) | (base & ((std::uint64_t{1} << hi) - 1));
         const cd in[4] = {psi[base], psi[base | m0], psi[base | m1], psi[base | m0 | m1]};
         cd out[4];
         for (int r = 0; r < 4; ++r)

@@ -90,6 +90,11 @@ def lib():
     return _LIB
 
 
+def ref_available() -> bool:
+    """Whether oracle/_ref/libref.so has been built (checked without loading it)."""
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref.so"))
+
+
 def ref_lib():
     """The reference's own host code compiled into oracle/_ref (None if absent)."""
     global _REF
@@ -98,6 +103,14 @@ def ref_lib():
         if not os.path.exists(path):
             return None
         R = ctypes.CDLL(path)
+        u32p, u64p, cp = _u32p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p
+        R.ref_density_header.restype = ctypes.c_int
+        R.ref_density_header.argtypes = [cp, u32p, u64p, _dp, ctypes.c_char_p, ctypes.c_int]
+        R.ref_density_records.restype = ctypes.c_int
+        R.ref_density_records.argtypes = [cp, u32p, u32p, u32p, _dp, _dp, ctypes.c_char_p, ctypes.c_int]
+        R.ref_tmi_resume_load.restype = ctypes.c_int
+        R.ref_tmi_resume_load.argtypes = [cp, cp, ctypes.c_int, ctypes.c_int, cp, u64p, _ip, ctypes.c_char_p,
+                                          ctypes.c_int]
         R.ref_measure_collapse_host.restype = ctypes.c_int
         R.ref_measure_collapse_host.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp]
         R.ref_entropy_subsystem_rdm.restype = ctypes.c_double
@@ -118,6 +131,55 @@ def ref_lib():
         R.ref_tensor_product.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp]
         _REF = R
     return _REF
+
+
+def ref_density_read(path: str) -> dict:
+    """A MIPTRHO2 v2 file read by the reference's own DensityMatrixReader
+    (cpp/include/mipt/density.hpp:253-347).  Raises ValueError with the
+    reader's message when it rejects the file."""
+    R = ref_lib()
+    err = ctypes.create_string_buffer(512)
+    u = np.zeros(10, np.uint32)
+    rc = ctypes.c_uint64(0)
+    pr = np.zeros(2)
+    if R.ref_density_header(path.encode(), u.ctypes.data_as(_u32p), ctypes.byref(rc), _d(pr), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    keys = ["spatial_dimension", "total_qubits", "kept_qubits", "matrix_dimension", "periods", "realizations",
+            "resolution", "grid_x", "grid_y", "subsystem_count"]
+    meta = {k: int(v) for k, v in zip(keys, u)}
+    meta["record_count"] = int(rc.value)
+    meta["p_min"], meta["p_max"] = float(pr[0]), float(pr[1])
+    nrec, nsub, kept, dim = meta["record_count"], meta["subsystem_count"], meta["kept_qubits"], meta["matrix_dimension"]
+    subs = np.zeros(max(nsub * kept, 1), np.uint32)
+    pidx = np.zeros(max(nrec, 1), np.uint32)
+    real = np.zeros(max(nrec, 1), np.uint32)
+    ps = np.zeros(max(nrec, 1))
+    rho = np.zeros(max(nrec * nsub * 2 * dim * dim, 1))
+    if R.ref_density_records(path.encode(), subs.ctypes.data_as(_u32p), pidx.ctypes.data_as(_u32p),
+                             real.ctypes.data_as(_u32p), _d(ps), _d(rho), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    meta["subsystems"] = subs[:nsub * kept].reshape(nsub, kept)
+    meta["p_index"], meta["realization"], meta["p"] = pidx[:nrec], real[:nrec], ps[:nrec]
+    r = rho[:nrec * nsub * 2 * dim * dim].reshape(nrec, nsub, dim, dim, 2)
+    meta["rho"] = r[..., 0] + 1j * r[..., 1]
+    return meta
+
+
+def ref_tmi_resume_load(path: str, expected_name: str, realizations: int, cycles: int, p_texts) -> dict:
+    """The reference's sim_tmi checkpoint scan (cpp/include/mipt/tmi/resume.hpp:205-333)
+    of an existing CSV.  Raises ValueError("Resume refused: ...") when it refuses."""
+    R = ref_lib()
+    err = ctypes.create_string_buffer(1024)
+    out = np.zeros(5, np.uint64)
+    fl = np.zeros(2, np.int32)
+    if R.ref_tmi_resume_load(path.encode(), expected_name.encode(), int(realizations), int(cycles),
+                             "\n".join(p_texts).encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                             _i(fl), err, 1024) != 0:
+        raise ValueError(err.value.decode())
+    keys = ["rows", "dropped_rows", "kept_bytes", "start_p_index", "start_realization"]
+    d = {k: int(v) for k, v in zip(keys, out)}
+    d["name_verified"], d["loaded"] = bool(fl[0]), bool(fl[1])
+    return d
 
 
 def _d(a):

@@ -49,6 +49,17 @@ def test_sim_tmi_entropy_order_env_is_checked(M, monkeypatch, capfd):
     assert "SIM_TMI_ENTROPY_ORDER" in capfd.readouterr().err
 
 
+@pytest.mark.parametrize("seed", ["abc", "12x", "-5", "", "99999999999999999999999"])
+def test_malformed_seed_is_refused(M, monkeypatch, capfd, seed):
+    # MIPT_SEED must parse completely as an unsigned 64-bit value; a typo is
+    # refused before any work instead of becoming seed 0 or a prefix
+    monkeypatch.setenv("MIPT_SEED", seed)
+    assert M.sim_tmi_main(["8", "1", "1", "1"]) == 1
+    assert "MIPT_SEED" in capfd.readouterr().err
+    assert M.mipt_main(["1", "8", "1", "1", "1"]) == 1
+    assert "MIPT_SEED" in capfd.readouterr().err
+
+
 def _write(path, text):
     path.parent.mkdir(parents=True, exist_ok=True)
     path.write_text(text)
@@ -72,6 +83,46 @@ def test_sim_tmi_resume_refuses_a_foreign_file(M, tmp_path, capfd):
     f = _write(tmp_path / "c.csv", "p,tmi\n0,0.1\n0,0.2\n0,0.3\n0,0.4\n")  # 4 rows for a 3-row group
     assert M.sim_tmi_main(args + [f]) == 1
     assert "more rows" in capfd.readouterr().err
+
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference tree absent (oracle/_ref not built)")
+
+
+@needs_ref
+def test_sim_tmi_resume_refusals_agree_with_reference_reader(M, tmp_path, capfd):
+    # every file the driver refuses, the reference's own checkpoint scan
+    # (cpp/include/mipt/tmi/resume.hpp:205-333) refuses with the same reason
+    args = ["8", "2", "3", "2", "0", "1", "1", "0"]
+    expected = "tmi_haar_n_8_real_3_0_1_res_2.csv"
+    cases = [("tmi_haar_n_8_real_5_0_1_res_2.csv", "p,tmi\n", "was produced with realizations=5"),
+             ("a.csv", "x,y\n", "expected header"),
+             ("b.csv", "p,tmi\n0.5,0.1\n", "switches away from p=0"),
+             ("c.csv", "p,tmi\n0,0.1\n0,0.2\n0,0.3\n0,0.4\n", "more than the 3")]
+    for name, text, why in cases:
+        f = _write(tmp_path / name, text)
+        assert M.sim_tmi_main(args + [f]) == 1
+        ours = capfd.readouterr().err
+        with pytest.raises(ValueError) as e:
+            O.ref_tmi_resume_load(f, expected, 3, 1, ["0", "1"])
+        assert why in str(e.value), str(e.value)
+        assert why.split(" ")[0] in ours, ours
+
+
+@needs_ref
+def test_sim_tmi_resume_checkpoints_agree_with_reference_reader(M, tmp_path):
+    # complete and whole-circuit files the driver accepts; the reference's scan
+    # finds the same checkpoint the driver acts on (nothing left to do for a
+    # complete file; whole circuits kept, a trailing partial one dropped)
+    text = "p,tmi\n" + "".join(f"{p},{0.1 * i}\n" for p in ("0", "1") for i in range(3))
+    f = _write(tmp_path / "done.csv", text)
+    ck = O.ref_tmi_resume_load(f, "tmi_haar_n_8_real_3_0_1_res_2.csv", 3, 1, ["0", "1"])
+    assert ck["loaded"] and ck["rows"] == 6 and ck["start_p_index"] == 2 and ck["kept_bytes"] == len(text)
+    # two cycles per circuit: 5 rows = 2 whole circuits + 1 row of a third
+    lines = ["p,tmi"] + [f"0,{0.1 * i}" for i in range(5)]
+    f = _write(tmp_path / "cut.csv", "\n".join(lines) + "\n0,0.9")
+    ck = O.ref_tmi_resume_load(f, "tmi_haar_all_n_8_real_3_0_1_res_2.csv", 3, 2, ["0", "1"])
+    assert (ck["rows"], ck["dropped_rows"], ck["start_p_index"], ck["start_realization"]) == (4, 1, 0, 2)
+    assert ck["kept_bytes"] == len("\n".join(lines[:5]) + "\n")
 
 
 def test_sim_tmi_resume_complete_file_is_left_alone(M, tmp_path, monkeypatch, capfd):
@@ -163,6 +214,15 @@ def test_sim_tmi_resume_after_interruption_reproduces_the_scan(M, tmp_path, monk
     assert len(lines) == 1 + 2 * 3 * 2
     cut = tmp_path / "cut.csv"
     cut.write_text("".join(lines[:6]) + lines[6][:5])  # header + 5 rows + a partial line
+    if O.ref_available():
+        # the reference's checkpoint scan of the driver's own file: whole
+        # circuits (2 cycles each) are kept, the fifth row is dropped
+        p_texts = list(dict.fromkeys(l.split(",")[0] for l in lines[1:]))
+        ck = O.ref_tmi_resume_load(str(cut), "tmi_haar_all_n_8_real_3_0.1_0.2_res_2.csv", 3, 2, p_texts)
+        assert (ck["rows"], ck["dropped_rows"], ck["start_p_index"], ck["start_realization"]) == (4, 1, 0, 2)
+        assert ck["kept_bytes"] == len("".join(lines[:5]))
+        full_ck = O.ref_tmi_resume_load(str(full), "tmi_haar_all_n_8_real_3_0.1_0.2_res_2.csv", 3, 2, p_texts)
+        assert full_ck["rows"] == 12 and full_ck["start_p_index"] == 2
     assert M.sim_tmi_main(args + [str(cut)]) == 0
     assert cut.read_text() == text
     monkeypatch.setenv("SIM_TMI_RESUME", "0")

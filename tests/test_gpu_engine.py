@@ -114,6 +114,29 @@ def test_entropy_rdm_renyi_match_oracle(M, gpu, n, kept):
         assert np.abs(M.rdm_subsystem(t32.data_ptr(), n, kept) - rho).max() < 2e-6
 
 
+@pytest.mark.parametrize("n,kept", [(12, list(range(6))), (16, list(range(8))), (20, list(range(10)))])
+def test_fp32_mfma_gram_within_its_tolerance(M, gpu, n, kept):
+    # the FP32-MFMA Gram (selected per process by mipt_hip_set_gram_precision;
+    # the default is FP64 MFMA): half-chain entropies within 2e-5 bits of the
+    # FP64 oracle and RDM entries within 2e-6; the default restored after
+    psi, _, _, _ = O.simulate(n, 2 * n, 0.1, 3000 + n)
+    want = O.entropy(psi, n, kept)
+    t = dev_state(gpu, psi)
+    M.set_gram_precision(32)
+    try:
+        s32 = M.entropy(t.data_ptr(), n, kept)
+        r32 = M.rdm_subsystem(t.data_ptr(), n, kept[:6])
+    finally:
+        M.set_gram_precision(64)
+    s64 = M.entropy(t.data_ptr(), n, kept)
+    print(f"  n={n}: FP32 Gram error {abs(s32 - want):.2e}, FP64 Gram error {abs(s64 - want):.2e}", flush=True)
+    assert abs(s32 - want) < 2e-5
+    assert abs(s64 - want) < 1e-5
+    assert np.abs(r32 - O.rdm(psi, n, kept[:6])).max() < 2e-6
+    with pytest.raises(M.MiptHipError):
+        M.set_gram_precision(16)
+
+
 def test_entropy_closed_forms_gpu(M, gpu):
     # Bell pairs across the half cut -> exactly 3 bits; product state -> 0
     n = 8

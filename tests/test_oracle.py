@@ -15,7 +15,14 @@ import pytest
 
 from oracle import oracle as O
 
-REF = O.ref_lib()
+class _LazyRef:
+    """oracle/_ref/libref.so, loaded on first use (GPU test runs never load it)."""
+
+    def __getattr__(self, name):
+        return getattr(O.ref_lib(), name)
+
+
+REF = _LazyRef() if O.ref_available() else None
 needs_ref = pytest.mark.skipif(REF is None, reason="reference tree absent (oracle/_ref not built)")
 
 
@@ -324,6 +331,53 @@ def test_tmi_closed_forms():
     # pairwise Bell pairs between regions: I3 = 0
     psi = bell_pairs_state(8, [(0, 2), (1, 5), (3, 6)])
     assert abs(O.tmi(psi, 8)) < 1e-12
+
+
+def ghz_state(n, sites):
+    psi = np.zeros(1 << n, np.complex128)
+    psi[0] = psi[sum(1 << q for q in sites)] = 1 / math.sqrt(2)
+    return psi
+
+
+def i3_of(psi, n, A, B, C, D):
+    s = lambda k: O.entropy(psi, n, k)
+    return s(A) + s(B) + s(C) + s(D) - s(A + B) - s(A + C) - s(B + C)
+
+
+def wrong_layouts(n, b, o):
+    """Plausible mistakes in the cyclic regions of compute.hpp:64-113: offset
+    off by one either way, the blocks laid out in the other direction, and a
+    D of width b (the last n - 4b sites dropped)."""
+    cyc = lambda start, k, step=1: [(start + step * i) % n for i in range(k)]
+    out = []
+    for oo in (o - 1, o + 1):
+        out.append((cyc(oo, b), cyc(oo + b, b), cyc(oo + 2 * b, b), cyc(oo + 3 * b, n - 3 * b)))
+    out.append((cyc(o, b, -1), cyc(o - b, b, -1), cyc(o - 2 * b, b, -1), cyc(o - 3 * b, n - 3 * b, -1)))
+    if n > 4 * b:
+        out.append((cyc(o, b), cyc(o + b, b), cyc(o + 2 * b, b), cyc(o + 3 * b, b)))
+    return out
+
+
+@pytest.mark.parametrize("n,b", [(8, 2), (10, 2), (12, 3), (14, 3)])
+def test_tmi_regions_pinned_by_ghz4(n, b):
+    # A 4-party GHZ state has I3 = +1 bit exactly when its four sites fall in
+    # four different regions and 0 otherwise (S_X = 1 iff X holds some but not
+    # all GHZ sites).  GHZ on (last of A, first of B, last of C, last of D)
+    # gives +1 for the regions of compute.hpp:64-113 at every cycle offset,
+    # and 0 under each plausible mistake in them (checked here explicitly, so
+    # the pin is known to discriminate).
+    for o in range(b):
+        A, B, C, D = O.tmi_regions(n, b, o)
+        assert A == [(o + i) % n for i in range(b)] and len(D) == n - 3 * b
+        sites = [(o + b - 1) % n, (o + b) % n, (o + 3 * b - 1) % n, (o + n - 1) % n]
+        psi = ghz_state(n, sites)
+        assert abs(O.tmi(psi, n, b, o) - 1.0) < 1e-12
+        for lay in wrong_layouts(n, b, o):
+            assert abs(i3_of(psi, n, *lay)) < 1e-12, (o, lay)
+        # and the offset is not interchangeable: the other cycles give 0
+        for o2 in range(b):
+            if o2 != o:
+                assert abs(O.tmi(psi, n, b, o2)) < 1e-12
 
 
 @needs_ref

@@ -105,6 +105,49 @@ def read_rho_file(path):
     return head, subs, recs
 
 
+def write_rho_file(path, head, subs, recs):
+    """The layout read_rho_file parses, written back (used to pin that layout
+    against the reference's own reader on the CPU)."""
+    out = bytearray(b"MIPTRHO2")
+    out += struct.pack("<12I", head["endian"], head["version"], head["dim"], head["total"], head["kept"], head["mdim"],
+                       head["periods"], head["real"], head["res"], head["gx"], head["gy"], head["nsub"])
+    out += struct.pack("<Q", head["records"]) + struct.pack("<2d", head["p_min"], head["p_max"])
+    out += np.asarray(subs, np.uint32).tobytes()
+    for (pi, r, p, rho) in recs:
+        out += struct.pack("<2I", pi, r) + struct.pack("<d", p)
+        out += np.stack([rho.real, rho.imag], -1).astype(np.float64).tobytes()
+    open(path, "wb").write(bytes(out))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference tree absent (oracle/_ref not built)")
+def test_rho_file_layout_matches_reference_reader(tmp_path):
+    # the layout this file's parser assumes, read back by the reference's
+    # DensityMatrixReader (cpp/include/mipt/density.hpp:253-347)
+    rng = np.random.default_rng(5)
+    n = 6
+    subs = ring_triples(n)
+    recs = []
+    for idx in range(4):
+        psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        psi /= np.linalg.norm(psi)
+        recs.append((idx // 2, idx % 2, [0.1, 0.3][idx // 2], np.stack([O.rdm(psi, n, list(q)) for q in subs])))
+    head = dict(endian=0x01020304, version=2, dim=1, total=n, kept=3, mdim=8, periods=2, real=2, res=2, gx=n, gy=1,
+                nsub=n, records=4, p_min=0.1, p_max=0.3)
+    path = str(tmp_path / "rho3.bin")
+    write_rho_file(path, head, subs, recs)
+    got = O.ref_density_read(path)
+    assert (got["spatial_dimension"], got["total_qubits"], got["kept_qubits"], got["matrix_dimension"]) == (1, n, 3, 8)
+    assert (got["realizations"], got["resolution"], got["record_count"], got["subsystem_count"]) == (2, 2, 4, n)
+    assert [tuple(int(x) for x in s) for s in got["subsystems"]] == subs
+    for i, (pi, r, p, rho) in enumerate(recs):
+        assert (got["p_index"][i], got["realization"][i], got["p"][i]) == (pi, r, p)
+        assert np.array_equal(got["rho"][i], rho)
+    # a truncated file is rejected by the reference reader
+    open(path, "r+b").truncate(len(open(path, "rb").read()) - 8)
+    with pytest.raises(ValueError, match="incomplete"):
+        O.ref_density_read(path)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("d,n", [(1, 6), (2, 3)])
 def test_mipt_rho3_file_matches_oracle(M, tmp_path, monkeypatch, d, n):
@@ -113,7 +156,16 @@ def test_mipt_rho3_file_matches_oracle(M, tmp_path, monkeypatch, d, n):
     periods, real, res = 2, 2, 2
     print(f"mipt d={d} n={n}", flush=True)
     assert M.mipt_main([d, n, periods, real, res, 0.1, 0.3, 1]) == 0
-    head, subs, recs = read_rho_file(tmp_path / ("rho3.bin" if d == 1 else "rho3_2d.bin"))
+    fname = tmp_path / ("rho3.bin" if d == 1 else "rho3_2d.bin")
+    head, subs, recs = read_rho_file(fname)
+    if O.ref_available():
+        # the driver's file read by the reference's own DensityMatrixReader
+        ref = O.ref_density_read(str(fname))
+        assert ref["record_count"] == head["records"] and ref["subsystem_count"] == head["nsub"]
+        assert (ref["grid_x"], ref["grid_y"], ref["periods"]) == (head["gx"], head["gy"], head["periods"])
+        for i, (pi, r, p, rho) in enumerate(recs):
+            assert (ref["p_index"][i], ref["realization"][i], ref["p"][i]) == (pi, r, p)
+            assert np.array_equal(ref["rho"][i], rho)
     L = n if d == 1 else n * n
     assert head == dict(endian=0x01020304, version=2, dim=d, total=L, kept=3, mdim=8, periods=periods, real=real,
                         res=res, gx=n, gy=1 if d == 1 else n, nsub=len(subs), records=real * res, p_min=0.1, p_max=0.3)
