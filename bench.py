@@ -100,6 +100,9 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile configuration")
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel eagerly (no HIP graph replay)")
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--bond-arith", type=int, default=0,
+                    help="1 VALU, 2 VALU + matrix cores, 3 matrix cores (0: the autotuner's choice)")
+    ap.add_argument("--streams", type=int, default=2, help="tiled groups in flight during the timed steps")
     ap.add_argument("--profile-every", type=int, default=7, help="odd, so both sweeps of a layer are sampled")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB plain copy (traffic-counter calibration)")
@@ -135,12 +138,13 @@ def main(argv=None):
             specs.append(M.Spec(L, layers, p, len(specs) + 1, True))
     ntraj = len(specs)
     log(f"setting up {ntraj} trajectories (L={L}, T={layers})")
-    b = M.Batch(specs, tile_bits=args.tile_bits, threads=args.threads)
+    b = M.Batch(specs, tile_bits=args.tile_bits, threads=args.threads, bond_arith=args.bond_arith)
+    b.set_streams(args.streams)
     tune = None
-    if not args.no_autotune and args.tile_bits == 0:
+    if not args.no_autotune and args.tile_bits == 0 and args.bond_arith == 0:
         b.generate()
         best, ms = b.autotune(4)
-        names = ["13b/256t+mfma", "13b/256t", "14b/256t", "13b/128t", "12b/64t"]
+        names = ["13b/256t valu", "13b/256t valu+mfma", "13b/256t mfma", "14b/256t", "13b/128t", "12b/64t"]
         tune = {"choice": names[best] if best >= 0 else None, "probe_ms": {n: round(float(m), 2) for n, m in zip(names, ms)}}
         log(f"autotune: {tune}")
     if not args.no_graph:
@@ -180,14 +184,22 @@ def main(argv=None):
     ms_per_step = ms.value / args.steps
     layer_trajs = ntraj * layers
     value = layer_trajs / (ms_per_step / 1e3)
-    if not args.no_graph:
-        # graph replays cannot be sampled per launch: one more eager step,
-        # outside the timed region, times the sweep kernels
-        b.set_graph(False)
-        b.set_profiling(args.profile_every)
-        step()
-        b.synchronize()
-        log("profiled eager step done")
+    # per-kernel timing: one more eager step outside the timed region, every
+    # group on the batch stream (so no other group's kernels overlap a sweep)
+    # and every profile_every-th sweep bracketed by HIP events on that stream
+    b.set_graph(False)
+    b.set_streams(1)
+    b.set_profiling(args.profile_every)
+    hip.hipEventRecord(e0, stream)
+    step()
+    hip.hipEventRecord(e1, stream)
+    hip.hipEventSynchronize(e1)
+    b.synchronize()
+    ms1 = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms1), e0, e1)
+    single_stream_ms = ms1.value
+    b.set_streams(args.streams)
+    log(f"profiled single-stream eager step done ({single_stream_ms:.1f} ms)")
     st = b.pass_stats()
     achieved = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
     # all sweeps of a step together: algorithmic bytes of the step over its wall time
@@ -232,7 +244,10 @@ def main(argv=None):
                      "traffic": None if ratio is None else round(ratio * st["bytes_avg"]),
                      "kernel": "pass_kernel (gate sweep)", "timed_launches": st["timed"],
                      "avg_launch_ms": round(st["avg_ms"], 4), "algorithmic_bytes_per_launch": round(st["bytes_avg"]),
-                     "streams_in_flight": 2,
+                     "streams_in_flight": 1, "launches_per_step": st["launches"],
+                     "single_stream_step_ms": round(single_stream_ms, 3),
+                     "sweep_time_share": round(st["launches"] * st["avg_ms"] / single_stream_ms, 4)
+                     if single_stream_ms > 0 else None,
                      "aggregate_sweep_GBps": round(aggregate, 1),
                      "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4)},
         "autotune": tune,

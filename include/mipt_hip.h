@@ -171,7 +171,10 @@ typedef struct mipt_hip_batch_config
 {
     int tile_bits;      /* 0 = the autotuner's choice (mipt_hip_batch_autotune; 13 before it runs); else 12..14 */
     int max_hist_bits;  /* measured sites per fused histogram chunk (<= 10); 0 = default */
-    int threads;        /* reserved: 0 */
+    int threads;        /* with tile_bits = 13: 128 selects 64-amplitude registers per thread; else 0 */
+    int bond_arith;     /* bond arithmetic of the tile sweeps: 0 = the autotuner's choice (VALU before it
+                           runs); 1 = VALU only; 2 = VALU and matrix cores; 3 = matrix cores only
+                           (v_mfma_f32_4x4x1_16b_f32) */
 } mipt_hip_batch_config;
 
 int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *hip_stream, mipt_hip_batch **out);
@@ -198,12 +201,14 @@ int mipt_hip_batch_evolve_range(mipt_hip_batch *batch, int layer_begin, int laye
 int mipt_hip_batch_synchronize(mipt_hip_batch *batch);
 /* Autotuner: times probe_layers layers of this batch (its own draws and
  * shapes) under each tile configuration -- 0: 2^13-amplitude tiles, 256
- * threads, one bond per stage on the matrix cores; 1: the same on the VALU
- * only; 2: 2^14 tiles, 256 threads; 3: 2^13 tiles, 128 threads; 4: 2^12 tiles,
- * 64 threads -- and keeps the fastest for the tile-swept trajectories of a
- * batch created with tile_bits = 0.  The batch is re-armed before and after
- * (draws kept).  best_index (may be NULL) receives the choice (-1: nothing is
- * tile-swept), ms (may be NULL, 5 doubles) each candidate's time. */
+ * threads, bond arithmetic on the VALU; 1: the same split between the VALU
+ * and the matrix cores (v_mfma_f32_4x4x1_16b_f32); 2: the same all on the
+ * matrix cores; 3: 2^14 tiles, 256 threads; 4: 2^13 tiles, 128 threads;
+ * 5: 2^12 tiles, 64 threads (3-5 on the VALU) -- and keeps the fastest for
+ * the tile-swept trajectories of a batch created with tile_bits = 0.  The
+ * batch is re-armed before and after (draws kept).  best_index (may be NULL)
+ * receives the choice (-1: nothing is tile-swept), ms (may be NULL, 6 doubles)
+ * each candidate's time. */
 int mipt_hip_batch_autotune(mipt_hip_batch *batch, int probe_layers, int *best_index, double *ms);
 /* 1: full evolutions run as one captured HIP graph (both streams), captured
  * on the second full evolve after the draws or configuration last changed
@@ -272,6 +277,11 @@ int mipt_hip_batch_reset(mipt_hip_batch *batch);
  * flags ([layers][L]) and uniforms ([layers][L]) to host (any may be NULL). */
 int mipt_hip_batch_export_layers(mipt_hip_batch *batch, int traj, double *gates_ri, uint8_t *flags,
                                  double *uniforms);
+/* Tiled groups in flight: 2 (default) alternates groups between the batch
+ * stream and a second stream so one group's small kernels overlap the
+ * other's sweeps; 1 runs every group on the batch stream, so a sweep's
+ * event-timed duration is its own (per-kernel roofline measurement). */
+int mipt_hip_batch_set_streams(mipt_hip_batch *batch, int streams);
 /* Gate-sweep timing: every sample_every-th sweep kernel is bracketed by HIP
  * events on the batch stream (0 = off); resets the statistics. */
 int mipt_hip_batch_set_profiling(mipt_hip_batch *batch, int sample_every);

@@ -92,6 +92,7 @@ def lib():
     L.mipt_hip_batch_reset.argtypes = [vp]
     L.mipt_hip_batch_export_layers.argtypes = [vp, i, dp, u8p, dp]
     L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
+    L.mipt_hip_batch_set_streams.argtypes = [vp, i]
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
@@ -342,7 +343,8 @@ class TrajSpec(ctypes.Structure):
 
 
 class BatchConfig(ctypes.Structure):
-    _fields_ = [("tile_bits", ctypes.c_int), ("max_hist_bits", ctypes.c_int), ("threads", ctypes.c_int)]
+    _fields_ = [("tile_bits", ctypes.c_int), ("max_hist_bits", ctypes.c_int), ("threads", ctypes.c_int),
+                ("bond_arith", ctypes.c_int)]
 
 
 @dataclass
@@ -370,8 +372,8 @@ class Spec:
 class Batch:
     """Device-resident batch of trajectories (mixed chain lengths allowed)."""
 
-    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None, threads: int = 0):
-        cfg = BatchConfig(tile_bits, 0, threads)
+    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None, threads: int = 0, bond_arith: int = 0):
+        cfg = BatchConfig(tile_bits, 0, threads, bond_arith)
         h = ctypes.c_void_p()
         _check(lib().mipt_hip_batch_create(ctypes.byref(cfg), ctypes.c_void_p(stream or 0), ctypes.byref(h)),
                "batch_create")
@@ -451,6 +453,10 @@ class Batch:
                "batch_export_layers")
         return g, f, u
 
+    def set_streams(self, streams: int):
+        """1: every tiled group on the batch stream; 2 (default): two groups in flight."""
+        _check(lib().mipt_hip_batch_set_streams(self._h, streams), "batch_set_streams")
+
     def set_profiling(self, sample_every: int):
         _check(lib().mipt_hip_batch_set_profiling(self._h, sample_every), "batch_set_profiling")
 
@@ -526,7 +532,7 @@ class Batch:
     def autotune(self, probe_layers: int = 4):
         """Pick the fastest tile configuration on this batch; returns (index, ms per candidate)."""
         best = ctypes.c_int()
-        ms = np.zeros(5)
+        ms = np.zeros(6)
         _check(lib().mipt_hip_batch_autotune(self._h, probe_layers, ctypes.byref(best), _dp(ms)), "batch_autotune")
         return best.value, ms
 

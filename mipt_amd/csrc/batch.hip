@@ -572,9 +572,9 @@ __global__ void __launch_bounds__(SMALL_THREADS) rho3_kernel(DevArena a, int fir
 // c..k-1 <-> physical bits s..s+k-c-1 (the window); the other physical bits
 // enumerate the tiles.  A "mapping" assigns each u bit to a register bit (RB
 // of them, so a thread holds 2^RB amplitudes), a lane bit (6) or a wave bit.
-// M0 (used for global loads/stores): lanes = u 0..5, registers = u 6..6+RB-1,
-// waves = the rest, so every load/store instruction covers runs of 2^c
-// consecutive amplitudes.
+// M0 (global loads/stores and the histogram; m0_u): register bit 0 = u 0,
+// lanes = u 1..6, register bits 1.. = u 7..5+RB, waves = the rest, so every
+// 16-byte load/store instruction covers runs of 2^c consecutive amplitudes.
 struct StagePlan
 {
     // 32-bit fields: a uniform runtime index into the kernel arguments then
@@ -583,9 +583,13 @@ struct StagePlan
     int lane_u[6];
     int wave_u[4];
     int nb;        // bonds in this stage
-    int bond[3];   // bond number (within the layer) on register pair i: (0,1), (2,3), (4,5); -1 none
+    // bond number (within the layer), -1 none: bond[0] on register bits
+    // (1,2), or with zpair 1 / 2 a bond of qubit 0 on bits (0,1) / (1,0)
+    // (q0, q1); bond[1] on bits (3,4).  Register bit 0 always holds u bit 0
+    // (qubit 0), on which the LDS exchanges split the tile in halves.
+    int bond[2];
+    int zpair;
     int use_lds;   // 0: M0 registers (no exchange)
-    int mbond;     // bond on lane bits (4,5) applied on the matrix cores; -1 none
 };
 
 struct PassPlan
@@ -610,6 +614,7 @@ struct PassArgs
 {
     DevArena a;
     const int *traj_list;  // group trajectories
+    int ntraj;             // trajectories in the group (tiles = ntraj * plan.ntiles)
     int layer;
     PassPlan plan;
 };
@@ -622,147 +627,176 @@ __device__ __forceinline__ std::uint32_t swz(std::uint32_t u)
 // LDS word of in-tile index u: the low nibble is XORed with the higher
 // nibbles so that lanes walking any 6 u bits with distinct residues mod 4 hit
 // distinct banks.  swz is GF(2)-linear: swz(a ^ b) = swz(a) ^ swz(b).
-// 4x4 complex gate (row-major, index 2*b(q1)+b(q0)) on register bits (RA <- q0, RC <- q1).
-// HALF = -1: all register groups; 0/1: only groups whose top register bit is HALF.
-template <int NV, int RA, int RC, int HALF = -1>
-__device__ __forceinline__ void gate_on_regs(float2 (&v)[NV], const float2 *__restrict__ g)
+// 4x4 complex gate (row-major, index 2*b(q1)+b(q0)) on one register group
+// (registers i0..i3 hold the basis states q1q0 = 00, 01, 10, 11) on the VALU,
+// with the gate's coefficients uniform over the wave (scalar registers).
+template <int NV, int i0, int i1, int i2, int i3>
+__device__ __forceinline__ void group_valu(float2 (&v)[NV], const float (&ur)[16], const float (&ui)[16])
 {
-    float ur[16], ui[16];
+    const float2 x0 = v[i0], x1 = v[i1], x2 = v[i2], x3 = v[i3];
+    float2 y[4];
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
+    for (int r = 0; r < 4; ++r)
     {
-        const float2 x = g[e];
-        ur[e] = x.x;
-        ui[e] = x.y;
+        float yr = ur[4 * r] * x0.x;
+        float yi = ur[4 * r] * x0.y;
+        yr = fmaf(-ui[4 * r], x0.y, yr);
+        yi = fmaf(ui[4 * r], x0.x, yi);
+        yr = fmaf(ur[4 * r + 1], x1.x, yr);
+        yi = fmaf(ur[4 * r + 1], x1.y, yi);
+        yr = fmaf(-ui[4 * r + 1], x1.y, yr);
+        yi = fmaf(ui[4 * r + 1], x1.x, yi);
+        yr = fmaf(ur[4 * r + 2], x2.x, yr);
+        yi = fmaf(ur[4 * r + 2], x2.y, yi);
+        yr = fmaf(-ui[4 * r + 2], x2.y, yr);
+        yi = fmaf(ui[4 * r + 2], x2.x, yi);
+        yr = fmaf(ur[4 * r + 3], x3.x, yr);
+        yi = fmaf(ur[4 * r + 3], x3.y, yi);
+        yr = fmaf(-ui[4 * r + 3], x3.y, yr);
+        yi = fmaf(ui[4 * r + 3], x3.x, yi);
+        y[r] = make_float2(yr, yi);
     }
-#pragma unroll
-    for (int base = 0; base < NV; ++base)
-    {
-        if (base & ((1 << RA) | (1 << RC))) continue;
-        if (HALF >= 0 && ((base & (NV / 2)) != 0) != (HALF == 1)) continue;
-        const int i1 = base | (1 << RA), i2 = base | (1 << RC), i3 = i1 | i2;
-        const float2 x0 = v[base], x1 = v[i1], x2 = v[i2], x3 = v[i3];
-        float2 y[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-            float yr = ur[4 * r] * x0.x;
-            float yi = ur[4 * r] * x0.y;
-            yr = fmaf(-ui[4 * r], x0.y, yr);
-            yi = fmaf(ui[4 * r], x0.x, yi);
-            yr = fmaf(ur[4 * r + 1], x1.x, yr);
-            yi = fmaf(ur[4 * r + 1], x1.y, yi);
-            yr = fmaf(-ui[4 * r + 1], x1.y, yr);
-            yi = fmaf(ui[4 * r + 1], x1.x, yi);
-            yr = fmaf(ur[4 * r + 2], x2.x, yr);
-            yi = fmaf(ur[4 * r + 2], x2.y, yi);
-            yr = fmaf(-ui[4 * r + 2], x2.y, yr);
-            yi = fmaf(ui[4 * r + 2], x2.x, yi);
-            yr = fmaf(ur[4 * r + 3], x3.x, yr);
-            yi = fmaf(ur[4 * r + 3], x3.y, yi);
-            yr = fmaf(-ui[4 * r + 3], x3.y, yr);
-            yi = fmaf(ui[4 * r + 3], x3.x, yi);
-            y[r] = make_float2(yr, yi);
-        }
-        v[base] = y[0];
-        v[i1] = y[1];
-        v[i2] = y[2];
-        v[i3] = y[3];
-    }
+    v[i0] = y[0];
+    v[i1] = y[1];
+    v[i2] = y[2];
+    v[i3] = y[3];
 }
 
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-// The bond on lane bits (4,5) on the matrix cores.  Lane l holds basis state
-// q = l >> 4 of group l & 15, so one v_mfma_f32_16x16x4f32 with B = Re (then
-// Im) of a register applies a 16x4 real matrix to the four amplitudes of
-// every group: rows 4q and 4q+1 carry Re and Im of output q (so each lane
-// receives its own outputs in D[0], D[1]); rows 4q+2, 4q+3 are zero.
-template <int NV, int HALF>
-__device__ __forceinline__ void gate_on_lanes45(float2 (&v)[NV], float a1, float a2)
+// The same group on the matrix cores.  v_mfma_f32_4x4x1_16b_f32 adds, in
+// every block of four lanes 4b..4b+3, A(lane 4b+i) * B(lane l) to register i
+// of lane l.  With A = row i = lane & 3 of the gate's real 8x8 form
+// [[Re U, -Im U], [Im U, Re U]] and B = the lane's own amplitude components,
+// eight steps leave Re y_i (re[i]) and Im y_i (im[i]) of the lane's own
+// group in its own registers: no data moves between lanes and every product
+// is used.  cr/ci/cn = Re U[i][c], Im U[i][c], -Im U[i][c].
+template <int NV, int i0, int i1, int i2, int i3>
+__device__ __forceinline__ void group_mfma(float2 (&v)[NV], const float (&cr)[4], const float (&ci)[4],
+                                           const float (&cn)[4])
 {
+    const float2 x[4] = {v[i0], v[i1], v[i2], v[i3]};
+    floatx4_t re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < NV; ++j)
+    for (int c = 0; c < 4; ++c)
     {
-        if (((j & (NV / 2)) != 0) != (HALF == 1)) continue;
-        floatx4_t d = {0.f, 0.f, 0.f, 0.f};
-        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, v[j].x, d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, v[j].y, d, 0, 0, 0);
-        v[j] = make_float2(d[0], d[1]);
+        re = __builtin_amdgcn_mfma_f32_4x4x1f32(cr[c], x[c].x, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_4x4x1f32(ci[c], x[c].x, im, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_4x4x1f32(cn[c], x[c].y, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_4x4x1f32(cr[c], x[c].y, im, 0, 0, 0);
+    }
+    v[i0] = make_float2(re[0], im[0]);
+    v[i1] = make_float2(re[1], im[1]);
+    v[i2] = make_float2(re[2], im[2]);
+    v[i3] = make_float2(re[3], im[3]);
+}
+
+// The groups of a bond on register bits RA, RC, enumerated at compile time
+// (every register index a constant): group number GI starting at register
+// BASE goes to the VALU while GI < VG, else to the matrix cores.
+template <int NV, int RA, int RC, int VG, int BASE, int GI>
+__device__ __forceinline__ void bond_groups(float2 (&v)[NV], const float (&ur)[16], const float (&ui)[16],
+                                            const float (&cr)[4], const float (&ci)[4], const float (&cn)[4])
+{
+    if constexpr (BASE < NV)
+    {
+        if constexpr ((BASE & ((1 << RA) | (1 << RC))) != 0)
+            bond_groups<NV, RA, RC, VG, BASE + 1, GI>(v, ur, ui, cr, ci, cn);
+        else
+        {
+            constexpr int i1 = BASE | (1 << RA), i2 = BASE | (1 << RC), i3 = i1 | i2;
+            if constexpr (GI < VG) group_valu<NV, BASE, i1, i2, i3>(v, ur, ui);
+            else group_mfma<NV, BASE, i1, i2, i3>(v, cr, ci, cn);
+            // a VALU-only bond keeps one group in flight at a time (its eight
+            // FMA chains are enough to fill the pipe), which bounds registers
+            if constexpr (VG == NV / 4) __builtin_amdgcn_sched_barrier(0);
+            bond_groups<NV, RA, RC, VG, BASE + 1, GI + 1>(v, ur, ui, cr, ci, cn);
+        }
     }
 }
 
-// Register-pair bonds of a stage: (0,1), (2,3), (4,5); -1 = none.
-template <int NV, int HALF>
-__device__ __forceinline__ void valu_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
-    __attribute__((always_inline));
-template <int NV, int HALF>
-__device__ __forceinline__ void valu_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl)
+// A bond on register bits (RA <- q0, RC <- q1): its NV/4 groups are split
+// between the VALU and the matrix cores by MX (0: all VALU; 1: NV/16 groups
+// on the VALU, the rest on the matrix cores, which keeps both pipes busy --
+// a matrix-core group costs 16 MFMAs of 8 cycles, a VALU group 64 FMAs of 4;
+// 2: all on the matrix cores).  The code is straight-line, so the scheduler
+// interleaves the two pipes' instructions.
+template <int NV, int RA, int RC, int MX>
+__device__ __forceinline__ void bond_on_regs(float2 (&v)[NV], const float2 *__restrict__ g, int lane)
 {
-    if (sp.bond[0] >= 0) gate_on_regs<NV, 0, 1, HALF>(v, gl + 16 * sp.bond[0]);
-    if (sp.bond[1] >= 0) gate_on_regs<NV, 2, 3, HALF>(v, gl + 16 * sp.bond[1]);
-    if constexpr (NV == 64 && HALF < 0)
-        if (sp.bond[2] >= 0) gate_on_regs<NV, 4, 5, -1>(v, gl + 16 * sp.bond[2]);
+    constexpr int VG = MX == 0 ? NV / 4 : (MX == 1 ? NV / 16 : 0);
+    float ur[16], ui[16];
+    if constexpr (VG > 0)
+    {
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+        {
+            const float2 x = g[e];
+            ur[e] = x.x;
+            ui[e] = x.y;
+        }
+    }
+    float cr[4], ci[4], cn[4];
+    if constexpr (VG < NV / 4)
+    {
+        // row i = lane & 3, selected from the wave-uniform coefficients (no
+        // vector memory access here: a sweep's next tile is in flight)
+        float sr[16], si[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+        {
+            const float2 x = g[e];
+            sr[e] = x.x;
+            si[e] = x.y;
+        }
+        const int i = lane & 3;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            cr[c] = i == 0 ? sr[c] : (i == 1 ? sr[4 + c] : (i == 2 ? sr[8 + c] : sr[12 + c]));
+            ci[c] = i == 0 ? si[c] : (i == 1 ? si[4 + c] : (i == 2 ? si[8 + c] : si[12 + c]));
+            cn[c] = -ci[c];
+        }
+    }
+    bond_groups<NV, RA, RC, VG, 0, 0>(v, ur, ui, cr, ci, cn);
 }
 
-// A stage's bonds.  With a matrix-core bond the registers are split on their
-// top bit and the stage runs as straight-line code (no branches between the
-// halves, so the scheduler can interleave): register-pair bonds on the first
-// half, then the MFMAs of the first half interleaved with the register-pair
-// bonds of the second half, then the MFMAs of the second half.  The planner
-// guarantees that such a stage uses pairs (0,1) [and (2,3)] only.
-template <int NV>
+// A stage's bonds: bond[0] on register bits (1,2) or, for a bond of qubit 0,
+// (0,1) / (1,0); bond[1] on (3,4).
+template <int NV, int MX>
 __device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl, int lane)
 {
     static_assert(NV == 32 || NV == 64, "register tiles hold 32 or 64 amplitudes");
-    if (sp.mbond < 0)
+    // The three forms differ only in which registers they touch; the empty
+    // markers keep the optimiser from merging them into one body that would
+    // address the tile through pointer selects (which moves it to scratch).
+    if (sp.bond[0] >= 0)
     {
-        valu_stage_gates<NV, -1>(v, sp, gl);
-        return;
-    }
-    const float2 um = gl[16 * sp.mbond + ((lane & 15) >> 2) * 4 + (lane >> 4)];
-    const int e = lane & 3;
-    const float a1 = e == 0 ? um.x : (e == 1 ? um.y : 0.f);
-    const float a2 = e == 0 ? -um.y : (e == 1 ? um.x : 0.f);
-    if (sp.bond[0] >= 0 && sp.bond[1] >= 0)
-    {
-        const float2 *g0 = gl + 16 * sp.bond[0], *g1 = gl + 16 * sp.bond[1];
-        gate_on_regs<NV, 0, 1, 0>(v, g0);
-        gate_on_regs<NV, 2, 3, 0>(v, g1);
-        gate_on_lanes45<NV, 0>(v, a1, a2);
-        gate_on_regs<NV, 0, 1, 1>(v, g0);
-        gate_on_regs<NV, 2, 3, 1>(v, g1);
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
+        if (sp.zpair == 1)
         {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 16, 0); // 16 VALU
+            bond_on_regs<NV, 0, 1, MX>(v, gl + 16 * sp.bond[0], lane);
+            asm volatile("; bond slot (0,1)");
         }
-        gate_on_lanes45<NV, 1>(v, a1, a2);
-    }
-    else if (sp.bond[0] >= 0)
-    {
-        const float2 *g0 = gl + 16 * sp.bond[0];
-        gate_on_regs<NV, 0, 1, 0>(v, g0);
-        gate_on_lanes45<NV, 0>(v, a1, a2);
-        gate_on_regs<NV, 0, 1, 1>(v, g0);
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
+        else if (sp.zpair == 2)
         {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
+            bond_on_regs<NV, 1, 0, MX>(v, gl + 16 * sp.bond[0], lane);
+            asm volatile("; bond slot (1,0)");
         }
-        gate_on_lanes45<NV, 1>(v, a1, a2);
+        else
+        {
+            bond_on_regs<NV, 1, 2, MX>(v, gl + 16 * sp.bond[0], lane);
+            asm volatile("; bond slot (1,2)");
+        }
     }
-    else
-    {
-        gate_on_lanes45<NV, 0>(v, a1, a2);
-        gate_on_lanes45<NV, 1>(v, a1, a2);
-    }
+    if (sp.bond[1] >= 0) bond_on_regs<NV, 3, 4, MX>(v, gl + 16 * sp.bond[1], lane);
 }
 
-// LDS address pieces of a mapping: thread part T (lane and wave bits) and the
-// per-register-bit XOR terms Q[r] = swz(1 << reg_u[r]).
+// LDS addressing.  u bit 0 (qubit 0) is register bit 0 in every mapping, and
+// an exchange moves the tile in two halves (u bit 0 = 0, then 1) through a
+// buffer of 2^(k-1) words: u's word is swz(u >> 1), so the buffer is half
+// the tile and more workgroups fit on a CU.  Address pieces of a mapping:
+// thread part T3 = 8 swz((lane and wave bits) >> 1) and per-register-bit
+// terms Q3[r] = 8 swz((1 << reg_u[r]) >> 1) for r >= 1 (Q3[0] unused).
 template <int WB>
 __device__ __forceinline__ std::uint32_t thread_term(const int *lane_u, const int *wave_u, int lane, int wave)
 {
@@ -771,36 +805,126 @@ __device__ __forceinline__ std::uint32_t thread_term(const int *lane_u, const in
     for (int b = 0; b < 6; ++b) o |= static_cast<std::uint32_t>((lane >> b) & 1) << lane_u[b];
 #pragma unroll
     for (int b = 0; b < WB; ++b) o |= static_cast<std::uint32_t>((wave >> b) & 1) << wave_u[b];
-    return swz(o);
+    return swz(o >> 1);
 }
 
-// LDS byte address of register j: T3 ^ (XOR of Q3[r] over the set bits r of j),
-// with T3 = 8 * swz(thread bits) and Q3[r] = 8 * swz(1 << reg_u[r]) (uniform).
+// LDS byte address of register j: T3 ^ (XOR of Q3[r] over the set bits r >= 1 of j).
 template <int NV>
-__device__ __forceinline__ void lds_write(char *lds, const float2 (&v)[NV], std::uint32_t T3, const std::uint32_t (&Q3)[6])
+__device__ __forceinline__ std::uint32_t reg_addr(std::uint32_t T3, const std::uint32_t (&Q3)[6], int j)
 {
+    std::uint32_t s = 0;
+#pragma unroll
+    for (int r = 1; r < 6; ++r)
+        if ((j >> r) & 1) s ^= Q3[r];
+    return T3 ^ s;
+}
+
+// One half of an exchange: registers j with j & 1 == HALF (u bit 0 = HALF).
+template <int NV, int HALF>
+__device__ __forceinline__ void lds_exchange_half(char *lds, float2 (&v)[NV], std::uint32_t T3s,
+                                                  const std::uint32_t (&Q3s)[6], std::uint32_t T3d,
+                                                  const std::uint32_t (&Q3d)[6], bool dirty)
+{
+    if (dirty) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV / 2; ++i)
+        *reinterpret_cast<float2 *>(lds + reg_addr<NV>(T3s, Q3s, 2 * i + HALF)) = v[2 * i + HALF];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV / 2; ++i)
+        v[2 * i + HALF] = *reinterpret_cast<const float2 *>(lds + reg_addr<NV>(T3d, Q3d, 2 * i + HALF));
+}
+
+// Moves the tile from one mapping (T3s, Q3s) to another (T3d, Q3d) through
+// LDS, half by half; dirty: an earlier read of the buffer may be pending.
+template <int NV>
+__device__ __forceinline__ void lds_exchange(char *lds, float2 (&v)[NV], std::uint32_t T3s, const std::uint32_t (&Q3s)[6],
+                                             std::uint32_t T3d, const std::uint32_t (&Q3d)[6], bool &dirty)
+{
+    lds_exchange_half<NV, 0>(lds, v, T3s, Q3s, T3d, Q3d, dirty);
+    lds_exchange_half<NV, 1>(lds, v, T3s, Q3s, T3d, Q3d, true);
+    dirty = true;
+}
+
+// The M0 mapping of a tile (global loads and stores, the histogram): lane bit
+// b <-> u bit b + 1, register bit 0 <-> u bit 0, register bit r >= 1 <-> u bit
+// 6 + r, wave bit w <-> u bit 6 + RB + w.  Registers 2i and 2i + 1 then hold
+// two consecutive amplitudes, so each load or store moves 16 bytes per lane.
+__host__ __device__ constexpr std::uint32_t m0_u(std::uint32_t lane, std::uint32_t wave, std::uint32_t j, int rb)
+{
+    return (j & 1u) | (lane << 1) | ((j >> 1) << 7) | (wave << (6 + rb));
+}
+
+// FP64 histogram partial of a tile over its local measured sites, from the
+// registers in the M0 mapping.  The local measured u bits split into lane
+// (ml), register (mr) and wave (mw) bits; a bin is pext(u, loc_u), the wave
+// bits highest.  Weights |psi|^2 are formed in FP64 and summed over the
+// unmeasured register bits by in-register folds, over the unmeasured lane
+// bits by butterfly exchanges (a + b in both lanes, exactly commutative) and
+// over the waves in LDS in increasing wave order: a fixed order, so the
+// partials are reproducible.  With cond, only amplitudes whose u bits under
+// cdm equal cdv count.  red: LDS scratch of NT/64 * 2^(|ml| + |mr|) doubles.
+// Every thread must call.
+template <int NV, int NT>
+__device__ __forceinline__ void tile_histogram(const float2 (&v)[NV], std::uint32_t loc_u, bool cond, std::uint32_t cdm,
+                                               std::uint32_t cdv, int lane, int wave, double *red, double *out)
+{
+    constexpr int RB = NV == 32 ? 5 : 6;
+    const std::uint32_t ml = (loc_u >> 1) & 63u;
+    const std::uint32_t mr = (loc_u & 1u) | (((loc_u >> 7) & (NV / 2 - 1u)) << 1);
+    const std::uint32_t mw = loc_u >> (6 + RB);
+    const std::uint32_t nw_mask = (1u << (6 + RB)) - 1u;  // u bits below the wave bits
+    double w[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j)
     {
-        std::uint32_t s = 0;
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-            if ((j >> r) & 1) s ^= Q3[r];
-        *reinterpret_cast<float2 *>(lds + (T3 ^ s)) = v[j];
+        const double xr = v[j].x, xi = v[j].y;
+        double q = fma(xr, xr, xi * xi);
+        if (cond && ((m0_u(lane, wave, j, RB) & cdm) != cdv)) q = 0.0;
+        w[j] = q;
     }
-}
-
-template <int NV>
-__device__ __forceinline__ void lds_read(const char *lds, float2 (&v)[NV], std::uint32_t T3, const std::uint32_t (&Q3)[6])
-{
+    // unmeasured register bits: fold j | 1 << r into j
 #pragma unroll
-    for (int j = 0; j < NV; ++j)
+    for (int r = 0; r < RB; ++r)
+        if (!((mr >> r) & 1u))
+        {
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                if (!(j & (1 << r))) w[j] += w[j | (1 << r)];
+        }
+    // unmeasured lane bits: butterfly over the surviving registers
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+        if (!((ml >> b) & 1u))
+        {
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                if ((static_cast<std::uint32_t>(j) & ~mr) == 0u) w[j] += __shfl_xor(w[j], 1 << b);
+        }
+    const int nlow = __popc(loc_u & nw_mask);
+    const std::uint32_t stride = 1u << nlow;
+    __syncthreads();  // the tile buffer is reused as scratch
+    if ((static_cast<std::uint32_t>(lane) & ~ml) == 0u)
     {
-        std::uint32_t s = 0;
 #pragma unroll
-        for (int r = 0; r < 6; ++r)
-            if ((j >> r) & 1) s ^= Q3[r];
-        v[j] = *reinterpret_cast<const float2 *>(lds + (T3 ^ s));
+        for (int j = 0; j < NV; ++j)
+            if ((static_cast<std::uint32_t>(j) & ~mr) == 0u)
+            {
+                const std::uint32_t low = static_cast<std::uint32_t>(pext64(m0_u(lane, 0, j, RB), loc_u & nw_mask));
+                red[static_cast<std::uint32_t>(wave) * stride + low] = w[j];
+            }
+    }
+    __syncthreads();
+    constexpr int NW = NT / 64;
+    const std::uint32_t bins = stride << __popc(mw);
+    for (std::uint32_t bidx = threadIdx.x; bidx < bins; bidx += NT)
+    {
+        const std::uint32_t low = bidx & (stride - 1u), wpart = bidx >> nlow;
+        double acc = 0.0;
+        for (int ww = 0; ww < NW; ++ww)
+            if ((static_cast<std::uint32_t>(ww) & mw) == static_cast<std::uint32_t>(pdep64(wpart, mw)))
+                acc += red[static_cast<std::uint32_t>(ww) * stride + low];
+        out[bidx] = acc;
     }
 }
 
@@ -932,113 +1056,166 @@ __device__ void finalize_chunk(const DevArena &a, int traj, int tl, int t, int c
     a.flagged[d.layer_off + t] = flg;
 }
 
-// One sweep over the tiles of a group of trajectories.  AUX = 0: a gate sweep
-// (the hot kernel); AUX = 1: a sweep without gates (the closing collapse, or
-// the histogram of an extra chunk of a layer with more than MAX_HIST_BITS
-// measured sites), kept as its own kernel so profiles separate the two.
-template <int RB, int NT, int AUX>
-__global__ void __launch_bounds__(NT, (RB == 5 ? 2 : 1)) pass_kernel(PassArgs args)
+// Wave-uniform context of one tile of a sweep: what its load, stages,
+// histogram and store need (kept small: two are live at once).
+struct TileCtx
 {
-    constexpr int NV = 1 << RB;
-    constexpr int WB = (NT == 256) ? 2 : (NT == 128 ? 1 : 0);
-    static_assert(RB == 5 || RB == 6, "register tiles hold 32 or 64 amplitudes");
-    extern __shared__ float2 lds[];
+    int id, tl, tile, traj, L, nq;
+    std::uint64_t B;              // tile base (physical index)
+    float2 *state;                // the trajectory's state
+    std::uint64_t goff;           // the layer's first gate (complex64 element offset)
+    std::uint32_t cml, cvl;       // previous collapse on local bits (physical)
+    float scale;                  // its renormalisation
+    bool zero, gen_init, tile_out;
+    std::uint32_t flags;          // the layer's measured sites
+    std::uint32_t cdm, cdv;       // decided outcomes on local bits (u bits; overflow-chunk histograms)
+};
+
+// Read-only arrays of a sweep, from kernel arguments declared __restrict__ so
+// that their wave-uniform reads become scalar loads: scalar loads complete on
+// their own counter, so reading them never waits for a tile prefetch.
+struct SweepIn
+{
+    const int *traj_list;
+    const TrajDesc *desc;
+    const std::uint32_t *flags;
+    const Collapse *coll;
+    const int *init;
+    const float2 *gates;
+};
+
+// Context of tile `id`; false when the tile has nothing to do in this sweep
+// (a collapsed-out tile the next sweep re-zeroes, or an overflow-chunk
+// histogram of a trajectory without that chunk).
+__device__ __forceinline__ bool tile_context(const PassArgs &args, const int *__restrict__ traj_list,
+                                             const TrajDesc *__restrict__ desc, const std::uint32_t *__restrict__ flags,
+                                             const Collapse *__restrict__ coll, const int *__restrict__ init,
+                                             const float2 *__restrict__ gates, int id, bool aux, TileCtx &x)
+{
     const PassPlan &pl = args.plan;
     const DevArena &a = args.a;
-    const int k = pl.k;
-    double *hred = reinterpret_cast<double *>(lds + (1u << k));
-
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int tl = blockIdx.x / pl.ntiles;
-    const int tile = blockIdx.x - tl * pl.ntiles;
-    const int traj = args.traj_list[tl];
-    const TrajDesc d = a.desc[traj];
     const int t = args.layer;
-    // an overflow-chunk histogram sweep (cond: layer t is one of the
-    // trajectory's layers) has nothing to add for a trajectory with too few
-    // measured sites (its finalize returns at once); uniform per block.  The
-    // closing collapse sweep also runs stage-free but has cond = 0.
-    if (AUX && pl.cond && __popc(a.flags[d.layer_off + t]) <= pl.chunk * MAX_HIST_BITS) return;
-    const int c = pl.c, s = pl.s;
-    const std::uint64_t B = pdep64(static_cast<std::uint64_t>(tile), pl.rest_mask);
-    char *tp = reinterpret_cast<char *>(a.states + d.state_off + B);  // tile base (uniform)
-    char *ldsb = reinterpret_cast<char *>(lds);
-    const std::uint32_t lowmask = (1u << c) - 1u;
-    // M0 element offset of (lane, wave, register j): thread part + j * jstride
-    const std::uint32_t jshift = static_cast<std::uint32_t>(s + 6 - c);
-    const std::uint32_t toff = (static_cast<std::uint32_t>(lane) & lowmask) |
-                               ((static_cast<std::uint32_t>(lane) >> c) << s) |
-                               (static_cast<std::uint32_t>(wave) << (jshift + RB));
-
-    // ---- load (M0), with the previous layer's collapse or the initial state
+    x.id = id;
+    x.tl = id / pl.ntiles;
+    x.tile = id - x.tl * pl.ntiles;
+    x.traj = traj_list[x.tl];
+    const TrajDesc d = desc[x.traj];
+    x.flags = flags[d.layer_off + t];
+    if (aux && pl.cond && __popc(x.flags) <= pl.chunk * MAX_HIST_BITS) return false;
+    x.L = d.L;
+    x.nq = d.nq;
+    x.B = pdep64(static_cast<std::uint64_t>(x.tile), pl.rest_mask);
+    x.state = a.states + d.state_off;
+    x.goff = (d.gate_off + gate_prefix(d, t)) * 16;
     std::uint64_t cm = 0, cv = 0;
-    float scale = 1.f;
+    x.scale = 1.f;
     if (pl.collapse && t > 0)
     {
-        const Collapse cc = a.coll[d.layer_off + t - 1];
+        const Collapse cc = coll[d.layer_off + t - 1];
         cm = cc.mask;
         cv = cc.val;
-        scale = static_cast<float>(cc.scale);
+        x.scale = static_cast<float>(cc.scale);
     }
     else if (pl.pmask && t > 0)
     {
         // the layer's first sweep skipped tiles of the zero branch; their
         // sites are untouched since, so the branch is still zero by definition
-        const Collapse cc = a.coll[d.layer_off + t - 1];
+        const Collapse cc = coll[d.layer_off + t - 1];
         cm = cc.mask & pl.pmask;
         cv = cc.val & cm;
     }
-    const bool tile_zero = (B & cm) != (cv & cm & pl.rest_mask);
-    if (pl.zskip && tile_zero) return;  // uniform over the workgroup, before any barrier
-    const std::uint32_t cml = static_cast<std::uint32_t>(cm & ~pl.rest_mask);
-    const std::uint32_t cvl = static_cast<std::uint32_t>(cv & ~pl.rest_mask);
-    const bool gen_init = pl.init && a.init[traj];
-    float2 v[NV];
-    if (gen_init)
+    x.zero = (x.B & cm) != (cv & cm & pl.rest_mask);
+    if (pl.zskip && x.zero) return false;
+    x.cml = static_cast<std::uint32_t>(cm & ~pl.rest_mask);
+    x.cvl = static_cast<std::uint32_t>(cv & ~pl.rest_mask);
+    x.gen_init = pl.init && init[x.traj];
+    x.tile_out = false;
+    x.cdm = x.cdv = 0;
+    if (aux && pl.cond)
     {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) v[j] = initial_amp(d, B | (toff + (static_cast<std::uint32_t>(j) << jshift)));
+        const Collapse cc = coll[d.layer_off + t];
+        x.tile_out = (x.B & cc.mask) != (cc.val & cc.mask & pl.rest_mask);
+        const std::uint32_t lowmask = (1u << pl.c) - 1u, wmask = (1u << (pl.k - pl.c)) - 1u;
+        x.cdm = static_cast<std::uint32_t>((cc.mask & lowmask) | (((cc.mask >> pl.s) & wmask) << pl.c));
+        x.cdv = static_cast<std::uint32_t>((cc.val & lowmask) | (((cc.val >> pl.s) & wmask) << pl.c));
     }
-    else if (tile_zero)
-    {
+    return true;
+}
+
+// Issues the loads of tile x in the M0 mapping (element (lane, wave, j) at
+// in-tile offset toff + j << jshift); the collapsed-out branch of the
+// previous layer's measurement reads as 0 (its scale is applied when the tile
+// is processed).  A tile of an initial state is generated when processed.
+template <int NV>
+__device__ __forceinline__ void tile_load(const TileCtx &x, bool have, const float2 *any_state, std::uint32_t toff,
+                                          std::uint32_t jshift, float2 (&v)[NV])
+{
+    // One branch-free sequence of 16-byte buffer loads for every case (a
+    // merge of differently loaded registers would force a wait): a lane's
+    // pair is fetched only when the tile exists, is neither all zero nor an
+    // initial state, and the pair is not wholly on the previous collapse's
+    // zero branch; every other lane takes an offset past the resource's
+    // range, which the hardware answers with 0 and no fetch (states up to
+    // 2^27 amplitudes, so in-range offsets stay below 2^31).  When qubit 0
+    // itself was measured, one amplitude of each pair is zeroed when the tile
+    // is processed.
+    const bool fetch = have && !x.zero && !x.gen_init;
+    const char *tp = fetch ? reinterpret_cast<const char *>(x.state + x.B) : reinterpret_cast<const char *>(any_state);
+    const std::uint32_t range = fetch ? static_cast<std::uint32_t>(((std::uint64_t{1} << x.nq) - x.B) << 3) : 0u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(tp), static_cast<short>(0),
+                                                                        static_cast<int>(range), 0x00020000);
+    const std::uint32_t m = fetch ? (x.cml & ~1u) : 0u, val = fetch ? (x.cvl & ~1u) : 0u;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) v[j] = make_float2(0.f, 0.f);
-    }
-    else if (cm == 0)
+    for (int i = 0; i < NV / 2; ++i)
     {
-        // 32-bit byte offsets keep the SGPR-base + VGPR-offset addressing form
+        const std::uint32_t o = toff + (static_cast<std::uint32_t>(i) << jshift);
+        const std::uint32_t off = (fetch && (o & m) == val) ? (o << 3) : 0x80000000u;
+        const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        v[2 * i] = make_float2(q.x, q.y);
+        v[2 * i + 1] = make_float2(q.z, q.w);
+    }
+}
+
+// Processes one tile whose amplitudes are in v (M0 mapping): the initial
+// state or the previous layer's collapse scale, the stages of the sweep plan,
+// back to M0, the store, and the FP64 histogram partial.
+template <int RB, int NT, int AUX, int MX>
+__device__ __forceinline__ void process_tile(const PassArgs &args, const float2 *__restrict__ gates, const TileCtx &cur,
+                                             float2 (&v)[1 << RB],
+                                             std::uint32_t toff, std::uint32_t jshift, std::uint32_t T3m0,
+                                             const std::uint32_t (&Q3m0)[6], int lane, int wave)
+{
+    constexpr int NV = 1 << RB;
+    constexpr int WB = (NT == 256) ? 2 : (NT == 128 ? 1 : 0);
+    extern __shared__ float2 lds[];
+    const PassPlan &pl = args.plan;
+    const DevArena &a = args.a;
+    const int k = pl.k;
+    const int c = pl.c, s = pl.s;
+    char *ldsb = reinterpret_cast<char *>(lds);
+    // ---- initial state, or the collapse scale of the current tile
+    if (cur.gen_init)
+    {
+        const TrajDesc d = a.desc[cur.traj];
 #pragma unroll
         for (int j = 0; j < NV; ++j)
-            v[j] = *reinterpret_cast<const float2 *>(tp + ((toff + (static_cast<std::uint32_t>(j) << jshift)) << 3));
+            v[j] = initial_amp(d, cur.B | (toff + ((static_cast<std::uint32_t>(j) >> 1) << jshift) + (j & 1)));
     }
-    else
+    else if (cur.scale != 1.f || (cur.cml & 1u))
     {
-        // the zero branch is not read: its lanes take a buffer offset past the
-        // resource's range, which the hardware answers with 0 and no fetch
-        // (states up to 2^27 amplitudes, so in-range offsets stay below 2^31)
-        const std::uint32_t range = static_cast<std::uint32_t>(((std::uint64_t{1} << d.nq) - B) << 3);
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc(tp, static_cast<short>(0), static_cast<int>(range), 0x00020000);
+        // qubit 0 measured: one amplitude of each pair is on the zero branch
 #pragma unroll
         for (int j = 0; j < NV; ++j)
         {
-            const std::uint32_t o = toff + (static_cast<std::uint32_t>(j) << jshift);
-            const std::uint32_t off = ((o & cml) == cvl) ? (o << 3) : 0x80000000u;
-            float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
-            x.x *= scale;
-            x.y *= scale;
-            v[j] = x;
+            const bool off_branch = (cur.cml & 1u) && ((static_cast<std::uint32_t>(j) & 1u) != (cur.cvl & 1u));
+            const float f = off_branch ? 0.f : cur.scale;
+            v[j].x *= f;
+            v[j].y *= f;
         }
     }
 
     // ---- stages
-    const float2 *gl = a.gates + (d.gate_off + gate_prefix(d, t)) * 16;
-    // current mapping: M0 (registers = u bits 6.., lanes = u bits 0..5)
-    const std::uint32_t T3m0 = swz(static_cast<std::uint32_t>(lane) | (static_cast<std::uint32_t>(wave) << (6 + RB))) << 3;
-    std::uint32_t Q3m0[6];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) Q3m0[r] = swz(1u << (6 + r)) << 3;
+    const float2 *__restrict__ gl = gates + cur.goff;
     std::uint32_t T3 = T3m0;
     std::uint32_t Q3[6];
 #pragma unroll
@@ -1050,127 +1227,115 @@ __global__ void __launch_bounds__(NT, (RB == 5 ? 2 : 1)) pass_kernel(PassArgs ar
         const StagePlan &sp = pl.st[si];
         if (sp.use_lds)
         {
-            if (lds_dirty) __syncthreads();
-            lds_write<NV>(ldsb, v, T3, Q3);
-            __syncthreads();
-            T3 = thread_term<WB>(sp.lane_u, sp.wave_u, lane, wave) << 3;
+            const std::uint32_t T3n = thread_term<WB>(sp.lane_u, sp.wave_u, lane, wave) << 3;
+            std::uint32_t Q3n[6];
+            Q3n[0] = 0;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) Q3[r] = swz(1u << sp.reg_u[r]) << 3;
-            lds_read<NV>(ldsb, v, T3, Q3);
-            lds_dirty = true;
+            for (int r = 1; r < 6; ++r) Q3n[r] = swz((1u << sp.reg_u[r]) >> 1) << 3;
+            lds_exchange<NV>(ldsb, v, T3, Q3, T3n, Q3n, lds_dirty);
+            T3 = T3n;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) Q3[r] = Q3n[r];
             in_m0 = false;
         }
-        apply_stage_gates<NV>(v, sp, gl, lane);
-    }
-    // ---- back to M0 through LDS (also leaves the tile in LDS for the histogram)
-    if (!in_m0 || pl.hist)
-    {
-        if (lds_dirty) __syncthreads();
-        lds_write<NV>(ldsb, v, T3, Q3);
-        __syncthreads();
-        if (!in_m0) lds_read<NV>(ldsb, v, T3m0, Q3m0);
+        apply_stage_gates<NV, MX>(v, sp, gl, lane);
     }
 
-    // ---- FP64 histogram partial of this tile over the chunk's local sites
-    if (pl.hist)
+    // ---- the chunk's measured sites that are local to this tile (u bits)
+    std::uint32_t loc_u = 0;
+    // a layer without measured sites needs no partials (its finalize
+    // records the identity collapse without reading them)
+    const bool do_hist = pl.hist && !(pl.chunk == 0 && cur.flags == 0u);
+    if (do_hist)
     {
-        auto local_phys = [&](std::uint32_t u) -> std::uint32_t { return (u & lowmask) | ((u >> c) << s); };
-        const std::uint32_t fl = a.flags[d.layer_off + t];
-        // chunk sites: set bits number [chunk*MAX, chunk*MAX + MAX) of fl
-        std::uint32_t chunk_sites = 0;
         int seen = 0;
-        for (int q = 0; q < d.L; ++q)
+        for (int q = 0; q < cur.L; ++q)
         {
-            if (!((fl >> q) & 1u)) continue;
-            if (seen >= pl.chunk * MAX_HIST_BITS && seen < (pl.chunk + 1) * MAX_HIST_BITS) chunk_sites |= 1u << q;
+            if (!((cur.flags >> q) & 1u)) continue;
+            if (seen >= pl.chunk * MAX_HIST_BITS && seen < (pl.chunk + 1) * MAX_HIST_BITS)
+            {
+                if (q < c) loc_u |= 1u << q;
+                else if (q >= s && q < s + (k - c)) loc_u |= 1u << (q - s + c);
+            }
             ++seen;
         }
-        std::uint32_t loc_u = 0;  // u bits of local chunk sites
-        for (int q = 0; q < d.L; ++q)
-        {
-            if (!((chunk_sites >> q) & 1u)) continue;
-            if (q < c) loc_u |= 1u << q;
-            else if (q >= s && q < s + (k - c)) loc_u |= 1u << (q - s + c);
-        }
-        std::uint64_t condm = 0, condv = 0;
-        if (AUX && pl.cond)
-        {
-            const Collapse cc = a.coll[d.layer_off + t];
-            condm = cc.mask;
-            condv = cc.val;
-        }
-        const bool tile_out = (B & condm) != (condv & condm & pl.rest_mask);
-        const std::uint32_t cdm = static_cast<std::uint32_t>(condm & ~pl.rest_mask);
-        const std::uint32_t cdv = static_cast<std::uint32_t>(condv & ~pl.rest_mask);
-        const int mloc = __popc(loc_u);
-        const std::uint32_t bins = 1u << mloc;
-        const std::uint32_t full = (1u << k) - 1u;
-        const std::uint32_t rmask = full & ~loc_u;
-        const std::uint32_t R = 1u << (k - mloc);
-        double *out = a.partials + (static_cast<std::size_t>(tl) * pl.ntiles + tile) * HSTRIDE;
-        if (bins >= static_cast<std::uint32_t>(NT))
-        {
-            for (std::uint32_t b = threadIdx.x; b < bins; b += NT)
-            {
-                double acc = 0.0;
-                if (!tile_out)
-                {
-                    const std::uint32_t bb = static_cast<std::uint32_t>(pdep64(b, loc_u));
-                    std::uint32_t x = 0;
-                    for (std::uint32_t e = 0; e < R; ++e)
-                    {
-                        const std::uint32_t u = bb | x;
-                        if (!AUX || (local_phys(u) & cdm) == cdv)
-                        {
-                            const float2 z = lds[swz(u)];
-                            acc = fma(static_cast<double>(z.x), static_cast<double>(z.x), acc);
-                            acc = fma(static_cast<double>(z.y), static_cast<double>(z.y), acc);
-                        }
-                        x = next_submask(x, rmask);
-                    }
-                }
-                out[b] = acc;
-            }
-        }
-        else
-        {
-            const std::uint32_t P = NT / bins;  // parts per bin
-            const std::uint32_t b = threadIdx.x / P, part = threadIdx.x % P;
-            const std::uint32_t per = R / P;
-            double acc = 0.0;
-            if (!tile_out)
-            {
-                const std::uint32_t bb = static_cast<std::uint32_t>(pdep64(b, loc_u));
-                std::uint32_t x = static_cast<std::uint32_t>(pdep64(part * per, rmask));
-                for (std::uint32_t e = 0; e < per; ++e)
-                {
-                    const std::uint32_t u = bb | x;
-                    if (!AUX || (local_phys(u) & cdm) == cdv)
-                    {
-                        const float2 z = lds[swz(u)];
-                        acc = fma(static_cast<double>(z.x), static_cast<double>(z.x), acc);
-                        acc = fma(static_cast<double>(z.y), static_cast<double>(z.y), acc);
-                    }
-                    x = next_submask(x, rmask);
-                }
-            }
-            hred[threadIdx.x] = acc;
-            __syncthreads();
-            if (threadIdx.x < bins)
-            {
-                double sum = 0.0;
-                for (std::uint32_t pp = 0; pp < P; ++pp) sum += hred[threadIdx.x * P + pp];
-                out[threadIdx.x] = sum;
-            }
-        }
     }
 
-    // ---- store (M0)
+    // ---- back to M0 through LDS
+    if (!in_m0) lds_exchange<NV>(ldsb, v, T3, Q3, T3m0, Q3m0, lds_dirty);
+
+    // ---- store (M0); before the histogram, so the tile's registers are
+    // free again as its FP64 weights are formed
     if (pl.store)
     {
+        char *tp = reinterpret_cast<char *>(cur.state + cur.B);
 #pragma unroll
-        for (int j = 0; j < NV; ++j)
-            *reinterpret_cast<float2 *>(tp + ((toff + (static_cast<std::uint32_t>(j) << jshift)) << 3)) = v[j];
+        for (int i = 0; i < NV / 2; ++i)
+            *reinterpret_cast<float4 *>(tp + ((toff + (static_cast<std::uint32_t>(i) << jshift)) << 3)) =
+                make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+    }
+    // ---- FP64 histogram partial of this tile over the chunk's local sites
+    if (do_hist)
+    {
+        double *out = a.partials + (static_cast<std::size_t>(cur.tl) * pl.ntiles + cur.tile) * HSTRIDE;
+        if (cur.tile_out)
+        {
+            // the tile lies outside the decided outcomes: every bin is 0
+            const std::uint32_t bins = 1u << __popc(loc_u);
+            for (std::uint32_t bb = threadIdx.x; bb < bins; bb += NT) out[bb] = 0.0;
+        }
+        else
+            tile_histogram<NV, NT>(v, loc_u, AUX && pl.cond, cur.cdm, cur.cdv, lane, wave,
+                                   reinterpret_cast<double *>(lds), out);
+    }
+}
+
+// One sweep over the tiles of a group of trajectories.  AUX = 0: a gate sweep
+// (the hot kernel); AUX = 1: a sweep without gates (the closing collapse, or
+// the histogram of an extra chunk of a layer with more than MAX_HIST_BITS
+// measured sites), kept as its own kernel so profiles separate the two.
+// One workgroup per tile; with the half-tile LDS buffer three 2^13-amplitude
+// workgroups of VALU bond arithmetic share a CU (the matrix-core forms need
+// more registers: two; 64-register tiles one), whose loads overlap one
+// another's LDS stages.
+template <int RB, int NT, int AUX, int MX>
+__global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 3 : 2))) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
+                                                                  const TrajDesc *__restrict__ desc,
+                                                                  const std::uint32_t *__restrict__ flags,
+                                                                  const Collapse *__restrict__ coll,
+                                                                  const int *__restrict__ init,
+                                                                  const float2 *__restrict__ gates)
+{
+    constexpr int NV = 1 << RB;
+    static_assert(RB == 5 || RB == 6, "register tiles hold 32 or 64 amplitudes");
+    const PassPlan &pl = args.plan;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = pl.c, s = pl.s;
+    const std::uint32_t lowmask = (1u << c) - 1u;
+    // M0 element offset of (lane, wave, register j): the thread part toff
+    // (lane bits on u bits 1..6, wave bits above the registers) plus (j & 1)
+    // + ((j >> 1) << jshift) (register bits 1.. on u bits 7..)
+    auto phys = [&](std::uint32_t u) { return (u & lowmask) | ((u >> c) << s); };
+    const std::uint32_t jshift = static_cast<std::uint32_t>(s + 7 - c);
+    const std::uint32_t toff = phys(m0_u(static_cast<std::uint32_t>(lane), static_cast<std::uint32_t>(wave), 0, RB));
+    // the M0 mapping in LDS terms
+    const std::uint32_t T3m0 = swz(m0_u(static_cast<std::uint32_t>(lane), static_cast<std::uint32_t>(wave), 0, RB) >> 1)
+                               << 3;
+    std::uint32_t Q3m0[6];
+    Q3m0[0] = 0;
+#pragma unroll
+    for (int r = 1; r < 6; ++r) Q3m0[r] = swz(m0_u(0, 0, 1u << r, RB) >> 1) << 3;
+
+    const int total = args.ntraj * pl.ntiles;
+    const SweepIn in{traj_list, desc, flags, coll, init, gates};
+    TileCtx x;
+    float2 v[NV];
+    const int id = static_cast<int>(blockIdx.x);
+    if (id < total && tile_context(args, in.traj_list, in.desc, in.flags, in.coll, in.init, in.gates, id, AUX != 0, x))
+    {
+        tile_load<NV>(x, true, args.a.states, toff, jshift, v);
+        process_tile<RB, NT, AUX, MX>(args, in.gates, x, v, toff, jshift, T3m0, Q3m0, lane, wave);
     }
 }
 
@@ -1211,7 +1376,7 @@ namespace
 struct Config
 {
     int rb, nt, c;
-    bool use_mfma = true;  // one bond per stage on the matrix cores
+    int mx = 0;  // bond arithmetic: 0 VALU, 1 VALU + matrix cores, 2 matrix cores
     int k() const { return 6 + rb + (nt == 256 ? 2 : nt == 128 ? 1 : 0); }
 };
 
@@ -1221,19 +1386,21 @@ struct LayerPlan
     std::vector<PassPlan> passes;
 };
 
-int ilog2i(int x)
-{
-    int r = 0;
-    while ((1 << r) < x) ++r;
-    return r;
-}
-
+// The M0 mapping as a stage plan (m0_u): register bit 0 <-> u bit 0,
+// register bit r >= 1 <-> u bit 6 + r, lane bit b <-> u bit b + 1, waves above.
 void set_m0(StagePlan &sp, int rb, int wb)
 {
-    sp.mbond = -1;
-    for (int r = 0; r < 6; ++r) sp.reg_u[r] = static_cast<int>(r < rb ? 6 + r : 0);
-    for (int b = 0; b < 6; ++b) sp.lane_u[b] = static_cast<int>(b);
+    for (int r = 0; r < 6; ++r) sp.reg_u[r] = static_cast<int>(r == 0 ? 0 : (r < rb ? 6 + r : 0));
+    for (int b = 0; b < 6; ++b) sp.lane_u[b] = static_cast<int>(b + 1);
     for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<int>(b < wb ? 6 + rb + b : 0);
+}
+
+// Register bit of u bit x under M0 (-1: a lane or wave bit).
+int m0_reg(int x, int rb)
+{
+    if (x == 0) return 0;
+    if (x >= 7 && x < 6 + rb) return x - 6;
+    return -1;
 }
 
 // Choose `count` lane bits (lane_u[0..count)) among `free` so that their
@@ -1352,7 +1519,6 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
                 best = static_cast<int>(p);
         assigned[best].push_back(b);
     }
-    const int per_stage = cf.rb / 2;
     for (std::size_t p = 0; p < starts.size(); ++p)
     {
         PassPlan pp{};
@@ -1367,101 +1533,92 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
         pp.bonded = 0;
         for (int b : assigned[p]) pp.bonded |= (std::uint64_t{1} << b0[b]) | (std::uint64_t{1} << b1[b]);
         auto u_of = [&](int q) { return q < c ? q : q - s + c; };
-        // M0 stage: bonds already on M0 register pairs, plus (matrix cores)
-        // the bond on u bits (4,5), which M0 puts on lane bits (4,5).  With a
-        // matrix-core bond the VALU pairs must leave the top register bit free.
-        const bool mfma_ok = cf.use_mfma;
+        // M0 stage: bonds already on M0 register slots (1,2) / (3,4) (no LDS
+        // exchange); u bit 0 is register bit 0 in every mapping
         std::vector<int> m0_bonds, other;
-        int m0_m = -1;
+        int m0_slot[2] = {-1, -1};
         for (int b : assigned[p])
         {
-            const int ua = u_of(b0[b]), ub = u_of(b1[b]);
-            if (mfma_ok && m0_m < 0 && ua == 4 && ub == 5)
-            {
-                m0_m = b;
-                continue;
-            }
-            const int ra = ua - 6, rc = ub - 6;
-            if (ra >= 0 && rc == ra + 1 && (ra % 2) == 0 && rc < cf.rb && static_cast<int>(m0_bonds.size()) < per_stage)
-                m0_bonds.push_back(b);
-            else
-                other.push_back(b);
-        }
-        if (m0_m >= 0)
-        {
-            // pairs on the top register bit cannot run beside the MFMAs
-            std::vector<int> keep;
-            for (int b : m0_bonds)
-                (u_of(b1[b]) - 6 >= cf.rb - 1 ? other : keep).push_back(b);
-            m0_bonds.swap(keep);
+            const int ra = m0_reg(u_of(b0[b]), cf.rb), rc = m0_reg(u_of(b1[b]), cf.rb);
+            if (ra == 1 && rc == 2 && m0_slot[0] < 0) m0_slot[0] = b;
+            else if (ra == 3 && rc == 4 && m0_slot[1] < 0) m0_slot[1] = b;
+            else other.push_back(b);
         }
         std::vector<StagePlan> stages;
-        if (!m0_bonds.empty() || m0_m >= 0)
+        if (m0_slot[0] >= 0 || m0_slot[1] >= 0)
         {
             StagePlan sp{};
             set_m0(sp, cf.rb, wb);
             sp.use_lds = 0;
-            sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
-            sp.mbond = m0_m;
-            for (std::size_t i = 0; i < m0_bonds.size(); ++i)
-                sp.bond[(u_of(b0[m0_bonds[i]]) - 6) / 2] = static_cast<int>(m0_bonds[i]);
-            sp.nb = static_cast<int>(m0_bonds.size()) + (m0_m >= 0 ? 1 : 0);
+            sp.zpair = 0;
+            sp.bond[0] = m0_slot[0];
+            sp.bond[1] = m0_slot[1];
+            sp.nb = (m0_slot[0] >= 0) + (m0_slot[1] >= 0);
             stages.push_back(sp);
         }
-        // LDS stages: up to two register-pair bonds plus one matrix-core bond
-        // (or RB/2 register-pair bonds without matrix cores)
+        // LDS stages: two bonds each -- a bond of qubit 0 on bits (0,1) or
+        // (1,0), any other on (1,2), and the second on (3,4) -- with u bit 0
+        // kept on register bit 0
         std::size_t i = 0;
         while (i < other.size())
         {
             StagePlan sp{};
             sp.use_lds = 1;
-            sp.bond[0] = sp.bond[1] = sp.bond[2] = -1;
-            sp.mbond = -1;
-            std::vector<int> used;
-            const std::size_t left = other.size() - i;
-            const bool with_m = mfma_ok && left >= 2;
-            const int nvalu = with_m ? static_cast<int>(std::min<std::size_t>(2, left - 1))
-                                     : static_cast<int>(std::min<std::size_t>(per_stage, left));
+            sp.bond[0] = sp.bond[1] = -1;
+            sp.zpair = 0;
+            for (int r = 0; r < 6; ++r) sp.reg_u[r] = -1;
+            sp.reg_u[0] = 0;
+            std::vector<int> used{0};
             int nbs = 0;
-            for (; nbs < nvalu; ++nbs, ++i)
+            // the first bond of the stage takes slot 0 (with qubit 0 if it has it)
+            for (int slot = 0; slot < 2 && i < other.size(); ++slot)
             {
                 const int b = other[i];
-                sp.reg_u[2 * nbs] = static_cast<int>(u_of(b0[b]));
-                sp.reg_u[2 * nbs + 1] = static_cast<int>(u_of(b1[b]));
-                used.push_back(u_of(b0[b]));
-                used.push_back(u_of(b1[b]));
-                sp.bond[nbs] = static_cast<int>(b);
+                const int ua = u_of(b0[b]), ub = u_of(b1[b]);
+                const bool has0 = ua == 0 || ub == 0;
+                if (slot == 1 && has0) break;  // a bond of qubit 0 needs slot 0
+                ++i;
+                ++nbs;
+                if (slot == 0)
+                {
+                    sp.bond[0] = b;
+                    if (has0)
+                    {
+                        sp.zpair = ua == 0 ? 1 : 2;  // q0 on bit 0 : q1 on bit 0
+                        sp.reg_u[1] = ua == 0 ? ub : ua;
+                        used.push_back(sp.reg_u[1]);
+                    }
+                    else
+                    {
+                        sp.reg_u[1] = ua;
+                        sp.reg_u[2] = ub;
+                        used.push_back(ua);
+                        used.push_back(ub);
+                    }
+                }
+                else
+                {
+                    sp.bond[1] = b;
+                    sp.reg_u[3] = ua;
+                    sp.reg_u[4] = ub;
+                    used.push_back(ua);
+                    used.push_back(ub);
+                }
             }
-            int lane45[2] = {-1, -1};
-            if (with_m)
-            {
-                const int b = other[i++];
-                sp.mbond = b;
-                lane45[0] = u_of(b0[b]);
-                lane45[1] = u_of(b1[b]);
-                used.push_back(lane45[0]);
-                used.push_back(lane45[1]);
-            }
-            sp.nb = nbs + (with_m ? 1 : 0);
+            sp.nb = nbs;
             std::vector<int> freeb;
             for (int u = 0; u < k; ++u)
                 if (std::find(used.begin(), used.end(), u) == used.end()) freeb.push_back(u);
             // fill the remaining register bits first from high u bits (keeps
             // lanes on bank-friendly bits), then lanes, then waves
-            int r = 2 * nbs;
-            while (r < cf.rb)
-            {
-                const int x = freeb.back();
-                freeb.pop_back();
-                sp.reg_u[r++] = static_cast<int>(x);
-            }
+            for (int r = 1; r < cf.rb; ++r)
+                if (sp.reg_u[r] < 0)
+                {
+                    sp.reg_u[r] = freeb.back();
+                    freeb.pop_back();
+                }
             for (int rr = cf.rb; rr < 6; ++rr) sp.reg_u[rr] = 0;
-            pick_lanes(freeb, sp.lane_u, with_m ? 4 : 6);
-            if (with_m)
-            {
-                sp.lane_u[4] = lane45[0];
-                sp.lane_u[5] = lane45[1];
-            }
+            pick_lanes(freeb, sp.lane_u, 6);
             for (int b = 0; b < 4; ++b) sp.wave_u[b] = static_cast<int>(b < wb ? freeb[b] : 0);
             stages.push_back(sp);
         }
@@ -1528,6 +1685,7 @@ struct Batch
 
     // Sampled HIP-event timing of the gate sweeps (pass kernels).
     int prof_every = 0;
+    int streams = 2;  // tiled groups in flight (1: every group on the batch stream)
     long prof_counter = 0;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, double>> ev_used;  // (event pair index, algorithmic bytes)
@@ -1614,12 +1772,8 @@ struct Batch
         if (cfg.tile_bits == 14) cf = Config{6, 256, 6};
         if (cfg.tile_bits == 13) cf = cfg.threads == 128 ? Config{6, 128, 5} : Config{5, 256, 5};
         if (cfg.tile_bits == 12) cf = Config{6, 64, 4};
-        // MIPT_HIP_MFMA_GATES=0 keeps every bond on the VALU
-        static const bool mfma = [] {
-            const char *e = std::getenv("MIPT_HIP_MFMA_GATES");
-            return !(e && e[0] == '0');
-        }();
-        cf.use_mfma = mfma && cf.rb == 5;  // 64-register tiles leave no room for the MFMA operands
+        // 64-register tiles are built with the split arithmetic only
+        if (cfg.bond_arith > 0 && cf.rb == 5) cf.mx = cfg.bond_arith - 1;
         return cf;
     }
 
@@ -1854,35 +2008,46 @@ struct Batch
         return OK;
     }
 
-    template <int RB, int NT, int AUX>
+    template <int RB, int NT, int AUX, int MX>
     int launch_pass(const PassPlan &pl, const int *list, int ntraj, int layer)
     {
-        PassArgs pa{al, list, layer, pl};
-        const std::size_t lds = (std::size_t{1} << pl.k) * sizeof(float2) + NT * sizeof(double);
+        PassArgs pa{al, list, ntraj, layer, pl};
+        const std::size_t lds = (std::size_t{1} << (pl.k - 1)) * sizeof(float2);  // half a tile
         static bool attr_set = false;
         if (!attr_set)
         {
-            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX>),
+            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               static_cast<int>((std::size_t{1} << 14) * sizeof(float2) + 256 * sizeof(double))));
+                                               static_cast<int>((std::size_t{1} << 13) * sizeof(float2))));
             attr_set = true;
         }
-        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX>), dim3(static_cast<unsigned>(ntraj * pl.ntiles)), dim3(NT), lds,
-                           ls, pa);
+        const unsigned grid = static_cast<unsigned>(static_cast<long>(ntraj) * pl.ntiles);
+        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX>), dim3(grid), dim3(NT), lds, ls, pa,
+                           static_cast<const int *>(list), static_cast<const TrajDesc *>(al.desc),
+                           static_cast<const std::uint32_t *>(al.flags), static_cast<const Collapse *>(al.coll),
+                           static_cast<const int *>(al.init), static_cast<const float2 *>(al.gates));
         MIPT_HIP_CHECK(hipGetLastError());
         return OK;
     }
 
+    // Stage-free sweeps (AUX) have no bond arithmetic; gate sweeps of 2^13
+    // tiles come in all three arithmetic mixes, the 64-register shapes on the
+    // VALU (the matrix-core operands do not fit beside 64 amplitudes).
     int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer)
     {
         const bool aux = pl.nstages == 0;
         if (cf.nt == 256 && cf.rb == 5)
-            return aux ? launch_pass<5, 256, 1>(pl, list, ntraj, layer) : launch_pass<5, 256, 0>(pl, list, ntraj, layer);
+        {
+            if (aux) return launch_pass<5, 256, 1, 0>(pl, list, ntraj, layer);
+            if (cf.mx == 0) return launch_pass<5, 256, 0, 0>(pl, list, ntraj, layer);
+            if (cf.mx == 2) return launch_pass<5, 256, 0, 2>(pl, list, ntraj, layer);
+            return launch_pass<5, 256, 0, 1>(pl, list, ntraj, layer);
+        }
         if (cf.nt == 256)
-            return aux ? launch_pass<6, 256, 1>(pl, list, ntraj, layer) : launch_pass<6, 256, 0>(pl, list, ntraj, layer);
+            return aux ? launch_pass<6, 256, 1, 0>(pl, list, ntraj, layer) : launch_pass<6, 256, 0, 0>(pl, list, ntraj, layer);
         if (cf.nt == 128)
-            return aux ? launch_pass<6, 128, 1>(pl, list, ntraj, layer) : launch_pass<6, 128, 0>(pl, list, ntraj, layer);
-        return aux ? launch_pass<6, 64, 1>(pl, list, ntraj, layer) : launch_pass<6, 64, 0>(pl, list, ntraj, layer);
+            return aux ? launch_pass<6, 128, 1, 0>(pl, list, ntraj, layer) : launch_pass<6, 128, 0, 0>(pl, list, ntraj, layer);
+        return aux ? launch_pass<6, 64, 1, 0>(pl, list, ntraj, layer) : launch_pass<6, 64, 0, 0>(pl, list, ntraj, layer);
     }
 
     // Tiled evolution of a group of same-size trajectories (nq > 14).
@@ -2088,7 +2253,7 @@ struct Batch
                 {
                     const int qe = std::min(end, q + g);
                     std::vector<int> grp(order.begin() + q, order.begin() + qe);
-                    MIPT_HIP_TRY(evolve_group_tiled(grp, q, t0, t1, tiled_groups & 1));
+                    MIPT_HIP_TRY(evolve_group_tiled(grp, q, t0, t1, streams > 1 ? (tiled_groups & 1) : 0));
                     ++tiled_groups;
                 }
             }
@@ -2134,8 +2299,8 @@ struct Batch
             }
         if (best_index) *best_index = -1;
         if (nqs.empty() || depth < 1) return OK;  // nothing tile-swept
-        const Config cand[] = {Config{5, 256, 5, true}, Config{5, 256, 5, false}, Config{6, 256, 6, false},
-                               Config{6, 128, 5, false}, Config{6, 64, 4, false}};
+        const Config cand[AUTOTUNE_CANDIDATES] = {Config{5, 256, 5, 0}, Config{5, 256, 5, 1}, Config{5, 256, 5, 2},
+                                                  Config{6, 256, 6, 0}, Config{6, 128, 5, 0}, Config{6, 64, 4, 0}};
         const int ncand = static_cast<int>(sizeof(cand) / sizeof(cand[0]));
         drop_graph();
         hipEvent_t e0, e1;
@@ -2298,6 +2463,13 @@ int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *stream, mip
     {
         delete b;
         set_error("batch_create: tile_bits must be 0 or 12..14");
+        return E_ARG;
+    }
+    if (b->cfg.bond_arith < 0 || b->cfg.bond_arith > 3 ||
+        (b->cfg.bond_arith >= 2 && ((b->cfg.tile_bits != 0 && b->cfg.tile_bits != 13) || b->cfg.threads == 128)))
+    {
+        delete b;
+        set_error("batch_create: bond_arith must be 0..3; 2 and 3 need 2^13 tiles of 256 threads");
         return E_ARG;
     }
     if (stream) b->stream = static_cast<hipStream_t>(stream);
@@ -2579,6 +2751,18 @@ int mipt_hip_batch_export_layers(mipt_hip_batch *b, int traj, double *g, uint8_t
 {
     if (!b) { set_error("batch_export_layers: null batch"); return E_ARG; }
     return b->export_layers(traj, g, f, u);
+}
+
+int mipt_hip_batch_set_streams(mipt_hip_batch *b, int streams)
+{
+    if (!b || (streams != 1 && streams != 2))
+    {
+        set_error("batch_set_streams: streams must be 1 or 2");
+        return E_ARG;
+    }
+    b->drop_graph();
+    b->streams = streams;
+    return OK;
 }
 
 int mipt_hip_batch_set_profiling(mipt_hip_batch *b, int sample_every)

@@ -594,7 +594,7 @@ int run_mipt(int argc, char **argv)
                                           0, -1, d == 1 ? 0 : n};
         }
         mipt_hip_batch *b = nullptr;
-        const mipt_hip_batch_config cfg{0, 0, 0};
+        const mipt_hip_batch_config cfg{0, 0, 0, 0};
         check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
         int st = mipt_hip_batch_setup(b, specs.data(), count);
         if (!st) st = mipt_hip_batch_generate(b);
@@ -702,7 +702,7 @@ int run_entropy(int argc, char **argv)
             specs[i] = mipt_hip_traj_spec{n, 2 * periods, p, trajectory_seed(master, static_cast<std::uint64_t>(first + i)),
                                           1, 1, -1, 0, -1, 0};
         mipt_hip_batch *b = nullptr;
-        const mipt_hip_batch_config cfg{0, 0, 0};
+        const mipt_hip_batch_config cfg{0, 0, 0, 0};
         check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
         std::vector<double> v1(static_cast<std::size_t>(count) * sizes), v2(v1.size());
         int st = mipt_hip_batch_setup(b, specs.data(), count);
@@ -917,7 +917,7 @@ int run_probed_triangles(int argc, char **argv, int n, int realizations, int cir
             s.attach_layer = t_eq;
         }
         mipt_hip_batch *b = nullptr;
-        const mipt_hip_batch_config cfg{0, 0, 0};
+        const mipt_hip_batch_config cfg{0, 0, 0, 0};
         check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
         int st = mipt_hip_batch_setup(b, specs.data(), count);
         if (!st) st = mipt_hip_batch_generate(b);
@@ -1059,7 +1059,7 @@ int run_probed_mode4(int argc, char **argv, int probes, int n, int realizations,
             s.attach_layer = t_eq;
         }
         mipt_hip_batch *b = nullptr;
-        const mipt_hip_batch_config cfg{0, 0, 0};
+        const mipt_hip_batch_config cfg{0, 0, 0, 0};
         check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
         int st = mipt_hip_batch_setup(b, specs.data(), count);
         if (!st) st = mipt_hip_batch_generate(b);
@@ -1290,7 +1290,7 @@ int run_probed(int argc, char **argv)
                 s.attach_stagger = delta_t;
             }
             mipt_hip_batch *b = nullptr;
-            const mipt_hip_batch_config cfg{0, 0, 0};
+            const mipt_hip_batch_config cfg{0, 0, 0, 0};
             check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
             int st = mipt_hip_batch_setup(b, specs.data(), count);
             if (!st) st = mipt_hip_batch_generate(b);
@@ -1581,7 +1581,7 @@ int run_free_energy(int argc, char **argv)
             specs[i].shared_gates = -1;
         }
         mipt_hip_batch *b = nullptr;
-        const mipt_hip_batch_config cfg{0, 0, 0};
+        const mipt_hip_batch_config cfg{0, 0, 0, 0};
         check(mipt_hip_batch_create(&cfg, nullptr, &b), "batch_create");
         std::vector<double> sh(count * (rec_t.size() + 1), 0.0);
         int st = mipt_hip_batch_setup(b, specs.data(), count);
@@ -1706,7 +1706,7 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
     mipt_hip_batch *b = nullptr;
     mipt_hip_engine *eng = nullptr;
     int st = 0;
-    const mipt_hip_batch_config cfg{0, 0, 0};
+    const mipt_hip_batch_config cfg{0, 0, 0, 0};
     if ((st = mipt_hip_batch_create(&cfg, hip_stream, &b)) != 0) return st;
     do
     {

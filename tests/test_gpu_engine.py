@@ -220,10 +220,10 @@ def test_plain_joint_mode_matches_oracle(M):
 
 # ------------------------------------------------------------------ batched engine
 
-def replay_batch(M, cases, tile_bits=0, threads=0):
+def replay_batch(M, cases, tile_bits=0, threads=0, arith=0):
     """cases: list of (n, layers, p, seed, closed); returns (batch, oracle results)."""
     specs = [M.Spec(n, layers, p, seed, closed) for (n, layers, p, seed, closed) in cases]
-    b = M.Batch(specs, tile_bits=tile_bits, threads=threads)
+    b = M.Batch(specs, tile_bits=tile_bits, threads=threads, bond_arith=arith)
     refs = []
     for i, (n, layers, p, seed, closed) in enumerate(cases):
         psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed, closed)
@@ -234,12 +234,13 @@ def replay_batch(M, cases, tile_bits=0, threads=0):
     return b, refs
 
 
-@pytest.mark.parametrize("tile_bits,threads", [(0, 0), (14, 0), (13, 0), (13, 128), (12, 0)])
-def test_batch_replay_mixed_lengths(M, tile_bits, threads):
+@pytest.mark.parametrize("tile_bits,threads,arith", [(0, 0, 0), (14, 0, 0), (13, 0, 1), (13, 0, 2), (13, 0, 3),
+                                                     (13, 128, 0), (12, 0, 0)])
+def test_batch_replay_mixed_lengths(M, tile_bits, threads, arith):
     cases = [(4, 8, 0.3, 1, True), (8, 12, 0.2, 2, True), (12, 14, 0.16, 3, False), (14, 10, 0.3, 4, True),
              (16, 8, 0.2, 5, True), (16, 9, 0.5, 6, False), (18, 6, 0.3, 7, True), (20, 5, 0.25, 8, True)]
-    print(f"batch replay tile_bits={tile_bits} threads={threads}", flush=True)
-    b, refs = replay_batch(M, cases, tile_bits, threads)
+    print(f"batch replay tile_bits={tile_bits} threads={threads} bond_arith={arith}", flush=True)
+    b, refs = replay_batch(M, cases, tile_bits, threads, arith)
     for i, (psi, out, p1, f, u) in enumerate(refs):
         got, flagged, gp1, _ = b.outcomes(i)
         check_outcomes(got, out, p1, u, f, f"traj {i} L={cases[i][0]}")
@@ -427,7 +428,7 @@ def test_batch_autotune_then_evolve_matches_oracle(M):
     b.generate()
     best, ms = b.autotune(3)
     print("autotune choice", best, "ms", ms, flush=True)
-    assert 0 <= best < 5 and ms[best] > 0
+    assert 0 <= best < 6 and ms[best] > 0
     b.evolve()
     b.synchronize()
     for i, (n, layers, p, seed) in enumerate(cases):
