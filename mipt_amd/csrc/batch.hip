@@ -1176,15 +1176,34 @@ __device__ __forceinline__ void tile_load(const TileCtx &x, bool have, const flo
     }
 }
 
+// The thread's index in its workgroup, as a value the compiler cannot tie to
+// an earlier computation: per-lane terms derived from it are recomputed where
+// they are needed instead of held in registers across the LDS stages.
+__device__ __forceinline__ std::uint32_t fresh_tid()
+{
+    std::uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // Processes one tile whose amplitudes are in v (M0 mapping): the initial
 // state or the previous layer's collapse scale, the stages of the sweep plan,
 // back to M0, the store, and the FP64 histogram partial.
 template <int RB, int NT, int AUX, int MX>
 __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 *__restrict__ gates, const TileCtx &cur,
-                                             float2 (&v)[1 << RB],
-                                             std::uint32_t toff, std::uint32_t jshift, std::uint32_t T3m0,
-                                             const std::uint32_t (&Q3m0)[6], int lane, int wave)
+                                             float2 (&v)[1 << RB], std::uint32_t jshift,
+                                             const std::uint32_t (&Q3m0)[6])
 {
+    const std::uint32_t lowmask = (1u << args.plan.c) - 1u;
+    // per-lane terms of the M0 mapping, from a fresh thread index at each use
+    auto m0_terms = [&](std::uint32_t &toff, std::uint32_t &T3m0, int &lane, int &wave) {
+        const std::uint32_t t = fresh_tid();
+        lane = static_cast<int>(t & 63u);
+        wave = static_cast<int>(t >> 6);
+        const std::uint32_t u = m0_u(t & 63u, t >> 6, 0, RB);
+        toff = (u & lowmask) | ((u >> args.plan.c) << args.plan.s);
+        T3m0 = swz(u >> 1) << 3;
+    };
     constexpr int NV = 1 << RB;
     constexpr int WB = (NT == 256) ? 2 : (NT == 128 ? 1 : 0);
     extern __shared__ float2 lds[];
@@ -1194,6 +1213,9 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     const int c = pl.c, s = pl.s;
     char *ldsb = reinterpret_cast<char *>(lds);
     // ---- initial state, or the collapse scale of the current tile
+    std::uint32_t toff, T3m0;
+    int lane, wave;
+    m0_terms(toff, T3m0, lane, wave);
     if (cur.gen_init)
     {
         const TrajDesc d = a.desc[cur.traj];
@@ -1227,7 +1249,9 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
         const StagePlan &sp = pl.st[si];
         if (sp.use_lds)
         {
-            const std::uint32_t T3n = thread_term<WB>(sp.lane_u, sp.wave_u, lane, wave) << 3;
+            const std::uint32_t tt = fresh_tid();
+            const std::uint32_t T3n =
+                thread_term<WB>(sp.lane_u, sp.wave_u, static_cast<int>(tt & 63u), static_cast<int>(tt >> 6)) << 3;
             std::uint32_t Q3n[6];
             Q3n[0] = 0;
 #pragma unroll
@@ -1262,6 +1286,7 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     }
 
     // ---- back to M0 through LDS
+    m0_terms(toff, T3m0, lane, wave);
     if (!in_m0) lds_exchange<NV>(ldsb, v, T3, Q3, T3m0, Q3m0, lds_dirty);
 
     // ---- store (M0); before the histogram, so the tile's registers are
@@ -1299,7 +1324,7 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
 // more registers: two; 64-register tiles one), whose loads overlap one
 // another's LDS stages.
 template <int RB, int NT, int AUX, int MX>
-__global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 3 : 2))) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
+__global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 4 : 3))) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
                                                                   const TrajDesc *__restrict__ desc,
                                                                   const std::uint32_t *__restrict__ flags,
                                                                   const Collapse *__restrict__ coll,
@@ -1319,9 +1344,7 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 3 : 2))) pass_ke
     auto phys = [&](std::uint32_t u) { return (u & lowmask) | ((u >> c) << s); };
     const std::uint32_t jshift = static_cast<std::uint32_t>(s + 7 - c);
     const std::uint32_t toff = phys(m0_u(static_cast<std::uint32_t>(lane), static_cast<std::uint32_t>(wave), 0, RB));
-    // the M0 mapping in LDS terms
-    const std::uint32_t T3m0 = swz(m0_u(static_cast<std::uint32_t>(lane), static_cast<std::uint32_t>(wave), 0, RB) >> 1)
-                               << 3;
+    // the M0 mapping's uniform LDS terms (its per-lane term is formed where used)
     std::uint32_t Q3m0[6];
     Q3m0[0] = 0;
 #pragma unroll
@@ -1335,7 +1358,7 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 3 : 2))) pass_ke
     if (id < total && tile_context(args, in.traj_list, in.desc, in.flags, in.coll, in.init, in.gates, id, AUX != 0, x))
     {
         tile_load<NV>(x, true, args.a.states, toff, jshift, v);
-        process_tile<RB, NT, AUX, MX>(args, in.gates, x, v, toff, jshift, T3m0, Q3m0, lane, wave);
+        process_tile<RB, NT, AUX, MX>(args, in.gates, x, v, jshift, Q3m0);
     }
 }
 
