@@ -49,7 +49,22 @@ def hip_runtime():
     h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
     h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    h.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     return h
+
+
+def host_cpu():
+    """nproc, the cores this process may use, and the CPU model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "model": model}
 
 
 def linspace(first, last, count):
@@ -94,7 +109,8 @@ def main(argv=None):
     ap.add_argument("--per-p", type=int, default=256)
     ap.add_argument("--np", type=int, default=16)
     ap.add_argument("--plain", type=int, default=4, help="trajectories timed on the plain path")
-    ap.add_argument("--replay", type=int, default=8, help="trajectories checked against the FP64 oracle")
+    ap.add_argument("--replay", type=int, default=16,
+                    help="trajectories checked against the FP64 oracle (one per host thread, up to 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg (profiling runs)")
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile configuration")
@@ -113,6 +129,7 @@ def main(argv=None):
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-length / 2D-grid leg")
     ap.add_argument("--no-observables", action="store_true", help="skip the S/TMI readout timing")
     ap.add_argument("--no-c1", action="store_true", help="skip the L=12 open-chain (CPU-path) workload")
+    ap.add_argument("--no-c4", action="store_true", help="skip the L=26 x 64 workload")
     ap.add_argument("--obs-traj", type=int, default=256)
     ap.add_argument("--uneven-per", type=int, default=512, help="trajectories per (L, p) in the uneven leg")
     ap.add_argument("--decoder-train", type=int, default=200_000)
@@ -259,7 +276,8 @@ def main(argv=None):
     # ---- oracle leg: CPU baseline + S/TMI/outcome parity on a replayed subset
     if not args.no_cpu and args.replay > 0:
         idx = [int(i * ntraj / args.replay) for i in range(args.replay)]
-        cores = max(1, min(len(idx), len(os.sched_getaffinity(0)), 16))
+        cpu = host_cpu()
+        cores = max(1, min(len(idx), cpu["usable_cores"], 16))
         jobs = [(L, layers, specs[i].p, specs[i].seed) for i in idx]
         from oracle import oracle as O
         O.lib()
@@ -287,12 +305,14 @@ def main(argv=None):
                 amp_err = max(amp_err, float(np.abs(b.get_state(i).astype(np.complex128) - psi).max()))
         result["cpu_baseline"] = {"value": round(cpu_lps, 2), "unit": "layers/s", "cores": cores, "kind": "port",
                                   "sample": f"{len(idx)} trajectories of the workload (L={L}, T={layers}) evolved "
-                                            f"in complex128 by the oracle, one thread per trajectory"}
+                                            f"in complex128 by the oracle, one host thread per trajectory",
+                                  "nproc": cpu["nproc"], "usable_cores": cpu["usable_cores"], "cpu_model": cpu["model"]}
         result["parity"] = {"replayed": len(idx), "outcome_mismatches": mism, "flagged_draws": flagged_n,
                             "s_half_err_max": s_err, "tmi_err_max": tmi_err, "amp_err_max": amp_err}
         result["batched_vs_cpu"] = round(value / cpu_lps, 1)
     else:
         result["cpu_baseline"] = None
+    b.close()  # the legs below need the device memory
     if args.rdm_L:
         result["rdm_gemm"] = rdm_timing(M, args.rdm_L, args.rdm_reps)
     if not args.no_cpu and not args.no_c1:
@@ -301,6 +321,12 @@ def main(argv=None):
         except Exception as e:
             result["c1"] = {"error": repr(e)}
             log(f"C1 leg failed: {e!r}")
+    if not args.no_c4:
+        try:
+            result["c4"] = c4_leg(M, hip)
+        except Exception as e:
+            result["c4"] = {"error": repr(e)}
+            log(f"C4 leg failed: {e!r}")
     if not args.no_observables:
         try:
             result["observables"] = observables_leg(M, L, layers, ps, args.obs_traj)
@@ -433,30 +459,104 @@ def _timed_evolve(M, hip, specs):
 
 
 def uneven_leg(M, hip, per):
-    """Uneven-size workload: chains L in {14,...,24} with T = 2L and p in
-    {0.10, 0.16, 0.22}, `per` trajectories per (L, p), one batch per L (the
-    engine groups by length inside a batch; one batch per L bounds memory at
-    L=24); plus the 4 x 5 grid (L=20, four bond colours, open) at p = 0.12,
-    1024 trajectories, T = 40.  Throughput in layer-trajectories/s."""
-    out = {"per_L": [], "workload": f"L in 14..24, T=2L, p in (0.10,0.16,0.22) x {per}; 4x5 grid p=0.12 x 1024, T=40"}
-    tot_ms = tot_work = 0.0
+    """Uneven-size workload in ONE batch: chains L in {14,...,24} with T = 2L
+    and p in {0.10, 0.16, 0.22}, `per` trajectories per (L, p), interleaved in
+    the spec list (one evolve call; the engine groups same-length trajectories
+    into tile sweeps and keeps two groups in flight); the state memory is
+    checked against the device's free memory first and reported.  Plus the
+    4 x 5 grid (L=20, four bond colours, open) at p = 0.12, 1024 trajectories,
+    T = 40.  Throughput in layer-trajectories/s."""
+    out = {"workload": f"one batch: L in 14..24 interleaved, T=2L, p in (0.10,0.16,0.22) x {per}; "
+                       f"4x5 grid p=0.12 x 1024, T=40"}
+    specs = []
     seed = 50_000_000
-    for L in range(14, 26, 2):
-        specs = []
+    for r in range(per):
         for p in (0.10, 0.16, 0.22):
-            for r in range(per):
+            for L in range(14, 26, 2):
                 seed += 1
                 specs.append(M.Spec(L, 2 * L, p, seed, True))
-        ms, work = _timed_evolve(M, hip, specs)
-        tot_ms += ms
-        tot_work += work
-        out["per_L"].append({"L": L, "trajectories": len(specs), "layers_per_s": round(work / (ms * 1e-3), 1)})
-        log(f"uneven leg: L={L} {len(specs)} trajectories, {work / (ms * 1e-3):.0f} layers/s")
-    out["chains_layers_per_s"] = round(tot_work / (tot_ms * 1e-3), 1)
+    state_bytes = sum(8 << s.L for s in specs)
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total))
+    out["state_GiB"] = round(state_bytes / 2**30, 2)
+    out["device_free_GiB"] = round(free.value / 2**30, 2)
+    if state_bytes > 0.97 * free.value:
+        # keep every length and the interleaving; scale the count per (L, p)
+        scale = 0.97 * free.value / state_bytes
+        keep = max(1, int(per * scale))
+        specs = [sp for i, sp in enumerate(specs) if (i // 18) < keep]
+        out["per_L_p_run"] = keep
+        out["state_GiB"] = round(sum(8 << s.L for s in specs) / 2**30, 2)
+        log(f"uneven leg: {state_bytes / 2**30:.1f} GiB of states exceeds free memory, running {keep} per (L, p)")
+    else:
+        out["per_L_p_run"] = per
+    log(f"uneven leg: {len(specs)} trajectories in one batch, {out['state_GiB']} GiB of states")
+    ms, work = _timed_evolve(M, hip, specs)
+    out["trajectories"] = len(specs)
+    out["chains_layers_per_s"] = round(work / (ms * 1e-3), 1)
+    log(f"uneven leg: mixed lengths {work / (ms * 1e-3):.0f} layers/s")
     specs = [M.Spec(20, 40, 0.12, 60_000_000 + i, False, grid_x=5) for i in range(1024)]
     ms, work = _timed_evolve(M, hip, specs)
     out["grid_4x5_layers_per_s"] = round(work / (ms * 1e-3), 1)
     log(f"uneven leg: 4x5 grid {work / (ms * 1e-3):.0f} layers/s")
+    return out
+
+
+def c4_leg(M, hip, n=64, L=26, layers=52, p=0.16):
+    """The large single-size workload: L=26 ring (512 MiB complex64 states),
+    T=52, p=0.16, 64 trajectories.  Evolution throughput (layer-trajectories/s
+    over HIP events on the batch stream), the gate sweeps' per-launch
+    algorithmic HBM GB/s on one stream, and the 8192 x 8192 half-chain RDM
+    Gram of one trajectory: kernel time and algorithmic FLOPs (lower triangle)
+    against the FP64 (default) and FP32 MFMA peaks."""
+    specs = [M.Spec(L, layers, p, 90_000_000 + i, True) for i in range(n)]
+    b = M.Batch(specs)
+    stream = ctypes.c_void_p(b.stream)
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    b.generate()
+    b.synchronize()
+    hip.hipEventRecord(e0, stream)
+    b.evolve()
+    hip.hipEventRecord(e1, stream)
+    hip.hipEventSynchronize(e1)
+    b.synchronize()
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    lps = n * layers / (ms.value * 1e-3)
+    log(f"C4: {n} x L={L} x {layers} layers in {ms.value:.0f} ms, {lps:.0f} layers/s")
+    # per-launch sweep timing on one stream (an extra run, outside the timed one)
+    b.reset()
+    b.set_streams(1)
+    b.set_profiling(5)
+    b.evolve()
+    b.synchronize()
+    st = b.pass_stats()
+    sweep_gbs = st["bytes_avg"] / (st["avg_ms"] * 1e-3) / 1e9 if st["avg_ms"] > 0 else 0.0
+    out = {"workload": f"L={L} ring, T={layers}, p={p}, {n} trajectories, complex64 (512 MiB states)",
+           "layers_per_s": round(lps, 1), "evolve_ms": round(ms.value, 1),
+           "sweep": {"achieved_GBps": round(sweep_gbs, 1), "peak_GBps": HBM_PEAK_GBS,
+                     "frac": round(sweep_gbs / HBM_PEAK_GBS, 4), "avg_launch_ms": round(st["avg_ms"], 4),
+                     "timed_launches": st["timed"], "algorithmic_bytes_per_launch": round(st["bytes_avg"])}}
+    # half-chain Gram (8192 x 8192 from an 8192 x 8192 reshape), FP64 then FP32 MFMA
+    ptr = b.state_ptr(0)
+    kept = list(range(L // 2))
+    grams = {}
+    for bits, peak in ((64, 78.6e12), (32, 157.3e12)):
+        M.set_gram_precision(bits)
+        M.set_gram_profiling(True)
+        s_half = M.entropy(ptr, L, kept)
+        g = M.gram_stats()
+        M.set_gram_profiling(False)
+        tf = g["flops_avg"] / (g["avg_ms"] * 1e-3) if g["avg_ms"] > 0 else 0.0
+        grams[f"fp{bits}_mfma"] = {"kernel_ms": round(g["avg_ms"], 3), "flop": g["flops_avg"],
+                                   "achieved_TFLOPs": round(tf / 1e12, 2), "peak_TFLOPs": peak / 1e12,
+                                   "frac": round(tf / peak, 4), "s_half_bits": s_half}
+        log(f"C4: half-chain Gram FP{bits} MFMA {g['avg_ms']:.1f} ms, {tf / 1e12:.1f} TFLOP/s, S(13)={s_half:.6f}")
+    M.set_gram_precision(64)
+    out["rdm_gram"] = grams
+    b.close()
     return out
 
 

@@ -71,6 +71,14 @@ int mipt_hip_entropy_svd_f64(const void *device_state_vector, int n_qubits, cons
  * MIPT_HIP_GRAM_PRECISION.  -1 for any other value. */
 int mipt_hip_set_gram_precision(int bits);
 
+/* Measurement helpers (no counterpart in the reference): time every Gram
+ * kernel of the complex64 RDM path with HIP events on its stream (enable 1;
+ * 0 stops; either clears earlier timings), then the average kernel duration
+ * in ms, how many were timed and their average algorithmic FLOPs
+ * (4 m (m + 1) K for the lower triangle of an m x K complex M M^H). */
+int mipt_hip_set_gram_profiling(int enable);
+int mipt_hip_gram_stats(double *avg_ms, int *count, double *flops_avg);
+
 /* Replace mipt_cuda_renyi2_f32/_f64 (tmi_cuda_svd.hpp:36-58): S_2 = -log2 Tr rho^2 (bits). */
 int mipt_hip_renyi2_f32(const void *device_state_vector, int n_qubits, const int *host_kept_modes, int kept_count,
                         int fermion_trace, double *entropy_out);

@@ -61,6 +61,8 @@ def lib():
         getattr(L, name).argtypes = [vp, i, i, dp]
     L.mipt_hip_tmi_f32.argtypes = [vp, i, i, i, i, dp, dp]
     L.mipt_hip_set_gram_precision.argtypes = [i]
+    L.mipt_hip_set_gram_profiling.argtypes = [i]
+    L.mipt_hip_gram_stats.argtypes = [dp, ip, dp]
     L.mipt_hip_engine_create.argtypes = [i, vp, ctypes.POINTER(vp)]
     L.mipt_hip_engine_destroy.argtypes = [vp]
     L.mipt_hip_engine_adopt.argtypes = [vp, vp, i]
@@ -136,6 +138,18 @@ def _dp(a):
 
 
 # ------------------------------------------------------------------ drop-ins
+def set_gram_profiling(enable: bool) -> None:
+    """Time every Gram kernel of the complex64 RDM path (clears earlier timings)."""
+    _check(lib().mipt_hip_set_gram_profiling(int(enable)), "set_gram_profiling")
+
+
+def gram_stats() -> dict:
+    """Average Gram kernel ms, count timed, average algorithmic FLOPs."""
+    ms, n, fl = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().mipt_hip_gram_stats(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl)), "gram_stats")
+    return {"avg_ms": ms.value, "timed": n.value, "flops_avg": fl.value}
+
+
 def set_gram_precision(bits: int) -> None:
     """Gram arithmetic of complex64 RDMs: 64 (FP64 MFMA, default) or 32 (FP32 MFMA)."""
     _check(lib().mipt_hip_set_gram_precision(int(bits)), "set_gram_precision")

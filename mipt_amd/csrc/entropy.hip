@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -415,6 +416,47 @@ int gram_precision(int &bits)
     return OK;
 }
 
+// Optional timing of the Gram kernel (HIP events on the calling stream): its
+// average duration and algorithmic FLOPs, 4 m (m + 1) K for the lower
+// triangle of M M^H (complex, as real products), for the MFMA roofline.
+struct GramTiming
+{
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<double> flops;
+};
+GramTiming &gram_timing()
+{
+    static GramTiming t;
+    return t;
+}
+
+int gram_timing_begin(hipStream_t stream, double flops, int &slot)
+{
+    GramTiming &g = gram_timing();
+    std::lock_guard<std::mutex> lock(g.mu);
+    slot = -1;
+    if (!g.on) return OK;
+    hipEvent_t a, b;
+    MIPT_HIP_CHECK(hipEventCreate(&a));
+    MIPT_HIP_CHECK(hipEventCreate(&b));
+    MIPT_HIP_CHECK(hipEventRecord(a, stream));
+    g.ev.emplace_back(a, b);
+    g.flops.push_back(flops);
+    slot = static_cast<int>(g.ev.size()) - 1;
+    return OK;
+}
+
+int gram_timing_end(hipStream_t stream, int slot)
+{
+    if (slot < 0) return OK;
+    GramTiming &g = gram_timing();
+    std::lock_guard<std::mutex> lock(g.mu);
+    MIPT_HIP_CHECK(hipEventRecord(g.ev[slot].second, stream));
+    return OK;
+}
+
 // rho (FP64, device, row-major m x m) of the kept modes of psi.
 int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int &m_out,
               double2 *dst = nullptr)
@@ -464,6 +506,8 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
         MIPT_HIP_TRY(grow(w.part, w.part_cap, static_cast<std::size_t>(ksplit) * ntri * GT * GT));
         int gbits = 64;
         MIPT_HIP_TRY(gram_precision(gbits));
+        int slot = -1;
+        MIPT_HIP_TRY(gram_timing_begin(stream, 4.0 * m * (m + 1.0) * static_cast<double>(K), slot));
         if (gbits == 32)
             hipLaunchKernelGGL(gram_mfma_kernel, dim3(static_cast<unsigned>(ntri * ksplit)), dim3(256), 0, stream,
                                w.ar, w.ai, m, static_cast<int>(K), ksplit, ntri, w.part);
@@ -471,6 +515,7 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
             hipLaunchKernelGGL(gram_mfma64_kernel, dim3(static_cast<unsigned>(ntri * ksplit)), dim3(256), 0, stream,
                                w.ar, w.ai, m, static_cast<int>(K), ksplit, ntri, w.part);
         MIPT_HIP_CHECK(hipGetLastError());
+        MIPT_HIP_TRY(gram_timing_end(stream, slot));
         const std::size_t mm = static_cast<std::size_t>(m) * m;
         hipLaunchKernelGGL(gram_reduce_kernel, dim3(static_cast<unsigned>((mm + 255) / 256)), dim3(256), 0, stream,
                            w.part, m, ksplit, ntri, dst);
@@ -886,6 +931,41 @@ int mipt_hip_rho1_complex_f64(const void *s, int n, int q, double *rho)
 {
     return rho1_impl<double2>(s, n, q, nullptr, rho);
 }
+int mipt_hip_set_gram_profiling(int enable)
+{
+    GramTiming &g = gram_timing();
+    std::lock_guard<std::mutex> lock(g.mu);
+    for (auto &e : g.ev)
+    {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    g.ev.clear();
+    g.flops.clear();
+    g.on = enable != 0;
+    return OK;
+}
+
+int mipt_hip_gram_stats(double *avg_ms, int *count, double *flops_avg)
+{
+    GramTiming &g = gram_timing();
+    std::lock_guard<std::mutex> lock(g.mu);
+    double ms_sum = 0.0, fl_sum = 0.0;
+    for (std::size_t i = 0; i < g.ev.size(); ++i)
+    {
+        MIPT_HIP_CHECK(hipEventSynchronize(g.ev[i].second));
+        float ms = 0.f;
+        MIPT_HIP_CHECK(hipEventElapsedTime(&ms, g.ev[i].first, g.ev[i].second));
+        ms_sum += ms;
+        fl_sum += g.flops[i];
+    }
+    const double n = static_cast<double>(g.ev.size());
+    if (avg_ms) *avg_ms = n > 0 ? ms_sum / n : 0.0;
+    if (count) *count = static_cast<int>(g.ev.size());
+    if (flops_avg) *flops_avg = n > 0 ? fl_sum / n : 0.0;
+    return OK;
+}
+
 int mipt_hip_set_gram_precision(int bits)
 {
     if (bits != 32 && bits != 64)
