@@ -53,6 +53,14 @@ int mipt_hip_measure_collapse_f64(void *device_state_vector, int n_qubits, int m
                                   int *outcome, double *realized_probability);
 /* Replaces mipt_cuda_measure_collapse_last_error (free_energy_measure.hpp:29). */
 const char *mipt_hip_measure_collapse_last_error(void);
+/* No counterpart in the reference: mipt_hip_measure_collapse_f32 for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels, sums, outcome, probability and collapse;
+ * its working memory is a fixed 1 MiB device array per GPU, so states of at
+ * most 29 qubits (larger n returns -5 with a message).  Calls that overlap on
+ * one GPU race on that array. */
+int mipt_hip_measure_collapse_f32_noalloc(void *device_state_vector, int n_qubits, int measured_qubit,
+                                          double uniform_01, int *outcome, double *realized_probability);
 
 /* Replace mipt_cuda_entropy_svd_f32/_f64 (cpp/include/mipt/cuda/tmi_cuda_svd.hpp:20-34):
  * von Neumann entropy (bits) of the kept modes (little-endian order) of a pure

@@ -52,7 +52,8 @@ def lib():
     L.mipt_hip_last_error.restype = ctypes.c_char_p
     L.mipt_hip_version.restype = ctypes.c_char_p
     L.mipt_hip_measure_collapse_last_error.restype = ctypes.c_char_p
-    for name in ("mipt_hip_measure_collapse_f32", "mipt_hip_measure_collapse_f64"):
+    for name in ("mipt_hip_measure_collapse_f32", "mipt_hip_measure_collapse_f64",
+                 "mipt_hip_measure_collapse_f32_noalloc"):
         getattr(L, name).argtypes = [vp, i, i, ctypes.c_double, ip, dp]
     for name in ("mipt_hip_entropy_svd_f32", "mipt_hip_entropy_svd_f64", "mipt_hip_renyi2_f32",
                  "mipt_hip_renyi2_f64", "mipt_hip_rdm_subsystem_f32", "mipt_hip_rdm_subsystem_f64"):

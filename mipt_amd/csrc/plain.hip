@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace mipt_hip
@@ -158,6 +159,164 @@ int measure_collapse_impl(void *state, int n, int q, double u, int *outcome, dou
     if (st == 0)
     {
         const hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess)
+        {
+            set_error("collapse synchronization: %s", hipGetErrorString(e));
+            st = static_cast<int>(e);
+        }
+    }
+    if (st != 0) std::snprintf(t_measure_error, sizeof(t_measure_error), "%s", last_error());
+    return st;
+}
+
+// ---------------------------------------------------------- allocation-free
+// The sequential measurement for callers that cannot allocate device memory.
+// Its reduction partials, histogram and masks live in `scratch`, file-scope
+// device memory of a fixed size (one copy per GPU, held until the process
+// ends): 4 mask words, 2 histogram bins, 2 spare words, then the partials of
+// the stage-1 reduction, 2^(n-12) doubles for n >= 13 -- at most 2^17 (1 MiB)
+// for n <= 29.  Every word a call reads is written earlier in the same call,
+// so nothing needs clearing; calls that overlap on one GPU race on it.
+constexpr int NOALLOC_MAX_QUBITS = 29;
+constexpr std::size_t NOALLOC_WORDS = 8 + (std::size_t{1} << (NOALLOC_MAX_QUBITS - 12));
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
+// The device address of `scratch` on each GPU, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+struct NoAllocScratch
+{
+    std::mutex mu;
+    double *ptr[64] = {};
+};
+NoAllocScratch &noalloc_scratch()
+{
+    static NoAllocScratch s;
+    return s;
+}
+
+int noalloc_scratch_ptr(double **out)
+{
+    int dev = 0;
+    MIPT_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+    {
+        set_error("allocation-free measurement: device %d out of range", dev);
+        return E_ARG;
+    }
+    NoAllocScratch &s = noalloc_scratch();
+    std::lock_guard<std::mutex> lock(s.mu);
+    if (!s.ptr[dev])
+    {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+        {
+            set_error("allocation-free measurement: scratch address on device %d: %s", dev, hipGetErrorString(e));
+            return e != hipSuccess ? static_cast<int>(e) : E_ARG;
+        }
+        s.ptr[dev] = static_cast<double *>(p);
+    }
+    *out = s.ptr[dev];
+    return OK;
+}
+
+__global__ void set_masks_kernel(std::uint64_t *m, std::uint64_t off, std::uint64_t mmask, std::uint64_t cmask,
+                                 std::uint64_t cval)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        m[0] = off;
+        m[1] = mmask;
+        m[2] = cmask;
+        m[3] = cval;
+    }
+}
+
+// measure_collapse_impl with its buffers in `scratch`: the same kernels, the
+// same fixed-order FP64 sums and the same decision, so the same outcome,
+// probability and collapsed state.
+template <typename Real>
+int measure_collapse_noalloc_impl(void *state, int n, int q, double u, int *outcome, double *prob)
+{
+    t_measure_error[0] = '\0';
+    if (state == nullptr || outcome == nullptr || prob == nullptr || n < 1 || n >= 63 || q < 0 || q >= n ||
+        !(u >= 0.0 && u < 1.0))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "Invalid sequential-measurement argument.");
+        set_error("%s", t_measure_error);
+        return E_ARG;
+    }
+    if (n > NOALLOC_MAX_QUBITS)
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error),
+                      "allocation-free measurement: %d qubits exceed its fixed scratch (at most %d qubits)", n,
+                      NOALLOC_MAX_QUBITS);
+        set_error("%s", t_measure_error);
+        return E_UNSUPPORTED;
+    }
+    double *base = nullptr;
+    int st = noalloc_scratch_ptr(&base);
+    if (st != OK)
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "%s", last_error());
+        return st;
+    }
+    std::uint64_t *masks = reinterpret_cast<std::uint64_t *>(base);
+    double *hist = base + 4;
+    double *partial = base + 8;
+    const std::uint64_t mmask = std::uint64_t{1} << q;
+    const std::uint64_t rest = std::uint64_t{1} << (n - 1);
+    const std::uint64_t per_block = static_cast<std::uint64_t>(HIST_THREADS) * HIST_PER_THREAD;
+    const int bpb = static_cast<int>((rest + per_block - 1) / per_block);
+    const std::size_t need_partial = std::size_t{2} * bpb;  // within NOALLOC_WORDS - 8 for n <= 29
+    hipLaunchKernelGGL(set_masks_kernel, dim3(1), dim3(64), 0, nullptr, masks, std::uint64_t{0}, mmask,
+                       std::uint64_t{0}, std::uint64_t{0});
+    hipLaunchKernelGGL(hist_partial_kernel<Real>, dim3(static_cast<unsigned>(2 * bpb), 1), dim3(HIST_THREADS), 0, nullptr,
+                       static_cast<const typename CxT<Real>::type *>(state), masks + 0, n, masks + 1, masks + 2,
+                       masks + 3, bpb, partial, static_cast<int>(need_partial));
+    hipLaunchKernelGGL(hist_final_kernel, dim3(1, 1), dim3(256), 0, nullptr, partial, static_cast<int>(need_partial), bpb,
+                       masks + 1, hist, 2);
+    double h[2] = {0.0, 0.0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess)
+    {
+        set_error("allocation-free measurement: %s", hipGetErrorString(e));
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "%s", last_error());
+        return static_cast<int>(e);
+    }
+    const double total = h[0] + h[1];
+    const double one = h[1];
+    if (!(total > 0.0) || !std::isfinite(total) || !std::isfinite(one) || one < 0.0 || one > total * (1.0 + 1.0e-10))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "Invalid branch norms: total=%.17g, one=%.17g",
+                      total, one);
+        set_error("%s", t_measure_error);
+        return E_NORM;
+    }
+    const double p1 = std::fmin(1.0, std::fmax(0.0, one / total));
+    *outcome = u < p1 ? 1 : 0;
+    *prob = *outcome ? p1 : 1.0 - p1;
+    const double branch = *outcome ? one : total - one;
+    if (!(*prob > 0.0) || !std::isfinite(*prob))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error),
+                      "Sampled an outcome with invalid conditional probability %.17g", *prob);
+        set_error("%s", t_measure_error);
+        return E_PROB;
+    }
+    if (!(branch > 0.0) || !std::isfinite(branch))
+    {
+        std::snprintf(t_measure_error, sizeof(t_measure_error), "Sampled an outcome with invalid branch norm %.17g",
+                      branch);
+        set_error("%s", t_measure_error);
+        return E_BRANCH;
+    }
+    st = collapse<Real>(state, n, mmask, *outcome ? mmask : 0, 1.0 / std::sqrt(branch), nullptr);
+    if (st == 0)
+    {
+        e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess)
         {
             set_error("collapse synchronization: %s", hipGetErrorString(e));
@@ -406,6 +565,10 @@ int mipt_hip_measure_collapse_f64(void *s, int n, int q, double u, int *o, doubl
     return measure_collapse_impl<double>(s, n, q, u, o, p);
 }
 const char *mipt_hip_measure_collapse_last_error(void) { return t_measure_error; }
+int mipt_hip_measure_collapse_f32_noalloc(void *s, int n, int q, double u, int *o, double *p)
+{
+    return measure_collapse_noalloc_impl<float>(s, n, q, u, o, p);
+}
 
 int mipt_hip_engine_create(int use_fp64, void *stream, mipt_hip_engine **out)
 {
