@@ -79,6 +79,24 @@ int mipt_hip_entropy_svd_f64(const void *device_state_vector, int n_qubits, cons
  * MIPT_HIP_GRAM_PRECISION.  -1 for any other value. */
 int mipt_hip_set_gram_precision(int bits);
 
+/* Eigenvalue path of every entropy (no counterpart in the reference, whose
+ * device entropy uses cuSOLVER heevd/gesvd, tmi_cuda_svd.cu:663-798):
+ * 0 = the library's FP64 kernels (default): blocked Householder reduction to
+ * a real tridiagonal and bisection on Sturm counts, one workgroup per matrix,
+ * for batches of RDMs up to 1024 x 1024 and single RDMs up to 512 x 512,
+ * rocSOLVER zheevd beyond; 1 = rocSOLVER zheevd for every size.
+ * Process-wide.  -1 for any other value. */
+int mipt_hip_set_eigensolver(int which);
+
+/* Replaces entropy_hermitian_inplace (cpp/include/mipt/tmi/linalg.hpp:524-583)
+ * for many matrices at once: von Neumann entropies (bits; lambda <= 1e-15
+ * dropped, |S| < 1e-12 -> 0 as linalg.hpp:125-141) of `count` Hermitian m x m
+ * host matrices, row-major interleaved FP64 like the RDM export
+ * (rho_ri[2 (i m^2 + r m + c) + {0,1}]), on the library's batched FP64
+ * eigensolver; m <= 1024.  eigenvalues (ascending, count * m), may be NULL. */
+int mipt_hip_entropy_hermitian_batched(const double *rho_ri, int m, int count, double *entropy_bits,
+                                       double *eigenvalues);
+
 /* Measurement helpers (no counterpart in the reference): time every Gram
  * kernel of the complex64 RDM path with HIP events on its stream (enable 1;
  * 0 stops; either clears earlier timings), then the average kernel duration

@@ -62,6 +62,8 @@ def lib():
         getattr(L, name).argtypes = [vp, i, i, dp]
     L.mipt_hip_tmi_f32.argtypes = [vp, i, i, i, i, dp, dp]
     L.mipt_hip_set_gram_precision.argtypes = [i]
+    L.mipt_hip_set_eigensolver.argtypes = [i]
+    L.mipt_hip_entropy_hermitian_batched.argtypes = [dp, i, i, dp, dp]
     L.mipt_hip_set_gram_profiling.argtypes = [i]
     L.mipt_hip_gram_stats.argtypes = [dp, ip, dp]
     L.mipt_hip_engine_create.argtypes = [i, vp, ctypes.POINTER(vp)]
@@ -154,6 +156,26 @@ def gram_stats() -> dict:
 def set_gram_precision(bits: int) -> None:
     """Gram arithmetic of complex64 RDMs: 64 (FP64 MFMA, default) or 32 (FP32 MFMA)."""
     _check(lib().mipt_hip_set_gram_precision(int(bits)), "set_gram_precision")
+
+
+def set_eigensolver(which: int) -> None:
+    """Eigenvalue path of the entropies: 0 = the library's batched FP64 kernels
+    (default), 1 = rocSOLVER zheevd."""
+    _check(lib().mipt_hip_set_eigensolver(int(which)), "set_eigensolver")
+
+
+def entropy_hermitian_batched(rhos, eigenvalues: bool = True):
+    """Entropies (bits) and ascending eigenvalues of a stack of Hermitian
+    matrices [count, m, m] on the library's batched FP64 eigensolver
+    (entropy_hermitian_inplace, linalg.hpp:524-583, for many matrices)."""
+    r = np.ascontiguousarray(rhos, np.complex128)
+    count, m = r.shape[0], r.shape[1]
+    ent = np.zeros(count)
+    lam = np.zeros((count, m)) if eigenvalues else None
+    _check(lib().mipt_hip_entropy_hermitian_batched(r.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                    m, count, _dp(ent), _dp(lam) if eigenvalues else None),
+           "entropy_hermitian_batched")
+    return (ent, lam) if eigenvalues else ent
 
 
 def measure_collapse(state_ptr: int, n_qubits: int, qubit: int, uniform: float, fp64: bool = False):

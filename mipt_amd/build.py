@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmipt_hip.so")
-SOURCES = ["plain.hip", "batch.hip", "entropy.hip", "decoder.hip", "driver.cpp", "probe_info.cpp"]
+SOURCES = ["plain.hip", "batch.hip", "entropy.hip", "eigen.hip", "decoder.hip", "driver.cpp", "probe_info.cpp"]
 BIN = os.path.join(HERE, "bin")
 TOOLS = {"sim_tmi": os.path.join(HERE, "host", "sim_tmi_main.cpp"), "mipt": os.path.join(HERE, "host", "mipt_main.cpp"),
          "entropy": os.path.join(HERE, "host", "entropy_main.cpp"),

@@ -2875,15 +2875,6 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cyc
         const int nq = b->desc[idx[g0]].nq, L = b->desc[idx[g0]].L, cnt = g1 - g0;
         std::vector<const float2 *> psis(cnt);
         for (int j = 0; j < cnt; ++j) psis[j] = b->a.states + b->desc[idx[g0 + j]].state_off;
-        if (s_half)
-        {
-            std::vector<int> kept(static_cast<std::size_t>(cnt) * (L / 2));
-            for (int j = 0; j < cnt; ++j)
-                for (int q = 0; q < L / 2; ++q) kept[static_cast<std::size_t>(j) * (L / 2) + q] = q;
-            std::vector<double> sv(cnt);
-            MIPT_HIP_TRY(entropy_many_bits_f32(psis.data(), nq, kept.data(), L / 2, cnt, b->stream, sv.data()));
-            for (int j = 0; j < cnt; ++j) s_half[idx[g0 + j]] = sv[j];
-        }
         if (tmi_cycles > 0)
         {
             const int bk = tmi_block > 0 ? tmi_block : L / 4;
@@ -2930,6 +2921,9 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cyc
                         add(static_cast<int>(sets[term].size()), sets[term], Item{j, c, term});
                 }
             std::vector<double> acc(static_cast<std::size_t>(cnt) * tmi_cycles, 0.0);
+            // cycle 0's A u B is the kept list {0, ..., L/2 - 1} of S(L/2) when
+            // 2 * block = L / 2: the same RDM, so S(L/2) is taken from it
+            const bool reuse_half = s_half && L % 4 == 0 && 2 * bk == L / 2;
             for (std::size_t k = 0; k < by_size.size(); ++k)
             {
                 const int sz = by_size[k].first;
@@ -2942,11 +2936,22 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cyc
                 {
                     const Item &it = items[k][i];
                     acc[static_cast<std::size_t>(it.traj_j) * tmi_cycles + it.cycle] += (it.term < 4 ? 1.0 : -1.0) * sv[i];
+                    if (reuse_half && it.cycle == 0 && it.term == 4) s_half[idx[g0 + it.traj_j]] = sv[i];
                 }
             }
             for (int j = 0; j < cnt; ++j)
                 for (int c = 0; c < tmi_cycles; ++c)
                     tmi[static_cast<std::size_t>(idx[g0 + j]) * tmi_cycles + c] = acc[static_cast<std::size_t>(j) * tmi_cycles + c];
+        }
+        const bool half_done = tmi_cycles > 0 && L % 4 == 0 && 2 * (tmi_block > 0 ? tmi_block : L / 4) == L / 2;
+        if (s_half && !half_done)
+        {
+            std::vector<int> kept(static_cast<std::size_t>(cnt) * (L / 2));
+            for (int j = 0; j < cnt; ++j)
+                for (int q = 0; q < L / 2; ++q) kept[static_cast<std::size_t>(j) * (L / 2) + q] = q;
+            std::vector<double> sv(cnt);
+            MIPT_HIP_TRY(entropy_many_bits_f32(psis.data(), nq, kept.data(), L / 2, cnt, b->stream, sv.data()));
+            for (int j = 0; j < cnt; ++j) s_half[idx[g0 + j]] = sv[j];
         }
         g0 = g1;
     }

@@ -73,4 +73,14 @@ int purity_many_f32(const float2 *const *psis, int n, const int *kept, int kept_
 int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
 int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4);
 
+// Batched FP64 Hermitian eigenvalues (eigen.hip): `count` m x m matrices at
+// mats + i * stride (m <= EIG_MAX_N; the lower triangle of the column-major
+// view is read, the matrices are overwritten), panel workspace wpanel of
+// count * m * 32 elements, d and e of count * m doubles; entropies in bits
+// to ent[count] and, when lam is not null, eigenvalues to lam[count * m].
+// Launches on `stream` without synchronising.
+constexpr int EIG_MAX_N = 1024;
+int launch_hermitian_entropies(double2 *mats, int m, std::size_t stride, int count, double2 *wpanel, double *d,
+                               double *e, double *ent, double *lam, hipStream_t stream);
+
 } // namespace mipt_hip

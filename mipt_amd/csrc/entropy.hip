@@ -15,8 +15,10 @@
 // lower-triangle 64x64 tiles only, split-K, then a fixed-order FP64 split-K
 // reduction.  The products run on FP64 MFMA (exact products of the complex64
 // inputs, FP64 accumulation) by default, or on FP32 MFMA with FP64 promotion
-// every 16-deep K step.  Eigenvalues of the FP64 rho come from rocSOLVER
-// zheevd; the -sum lambda log2 lambda is FP64.
+// every 16-deep K step.  Eigenvalues of the FP64 rho come from the library's
+// batched Householder + bisection kernels (eigen.hip) for batches and for
+// single RDMs up to 512 x 512, from rocSOLVER zheevd for larger single RDMs
+// (or everywhere when selected); the -sum lambda log2 lambda is FP64.
 #include "batch.hpp"
 #include "state_ops.hpp"
 
@@ -352,6 +354,11 @@ struct Workspace
     std::size_t ev_b_cap = 0, e_b_cap = 0;
     int *info_b = nullptr;
     std::size_t info_b_cap = 0;
+    // the project's batched eigensolver: panel workspace, tridiagonals, entropies
+    double2 *eig_w = nullptr;
+    std::size_t eig_w_cap = 0;
+    double *eig_d = nullptr, *eig_e = nullptr, *eig_s = nullptr;
+    std::size_t eig_d_cap = 0, eig_e_cap = 0, eig_s_cap = 0;
 };
 
 // Process-lifetime per-thread workspace (never freed: the reference keeps its
@@ -413,6 +420,28 @@ int gram_precision(int &bits)
         b = g_gram_bits.load();
     }
     bits = b;
+    return OK;
+}
+
+// Eigenvalue path of the entropies: 0 (default) = the library's batched
+// Householder + bisection kernels (eigen.hip) for RDMs up to EIG_MAX_N;
+// 1 = rocSOLVER zheevd for every size.  Chosen by mipt_hip_set_eigensolver.
+std::atomic<int> g_eig_solver{0};
+
+// Entropies (bits) of `count` RDMs already in w.rho_b (stride m*m) with the
+// library's batched eigensolver; results to host out[count].
+int own_entropies(int m, int count, hipStream_t stream, double2 *mats, double *out)
+{
+    Workspace &w = ws();
+    MIPT_HIP_TRY(grow(w.eig_w, w.eig_w_cap, static_cast<std::size_t>(count) * m * 32));
+    MIPT_HIP_TRY(grow(w.eig_d, w.eig_d_cap, static_cast<std::size_t>(count) * m));
+    MIPT_HIP_TRY(grow(w.eig_e, w.eig_e_cap, static_cast<std::size_t>(count) * m));
+    MIPT_HIP_TRY(grow(w.eig_s, w.eig_s_cap, static_cast<std::size_t>(count)));
+    MIPT_HIP_TRY(launch_hermitian_entropies(mats, m, static_cast<std::size_t>(m) * m, count, w.eig_w, w.eig_d, w.eig_e,
+                                            w.eig_s, nullptr, stream));
+    MIPT_HIP_CHECK(hipMemcpyAsync(out, w.eig_s, static_cast<std::size_t>(count) * sizeof(double), hipMemcpyDeviceToHost,
+                                  stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(stream));
     return OK;
 }
 
@@ -614,6 +643,12 @@ int entropy_impl(const void *psi, bool f64, int n, const int *kept, int count, h
         }
         return OK;
     }
+    if (g_eig_solver.load() == 0 && m <= 512)
+    {
+        // one workgroup per matrix: cheaper than rocSOLVER's launch sequence up to this size
+        Workspace &w = ws();
+        return own_entropies(m, 1, stream, w.rho, out);
+    }
     std::vector<double> lam;
     MIPT_HIP_TRY(eigvalsh(m, stream, lam));
     *out = entropy_from_eigs(lam);
@@ -690,9 +725,10 @@ int ensure_handle(hipStream_t stream)
 
 // Von Neumann entropies (bits) of `count` items (state psis[i], kept set
 // kept + i*kept_count), all of the same n and kept size: the RDMs are built
-// into one strided buffer and their eigenvalues come from one
-// rocsolver_zheevd_strided_batched call per chunk (measured on MI355X at
-// 1024 x 1024: 2.9 ms per matrix batched against 24 ms one by one).
+// into one strided buffer per chunk and their eigenvalues come from the
+// library's batched eigensolver (one workgroup per matrix, eigen.hip) or,
+// when selected or beyond EIG_MAX_N, from one
+// rocsolver_zheevd_strided_batched call per chunk.
 int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
                  double *out)
 {
@@ -707,12 +743,13 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
         return OK;
     }
     Workspace &w = ws();
-    MIPT_HIP_TRY(ensure_handle(stream));
     const int m = 1 << kept_count;
     const std::size_t mm = static_cast<std::size_t>(m) * m;
-    if (m < 32)
+    const bool own = g_eig_solver.load() == 0 && m <= EIG_MAX_N;
+    if (!own) MIPT_HIP_TRY(ensure_handle(stream));
+    if (!own && m < 32)
     {
-        // tiny RDMs: one solve each (cheap at this size)
+        // tiny RDMs on rocSOLVER: one solve each (cheap at this size)
         for (int i = 0; i < count; ++i)
         {
             int mo = 0;
@@ -723,13 +760,24 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
         }
         return OK;
     }
-    const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, (std::size_t{1} << 30) / (mm * 16))));
+    // RDMs per chunk: the library's eigensolver runs one workgroup per matrix,
+    // so a chunk should hold a few hundred large matrices; bounded by a quarter
+    // of the free device memory and by 8 GiB (matrix + panel workspace)
+    std::size_t free_b = 0, total_b = 0;
+    MIPT_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const std::size_t per = mm * 16 + (own ? static_cast<std::size_t>(m) * 32 * 16 + static_cast<std::size_t>(m) * 16 : 0);
+    const std::size_t budget = own ? std::min<std::size_t>(std::size_t{8} << 30, (free_b + w.rho_b_cap * 16) / 4)
+                                   : (std::size_t{1} << 30);
+    const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, budget / per)));
     MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, chunk * mm));
-    MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk) * m));
-    MIPT_HIP_TRY(grow(w.e_b, w.e_b_cap, static_cast<std::size_t>(chunk) * m));
-    MIPT_HIP_TRY(grow(w.info_b, w.info_b_cap, static_cast<std::size_t>(chunk)));
-    std::vector<double> lam(static_cast<std::size_t>(chunk) * m);
-    std::vector<int> info(chunk);
+    if (!own)
+    {
+        MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk) * m));
+        MIPT_HIP_TRY(grow(w.e_b, w.e_b_cap, static_cast<std::size_t>(chunk) * m));
+        MIPT_HIP_TRY(grow(w.info_b, w.info_b_cap, static_cast<std::size_t>(chunk)));
+    }
+    std::vector<double> lam(own ? 0 : static_cast<std::size_t>(chunk) * m);
+    std::vector<int> info(own ? 0 : chunk);
     for (int c0 = 0; c0 < count; c0 += chunk)
     {
         const int nb = std::min(chunk, count - c0);
@@ -738,6 +786,11 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
             int mo = 0;
             MIPT_HIP_TRY(build_rho(psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
                                    kept_count, stream, mo, w.rho_b + i * mm));
+        }
+        if (own)
+        {
+            MIPT_HIP_TRY(own_entropies(m, nb, stream, w.rho_b, out + c0));
+            continue;
         }
         const rocblas_status st = rocsolver_zheevd_strided_batched(
             w.handle, rocblas_evect_none, rocblas_fill_lower, m, reinterpret_cast<rocblas_double_complex *>(w.rho_b), m,
@@ -974,6 +1027,49 @@ int mipt_hip_set_gram_precision(int bits)
         return E_ARG;
     }
     g_gram_bits.store(bits);
+    return OK;
+}
+
+int mipt_hip_entropy_hermitian_batched(const double *rho_ri, int m, int count, double *entropy_bits,
+                                       double *eigenvalues)
+{
+    if (!rho_ri || !entropy_bits || m < 1 || m > EIG_MAX_N || count < 0)
+    {
+        set_error("entropy_hermitian_batched: bad arguments (m=%d in [1, %d], count=%d)", m, EIG_MAX_N, count);
+        return E_ARG;
+    }
+    if (count == 0) return OK;
+    Workspace &w = ws();
+    hipStream_t stream = nullptr;
+    const std::size_t mm = static_cast<std::size_t>(m) * m, total = mm * count;
+    MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, total));
+    MIPT_HIP_TRY(grow(w.eig_w, w.eig_w_cap, static_cast<std::size_t>(count) * m * 32));
+    MIPT_HIP_TRY(grow(w.eig_d, w.eig_d_cap, static_cast<std::size_t>(count) * m));
+    MIPT_HIP_TRY(grow(w.eig_e, w.eig_e_cap, static_cast<std::size_t>(count) * m));
+    MIPT_HIP_TRY(grow(w.eig_s, w.eig_s_cap, static_cast<std::size_t>(count)));
+    double *lam = nullptr;
+    if (eigenvalues) MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(count) * m));
+    if (eigenvalues) lam = w.ev_b;
+    // row-major rho read as column-major is rho^T = conj(rho): the same spectrum
+    MIPT_HIP_CHECK(hipMemcpyAsync(w.rho_b, rho_ri, total * sizeof(double2), hipMemcpyHostToDevice, stream));
+    MIPT_HIP_TRY(launch_hermitian_entropies(w.rho_b, m, mm, count, w.eig_w, w.eig_d, w.eig_e, w.eig_s, lam, stream));
+    MIPT_HIP_CHECK(hipMemcpyAsync(entropy_bits, w.eig_s, static_cast<std::size_t>(count) * sizeof(double),
+                                  hipMemcpyDeviceToHost, stream));
+    if (eigenvalues)
+        MIPT_HIP_CHECK(hipMemcpyAsync(eigenvalues, lam, static_cast<std::size_t>(count) * m * sizeof(double),
+                                      hipMemcpyDeviceToHost, stream));
+    MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+    return OK;
+}
+
+int mipt_hip_set_eigensolver(int which)
+{
+    if (which != 0 && which != 1)
+    {
+        set_error("eigensolver must be 0 (library kernels) or 1 (rocSOLVER), got %d", which);
+        return E_ARG;
+    }
+    g_eig_solver.store(which);
     return OK;
 }
 
