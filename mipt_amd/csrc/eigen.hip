@@ -195,19 +195,30 @@ __global__ void __launch_bounds__(NT) hetrd_kernel(HetrdArgs g)
 #pragma unroll
                 for (int cc = 0; cc < 8; ++cc) cx[cc] = cy[cc] = 0.0;
                 const int lc0 = EIG_TILE * J + 8 * g8;
+                const double2 *colp = A + s + (s + min(lc0, len - 1)) * nn;
+                // row block I's entries; the next block's loads are issued
+                // before this block's arithmetic (two blocks in flight)
+                double2 a[8], an[8];
+                {
+                    const int lr = 16 * (2 * J) + q16;
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc)
+                    {
+                        const int lc = lc0 + cc;
+                        a[cc] = (lr < len && lc < len && lr >= lc) ? colp[min(lr, len - 1) + cc * nn] : zero;
+                    }
+                }
                 for (int I = 2 * J; I < nrb; ++I)
                 {
                     const int lr = 16 * I + q16;
                     const bool rv = lr < len;
                     const double2 vi = rv ? vs[lr] : zero;
-                    const double2 *rowp = A + (s + min(lr, len - 1)) + (s + min(lc0, len - 1)) * nn;
-                    double2 a[8];
+                    const int lrn = lr + 16;
 #pragma unroll
                     for (int cc = 0; cc < 8; ++cc)
                     {
                         const int lc = lc0 + cc;
-                        const bool ok = rv && lc < len && lr >= lc;
-                        a[cc] = ok ? rowp[cc * nn] : zero;
+                        an[cc] = (lrn < len && lc < len && lrn >= lc) ? colp[min(lrn, len - 1) + cc * nn] : zero;
                     }
                     double rx = 0.0, ry = 0.0;
 #pragma unroll
@@ -239,6 +250,8 @@ __global__ void __launch_bounds__(NT) hetrd_kernel(HetrdArgs g)
                         t.y += ry;
                         yp[wv * n + lr] = t;
                     }
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) a[cc] = an[cc];
                 }
 #pragma unroll
                 for (int cc = 0; cc < 8; ++cc)

@@ -256,6 +256,106 @@ __global__ void gram_reduce_kernel(const double2 *__restrict__ part, int m, int 
     rho[id] = lower ? make_double2(sr, si) : make_double2(sr, -si);
 }
 
+// Batched Schmidt gather for many (state, kept set) items of one shape:
+// item t's planes at t * m * K, M_t[r, c] = psi_t[pdep(c, env_mask_t) | row_off_t[r]].
+__global__ void gather_batch_kernel(const float2 *const *__restrict__ psis, const std::uint64_t *__restrict__ row_off,
+                                    const std::uint64_t *__restrict__ env_mask, int kept, int env, int count,
+                                    float *__restrict__ ar, float *__restrict__ ai)
+{
+    const std::uint64_t per = std::uint64_t{1} << (kept + env);
+    const std::uint64_t total = per * static_cast<std::uint64_t>(count);
+    const std::uint64_t cols = std::uint64_t{1} << env;
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t id = blockIdx.x * static_cast<std::uint64_t>(blockDim.x) + threadIdx.x; id < total; id += stride)
+    {
+        const std::uint64_t t = id >> (kept + env);
+        const std::uint64_t loc = id & (per - 1);
+        const std::uint64_t r = loc >> env, c = loc & (cols - 1);
+        const float2 v = psis[t][pdep64(c, env_mask[t]) | row_off[(t << kept) + r]];
+        ar[id] = v.x;
+        ai[id] = v.y;
+    }
+}
+
+// Batched Gram on FP64 MFMA for the library's eigensolver: item blockIdx.y,
+// lower tile (bi >= bj) of rho = M M^H over the full K (no split), written as
+// the UPPER triangle of the row-major rho, rho[a][b] for a <= b, i.e. operand
+// A from the rows of tile bj and B from the rows of tile bi (each wave store
+// covers 16 consecutive columns).  The eigensolver reads that triangle as
+// the lower triangle of the column-major conj(rho), which has the same
+// spectrum.  Exact products of the complex64 inputs, FP64 sums, real diagonal.
+__global__ void __launch_bounds__(256) gram_batch64_kernel(const float *__restrict__ ar_all,
+                                                           const float *__restrict__ ai_all, int m, int K,
+                                                           int ntiles_tri, double2 *__restrict__ rho_all)
+{
+    __shared__ float sA_r[GT][GK + 1], sA_i[GT][GK + 1], sB_r[GT][GK + 1], sB_i[GT][GK + 1];
+    const int tile = blockIdx.x;
+    const std::size_t item = blockIdx.y;
+    const float *ar = ar_all + item * static_cast<std::size_t>(m) * K;
+    const float *ai = ai_all + item * static_cast<std::size_t>(m) * K;
+    double2 *rho = rho_all + item * static_cast<std::size_t>(m) * m;
+    int bi = static_cast<int>((sqrt(8.0 * tile + 1.0) - 1.0) / 2.0);
+    while ((bi + 1) * (bi + 2) / 2 <= tile) ++bi;
+    while (bi * (bi + 1) / 2 > tile) --bi;
+    const int bj = tile - bi * (bi + 1) / 2;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wi = wave >> 1, wj = wave & 1;
+    doublex4 accr[2][2], acci[2][2];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+        {
+            accr[x][y] = doublex4{0, 0, 0, 0};
+            acci[x][y] = doublex4{0, 0, 0, 0};
+        }
+    const int lrow = threadIdx.x >> 2;
+    const int lk = (threadIdx.x & 3) * 4;
+    const int ga = bj * GT + lrow, gb = bi * GT + lrow;
+    for (int kk = 0; kk < K; kk += GK)
+    {
+        for (int e = 0; e < 4; ++e)
+        {
+            const int k = kk + lk + e;
+            const bool okk = k < K;
+            sA_r[lrow][lk + e] = (ga < m && okk) ? ar[static_cast<std::size_t>(ga) * K + k] : 0.f;
+            sA_i[lrow][lk + e] = (ga < m && okk) ? ai[static_cast<std::size_t>(ga) * K + k] : 0.f;
+            sB_r[lrow][lk + e] = (gb < m && okk) ? ar[static_cast<std::size_t>(gb) * K + k] : 0.f;
+            sB_i[lrow][lk + e] = (gb < m && okk) ? ai[static_cast<std::size_t>(gb) * K + k] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GK; ks += 4)
+        {
+            const int kq = ks + (lane >> 4);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+            {
+                const int ra = wi * 32 + x * 16 + (lane & 15);
+                const double a_r = sA_r[ra][kq], a_i = sA_i[ra][kq];
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                {
+                    const int rb = wj * 32 + y * 16 + (lane & 15);
+                    const double b_r = sB_r[rb][kq], b_i = sB_i[rb][kq];
+                    accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_r, accr[x][y], 0, 0, 0);
+                    accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_i, accr[x][y], 0, 0, 0);
+                    acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_r, acci[x][y], 0, 0, 0);
+                    acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_r, b_i, acci[x][y], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+            for (int e = 0; e < 4; ++e)
+            {
+                const int a = bj * GT + wi * 32 + x * 16 + (lane >> 4) + 4 * e;
+                const int b = bi * GT + wj * 32 + y * 16 + (lane & 15);
+                if (a < m && b < m && a <= b)
+                    rho[static_cast<std::size_t>(a) * m + b] = make_double2(accr[x][y][e], a == b ? 0.0 : acci[x][y][e]);
+            }
+}
+
 // FP64 Gram for complex128 states (drop-in _f64 path): rho[r][c] = sum_k M[r,k] conj(M[c,k]).
 __global__ void gram_f64_kernel(const double2 *__restrict__ psi, const std::uint64_t *__restrict__ row_off, int m,
                                 int env, std::uint64_t env_mask, double2 *__restrict__ rho)
@@ -359,6 +459,9 @@ struct Workspace
     std::size_t eig_w_cap = 0;
     double *eig_d = nullptr, *eig_e = nullptr, *eig_s = nullptr;
     std::size_t eig_d_cap = 0, eig_e_cap = 0, eig_s_cap = 0;
+    // batched gather tables: row offsets, environment masks, state pointers
+    std::uint64_t *rowb = nullptr, *emb = nullptr, *psib = nullptr;
+    std::size_t rowb_cap = 0, emb_cap = 0, psib_cap = 0;
 };
 
 // Process-lifetime per-thread workspace (never freed: the reference keeps its
@@ -723,6 +826,114 @@ int ensure_handle(hipStream_t stream)
     return OK;
 }
 
+// RDMs of `count` (state, kept set) items of one shape into dst (stride
+// m*m), as the upper triangle of the row-major rho that the batched
+// eigensolver reads: one gather and one Gram launch for all items.  roff and
+// emask hold the uploaded tables; the caller keeps them until the stream
+// has passed this work.
+int build_rho_batch(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+                    double2 *dst, std::vector<std::uint64_t> &roff, std::vector<std::uint64_t> &emask,
+                    std::vector<std::uint64_t> &pv)
+{
+    Workspace &w = ws();
+    const int m = 1 << kept_count, env = n - kept_count;
+    const std::size_t K = std::size_t{1} << env;
+    roff.assign(static_cast<std::size_t>(count) * m, 0);
+    emask.assign(count, 0);
+    pv.assign(count, 0);
+    const std::uint64_t full = (n >= 64) ? ~std::uint64_t{0} : ((std::uint64_t{1} << n) - 1);
+    for (int t = 0; t < count; ++t)
+    {
+        const int *kt = kept + static_cast<std::size_t>(t) * kept_count;
+        std::uint64_t kmask;
+        if (!psis[t] || validate_kept(n, kt, kept_count, kmask) != OK)
+        {
+            set_error("entropy batch: invalid state or kept-mode list (item %d)", t);
+            return E_ARG;
+        }
+        emask[t] = full & ~kmask;
+        pv[t] = reinterpret_cast<std::uint64_t>(psis[t]);
+        for (int r = 0; r < m; ++r)
+        {
+            std::uint64_t o = 0;
+            for (int j = 0; j < kept_count; ++j)
+                if ((r >> j) & 1) o |= std::uint64_t{1} << kt[j];
+            roff[static_cast<std::size_t>(t) * m + r] = o;
+        }
+    }
+    MIPT_HIP_TRY(grow(w.rowb, w.rowb_cap, roff.size()));
+    MIPT_HIP_TRY(grow(w.emb, w.emb_cap, emask.size()));
+    MIPT_HIP_TRY(grow(w.psib, w.psib_cap, pv.size()));
+    MIPT_HIP_CHECK(hipMemcpyAsync(w.rowb, roff.data(), roff.size() * 8, hipMemcpyHostToDevice, stream));
+    MIPT_HIP_CHECK(hipMemcpyAsync(w.emb, emask.data(), emask.size() * 8, hipMemcpyHostToDevice, stream));
+    MIPT_HIP_CHECK(hipMemcpyAsync(w.psib, pv.data(), pv.size() * 8, hipMemcpyHostToDevice, stream));
+    const std::size_t plane = static_cast<std::size_t>(count) * m * K;
+    MIPT_HIP_TRY(grow(w.ar, w.plane_cap, plane));
+    MIPT_HIP_TRY(grow(w.ai, w.plane_cap_i, plane));
+    const unsigned gblocks = static_cast<unsigned>(std::min<std::size_t>((plane + 255) / 256, 1u << 18));
+    hipLaunchKernelGGL(gather_batch_kernel, dim3(gblocks), dim3(256), 0, stream,
+                       reinterpret_cast<const float2 *const *>(w.psib), w.rowb, w.emb, kept_count, env, count, w.ar, w.ai);
+    MIPT_HIP_CHECK(hipGetLastError());
+    const int nb = (m + GT - 1) / GT;
+    const int ntri = nb * (nb + 1) / 2;
+    int slot = -1;
+    MIPT_HIP_TRY(gram_timing_begin(stream, 4.0 * m * (m + 1.0) * static_cast<double>(K) * count, slot));
+    hipLaunchKernelGGL(gram_batch64_kernel, dim3(static_cast<unsigned>(ntri), static_cast<unsigned>(count)), dim3(256), 0,
+                       stream, w.ar, w.ai, m, static_cast<int>(K), ntri, dst);
+    MIPT_HIP_CHECK(hipGetLastError());
+    MIPT_HIP_TRY(gram_timing_end(stream, slot));
+    return OK;
+}
+
+// Release batch-sized workspaces above 2 GiB so later allocations (other
+// batches, other legs of a run) see the memory again.
+int trim_batch_workspace()
+{
+    Workspace &w = ws();
+    const std::size_t held = w.rho_b_cap * 16 + (w.plane_cap + w.plane_cap_i) * 4 + w.eig_w_cap * 16;
+    if (held <= (std::size_t{2} << 30)) return OK;
+    MIPT_HIP_CHECK(hipFree(w.rho_b));
+    MIPT_HIP_CHECK(hipFree(w.ar));
+    MIPT_HIP_CHECK(hipFree(w.ai));
+    MIPT_HIP_CHECK(hipFree(w.eig_w));
+    w.rho_b = nullptr;
+    w.ar = w.ai = nullptr;
+    w.eig_w = nullptr;
+    w.rho_b_cap = w.plane_cap = w.plane_cap_i = w.eig_w_cap = 0;
+    return OK;
+}
+
+// entropy_many on the library's eigensolver: chunks of whole GPU rounds,
+// each with one gather, one Gram, one tridiagonal reduction and one
+// eigenvalue launch.
+int entropy_many_own(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+                     double *out)
+{
+    Workspace &w = ws();
+    const int m = 1 << kept_count;
+    const std::size_t K = std::size_t{1} << (n - kept_count);
+    const std::size_t mm = static_cast<std::size_t>(m) * m;
+    std::size_t free_b = 0, total_b = 0;
+    MIPT_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    // per item: rho, the two FP32 planes, panel workspace, tridiagonal and result
+    const std::size_t per = mm * 16 + static_cast<std::size_t>(m) * K * 8 + static_cast<std::size_t>(m) * 32 * 16 +
+                            static_cast<std::size_t>(m) * 16 + 64;
+    const std::size_t budget = std::min<std::size_t>(std::size_t{32} << 30, free_b / 4);
+    const int maxchunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, budget / per)));
+    const int nchunks = (count + maxchunk - 1) / maxchunk;
+    const int chunk = (count + nchunks - 1) / nchunks;
+    MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, static_cast<std::size_t>(chunk) * mm));
+    std::vector<std::uint64_t> roff, emask, pv;
+    for (int c0 = 0; c0 < count; c0 += chunk)
+    {
+        const int nb = std::min(chunk, count - c0);
+        MIPT_HIP_TRY(build_rho_batch(psis + c0, n, kept + static_cast<std::size_t>(c0) * kept_count, kept_count, nb, stream,
+                                     w.rho_b, roff, emask, pv));
+        MIPT_HIP_TRY(own_entropies(m, nb, stream, w.rho_b, out + c0));  // synchronises the stream
+    }
+    return trim_batch_workspace();
+}
+
 // Von Neumann entropies (bits) of `count` items (state psis[i], kept set
 // kept + i*kept_count), all of the same n and kept size: the RDMs are built
 // into one strided buffer per chunk and their eigenvalues come from the
@@ -746,8 +957,9 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
     const int m = 1 << kept_count;
     const std::size_t mm = static_cast<std::size_t>(m) * m;
     const bool own = g_eig_solver.load() == 0 && m <= EIG_MAX_N;
-    if (!own) MIPT_HIP_TRY(ensure_handle(stream));
-    if (!own && m < 32)
+    if (own) return entropy_many_own(psis, n, kept, kept_count, count, stream, out);
+    MIPT_HIP_TRY(ensure_handle(stream));
+    if (m < 32)
     {
         // tiny RDMs on rocSOLVER: one solve each (cheap at this size)
         for (int i = 0; i < count; ++i)
@@ -760,24 +972,13 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
         }
         return OK;
     }
-    // RDMs per chunk: the library's eigensolver runs one workgroup per matrix,
-    // so a chunk should hold a few hundred large matrices; bounded by a quarter
-    // of the free device memory and by 8 GiB (matrix + panel workspace)
-    std::size_t free_b = 0, total_b = 0;
-    MIPT_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const std::size_t per = mm * 16 + (own ? static_cast<std::size_t>(m) * 32 * 16 + static_cast<std::size_t>(m) * 16 : 0);
-    const std::size_t budget = own ? std::min<std::size_t>(std::size_t{8} << 30, (free_b + w.rho_b_cap * 16) / 4)
-                                   : (std::size_t{1} << 30);
-    const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, budget / per)));
+    const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, (std::size_t{1} << 30) / (mm * 16))));
     MIPT_HIP_TRY(grow(w.rho_b, w.rho_b_cap, chunk * mm));
-    if (!own)
-    {
-        MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk) * m));
-        MIPT_HIP_TRY(grow(w.e_b, w.e_b_cap, static_cast<std::size_t>(chunk) * m));
-        MIPT_HIP_TRY(grow(w.info_b, w.info_b_cap, static_cast<std::size_t>(chunk)));
-    }
-    std::vector<double> lam(own ? 0 : static_cast<std::size_t>(chunk) * m);
-    std::vector<int> info(own ? 0 : chunk);
+    MIPT_HIP_TRY(grow(w.ev_b, w.ev_b_cap, static_cast<std::size_t>(chunk) * m));
+    MIPT_HIP_TRY(grow(w.e_b, w.e_b_cap, static_cast<std::size_t>(chunk) * m));
+    MIPT_HIP_TRY(grow(w.info_b, w.info_b_cap, static_cast<std::size_t>(chunk)));
+    std::vector<double> lam(static_cast<std::size_t>(chunk) * m);
+    std::vector<int> info(chunk);
     for (int c0 = 0; c0 < count; c0 += chunk)
     {
         const int nb = std::min(chunk, count - c0);
@@ -786,11 +987,6 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
             int mo = 0;
             MIPT_HIP_TRY(build_rho(psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
                                    kept_count, stream, mo, w.rho_b + i * mm));
-        }
-        if (own)
-        {
-            MIPT_HIP_TRY(own_entropies(m, nb, stream, w.rho_b, out + c0));
-            continue;
         }
         const rocblas_status st = rocsolver_zheevd_strided_batched(
             w.handle, rocblas_evect_none, rocblas_fill_lower, m, reinterpret_cast<rocblas_double_complex *>(w.rho_b), m,
