@@ -108,7 +108,7 @@ def main(argv=None):
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--per-p", type=int, default=256)
     ap.add_argument("--np", type=int, default=16)
-    ap.add_argument("--plain", type=int, default=4, help="trajectories timed on the plain path")
+    ap.add_argument("--plain", type=int, default=8, help="trajectories timed on the plain path")
     ap.add_argument("--replay", type=int, default=16,
                     help="trajectories checked against the FP64 oracle (one per host thread, up to 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle leg (profiling runs)")
@@ -226,19 +226,26 @@ def main(argv=None):
     log(f"timed steps done: {ms_per_step:.1f} ms/step, {value:.0f} layers/s; host issue of evolve "
         f"{1e3 * sum(host_issue[-args.steps:]) / args.steps:.1f} ms/step")
     # plain one-trajectory-per-launch path on the same trajectories' device draws
-    plain_lps = None
+    plain_lps = plain_host_lps = None
     if args.plain > 0:
+        # one C-ABI call per trajectory (blocked matrices staged once, the
+        # per-layer histogram round trip kept); the layer-by-layer host loop
+        # over the same engine is timed beside it
         eng = M.PlainEngine(False)
         idx = [int(i * ntraj / args.plain) for i in range(args.plain)]
         data = [b.export_layers(i) for i in idx]
-        M.run_plain_trajectory(eng, L, *data[0], True, 4, 0, 10)  # warm
+        M.run_plain_trajectory_native(eng, L, *data[0], True, 4, 0, 10)  # warm
+        t = time.perf_counter()
+        for g, f, u in data:
+            M.run_plain_trajectory_native(eng, L, g, f, u, True, 4, 0, 10)
+        dt = time.perf_counter() - t
+        plain_lps = len(data) * layers / dt
         t = time.perf_counter()
         for g, f, u in data:
             M.run_plain_trajectory(eng, L, g, f, u, True, 4, 0, 10)
-        dt = time.perf_counter() - t
-        plain_lps = len(data) * layers / dt
+        plain_host_lps = len(data) * layers / (time.perf_counter() - t)
         eng.close()
-        log(f"plain path: {plain_lps:.0f} layers/s")
+        log(f"plain path: {plain_lps:.0f} layers/s one call per trajectory, {plain_host_lps:.0f} layers/s host loop")
 
     result = {
         "metric": METRIC,
@@ -270,6 +277,10 @@ def main(argv=None):
         "autotune": tune,
         "graph": not args.no_graph,
         "plain_layers_per_s": None if plain_lps is None else round(plain_lps, 1),
+        "plain": None if plain_lps is None else {
+            "trajectories": args.plain, "block_targets": 4,
+            "driver": "one C-ABI call per trajectory (mipt_hip_engine_run_trajectory)",
+            "host_loop_layers_per_s": round(plain_host_lps, 1)},
         "batched_vs_plain": None if plain_lps is None else round(value / plain_lps, 2),
     }
 
@@ -384,9 +395,10 @@ def c1_leg(M):
     gpu_s = time.perf_counter() - t
     eng = M.PlainEngine(True)
     draws = [b.export_layers(i) for i in range(16)]
+    M.run_plain_trajectory_native(eng, n, *draws[0], False, 4, 0, 10)  # warm
     t = time.perf_counter()
     for g, f, u in draws:
-        M.run_plain_trajectory(eng, n, g, f, u, False, 4, 0, 10)
+        M.run_plain_trajectory_native(eng, n, g, f, u, False, 4, 0, 10)
     plain_s = time.perf_counter() - t
     eng.close()
     sh, tm = b.observables_all(3, 1)

@@ -160,6 +160,19 @@ int mipt_hip_engine_apply_bonds(mipt_hip_engine *engine, const int *q0, const in
  * may be NULL) receives each conditional P(1); surprisal = -log P(outcomes). */
 int mipt_hip_engine_measure_layer(mipt_hip_engine *engine, const int *sites, int count, const double *uniforms,
                                   int mode, int chunk_bits, int *outcomes, double *p1, double *surprisal);
+/* Replaces the per-trajectory loop Circuit1D::try_cusv
+ * (cpp/include/mipt/circuit.hpp:240-282) on this engine in one call: reset to
+ * |0...0>, then per layer the bonds of geometry (n, closed, grid_x: 0 = chain,
+ * else grid columns) fused into Kronecker blocks of at most block_targets
+ * qubits and applied, then the flagged sites measured as
+ * mipt_hip_engine_measure_layer does (mode, chunk_bits), with the per-layer
+ * histogram copied to the host.  Every layer's blocked matrices are staged to
+ * the device in one copy before the first layer.  gates: interleaved complex128
+ * [total bonds][4][4] in layer order; flags, uniforms: [layers][n] (mode 1:
+ * uniforms[t][chunk]); outcomes [layers][n] out, -1 where unmeasured. */
+int mipt_hip_engine_run_trajectory(mipt_hip_engine *engine, int n, int layers, int closed, int grid_x,
+                                   const double *gates_ri, const uint8_t *flags, const double *uniforms,
+                                   int block_targets, int mode, int chunk_bits, int *outcomes);
 int mipt_hip_engine_to_host(mipt_hip_engine *engine, void *host_buffer);
 int mipt_hip_engine_set_from_host(mipt_hip_engine *engine, const void *host_buffer);
 int mipt_hip_engine_squared_norm(mipt_hip_engine *engine, double *out);

@@ -75,6 +75,7 @@ def lib():
     L.mipt_hip_engine_apply_matrix.argtypes = [vp, ip, i, dp]
     L.mipt_hip_engine_apply_bonds.argtypes = [vp, ip, ip, dp, i, i]
     L.mipt_hip_engine_measure_layer.argtypes = [vp, ip, i, dp, i, i, ip, dp, dp]
+    L.mipt_hip_engine_run_trajectory.argtypes = [vp, i, i, i, i, dp, u8p, dp, i, i, i, ip]
     L.mipt_hip_engine_to_host.argtypes = [vp, vp]
     L.mipt_hip_engine_set_from_host.argtypes = [vp, vp]
     L.mipt_hip_engine_squared_norm.argtypes = [vp, dp]
@@ -295,12 +296,16 @@ class PlainEngine:
         return o, p1, sp.value
 
     def to_host(self) -> np.ndarray:
+        if self.n < 1:
+            raise ValueError("engine holds no state (reset, adopt or run a trajectory first)")
         out = np.zeros(1 << self.n, np.complex128 if self.fp64 else np.complex64)
         _check(lib().mipt_hip_engine_to_host(self._h, out.ctypes.data_as(ctypes.c_void_p)), "to_host")
         return out
 
     def set_from_host(self, psi: np.ndarray):
         a = np.ascontiguousarray(psi, np.complex128 if self.fp64 else np.complex64)
+        if self.n < 1 or a.size != (1 << self.n):
+            raise ValueError(f"state of {a.size} amplitudes for an engine of {self.n} qubits")
         _check(lib().mipt_hip_engine_set_from_host(self._h, a.ctypes.data_as(ctypes.c_void_p)), "set_from_host")
 
     def squared_norm(self) -> float:
@@ -349,10 +354,30 @@ def layer_bonds(n: int, layer: int, closed: bool = True, grid_x: int = 0):
     return out
 
 
+def run_plain_trajectory_native(engine: PlainEngine, n: int, gates, flags, uniforms, closed: bool = True,
+                                block_targets: int = 4, mode: int = 0, chunk_bits: int = 10, grid_x: int = 0):
+    """The try_cusv loop (cpp/include/mipt/circuit.hpp:240-282) in one C-ABI call
+    (mipt_hip_engine_run_trajectory).  Same arguments and result as
+    run_plain_trajectory."""
+    flags = np.ascontiguousarray(flags, np.uint8)
+    layers = flags.shape[0]
+    u = np.ascontiguousarray(uniforms, np.float64)
+    g = np.ascontiguousarray(gates, np.complex128)
+    outcomes = np.full((layers, n), -1, np.int32)
+    _check(lib().mipt_hip_engine_run_trajectory(engine._h, n, layers, int(closed), grid_x,
+                                                g.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(u),
+                                                block_targets, mode, chunk_bits, _ip(outcomes)),
+           "engine_run_trajectory")
+    engine.n = n  # the call reset the engine to n qubits
+    return outcomes
+
+
 def run_plain_trajectory(engine: PlainEngine, n: int, gates, flags, uniforms, closed: bool = True,
                          block_targets: int = 4, mode: int = 0, chunk_bits: int = 10, grid_x: int = 0):
     """The try_cusv loop (cpp/include/mipt/circuit.hpp:240-282) on the plain
-    engine: per layer, blocked bond gates, then the measured sites.
+    engine, driven layer by layer from Python: per layer, blocked bond gates,
+    then the measured sites.
     gates: complex128 [bonds,4,4]; flags/uniforms: [layers, n].  Returns outcomes [layers, n] (-1 unmeasured)."""
     engine.reset(n)
     layers = flags.shape[0]

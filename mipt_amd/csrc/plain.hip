@@ -340,12 +340,19 @@ struct PlainEngine
     DenseTargets *tg_dev = nullptr;
     HistScratch hs;
     std::vector<double> hist;
+    // one trajectory's blocked matrices and target tables, staged at once
+    void *traj_mat = nullptr;
+    std::size_t traj_mat_cap = 0;
+    DenseTargets *traj_tg = nullptr;
+    std::size_t traj_tg_cap = 0;
 
     ~PlainEngine()
     {
         if (owned) (void)hipFree(owned);
         if (mat_dev) (void)hipFree(mat_dev);
         if (tg_dev) (void)hipFree(tg_dev);
+        if (traj_mat) (void)hipFree(traj_mat);
+        if (traj_tg) (void)hipFree(traj_tg);
     }
 
     std::size_t elem() const { return fp64 ? 16 : 8; }
@@ -419,7 +426,9 @@ struct PlainEngine
     // Kronecker blocking of consecutive disjoint bonds up to block_targets
     // (cpp/include/mipt/cusv/gates.hpp:295-333; tensor order of
     // cpp/include/mipt/cusv/matrix.hpp:214-235: later ops take the high bits).
-    int apply_bonds(const int *q0, const int *q1, const double *gates, int count, int block_targets)
+    // Calls emit(targets, k, matrix) once per block, in order.
+    template <typename Emit>
+    static int block_bonds(const int *q0, const int *q1, const double *gates, int count, int block_targets, Emit emit)
     {
         int i = 0;
         while (i < count)
@@ -453,11 +462,147 @@ struct PlainEngine
                 targets.push_back(q1[j]);
                 ++j;
             }
-            MIPT_HIP_TRY(apply_matrix(targets.data(), static_cast<int>(targets.size()), m.data()));
+            MIPT_HIP_TRY(emit(targets.data(), static_cast<int>(targets.size()), m.data()));
             i = j;
         }
         return OK;
     }
+
+    int apply_bonds(const int *q0, const int *q1, const double *gates, int count, int block_targets)
+    {
+        return block_bonds(q0, q1, gates, count, block_targets,
+                           [&](const int *t, int k, const double *m) { return apply_matrix(t, k, m); });
+    }
+
+    template <typename Real>
+    int launch_dense_at(int k, std::uint64_t groups, const DenseTargets *tg, const void *mat)
+    {
+        using C = typename CxT<Real>::type;
+        auto *psi = static_cast<C *>(state);
+        const auto *m = static_cast<const C *>(mat);
+        const dim3 grid(grid_for(groups, 256));
+        switch (k)
+        {
+        case 1: hipLaunchKernelGGL((apply_dense_kernel<Real, 1>), grid, dim3(256), 0, stream, psi, groups, tg, m); break;
+        case 2: hipLaunchKernelGGL((apply_dense_kernel<Real, 2>), grid, dim3(256), 0, stream, psi, groups, tg, m); break;
+        case 3: hipLaunchKernelGGL((apply_dense_kernel<Real, 3>), grid, dim3(256), 0, stream, psi, groups, tg, m); break;
+        case 4: hipLaunchKernelGGL((apply_dense_kernel<Real, 4>), grid, dim3(256), 0, stream, psi, groups, tg, m); break;
+        case 5: hipLaunchKernelGGL((apply_dense_kernel<Real, 5>), grid, dim3(256), 0, stream, psi, groups, tg, m); break;
+        default: set_error("dense apply supports 1..5 targets, got %d", k); return E_UNSUPPORTED;
+        }
+        MIPT_HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+
+    // One whole trajectory (Circuit1D::try_cusv, cpp/include/mipt/circuit.hpp:240-282)
+    // from |0...0>: every layer's blocked matrices and target tables are built
+    // on the host first and staged to the device in one copy, so the block
+    // applies run back to back with no host round trip; each measured layer
+    // keeps the reference's structure (engine.hpp:293-368): FP64 histogram,
+    // copy to the host, draw, collapse.
+    int run_trajectory(int nq, int layers, int closed, int gx, const double *gates, const std::uint8_t *flags,
+                       const double *uniforms, int block_targets, int mode, int chunk_bits, int *outcomes)
+    {
+        MIPT_HIP_TRY(mipt_hip_engine_reset_impl(nq));
+        struct Blk
+        {
+            int k;
+            std::size_t mat_off;  // elements (complex) into the staged matrices
+        };
+        std::vector<Blk> blocks;
+        std::vector<int> layer_first(layers + 1, 0);
+        std::vector<DenseTargets> tgs;
+        std::vector<double> mats;  // interleaved complex, FP64 (narrowed below for complex64)
+        std::size_t g = 0;
+        std::vector<int> q0, q1;
+        for (int t = 0; t < layers; ++t)
+        {
+            layer_first[t] = static_cast<int>(blocks.size());
+            const int nb = geo_bond_count(nq, gx, t, closed);
+            q0.resize(nb);
+            q1.resize(nb);
+            for (int b = 0; b < nb; ++b) geo_bond(nq, gx, t, b, closed, q0[b], q1[b]);
+            MIPT_HIP_TRY(block_bonds(q0.data(), q1.data(), gates + 32 * g, nb, block_targets,
+                                     [&](const int *tt, int k, const double *m) {
+                                         DenseTargets tg{};
+                                         tg.k = k;
+                                         for (int j = 0; j < k; ++j) tg.sorted[j] = tt[j];
+                                         std::sort(tg.sorted, tg.sorted + k);
+                                         for (int a = 0; a < (1 << k); ++a)
+                                             for (int j = 0; j < k; ++j)
+                                                 if ((a >> j) & 1) tg.off[a] |= std::uint64_t{1} << tt[j];
+                                         tgs.push_back(tg);
+                                         blocks.push_back({k, mats.size() / 2});
+                                         mats.insert(mats.end(), m, m + 2 * (std::size_t{1} << (2 * k)));
+                                         return OK;
+                                     }));
+            g += nb;
+        }
+        layer_first[layers] = static_cast<int>(blocks.size());
+        const std::size_t celem = fp64 ? 16 : 8;
+        const std::size_t ncx = mats.size() / 2;
+        if (ncx * celem > traj_mat_cap)
+        {
+            if (traj_mat) MIPT_HIP_CHECK(hipFree(traj_mat));
+            traj_mat = nullptr;
+            MIPT_HIP_CHECK(hipMalloc(&traj_mat, ncx * celem));
+            traj_mat_cap = ncx * celem;
+        }
+        if (tgs.size() > traj_tg_cap)
+        {
+            if (traj_tg) MIPT_HIP_CHECK(hipFree(traj_tg));
+            traj_tg = nullptr;
+            MIPT_HIP_CHECK(hipMalloc(&traj_tg, tgs.size() * sizeof(DenseTargets)));
+            traj_tg_cap = tgs.size();
+        }
+        std::vector<float> narrow;
+        if (!fp64)
+        {
+            narrow.resize(mats.size());
+            for (std::size_t i = 0; i < mats.size(); ++i) narrow[i] = static_cast<float>(mats[i]);
+        }
+        if (ncx)
+            MIPT_HIP_CHECK(hipMemcpyAsync(traj_mat, fp64 ? static_cast<const void *>(mats.data()) : narrow.data(),
+                                          ncx * celem, hipMemcpyHostToDevice, stream));
+        if (!tgs.empty())
+            MIPT_HIP_CHECK(hipMemcpyAsync(traj_tg, tgs.data(), tgs.size() * sizeof(DenseTargets), hipMemcpyHostToDevice,
+                                          stream));
+        std::vector<int> sites;
+        std::vector<double> u;
+        for (int t = 0; t < layers; ++t)
+        {
+            for (int bi = layer_first[t]; bi < layer_first[t + 1]; ++bi)
+            {
+                const Blk &b = blocks[bi];
+                const void *m = static_cast<const char *>(traj_mat) + b.mat_off * celem;
+                MIPT_HIP_TRY(fp64 ? launch_dense_at<double>(b.k, dim() >> b.k, traj_tg + bi, m)
+                                  : launch_dense_at<float>(b.k, dim() >> b.k, traj_tg + bi, m));
+            }
+            sites.clear();
+            u.clear();
+            for (int q = 0; q < nq; ++q)
+            {
+                outcomes[static_cast<std::size_t>(t) * nq + q] = -1;
+                if (flags[static_cast<std::size_t>(t) * nq + q])
+                {
+                    sites.push_back(q);
+                    if (mode == 0) u.push_back(uniforms[static_cast<std::size_t>(t) * nq + q]);
+                }
+            }
+            if (sites.empty()) continue;
+            if (mode == 1) u.assign(uniforms + static_cast<std::size_t>(t) * nq, uniforms + static_cast<std::size_t>(t + 1) * nq);
+            std::vector<int> o(sites.size());
+            MIPT_HIP_TRY(fp64 ? measure_layer_t<double>(sites.data(), static_cast<int>(sites.size()), u.data(), mode,
+                                                        chunk_bits, o.data(), nullptr, nullptr)
+                              : measure_layer_t<float>(sites.data(), static_cast<int>(sites.size()), u.data(), mode,
+                                                       chunk_bits, o.data(), nullptr, nullptr));
+            for (std::size_t j = 0; j < sites.size(); ++j) outcomes[static_cast<std::size_t>(t) * nq + sites[j]] = o[j];
+        }
+        MIPT_HIP_CHECK(hipStreamSynchronize(stream));
+        return OK;
+    }
+
+    int mipt_hip_engine_reset_impl(int nn);
 
     template <typename Real>
     int measure_layer_t(const int *sites, int count, const double *uniforms, int mode, int chunk_bits, int *outcomes,
@@ -522,6 +667,28 @@ struct PlainEngine
         return OK;
     }
 };
+
+int PlainEngine::mipt_hip_engine_reset_impl(int nn)
+{
+    if (nn < 1 || nn > 34) { set_error("engine_reset: bad arguments"); return E_ARG; }
+    if (!owned || n != nn)
+    {
+        if (owned) MIPT_HIP_CHECK(hipFree(owned));
+        owned = nullptr;
+        MIPT_HIP_CHECK(hipMalloc(&owned, (std::size_t{1} << nn) * (fp64 ? 16 : 8)));
+    }
+    state = owned;
+    n = nn;
+    const std::uint64_t d = std::uint64_t{1} << nn;
+    if (fp64)
+        hipLaunchKernelGGL(set_basis_kernel<double>, dim3(grid_for(d, 256)), dim3(256), 0, stream,
+                           static_cast<double2 *>(state), d, std::uint64_t{0});
+    else
+        hipLaunchKernelGGL(set_basis_kernel<float>, dim3(grid_for(d, 256)), dim3(256), 0, stream,
+                           static_cast<float2 *>(state), d, std::uint64_t{0});
+    MIPT_HIP_CHECK(hipGetLastError());
+    return OK;
+}
 
 // Plain device copy used to calibrate hardware traffic counters
 // (FETCH_SIZE / WRITE_SIZE) against a known byte count.
@@ -598,24 +765,29 @@ int mipt_hip_engine_adopt(mipt_hip_engine *e, void *state, int n)
 
 int mipt_hip_engine_reset(mipt_hip_engine *e, int n)
 {
-    if (!e || n < 1 || n > 34) { set_error("engine_reset: bad arguments"); return E_ARG; }
-    if (!e->owned || e->n != n)
+    if (!e) { set_error("engine_reset: null engine"); return E_ARG; }
+    return e->mipt_hip_engine_reset_impl(n);
+}
+
+int mipt_hip_engine_run_trajectory(mipt_hip_engine *e, int n, int layers, int closed, int grid_x, const double *gates,
+                                   const uint8_t *flags, const double *uniforms, int block_targets, int mode,
+                                   int chunk_bits, int *outcomes)
+{
+    if (!e || n < 2 || n > 34 || layers < 0 || !flags || !uniforms || !outcomes || (!gates && layers > 0) ||
+        block_targets < 2 || block_targets > 5 || (mode != 0 && mode != 1) || chunk_bits < 1 || chunk_bits > 16 ||
+        grid_x < 0 || (grid_x > 0 && (n % grid_x != 0 || grid_x < 2)))
     {
-        if (e->owned) MIPT_HIP_CHECK(hipFree(e->owned));
-        e->owned = nullptr;
-        MIPT_HIP_CHECK(hipMalloc(&e->owned, (std::size_t{1} << n) * (e->fp64 ? 16 : 8)));
+        set_error("engine_run_trajectory: bad arguments");
+        return E_ARG;
     }
-    e->state = e->owned;
-    e->n = n;
-    const std::uint64_t dim = std::uint64_t{1} << n;
-    if (e->fp64)
-        hipLaunchKernelGGL(set_basis_kernel<double>, dim3(grid_for(dim, 256)), dim3(256), 0, e->stream,
-                           static_cast<double2 *>(e->state), dim, std::uint64_t{0});
-    else
-        hipLaunchKernelGGL(set_basis_kernel<float>, dim3(grid_for(dim, 256)), dim3(256), 0, e->stream,
-                           static_cast<float2 *>(e->state), dim, std::uint64_t{0});
-    MIPT_HIP_CHECK(hipGetLastError());
-    return OK;
+    for (std::size_t i = 0; i < static_cast<std::size_t>(layers) * n; ++i)
+        if (flags[i] && mode == 0 && !(uniforms[i] >= 0.0 && uniforms[i] < 1.0))
+        {
+            set_error("engine_run_trajectory: uniform out of [0,1)");
+            return E_ARG;
+        }
+    return e->run_trajectory(n, layers, closed ? 1 : 0, grid_x, gates, flags, uniforms, block_targets, mode, chunk_bits,
+                             outcomes);
 }
 
 void *mipt_hip_engine_data(mipt_hip_engine *e) { return e ? e->state : nullptr; }

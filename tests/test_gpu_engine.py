@@ -204,6 +204,51 @@ def test_plain_trajectory_replay(M, fp64):
     assert np.abs(e.to_host() - psi).max() < tol
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("n,layers,p,closed,grid_x,blk", [(10, 16, 0.3, True, 0, 4), (12, 12, 0.2, False, 0, 5),
+                                                         (12, 8, 0.15, False, 4, 4), (16, 6, 0.1, True, 0, 2)])
+def test_plain_trajectory_one_call_matches_loop_and_oracle(M, fp64, n, layers, p, closed, grid_x, blk):
+    # mipt_hip_engine_run_trajectory: the whole try_cusv loop in one C-ABI call,
+    # against the layer-by-layer host loop (same outcomes, bit-identical state)
+    # and against the oracle
+    print(f"plain one-call n={n} grid_x={grid_x} fp64={fp64}", flush=True)
+    if grid_x:
+        psi, want, p1, f, u = O.simulate_grid(grid_x, n // grid_x, layers, p, 77, closed)
+        g = []
+        for t in range(layers):
+            for b, _ in enumerate(O.grid_layer_bonds(grid_x, n // grid_x, t, closed)):
+                gg = np.zeros(32)
+                O.lib().orc_philox_haar(77, t, b, gg.ctypes.data_as(O._dp))
+                g.append((gg[0::2] + 1j * gg[1::2]).reshape(4, 4))
+        g = np.array(g)
+    else:
+        psi, want, p1, (g, f, u) = O.simulate(n, layers, p, 77, closed)
+    e1, e2 = M.PlainEngine(fp64), M.PlainEngine(fp64)
+    got1 = M.run_plain_trajectory_native(e1, n, g, f, u, closed, blk, 0, 10, grid_x)
+    got2 = M.run_plain_trajectory(e2, n, g, f, u, closed, blk, 0, 10, grid_x)
+    assert np.array_equal(got1, got2)
+    assert np.array_equal(e1.to_host(), e2.to_host())
+    check_outcomes(got1, want, p1, u, f, "plain one-call")
+    tol = 1e-11 if fp64 else 1e-5
+    if not (np.abs(u - p1) < FLAG_EPS).any():
+        assert np.abs(e1.to_host() - psi).max() < tol
+    e1.close()
+    e2.close()
+
+
+def test_plain_trajectory_one_call_joint_mode(M):
+    n, layers = 8, 10
+    gates, flags, _ = O.build_layers(n, layers, 0.4, 9)
+    rng = np.random.default_rng(1)
+    unif = rng.random((layers, n))
+    ref = O.zero_state(n)
+    want, _ = O.run_layers(ref, n, gates, flags, unif, mode=1, chunk_bits=3)
+    e = M.PlainEngine(True)
+    got = M.run_plain_trajectory_native(e, n, gates, flags, unif, True, 4, 1, 3)
+    assert np.array_equal(got, want)
+    assert np.abs(e.to_host() - ref).max() < 1e-11
+
+
 def test_plain_joint_mode_matches_oracle(M):
     # engine.hpp:293-368 joint chunk sampling (one uniform per chunk)
     n, layers = 8, 10
