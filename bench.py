@@ -160,9 +160,11 @@ def main(argv=None):
     tune = None
     if not args.no_autotune and args.tile_bits == 0 and args.bond_arith == 0:
         b.generate()
-        best, ms = b.autotune(4)
+        best, plan, ms = b.autotune(4, with_plan=True)
         names = ["13b/256t valu", "13b/256t valu+mfma", "13b/256t mfma", "14b/256t", "13b/128t", "12b/64t"]
-        tune = {"choice": names[best] if best >= 0 else None, "probe_ms": {n: round(float(m), 2) for n, m in zip(names, ms)}}
+        plans = ["plan 1 (register-slot bonds first)", "plan 2 (spread windows)", "plan 3 (both)"]
+        tune = {"choice": names[best] if best >= 0 else None, "plan_variant": plan,
+                "probe_ms": {n: round(float(m), 2) for n, m in zip(names + plans, ms)}}
         log(f"autotune: {tune}")
     if not args.no_graph:
         b.set_graph(True)

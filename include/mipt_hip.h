@@ -222,6 +222,8 @@ typedef struct mipt_hip_batch_config
     int bond_arith;     /* bond arithmetic of the tile sweeps: 0 = the autotuner's choice (VALU before it
                            runs); 1 = VALU only; 2 = VALU and matrix cores; 3 = matrix cores only
                            (v_mfma_f32_4x4x1_16b_f32) */
+    int plan;           /* layer plan variant 0..3 (qubit-block assignment, see mipt_hip_batch_autotune);
+                           replaced by the autotuner's choice once it has run */
 } mipt_hip_batch_config;
 
 int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *hip_stream, mipt_hip_batch **out);
@@ -253,10 +255,16 @@ int mipt_hip_batch_synchronize(mipt_hip_batch *batch);
  * matrix cores; 3: 2^14 tiles, 256 threads; 4: 2^13 tiles, 128 threads;
  * 5: 2^12 tiles, 64 threads (3-5 on the VALU) -- and keeps the fastest for
  * the tile-swept trajectories of a batch created with tile_bits = 0.  The
- * batch is re-armed before and after (draws kept).  best_index (may be NULL)
- * receives the choice (-1: nothing is tile-swept), ms (may be NULL, 6 doubles)
- * each candidate's time. */
-int mipt_hip_batch_autotune(mipt_hip_batch *batch, int probe_layers, int *best_index, double *ms);
+ * batch is re-armed before and after (draws kept).  Then, on the winning
+ * configuration, the qubit-block assignment: layer plan variants 1..3 (bit 0:
+ * a bond local to several windows goes where it lands on a free register-slot
+ * pair, needing no LDS exchange, before counts are balanced; bit 1: three or
+ * more chain windows spread their overlaps evenly) against the default plan 0.
+ * best_index (may be NULL) receives the configuration (-1: nothing is
+ * tile-swept), best_plan (may be NULL) the plan variant, ms (may be NULL,
+ * 9 doubles) each configuration's time, then plan variants 1..3 (0 where a
+ * candidate does not apply). */
+int mipt_hip_batch_autotune(mipt_hip_batch *batch, int probe_layers, int *best_index, int *best_plan, double *ms);
 /* 1: full evolutions run as one captured HIP graph (both streams), captured
  * on the second full evolve after the draws or configuration last changed
  * and replayed afterwards; 0 (default): eager launches. */

@@ -104,7 +104,7 @@ def lib():
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
     L.mipt_hip_batch_records.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_int8)]
     L.mipt_hip_batch_observables_all.argtypes = [vp, i, i, dp, dp]
-    L.mipt_hip_batch_autotune.argtypes = [vp, i, ip, dp]
+    L.mipt_hip_batch_autotune.argtypes = [vp, i, ip, ip, dp]
     L.mipt_hip_batch_rho3.argtypes = [vp, i, i, ip, i, dp]
     L.mipt_hip_mipt_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
     L.mipt_hip_entropy_main.argtypes = [i, ctypes.POINTER(ctypes.c_char_p)]
@@ -406,7 +406,7 @@ class TrajSpec(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("tile_bits", ctypes.c_int), ("max_hist_bits", ctypes.c_int), ("threads", ctypes.c_int),
-                ("bond_arith", ctypes.c_int)]
+                ("bond_arith", ctypes.c_int), ("plan", ctypes.c_int)]
 
 
 @dataclass
@@ -434,8 +434,9 @@ class Spec:
 class Batch:
     """Device-resident batch of trajectories (mixed chain lengths allowed)."""
 
-    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None, threads: int = 0, bond_arith: int = 0):
-        cfg = BatchConfig(tile_bits, 0, threads, bond_arith)
+    def __init__(self, specs, tile_bits: int = 0, stream: int | None = None, threads: int = 0, bond_arith: int = 0,
+                 plan: int = 0):
+        cfg = BatchConfig(tile_bits, 0, threads, bond_arith, plan)
         h = ctypes.c_void_p()
         _check(lib().mipt_hip_batch_create(ctypes.byref(cfg), ctypes.c_void_p(stream or 0), ctypes.byref(h)),
                "batch_create")
@@ -591,12 +592,17 @@ class Batch:
                "batch_translation_entropies")
         return s1, s2
 
-    def autotune(self, probe_layers: int = 4):
-        """Pick the fastest tile configuration on this batch; returns (index, ms per candidate)."""
-        best = ctypes.c_int()
-        ms = np.zeros(6)
-        _check(lib().mipt_hip_batch_autotune(self._h, probe_layers, ctypes.byref(best), _dp(ms)), "batch_autotune")
-        return best.value, ms
+    def autotune(self, probe_layers: int = 4, with_plan: bool = False):
+        """Pick the fastest tile configuration, then layer plan variant, on this
+        batch; returns (configuration index, ms per configuration) or, with
+        with_plan, (configuration index, plan variant, ms [9])."""
+        best, plan = ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(9)
+        _check(lib().mipt_hip_batch_autotune(self._h, probe_layers, ctypes.byref(best), ctypes.byref(plan), _dp(ms)),
+               "batch_autotune")
+        if with_plan:
+            return best.value, plan.value, ms
+        return best.value, ms[:6]
 
     def set_graph(self, enable: bool = True):
         _check(lib().mipt_hip_batch_set_graph(self._h, int(enable)), "batch_set_graph")

@@ -28,6 +28,7 @@
 #include "state_ops.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1400,6 +1401,11 @@ struct Config
 {
     int rb, nt, c;
     int mx = 0;  // bond arithmetic: 0 VALU, 1 VALU + matrix cores, 2 matrix cores
+    // layer plan variant (qubit-block assignment): bit 0 puts a bond local to
+    // several windows where it lands on a free register-slot pair (no LDS
+    // exchange) before balancing counts; bit 1 spreads the overlaps of three
+    // or more chain windows evenly instead of packing the slack into the last
+    int plan = 0;
     int k() const { return 6 + rb + (nt == 256 ? 2 : nt == 128 ? 1 : 0); }
 };
 
@@ -1482,6 +1488,16 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
             if (in_window(s0, b0[b]) && in_window(s0, b1[b])) return true;
         return false;
     };
+    if ((cf.plan & 2) && starts.size() >= 3)
+    {
+        const std::vector<int> packed = starts;
+        const int W = static_cast<int>(starts.size());
+        for (int i = 0; i < W; ++i)
+            starts[i] = c + static_cast<int>(std::lround(static_cast<double>(i) * (nq - wbits - c) / (W - 1)));
+        bool cov = true;
+        for (int b = 0; b < nb && cov; ++b) cov = covered_by_any(b);
+        if (!cov) starts = packed;
+    }
     bool all = true;
     for (int b = 0; b < nb && all; ++b) all = covered_by_any(b);
     if (!all)
@@ -1518,6 +1534,20 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
     }
     auto local = [&](int p, int q) { return in_window(starts[p], q); };
     std::vector<std::vector<int>> assigned(starts.size());
+    // register-slot pair a bond takes in window p without an LDS stage (0: bits
+    // (1,2), 1: bits (3,4)), or -1
+    std::vector<std::array<int, 2>> slot_used(starts.size(), std::array<int, 2>{0, 0});
+    auto m0_slot_of = [&](int p, int b) {
+        const int s0 = starts[p];
+        auto uq = [&](int q) { return q < c ? q : q - s0 + c; };
+        const int ra = m0_reg(uq(b0[b]), cf.rb), rc = m0_reg(uq(b1[b]), cf.rb);
+        return (ra == 1 && rc == 2) ? 0 : ((ra == 3 && rc == 4) ? 1 : -1);
+    };
+    auto take = [&](int p, int b) {
+        assigned[p].push_back(b);
+        const int sl = m0_slot_of(p, b);
+        if (sl >= 0) slot_used[p][sl] = 1;
+    };
     // bonds local to exactly one window first, then balance the rest
     std::vector<int> flexible;
     for (int b = 0; b < nb; ++b)
@@ -1530,17 +1560,25 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
             set_error("tile plan: bond (%d,%d) is local to no window (L=%d k=%d)", b0[b], b1[b], L, k);
             return E_UNSUPPORTED;
         }
-        if (cand.size() == 1) assigned[cand[0]].push_back(b);
+        if (cand.size() == 1) take(cand[0], b);
         else flexible.push_back(b);
     }
     for (int b : flexible)
     {
         int best = -1;
-        for (std::size_t p = 0; p < starts.size(); ++p)
-            if (local(static_cast<int>(p), b0[b]) && local(static_cast<int>(p), b1[b]) &&
-                (best < 0 || assigned[p].size() < assigned[best].size()))
-                best = static_cast<int>(p);
-        assigned[best].push_back(b);
+        if (cf.plan & 1)
+            for (std::size_t p = 0; p < starts.size() && best < 0; ++p)
+            {
+                const int pi = static_cast<int>(p);
+                const int sl = (local(pi, b0[b]) && local(pi, b1[b])) ? m0_slot_of(pi, b) : -1;
+                if (sl >= 0 && !slot_used[p][sl]) best = pi;
+            }
+        if (best < 0)
+            for (std::size_t p = 0; p < starts.size(); ++p)
+                if (local(static_cast<int>(p), b0[b]) && local(static_cast<int>(p), b1[b]) &&
+                    (best < 0 || assigned[p].size() < assigned[best].size()))
+                    best = static_cast<int>(p);
+        take(best, b);
     }
     for (std::size_t p = 0; p < starts.size(); ++p)
     {
@@ -1797,6 +1835,7 @@ struct Batch
         if (cfg.tile_bits == 12) cf = Config{6, 64, 4};
         // 64-register tiles are built with the split arithmetic only
         if (cfg.bond_arith > 0 && cf.rb == 5) cf.mx = cfg.bond_arith - 1;
+        cf.plan = cfg.plan;
         return cf;
     }
 
@@ -2231,7 +2270,14 @@ struct Batch
         }
         MIPT_HIP_CHECK(hipEventRecord(ev_fork, stream));
         MIPT_HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
-        int tiled_groups = 0;
+        // tiled groups of every shape, dealt to the in-flight streams below
+        struct TiledTask
+        {
+            int q, qe, t1;
+            double cost;  // amplitude-layers x sweeps per layer
+        };
+        std::vector<TiledTask> tasks;
+        double small_cost = 0.0;
         int pos = 0;
         while (pos < count)
         {
@@ -2261,26 +2307,43 @@ struct Batch
                 hipLaunchKernelGGL(small_evolve_kernel, dim3(end - pos), dim3(SMALL_THREADS), lds, stream, a,
                                    d_traj_list + pos, t0, t1, t0 == 0 ? 1 : 0);
                 MIPT_HIP_CHECK(hipGetLastError());
+                small_cost += static_cast<double>(end - pos) * static_cast<double>(std::size_t{1} << nq) * (t1 - t0);
             }
             else
             {
-                // groups of trajectories, two in flight (one per stream)
+                // groups of trajectories of at most the per-stream budget
                 const std::size_t bytes = (std::size_t{1} << nq) * 8;
                 static const std::size_t budget = [] {
                     const char *e = std::getenv("MIPT_HIP_GROUP_MIB");  // per-stream group budget
                     const long v = e ? std::atol(e) : 0;
                     return static_cast<std::size_t>(v > 0 ? v : 896) << 20;
                 }();
-                int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
+                const int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
+                const TrajDesc &d0 = desc[order[pos]];
+                LayerPlan lp0;
+                const int sweeps = build_layer_plan(d0.L, d0.nq, d0.closed, d0.gx, 0, config_for(d0.nq), lp0) == OK
+                                       ? static_cast<int>(lp0.passes.size())
+                                       : 2;
                 for (int q = pos; q < end; q += g)
                 {
                     const int qe = std::min(end, q + g);
-                    std::vector<int> grp(order.begin() + q, order.begin() + qe);
-                    MIPT_HIP_TRY(evolve_group_tiled(grp, q, t0, t1, streams > 1 ? (tiled_groups & 1) : 0));
-                    ++tiled_groups;
+                    tasks.push_back({q, qe, t1, static_cast<double>(qe - q) * static_cast<double>(std::size_t{1} << nq) *
+                                                    (t1 - t0) * sweeps});
                 }
             }
             pos = end;
+        }
+        // a mixed-size batch is one queue: groups longest first, each to the
+        // stream with the least work so far (the small kernels run on the
+        // batch stream), so the streams finish together
+        std::stable_sort(tasks.begin(), tasks.end(), [](const TiledTask &x, const TiledTask &y) { return x.cost > y.cost; });
+        double load[2] = {small_cost, 0.0};
+        for (const TiledTask &tk : tasks)
+        {
+            const int lane = (streams > 1 && load[1] < load[0]) ? 1 : 0;
+            load[lane] += tk.cost;
+            std::vector<int> grp(order.begin() + tk.q, order.begin() + tk.qe);
+            MIPT_HIP_TRY(evolve_group_tiled(grp, tk.q, t0, tk.t1, lane));
         }
         ls = stream;
         MIPT_HIP_CHECK(hipEventRecord(ev_join, side));
@@ -2300,7 +2363,7 @@ struct Batch
     // shapes) under each tile configuration and keeps the fastest for the
     // tile-swept qubit counts; the batch is re-armed before and after, so the
     // draws stay and the next evolve starts from the initial states.
-    int autotune(int probe_layers, int *best_index, double *ms_out)
+    int autotune(int probe_layers, int *best_index, int *best_plan, double *ms_out)
     {
         if (!flags_on_host)
         {
@@ -2321,51 +2384,84 @@ struct Batch
                 if (std::find(nqs.begin(), nqs.end(), d.nq) == nqs.end()) nqs.push_back(d.nq);
             }
         if (best_index) *best_index = -1;
+        if (best_plan) *best_plan = 0;
+        if (ms_out)
+            for (int c = 0; c < AUTOTUNE_CANDIDATES + AUTOTUNE_PLANS - 1; ++c) ms_out[c] = 0.0;
         if (nqs.empty() || depth < 1) return OK;  // nothing tile-swept
         const Config cand[AUTOTUNE_CANDIDATES] = {Config{5, 256, 5, 0}, Config{5, 256, 5, 1}, Config{5, 256, 5, 2},
                                                   Config{6, 256, 6, 0}, Config{6, 128, 5, 0}, Config{6, 64, 4, 0}};
-        const int ncand = static_cast<int>(sizeof(cand) / sizeof(cand[0]));
         drop_graph();
         hipEvent_t e0, e1;
         MIPT_HIP_CHECK(hipEventCreate(&e0));
         MIPT_HIP_CHECK(hipEventCreate(&e1));
-        int best = -1;
-        double best_ms = 0.0;
         int st = OK;
-        for (int c = 0; c < ncand && st == OK; ++c)
-        {
-            for (int nq : nqs) tuned[nq] = cand[c];
-            // every plan must exist for every shape (a candidate may not fit)
-            bool ok = true;
+        // time one configuration on this batch's own draws (second of two runs);
+        // < 0 when some layer plan does not exist for it
+        auto probe = [&](const Config &cf, double &ms_best) -> int {
+            for (int nq : nqs) tuned[nq] = cf;
             for (const TrajDesc &d : desc)
                 if (d.nq > SMALL_MAX_QUBITS)
-                    for (int j = 0; j < period_layers(d.gx) && ok; ++j)
+                    for (int j = 0; j < period_layers(d.gx); ++j)
                     {
                         LayerPlan lp;
-                        ok = build_layer_plan(d.L, d.nq, d.closed, d.gx, j, cand[c], lp) == OK;
+                        if (build_layer_plan(d.L, d.nq, d.closed, d.gx, j, cf, lp) != OK) return -1;
                     }
-            if (!ok) continue;
-            for (int rep = 0; rep < 2 && st == OK; ++rep)  // first run warms buffers and code
+            for (int rep = 0; rep < 2; ++rep)  // the first run warms buffers and code
             {
-                if ((st = reset()) != OK) break;
-                if (hipEventRecord(e0, stream) != hipSuccess) { st = E_ARG; break; }
-                if ((st = evolve_body(0, depth)) != OK) break;
-                if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { st = E_ARG; break; }
+                MIPT_HIP_TRY(reset());
+                MIPT_HIP_CHECK(hipEventRecord(e0, stream));
+                MIPT_HIP_TRY(evolve_body(0, depth));
+                MIPT_HIP_CHECK(hipEventRecord(e1, stream));
+                MIPT_HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms_out) ms_out[c] = ms;
-                if (rep == 1 && (best < 0 || ms < best_ms))
-                {
-                    best = c;
-                    best_ms = ms;
-                }
+                MIPT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                ms_best = ms;
+            }
+            return OK;
+        };
+        // tile shapes and bond arithmetic on the default layer plans ...
+        int best = -1;
+        double best_ms = 0.0;
+        for (int c = 0; c < AUTOTUNE_CANDIDATES && st == OK; ++c)
+        {
+            double ms = 0.0;
+            const int r = probe(cand[c], ms);
+            if (r < 0) continue;
+            st = r;
+            if (st != OK) break;
+            if (ms_out) ms_out[c] = ms;
+            if (best < 0 || ms < best_ms)
+            {
+                best = c;
+                best_ms = ms;
+            }
+        }
+        // ... then the qubit-block assignment (layer plan variants) of the winner
+        int plan = 0;
+        for (int v = 1; v < AUTOTUNE_PLANS && st == OK && best >= 0; ++v)
+        {
+            Config cf = cand[best];
+            cf.plan = v;
+            double ms = 0.0;
+            const int r = probe(cf, ms);
+            if (r < 0) continue;
+            st = r;
+            if (st != OK) break;
+            if (ms_out) ms_out[AUTOTUNE_CANDIDATES + v - 1] = ms;
+            if (ms < best_ms)
+            {
+                plan = v;
+                best_ms = ms;
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (st != OK) return st;
-        for (int nq : nqs) tuned[nq] = cand[best < 0 ? 0 : best];
+        Config win = cand[best < 0 ? 0 : best];
+        win.plan = plan;
+        for (int nq : nqs) tuned[nq] = win;
         if (best_index) *best_index = best;
+        if (best_plan) *best_plan = plan;
         return reset();
     }
 
@@ -2488,6 +2584,12 @@ int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *stream, mip
         set_error("batch_create: tile_bits must be 0 or 12..14");
         return E_ARG;
     }
+    if (b->cfg.plan < 0 || b->cfg.plan >= AUTOTUNE_PLANS)
+    {
+        delete b;
+        set_error("batch_create: plan must be 0..%d", AUTOTUNE_PLANS - 1);
+        return E_ARG;
+    }
     if (b->cfg.bond_arith < 0 || b->cfg.bond_arith > 3 ||
         (b->cfg.bond_arith >= 2 && ((b->cfg.tile_bits != 0 && b->cfg.tile_bits != 13) || b->cfg.threads == 128)))
     {
@@ -2568,10 +2670,10 @@ int mipt_hip_batch_evolve_range(mipt_hip_batch *b, int t0, int t1)
     return b->evolve(t0, t1);
 }
 
-int mipt_hip_batch_autotune(mipt_hip_batch *b, int probe_layers, int *best_index, double *ms)
+int mipt_hip_batch_autotune(mipt_hip_batch *b, int probe_layers, int *best_index, int *best_plan, double *ms)
 {
     if (!b || probe_layers < 1) { set_error("batch_autotune: bad arguments"); return E_ARG; }
-    return b->autotune(probe_layers, best_index, ms);
+    return b->autotune(probe_layers, best_index, best_plan, ms);
 }
 
 int mipt_hip_batch_set_graph(mipt_hip_batch *b, int enable)

@@ -12,6 +12,7 @@ namespace mipt_hip
 constexpr int MAX_HIST_BITS = 10;          // measured sites per fused histogram chunk
 constexpr int HSTRIDE = 1 << MAX_HIST_BITS;  // doubles per tile partial
 constexpr int AUTOTUNE_CANDIDATES = 6;    // tile shapes x bond arithmetic timed by the autotuner
+constexpr int AUTOTUNE_PLANS = 4;         // layer plan variants (qubit-block assignment) timed on the winner
 constexpr int SMALL_MAX_QUBITS = 14;       // whole state LDS-resident (128 KiB)
 
 // Read-only per-trajectory description (device).

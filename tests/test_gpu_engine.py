@@ -471,9 +471,9 @@ def test_batch_autotune_then_evolve_matches_oracle(M):
     print("batch autotune", flush=True)
     b = M.Batch(specs)
     b.generate()
-    best, ms = b.autotune(3)
-    print("autotune choice", best, "ms", ms, flush=True)
-    assert 0 <= best < 6 and ms[best] > 0
+    best, plan, ms = b.autotune(3, with_plan=True)
+    print("autotune choice", best, "plan", plan, "ms", ms, flush=True)
+    assert 0 <= best < 6 and ms[best] > 0 and 0 <= plan < 4 and ms[6:].min() > 0
     b.evolve()
     b.synchronize()
     for i, (n, layers, p, seed) in enumerate(cases):

@@ -125,17 +125,28 @@ def test_page_value_of_deep_haar_circuits_n16(M):
 
 def _oracle_traj(args):
     n, layers, p, seed, with_obs = args
-    with threadpool_limits(1):
-        psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed)
-        obs = None
-        if with_obs:
-            obs = (O.half_chain_entropy(psi, n), O.tmi(psi, n, n // 4, 0))
+    psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed)
+    obs = None
+    if with_obs:
+        obs = (O.half_chain_entropy(psi, n), O.tmi(psi, n, n // 4, 0))
     return psi, out, p1, f, u, obs
 
 
-def _run_device(M, cases):
+def _oracle_many(cases):
+    """The oracle runs of `cases` on a thread pool.  The oracle library is
+    loaded and the BLAS thread limit set here, in the calling thread: a first
+    dlopen in one worker while another scans the loaded libraries for
+    threadpool_limits (dl_iterate_phdr holds the loader lock and calls back
+    into Python for the GIL; dlopen holds the GIL and waits for the loader
+    lock) deadlocks the pool."""
+    O.lib()
+    with threadpool_limits(1), ThreadPoolExecutor(min(POOL, len(cases))) as ex:
+        return list(ex.map(_oracle_traj, cases))
+
+
+def _run_device(M, cases, plan=0):
     specs = [M.Spec(n, layers, p, seed) for (n, layers, p, seed, _) in cases]
-    batch = M.Batch(specs)
+    batch = M.Batch(specs, plan=plan)
     batch.generate()
     batch.evolve()
     batch.synchronize()
@@ -143,15 +154,16 @@ def _run_device(M, cases):
 
 
 @pytest.mark.timeout(300)
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
 @pytest.mark.parametrize("n", [22, 24])
-def test_three_sweep_chains_match_oracle(M, n):
+def test_three_sweep_chains_match_oracle(M, n, plan):
     # L = 22 and 24 take three sweeps per layer (windows of 8 high bits over
-    # the 5 lowest); device-drawn streams against the oracle's
+    # the 5 lowest); device-drawn streams against the oracle's, under every
+    # layer plan variant (window placement and bond-to-window assignment)
     cases = [(n, 6, 0.15, 2200 + 10 * n + i, n == 22) for i in range(2)] + [(n, 5, 0.3, 2250 + n, False)]
-    print(f"L={n}: {len(cases)} trajectories", flush=True)
-    batch = _run_device(M, cases)
-    with ThreadPoolExecutor(min(POOL, len(cases))) as ex:
-        refs = list(ex.map(_oracle_traj, cases))
+    print(f"L={n} plan {plan}: {len(cases)} trajectories", flush=True)
+    batch = _run_device(M, cases, plan)
+    refs = _oracle_many(cases)
     try:
         sh = batch.observables_all(tmi_block=0, tmi_cycles=0)[0] if n == 22 else None
         for i, (psi, out, p1, f, u, obs) in enumerate(refs):
@@ -183,8 +195,7 @@ def test_metric_workload_full_depth_32_trajectories(M):
         outs = [batch.outcomes(i) for i in range(len(cases))]
     finally:
         batch.close()
-    with ThreadPoolExecutor(POOL) as ex:
-        refs = list(ex.map(_oracle_traj, cases))
+    refs = _oracle_many(cases)
     worst = {"amp": 0.0, "s": 0.0, "tmi": 0.0}
     for i, (psi, out, p1, f, u, (s_ref, t_ref)) in enumerate(refs):
         got, flagged, gp1, _ = outs[i]
