@@ -131,6 +131,9 @@ int mipt_hip_rho1_complex_f64(const void *device_state_vector, int n_qubits, int
  * (cpp/include/mipt/free_energy.hpp:274-287) is the entropy entry point above. */
 int mipt_hip_tmi_f32(const void *device_state_vector, int n_qubits, int chain_sites, int block, int cycle,
                      double *tmi_out, double *terms_out /* 7: A,B,C,D,AB,AC,BC; may be NULL */);
+/* The same TMI of a complex128 device state, FP64 Gram. */
+int mipt_hip_tmi_f64(const void *device_state_vector, int n_qubits, int chain_sites, int block, int cycle,
+                     double *tmi_out, double *terms_out /* 7: A,B,C,D,AB,AC,BC; may be NULL */);
 
 /* ---------------------------------------------------------- plain engine */
 /* One trajectory at a time, one launch per blocked op, a host round trip per
@@ -212,6 +215,9 @@ typedef struct mipt_hip_traj_spec
     int attach_stagger; /* with probe_count >= 2: reference j attaches at attach_layer + j * attach_stagger
                            (0: all together); a site may then serve several references, as the staggered
                            insertion of cpp/include/mipt/probed/runner.hpp:230-256 does */
+    int fp64_check;     /* 1: mipt_hip_run_batch states this trajectory's errors against a complex128 replay
+                           of the same FP64 gates, flags and uniforms (traj_result amp_err, s_half_err,
+                           tmi_err); 0: those fields are NaN */
 } mipt_hip_traj_spec;
 
 typedef struct mipt_hip_batch_config
@@ -393,6 +399,17 @@ typedef struct mipt_hip_traj_result
     double norm_error;    /* | <psi|psi> - 1 | of the final complex64 state */
     int measured;         /* measured draws of the trajectory */
     int flagged_draws;    /* draws within 1e-6 of their Born threshold */
+    /* Errors of this result, stated when the spec asks for fp64_check: the
+     * trajectory is replayed on the plain engine in complex128 with its gates
+     * drawn in FP64 by the same counter-based sampler (before the rounding to
+     * complex64) and its flags and uniforms, and S and TMI of that state are
+     * taken in FP64.  NaN when not asked for, for trajectories the plain
+     * engine does not replay (probes, references, an unmeasured prefix,
+     * shared gates), when a near-threshold draw came out differently in the
+     * replay, and tmi_err for entropy order 2. */
+    double amp_err;       /* max |psi - psi_128| over the amplitudes */
+    double s_half_err;    /* |S(L/2) - S(L/2) of the replay|, bits */
+    double tmi_err;       /* max over the cycles |TMI - TMI of the replay|, bits */
 } mipt_hip_traj_result;
 
 /* One call per batch of trajectories (the batched entry point a scan driver

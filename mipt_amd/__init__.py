@@ -61,6 +61,7 @@ def lib():
     for name in ("mipt_hip_rho1_complex_f32", "mipt_hip_rho1_complex_f64"):
         getattr(L, name).argtypes = [vp, i, i, dp]
     L.mipt_hip_tmi_f32.argtypes = [vp, i, i, i, i, dp, dp]
+    L.mipt_hip_tmi_f64.argtypes = [vp, i, i, i, i, dp, dp]
     L.mipt_hip_set_gram_precision.argtypes = [i]
     L.mipt_hip_set_eigensolver.argtypes = [i]
     L.mipt_hip_entropy_hermitian_batched.argtypes = [dp, i, i, dp, dp]
@@ -401,7 +402,7 @@ class TrajSpec(ctypes.Structure):
                 ("closed", ctypes.c_int), ("gate_family", ctypes.c_int), ("probe_site", ctypes.c_int),
                 ("p_zero_layers", ctypes.c_int), ("shared_gates", ctypes.c_int), ("grid_x", ctypes.c_int),
                 ("probe_count", ctypes.c_int), ("probe_sites", ctypes.c_int * 4), ("attach_layer", ctypes.c_int),
-                ("attach_stagger", ctypes.c_int)]
+                ("attach_stagger", ctypes.c_int), ("fp64_check", ctypes.c_int)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -423,12 +424,14 @@ class Spec:
     probe_sites: tuple = ()  # several references (qubits L, L+1, ...); probe_site stays -1
     attach_layer: int = -1   # with probe_sites: the attach layer (-1: p_zero_layers)
     attach_stagger: int = 0  # reference j attaches at attach_layer + j * attach_stagger
+    fp64_check: bool = False  # run_batch states this result's errors against a complex128 replay
 
     def to_c(self) -> TrajSpec:
         sites = list(self.probe_sites) + [0] * (4 - len(self.probe_sites))
         return TrajSpec(self.L, self.layers, float(self.p), self.seed & (2**64 - 1), int(self.closed), 1,
                         self.probe_site, self.p_zero_layers, self.shared_gates, self.grid_x,
-                        len(self.probe_sites), (ctypes.c_int * 4)(*sites[:4]), self.attach_layer, self.attach_stagger)
+                        len(self.probe_sites), (ctypes.c_int * 4)(*sites[:4]), self.attach_layer, self.attach_stagger,
+                        int(self.fp64_check))
 
 
 class Batch:
@@ -630,7 +633,8 @@ class Batch:
 
 class TrajResult(ctypes.Structure):
     _fields_ = [("s_half", ctypes.c_double), ("probe_entropy", ctypes.c_double), ("bloch", ctypes.c_double * 3),
-                ("norm_error", ctypes.c_double), ("measured", ctypes.c_int), ("flagged_draws", ctypes.c_int)]
+                ("norm_error", ctypes.c_double), ("measured", ctypes.c_int), ("flagged_draws", ctypes.c_int),
+                ("amp_err", ctypes.c_double), ("s_half_err", ctypes.c_double), ("tmi_err", ctypes.c_double)]
 
 
 PROBE_FIELDS = ("sq", "i2", "i3", "i4", "negativity", "log_negativity", "tmi", "average_mi",
@@ -658,7 +662,8 @@ def run_batch(specs, tmi_cycles: int = 1, tmi_block: int = 0, entropy_order: int
     _check(lib().mipt_hip_run_batch(ctypes.cast(arr, ctypes.c_void_p), len(specs), tmi_block, tmi_cycles,
                                     entropy_order, _dp(tmi), ctypes.cast(res, ctypes.c_void_p), None), "run_batch")
     out = [{"s_half": r.s_half, "probe_entropy": r.probe_entropy, "bloch": np.array(r.bloch[:]),
-            "norm_error": r.norm_error, "measured": r.measured, "flagged_draws": r.flagged_draws} for r in res]
+            "norm_error": r.norm_error, "measured": r.measured, "flagged_draws": r.flagged_draws,
+            "amp_err": r.amp_err, "s_half_err": r.s_half_err, "tmi_err": r.tmi_err} for r in res]
     return out, tmi[:, :tmi_cycles]
 
 

@@ -1694,6 +1694,70 @@ int run_free_energy(int argc, char **argv)
 
 extern "C" {
 
+// Per-result errors of trajectory i of batch b: its gates drawn again in FP64
+// by the batch's counter-based sampler (the device rounds them to complex64),
+// its flags and uniforms exported, the whole trajectory replayed on the plain
+// engine in complex128, and the amplitudes, S(L/2) and TMI of that state (FP64
+// Gram and eigenvalues) compared with the batch's.  A draw that comes out
+// differently in the replay (only near its threshold) leaves the errors NaN.
+static int fp64_replay_errors(mipt_hip_batch *b, mipt_hip_engine *eng128, int i, const mipt_hip_traj_spec &s,
+                              const std::vector<int> &outcomes, int tmi_block, int tmi_cycles, const double *tmi_out,
+                              mipt_hip_traj_result &r)
+{
+    using namespace mipt_hip;
+    const int L = s.L, layers = s.layers, gx = s.grid_x, closed = s.closed ? 1 : 0;
+    std::vector<double> g;
+    for (int t = 0; t < layers; ++t)
+        for (int k = 0; k < geo_bond_count(L, gx, t, closed); ++k)
+        {
+            double qr[16], qi[16];
+            haar_u4_philox(s.seed, static_cast<std::uint32_t>(t), static_cast<std::uint32_t>(k), qr, qi);
+            for (int e = 0; e < 16; ++e)
+            {
+                g.push_back(qr[e]);
+                g.push_back(qi[e]);
+            }
+        }
+    const std::size_t draws = static_cast<std::size_t>(layers) * L;
+    std::vector<std::uint8_t> flags(draws);
+    std::vector<double> unif(draws);
+    int st = mipt_hip_batch_export_layers(b, i, nullptr, flags.data(), unif.data());
+    if (st) return st;
+    std::vector<int> out128(draws);
+    if ((st = mipt_hip_engine_run_trajectory(eng128, L, layers, closed, gx, g.data(), flags.data(), unif.data(), 4, 0,
+                                             10, out128.data())) != 0)
+        return st;
+    for (std::size_t k = 0; k < draws; ++k)
+        if (out128[k] != outcomes[k]) return OK;  // diverged at a near-threshold draw: errors stay NaN
+    const std::size_t dim = std::size_t{1} << L;
+    std::vector<float> psi32(2 * dim);
+    std::vector<double> psi64(2 * dim);
+    if ((st = mipt_hip_batch_get_state(b, i, psi32.data())) != 0) return st;
+    if ((st = mipt_hip_engine_to_host(eng128, psi64.data())) != 0) return st;
+    double amp = 0.0;
+    for (std::size_t k = 0; k < dim; ++k)
+        amp = std::max(amp, std::hypot(psi32[2 * k] - psi64[2 * k], psi32[2 * k + 1] - psi64[2 * k + 1]));
+    r.amp_err = amp;
+    const void *dev = mipt_hip_engine_data(eng128);
+    std::vector<int> kept(L / 2);
+    for (int q = 0; q < L / 2; ++q) kept[q] = q;
+    double s64 = 0.0;
+    if ((st = mipt_hip_entropy_svd_f64(dev, L, kept.data(), L / 2, 0, &s64)) != 0) return st;
+    r.s_half_err = std::abs(r.s_half - s64);
+    if (tmi_cycles > 0)
+    {
+        double worst = 0.0;
+        for (int c = 0; c < tmi_cycles; ++c)
+        {
+            double t64 = 0.0;
+            if ((st = mipt_hip_tmi_f64(dev, L, L, tmi_block > 0 ? tmi_block : L / 4, c, &t64, nullptr)) != 0) return st;
+            worst = std::max(worst, std::abs(tmi_out[static_cast<std::size_t>(i) * tmi_cycles + c] - t64));
+        }
+        r.tmi_err = worst;
+    }
+    return OK;
+}
+
 int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block, int tmi_cycles, int entropy_order,
                        double *tmi_out, mipt_hip_traj_result *results, void *hip_stream)
 {
@@ -1704,7 +1768,7 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
         return mipt_hip::E_ARG;
     }
     mipt_hip_batch *b = nullptr;
-    mipt_hip_engine *eng = nullptr;
+    mipt_hip_engine *eng = nullptr, *eng128 = nullptr;
     int st = 0;
     const mipt_hip_batch_config cfg{0, 0, 0, 0};
     if ((st = mipt_hip_batch_create(&cfg, hip_stream, &b)) != 0) return st;
@@ -1757,9 +1821,19 @@ int mipt_hip_run_batch(const mipt_hip_traj_spec *specs, int count, int tmi_block
             if ((st = mipt_hip_engine_adopt(eng, mipt_hip_batch_state_ptr(b, i), nq)) != 0) break;
             if ((st = mipt_hip_engine_squared_norm(eng, &sq)) != 0) break;
             r.norm_error = std::abs(sq - 1.0);
+            r.amp_err = r.s_half_err = r.tmi_err = NAN;
+            const bool replayable = s.probe_site < 0 && s.probe_count == 0 && s.p_zero_layers == 0 && s.shared_gates < 0;
+            if (s.fp64_check && replayable)
+            {
+                if (!eng128 && (st = mipt_hip_engine_create(1, mipt_hip_batch_stream(b), &eng128)) != 0) break;
+                if ((st = fp64_replay_errors(b, eng128, i, s, out, tmi_block, entropy_order == 1 ? tmi_cycles : 0,
+                                             tmi_out, r)) != 0)
+                    break;
+            }
         }
     } while (false);
     if (eng) (void)mipt_hip_engine_destroy(eng);
+    if (eng128) (void)mipt_hip_engine_destroy(eng128);
     (void)mipt_hip_batch_destroy(b);
     return st;
 }

@@ -1274,4 +1274,9 @@ int mipt_hip_tmi_f32(const void *s, int n, int L, int block, int cycle, double *
     return tmi_impl(s, false, n, L, block, cycle, nullptr, out, terms);
 }
 
+int mipt_hip_tmi_f64(const void *s, int n, int L, int block, int cycle, double *out, double *terms)
+{
+    return tmi_impl(s, true, n, L, block, cycle, nullptr, out, terms);
+}
+
 } // extern "C"

@@ -162,6 +162,35 @@ def test_run_batch_matches_oracle(M):
 
 
 @pytest.mark.gpu
+def test_run_batch_states_errors_per_result(M):
+    # fp64_check: each listed result carries its error against a complex128
+    # replay of the same FP64 gates, flags and uniforms; the replay is the
+    # oracle's computation, so the stated errors equal the errors against the
+    # oracle (to FP64 rounding) and stay within the 1e-5 bar; unlisted results
+    # and trajectories the plain engine does not replay carry NaN
+    specs = [M.Spec(12, 10, 0.15, 611, fp64_check=True), M.Spec(16, 8, 0.2, 612, fp64_check=True),
+             M.Spec(20, 6, 0.1, 613, fp64_check=True), M.Spec(12, 8, 0.3, 614, False, grid_x=4, fp64_check=True),
+             M.Spec(16, 6, 0.2, 615), M.Spec(12, 6, 0.2, 616, probe_site=6, fp64_check=True)]
+    print("run_batch with per-result FP64 errors", flush=True)
+    res, tmi = M.run_batch(specs, tmi_cycles=2, entropy_order=1)
+    for i, s in enumerate(specs[:4]):
+        if s.grid_x:
+            psi, out, p1, f, u = O.simulate_grid(s.grid_x, s.L // s.grid_x, s.layers, s.p, s.seed, s.closed)
+        else:
+            psi, out, p1, f, u = _oracle_run(s.L, s.layers, s.p, s.seed)
+        r = res[i]
+        if r["flagged_draws"]:
+            continue
+        assert r["amp_err"] < 1e-5 and r["s_half_err"] < 1e-5 and r["tmi_err"] < 1e-5, r
+        s_or = O.half_chain_entropy(psi, s.L)
+        t_or = max(abs(tmi[i, c] - O.tmi(psi, s.L, s.L // 4, c)) for c in range(2))
+        assert abs(r["s_half_err"] - abs(r["s_half"] - s_or)) < 1e-9
+        assert abs(r["tmi_err"] - t_or) < 1e-9
+    for r in res[4:]:
+        assert math.isnan(r["amp_err"]) and math.isnan(r["s_half_err"]) and math.isnan(r["tmi_err"])
+
+
+@pytest.mark.gpu
 def test_sim_tmi_csv_matches_oracle(M, tmp_path, monkeypatch):
     n, periods, R, res, pmin, pmax = 8, 3, 2, 2, 0.1, 0.3
     master = 12345
