@@ -804,15 +804,24 @@ int probe_information_all(const double *rho, int count, int k, std::vector<doubl
     v.assign(static_cast<std::size_t>(count) * 11, 0.0);
     const int nthreads = std::max(1, std::min({16, count, static_cast<int>(std::thread::hardware_concurrency())}));
     std::vector<int> status(nthreads, 0);
+    // the library's error text is thread-local: each worker keeps its own copy
+    std::vector<std::string> message(nthreads);
     std::vector<std::thread> pool;
     for (int w = 0; w < nthreads; ++w)
         pool.emplace_back([&, w]() {
             for (int i = w; i < count && status[w] == 0; i += nthreads)
+            {
                 status[w] = mipt_hip_probe_information(rho + i * stride, k, v.data() + static_cast<std::size_t>(i) * 11);
+                if (status[w]) message[w] = mipt_hip_last_error();
+            }
         });
     for (auto &t : pool) t.join();
-    for (int st : status)
-        if (st) return st;
+    for (int w = 0; w < nthreads; ++w)
+        if (status[w])
+        {
+            mipt_hip::set_error("%s", message[w].c_str());  // re-raised in the calling thread
+            return status[w];
+        }
     return 0;
 }
 
