@@ -666,64 +666,111 @@ __device__ __forceinline__ void group_valu(float2 (&v)[NV], const float (&ur)[16
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 // The same group on the matrix cores.  v_mfma_f32_4x4x1_16b_f32 adds, in
-// every block of four lanes 4b..4b+3, A(lane 4b+i) * B(lane l) to register i
-// of lane l.  With A = row i = lane & 3 of the gate's real 8x8 form
-// [[Re U, -Im U], [Im U, Re U]] and B = the lane's own amplitude components,
-// eight steps leave Re y_i (re[i]) and Im y_i (im[i]) of the lane's own
-// group in its own registers: no data moves between lanes and every product
-// is used.  cr/ci/cn = Re U[i][c], Im U[i][c], -Im U[i][c].
-template <int NV, int i0, int i1, int i2, int i3>
-__device__ __forceinline__ void group_mfma(float2 (&v)[NV], const float (&cr)[4], const float (&ci)[4],
-                                           const float (&cn)[4])
+// every block of four lanes 4b..4b+3, A(lane 4m + i) * B(lane l) to register
+// i of lane l, where m = ABID is the block whose A values are broadcast to
+// all 16 blocks (CBSZ = 4).  B is the lane's own amplitude component and the
+// A column of block m = 8 T + 2 c + r is column (c, r) of a real 4 x 8 block
+// T of the gate (r = 0: the coefficients of Re x_c, 1: of Im x_c), so the
+// whole gate sits in ONE register kv (16 columns x 4 rows) and eight steps
+// per block leave four outputs of the lane's own group in its own registers:
+// no data moves between lanes and every product is used.  The rows of block
+// T are (Re y_a, Im y_a, Re y_b, Im y_b) with (a, b) = (2T, 2T + 1), or (T,
+// T + 2) with PQ1, so each accumulator holds two complex amplitudes in place;
+// the pairing is chosen so that the two share register bit 1 where the bond
+// allows, which lets the register allocator keep one layout of the tile
+// across bond forms.
+#define MIPT_MFMA_STEP(acc, xv, m) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(kv, xv, acc, 4, m, 0)
+template <int NV, int i0, int i1, int i2, int i3, bool PQ1>
+__device__ __forceinline__ void group_mfma(float2 (&v)[NV], float kv)
 {
-    const float2 x[4] = {v[i0], v[i1], v[i2], v[i3]};
-    floatx4_t re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
+    const float2 x0 = v[i0], x1 = v[i1], x2 = v[i2], x3 = v[i3];
+    floatx4_t t0 = {0.f, 0.f, 0.f, 0.f}, t1 = {0.f, 0.f, 0.f, 0.f};
+    MIPT_MFMA_STEP(t0, x0.x, 0);
+    MIPT_MFMA_STEP(t1, x0.x, 8);
+    MIPT_MFMA_STEP(t0, x0.y, 1);
+    MIPT_MFMA_STEP(t1, x0.y, 9);
+    MIPT_MFMA_STEP(t0, x1.x, 2);
+    MIPT_MFMA_STEP(t1, x1.x, 10);
+    MIPT_MFMA_STEP(t0, x1.y, 3);
+    MIPT_MFMA_STEP(t1, x1.y, 11);
+    MIPT_MFMA_STEP(t0, x2.x, 4);
+    MIPT_MFMA_STEP(t1, x2.x, 12);
+    MIPT_MFMA_STEP(t0, x2.y, 5);
+    MIPT_MFMA_STEP(t1, x2.y, 13);
+    MIPT_MFMA_STEP(t0, x3.x, 6);
+    MIPT_MFMA_STEP(t1, x3.x, 14);
+    MIPT_MFMA_STEP(t0, x3.y, 7);
+    MIPT_MFMA_STEP(t1, x3.y, 15);
+    // block T holds outputs (2T, 2T + 1), or with PQ1 (T, T + 2)
+    constexpr int o1 = PQ1 ? i2 : i1, o2 = PQ1 ? i1 : i2;
+    v[i0] = make_float2(t0[0], t0[1]);
+    v[o1] = make_float2(t0[2], t0[3]);
+    v[o2] = make_float2(t1[0], t1[1]);
+    v[i3] = make_float2(t1[2], t1[3]);
+}
+#undef MIPT_MFMA_STEP
+
+// Matrix-core coefficient table of a sweep in LDS: per bond slot (2 per
+// stage: slot 2 si + j is bond[j] of stage si) 8 entries of 8 floats, entry
+// 2 a + p for output a and part p (0: real, 1: imaginary): (U_r[a][c],
+// -U_i[a][c]) for the real part, (U_i[a][c], U_r[a][c]) for the imaginary
+// part, c = 0..3 -- the coefficients of (Re x_c, Im x_c).  256 bytes per slot.
+constexpr int MX_SLOT_BYTES = 256;
+constexpr int MX_MAX_SLOTS = 12;
+
+// Writes the table of the sweep's bonds (gates gl) into tab: wave w fills
+// slots w, w + NT/64, ...; lane e < 16 converts gate entry e = 4 a + c.
+// The stage records are read with a wave-uniform slot index (scalar loads).
+template <int NT>
+__device__ __forceinline__ void fill_mx_table(const PassPlan &pl, const float2 *__restrict__ gl, float *tab)
+{
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int e = static_cast<int>(threadIdx.x & 63u);
+    for (int slot = w; slot < 2 * pl.nstages; slot += NT / 64)
     {
-        re = __builtin_amdgcn_mfma_f32_4x4x1f32(cr[c], x[c].x, re, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_4x4x1f32(ci[c], x[c].x, im, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_4x4x1f32(cn[c], x[c].y, re, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_4x4x1f32(cr[c], x[c].y, im, 0, 0, 0);
+        const int b = pl.st[slot >> 1].bond[slot & 1];
+        if (b >= 0 && e < 16)
+        {
+            const float2 g = gl[16 * b + e];
+            const int a = e >> 2, c = e & 3;
+            float *row = tab + slot * (MX_SLOT_BYTES / 4) + 2 * a * 8 + 2 * c;
+            *reinterpret_cast<float2 *>(row) = make_float2(g.x, -g.y);
+            *reinterpret_cast<float2 *>(row + 8) = make_float2(g.y, g.x);
+        }
     }
-    v[i0] = make_float2(re[0], im[0]);
-    v[i1] = make_float2(re[1], im[1]);
-    v[i2] = make_float2(re[2], im[2]);
-    v[i3] = make_float2(re[3], im[3]);
 }
 
 // The groups of a bond on register bits RA, RC, enumerated at compile time
 // (every register index a constant): group number GI starting at register
 // BASE goes to the VALU while GI < VG, else to the matrix cores.
 template <int NV, int RA, int RC, int VG, int BASE, int GI>
-__device__ __forceinline__ void bond_groups(float2 (&v)[NV], const float (&ur)[16], const float (&ui)[16],
-                                            const float (&cr)[4], const float (&ci)[4], const float (&cn)[4])
+__device__ __forceinline__ void bond_groups(float2 (&v)[NV], const float (&ur)[16], const float (&ui)[16], float kv)
 {
     if constexpr (BASE < NV)
     {
         if constexpr ((BASE & ((1 << RA) | (1 << RC))) != 0)
-            bond_groups<NV, RA, RC, VG, BASE + 1, GI>(v, ur, ui, cr, ci, cn);
+            bond_groups<NV, RA, RC, VG, BASE + 1, GI>(v, ur, ui, kv);
         else
         {
             constexpr int i1 = BASE | (1 << RA), i2 = BASE | (1 << RC), i3 = i1 | i2;
             if constexpr (GI < VG) group_valu<NV, BASE, i1, i2, i3>(v, ur, ui);
-            else group_mfma<NV, BASE, i1, i2, i3>(v, cr, ci, cn);
+            else group_mfma<NV, BASE, i1, i2, i3, RC == 1>(v, kv);
             // a VALU-only bond keeps one group in flight at a time (its eight
             // FMA chains are enough to fill the pipe), which bounds registers
             if constexpr (VG == NV / 4) __builtin_amdgcn_sched_barrier(0);
-            bond_groups<NV, RA, RC, VG, BASE + 1, GI + 1>(v, ur, ui, cr, ci, cn);
+            bond_groups<NV, RA, RC, VG, BASE + 1, GI + 1>(v, ur, ui, kv);
         }
     }
 }
 
 // A bond on register bits (RA <- q0, RC <- q1): its NV/4 groups are split
 // between the VALU and the matrix cores by MX (0: all VALU; 1: NV/16 groups
-// on the VALU, the rest on the matrix cores, which keeps both pipes busy --
-// a matrix-core group costs 16 MFMAs of 8 cycles, a VALU group 64 FMAs of 4;
-// 2: all on the matrix cores).  The code is straight-line, so the scheduler
-// interleaves the two pipes' instructions.
+// on the VALU, the rest on the matrix cores; 2: all on the matrix cores).
+// A matrix-core group costs 16 MFMAs of 4x4x1 (256 products each, all
+// useful), a VALU group 64 FMAs.  tab: the bond's slot in the coefficient
+// table (matrix-core forms).
 template <int NV, int RA, int RC, int MX>
-__device__ __forceinline__ void bond_on_regs(float2 (&v)[NV], const float2 *__restrict__ g, int lane)
+__device__ __forceinline__ void bond_on_regs(float2 (&v)[NV], const float2 *__restrict__ g, const float *tab, int lane)
 {
     constexpr int VG = MX == 0 ? NV / 4 : (MX == 1 ? NV / 16 : 0);
     float ur[16], ui[16];
@@ -737,59 +784,61 @@ __device__ __forceinline__ void bond_on_regs(float2 (&v)[NV], const float2 *__re
             ui[e] = x.y;
         }
     }
-    float cr[4], ci[4], cn[4];
+    float kv = 0.f;
     if constexpr (VG < NV / 4)
     {
-        // row i = lane & 3, selected from the wave-uniform coefficients (no
-        // vector memory access here: a sweep's next tile is in flight)
-        float sr[16], si[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e)
-        {
-            const float2 x = g[e];
-            sr[e] = x.x;
-            si[e] = x.y;
-        }
-        const int i = lane & 3;
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-        {
-            cr[c] = i == 0 ? sr[c] : (i == 1 ? sr[4 + c] : (i == 2 ? sr[8 + c] : sr[12 + c]));
-            ci[c] = i == 0 ? si[c] : (i == 1 ? si[4 + c] : (i == 2 ? si[8 + c] : si[12 + c]));
-            cn[c] = -ci[c];
-        }
+        // lane 4 m + i holds row i of column m = 8 T + 2 c + r: table entry
+        // 2 a + (i & 1), element 2 c + r, with a = 2 T + (i >> 1), or
+        // T + 2 (i >> 1) when the blocks pair outputs across q1 (RC == 1)
+        const int i = lane & 3, m = lane >> 2;
+        const int T = m >> 3;
+        const int a = RC == 1 ? T + 2 * (i >> 1) : 2 * T + (i >> 1);
+        kv = tab[(2 * a + (i & 1)) * 8 + (m & 7)];
     }
-    bond_groups<NV, RA, RC, VG, 0, 0>(v, ur, ui, cr, ci, cn);
+    bond_groups<NV, RA, RC, VG, 0, 0>(v, ur, ui, kv);
 }
 
 // A stage's bonds: bond[0] on register bits (1,2) or, for a bond of qubit 0,
-// (0,1) / (1,0); bond[1] on (3,4).
-template <int NV, int MX>
-__device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl, int lane)
+// (0,1) / (1,0) (zpair 1 / 2); bond[1] on (3,4).  tab: the stage's first
+// table slot.  ZP: 0 -- the stage has no bond of qubit 0; 1 / 2 -- its
+// bond[0] is the zpair 1 / 2 form; 3 -- any form, chosen at run time.  A
+// sweep kernel that knows the form at compile time keeps one register
+// layout of the tile across its stage loop (no join of differently laid out
+// bond forms), which saves registers.
+template <int NV, int MX, int ZP>
+__device__ __forceinline__ void apply_stage_gates(float2 (&v)[NV], const StagePlan &sp, const float2 *gl,
+                                                  const float *tab, int lane)
 {
     static_assert(NV == 32 || NV == 64, "register tiles hold 32 or 64 amplitudes");
-    // The three forms differ only in which registers they touch; the empty
-    // markers keep the optimiser from merging them into one body that would
-    // address the tile through pointer selects (which moves it to scratch).
     if (sp.bond[0] >= 0)
     {
-        if (sp.zpair == 1)
-        {
-            bond_on_regs<NV, 0, 1, MX>(v, gl + 16 * sp.bond[0], lane);
-            asm volatile("; bond slot (0,1)");
-        }
-        else if (sp.zpair == 2)
-        {
-            bond_on_regs<NV, 1, 0, MX>(v, gl + 16 * sp.bond[0], lane);
-            asm volatile("; bond slot (1,0)");
-        }
+        if constexpr (ZP == 1) bond_on_regs<NV, 0, 1, MX>(v, gl + 16 * sp.bond[0], tab, lane);
+        else if constexpr (ZP == 2) bond_on_regs<NV, 1, 0, MX>(v, gl + 16 * sp.bond[0], tab, lane);
+        else if constexpr (ZP == 0) bond_on_regs<NV, 1, 2, MX>(v, gl + 16 * sp.bond[0], tab, lane);
         else
         {
-            bond_on_regs<NV, 1, 2, MX>(v, gl + 16 * sp.bond[0], lane);
-            asm volatile("; bond slot (1,2)");
+            // The three forms differ only in which registers they touch; the
+            // empty markers keep the optimiser from merging them into one body
+            // that would address the tile through pointer selects (which moves
+            // it to scratch).
+            if (sp.zpair == 1)
+            {
+                bond_on_regs<NV, 0, 1, MX>(v, gl + 16 * sp.bond[0], tab, lane);
+                asm volatile("; bond slot (0,1)");
+            }
+            else if (sp.zpair == 2)
+            {
+                bond_on_regs<NV, 1, 0, MX>(v, gl + 16 * sp.bond[0], tab, lane);
+                asm volatile("; bond slot (1,0)");
+            }
+            else
+            {
+                bond_on_regs<NV, 1, 2, MX>(v, gl + 16 * sp.bond[0], tab, lane);
+                asm volatile("; bond slot (1,2)");
+            }
         }
     }
-    if (sp.bond[1] >= 0) bond_on_regs<NV, 3, 4, MX>(v, gl + 16 * sp.bond[1], lane);
+    if (sp.bond[1] >= 0) bond_on_regs<NV, 3, 4, MX>(v, gl + 16 * sp.bond[1], tab + MX_SLOT_BYTES / 4, lane);
 }
 
 // LDS addressing.  u bit 0 (qubit 0) is register bit 0 in every mapping, and
@@ -1189,8 +1238,10 @@ __device__ __forceinline__ std::uint32_t fresh_tid()
 
 // Processes one tile whose amplitudes are in v (M0 mapping): the initial
 // state or the previous layer's collapse scale, the stages of the sweep plan,
-// back to M0, the store, and the FP64 histogram partial.
-template <int RB, int NT, int AUX, int MX>
+// back to M0, the store, and the FP64 histogram partial.  ZP 1 / 2: the last
+// stage holds the sweep's bond of qubit 0 in that form and the others none;
+// ZP 0: no stage has one; ZP 3: any stage may, in either form.
+template <int RB, int NT, int AUX, int MX, int ZP>
 __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 *__restrict__ gates, const TileCtx &cur,
                                              float2 (&v)[1 << RB], std::uint32_t jshift,
                                              const std::uint32_t (&Q3m0)[6])
@@ -1245,9 +1296,15 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     for (int r = 0; r < 6; ++r) Q3[r] = Q3m0[r];
     bool in_m0 = true;
     bool lds_dirty = false;  // a previous LDS read must finish before the next write
-    for (int si = 0; si < (AUX ? 0 : pl.nstages); ++si)
-    {
-        const StagePlan &sp = pl.st[si];
+    // matrix-core coefficient table, after the half-tile buffer; its writes
+    // are published by the first exchange's barrier (or one of its own)
+    const float *mxtab = reinterpret_cast<const float *>(ldsb + (std::size_t{8} << (k - 1)));
+    bool tab_ready = MX == 0;
+    if constexpr (MX > 0 && AUX == 0) fill_mx_table<NT>(pl, gates + cur.goff, const_cast<float *>(mxtab));
+    // moves the tile into stage sp's mapping (no move for an M0 stage)
+    auto enter_stage = [&](const StagePlan &sp) {
+        if (!tab_ready && !sp.use_lds) __syncthreads();
+        tab_ready = true;
         if (sp.use_lds)
         {
             const std::uint32_t tt = fresh_tid();
@@ -1263,7 +1320,21 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
             for (int r = 0; r < 6; ++r) Q3[r] = Q3n[r];
             in_m0 = false;
         }
-        apply_stage_gates<NV, MX>(v, sp, gl, lane);
+    };
+    constexpr bool zlast = ZP == 1 || ZP == 2;
+    const int nloop = AUX ? 0 : pl.nstages - (zlast ? 1 : 0);
+    for (int si = 0; si < nloop; ++si)
+    {
+        const StagePlan &sp = pl.st[si];
+        enter_stage(sp);
+        apply_stage_gates<NV, MX, ZP == 3 ? 3 : 0>(v, sp, gl, mxtab + si * (2 * MX_SLOT_BYTES / 4), lane);
+    }
+    if constexpr (zlast && !AUX)
+    {
+        const int si = pl.nstages - 1;
+        const StagePlan &sp = pl.st[si];
+        enter_stage(sp);
+        apply_stage_gates<NV, MX, ZP>(v, sp, gl, mxtab + si * (2 * MX_SLOT_BYTES / 4), lane);
     }
 
     // ---- the chunk's measured sites that are local to this tile (u bits)
@@ -1320,12 +1391,11 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
 // (the hot kernel); AUX = 1: a sweep without gates (the closing collapse, or
 // the histogram of an extra chunk of a layer with more than MAX_HIST_BITS
 // measured sites), kept as its own kernel so profiles separate the two.
-// One workgroup per tile; with the half-tile LDS buffer three 2^13-amplitude
-// workgroups of VALU bond arithmetic share a CU (the matrix-core forms need
-// more registers: two; 64-register tiles one), whose loads overlap one
-// another's LDS stages.
-template <int RB, int NT, int AUX, int MX>
-__global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 4 : 3))) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
+// One workgroup per tile; with the half-tile LDS buffer four 2^13-amplitude
+// workgroups share a CU (64-register tiles: one), whose loads overlap one
+// another's LDS stages and bond arithmetic.
+template <int RB, int NT, int AUX, int MX, int ZP>
+__global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
                                                                   const TrajDesc *__restrict__ desc,
                                                                   const std::uint32_t *__restrict__ flags,
                                                                   const Collapse *__restrict__ coll,
@@ -1359,7 +1429,7 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : (MX == 0 ? 4 : 3))) pass_ke
     if (id < total && tile_context(args, in.traj_list, in.desc, in.flags, in.coll, in.init, in.gates, id, AUX != 0, x))
     {
         tile_load<NV>(x, true, args.a.states, toff, jshift, v);
-        process_tile<RB, NT, AUX, MX>(args, in.gates, x, v, jshift, Q3m0);
+        process_tile<RB, NT, AUX, MX, ZP>(args, in.gates, x, v, jshift, Q3m0);
     }
 }
 
@@ -1688,6 +1758,10 @@ int build_layer_plan(int L, int nq, int closed, int gx, int layer, const Config 
             set_error("tile plan: too many stages");
             return E_UNSUPPORTED;
         }
+        // the stage with the bond of qubit 0 (at most one: a layer's bonds are
+        // disjoint) goes last, where the sweep kernel applies it in a form
+        // fixed at compile time; the bonds of one sweep commute
+        std::stable_partition(stages.begin(), stages.end(), [](const StagePlan &sp) { return sp.zpair == 0; });
         pp.nstages = static_cast<int>(stages.size());
         for (std::size_t i = 0; i < stages.size(); ++i) pp.st[i] = stages[i];
         pp.store = 1;
@@ -2070,21 +2144,24 @@ struct Batch
         return OK;
     }
 
-    template <int RB, int NT, int AUX, int MX>
+    template <int RB, int NT, int AUX, int MX, int ZP>
     int launch_pass(const PassPlan &pl, const int *list, int ntraj, int layer)
     {
         PassArgs pa{al, list, ntraj, layer, pl};
-        const std::size_t lds = (std::size_t{1} << (pl.k - 1)) * sizeof(float2);  // half a tile
+        // half a tile, then the matrix-core coefficient table
+        const std::size_t lds = (std::size_t{1} << (pl.k - 1)) * sizeof(float2) +
+                                (MX > 0 ? static_cast<std::size_t>(2 * pl.nstages) * MX_SLOT_BYTES : 0);
         static bool attr_set = false;
         if (!attr_set)
         {
-            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX>),
+            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX, ZP>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               static_cast<int>((std::size_t{1} << 13) * sizeof(float2))));
+                                               static_cast<int>((std::size_t{1} << 13) * sizeof(float2) +
+                                                                MX_MAX_SLOTS * MX_SLOT_BYTES)));
             attr_set = true;
         }
         const unsigned grid = static_cast<unsigned>(static_cast<long>(ntraj) * pl.ntiles);
-        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX>), dim3(grid), dim3(NT), lds, ls, pa,
+        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX, ZP>), dim3(grid), dim3(NT), lds, ls, pa,
                            static_cast<const int *>(list), static_cast<const TrajDesc *>(al.desc),
                            static_cast<const std::uint32_t *>(al.flags), static_cast<const Collapse *>(al.coll),
                            static_cast<const int *>(al.init), static_cast<const float2 *>(al.gates));
@@ -2100,16 +2177,38 @@ struct Batch
         const bool aux = pl.nstages == 0;
         if (cf.nt == 256 && cf.rb == 5)
         {
-            if (aux) return launch_pass<5, 256, 1, 0>(pl, list, ntraj, layer);
-            if (cf.mx == 0) return launch_pass<5, 256, 0, 0>(pl, list, ntraj, layer);
-            if (cf.mx == 2) return launch_pass<5, 256, 0, 2>(pl, list, ntraj, layer);
-            return launch_pass<5, 256, 0, 1>(pl, list, ntraj, layer);
+            if (aux) return launch_pass<5, 256, 1, 0, 0>(pl, list, ntraj, layer);
+            // the planner puts the stage with a bond of qubit 0 last
+            int zp = 0;
+            for (int si = 0; si < pl.nstages; ++si)
+                if (pl.st[si].zpair)
+                {
+                    if (si != pl.nstages - 1)
+                    {
+                        set_error("tile plan: the bond of qubit 0 is not in the last stage");
+                        return E_UNSUPPORTED;
+                    }
+                    zp = pl.st[si].zpair;
+                }
+            if (cf.mx == 0)
+                return zp == 0   ? launch_pass<5, 256, 0, 0, 0>(pl, list, ntraj, layer)
+                       : zp == 1 ? launch_pass<5, 256, 0, 0, 1>(pl, list, ntraj, layer)
+                                 : launch_pass<5, 256, 0, 0, 2>(pl, list, ntraj, layer);
+            if (cf.mx == 2)
+                return zp == 0   ? launch_pass<5, 256, 0, 2, 0>(pl, list, ntraj, layer)
+                       : zp == 1 ? launch_pass<5, 256, 0, 2, 1>(pl, list, ntraj, layer)
+                                 : launch_pass<5, 256, 0, 2, 2>(pl, list, ntraj, layer);
+            return zp == 0   ? launch_pass<5, 256, 0, 1, 0>(pl, list, ntraj, layer)
+                   : zp == 1 ? launch_pass<5, 256, 0, 1, 1>(pl, list, ntraj, layer)
+                             : launch_pass<5, 256, 0, 1, 2>(pl, list, ntraj, layer);
         }
         if (cf.nt == 256)
-            return aux ? launch_pass<6, 256, 1, 0>(pl, list, ntraj, layer) : launch_pass<6, 256, 0, 0>(pl, list, ntraj, layer);
+            return aux ? launch_pass<6, 256, 1, 0, 0>(pl, list, ntraj, layer)
+                       : launch_pass<6, 256, 0, 0, 3>(pl, list, ntraj, layer);
         if (cf.nt == 128)
-            return aux ? launch_pass<6, 128, 1, 0>(pl, list, ntraj, layer) : launch_pass<6, 128, 0, 0>(pl, list, ntraj, layer);
-        return aux ? launch_pass<6, 64, 1, 0>(pl, list, ntraj, layer) : launch_pass<6, 64, 0, 0>(pl, list, ntraj, layer);
+            return aux ? launch_pass<6, 128, 1, 0, 0>(pl, list, ntraj, layer)
+                       : launch_pass<6, 128, 0, 0, 3>(pl, list, ntraj, layer);
+        return aux ? launch_pass<6, 64, 1, 0, 0>(pl, list, ntraj, layer) : launch_pass<6, 64, 0, 0, 3>(pl, list, ntraj, layer);
     }
 
     // Tiled evolution of a group of same-size trajectories (nq > 14).
