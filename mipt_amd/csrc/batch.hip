@@ -1148,46 +1148,46 @@ __device__ __forceinline__ bool tile_context(const PassArgs &args, const int *__
     x.id = id;
     x.tl = id / pl.ntiles;
     x.tile = id - x.tl * pl.ntiles;
+    // Every record of the tile is read up front, unconditionally, in three
+    // dependent rounds of scalar loads (trajectory; its descriptor and init
+    // flag; its layer's flags and collapse records): the tile's global loads
+    // wait for no more round trips than these.
     x.traj = traj_list[x.tl];
     const TrajDesc d = desc[x.traj];
-    x.flags = flags[d.layer_off + t];
+    const int init_v = init[x.traj];
+    const std::uint32_t lt = d.layer_off + static_cast<std::uint32_t>(t);
+    const std::uint32_t fl = flags[lt];
+    const Collapse cprev = coll[t > 0 ? lt - 1 : lt];
+    const Collapse ccur = coll[lt];
+    x.flags = fl;
     if (aux && pl.cond && __popc(x.flags) <= pl.chunk * MAX_HIST_BITS) return false;
     x.L = d.L;
     x.nq = d.nq;
     x.B = pdep64(static_cast<std::uint64_t>(x.tile), pl.rest_mask);
     x.state = a.states + d.state_off;
     x.goff = (d.gate_off + gate_prefix(d, t)) * 16;
-    std::uint64_t cm = 0, cv = 0;
-    x.scale = 1.f;
-    if (pl.collapse && t > 0)
-    {
-        const Collapse cc = coll[d.layer_off + t - 1];
-        cm = cc.mask;
-        cv = cc.val;
-        x.scale = static_cast<float>(cc.scale);
-    }
-    else if (pl.pmask && t > 0)
-    {
-        // the layer's first sweep skipped tiles of the zero branch; their
-        // sites are untouched since, so the branch is still zero by definition
-        const Collapse cc = coll[d.layer_off + t - 1];
-        cm = cc.mask & pl.pmask;
-        cv = cc.val & cm;
-    }
+    // the previous layer's collapse, applied at load (collapse), or, where
+    // the layer's first sweep skipped tiles of its zero branch, that branch
+    // on the pmask sites (untouched since, so still zero by definition);
+    // selects rather than branches, so no load waits behind a branch
+    const bool use_c = pl.collapse && t > 0;
+    const bool use_p = !use_c && pl.pmask != 0 && t > 0;
+    const std::uint64_t pm = use_c ? ~std::uint64_t{0} : (use_p ? pl.pmask : 0);
+    const std::uint64_t cm = cprev.mask & pm, cv = cprev.val & cm;
+    x.scale = use_c ? static_cast<float>(cprev.scale) : 1.f;
     x.zero = (x.B & cm) != (cv & cm & pl.rest_mask);
     if (pl.zskip && x.zero) return false;
     x.cml = static_cast<std::uint32_t>(cm & ~pl.rest_mask);
     x.cvl = static_cast<std::uint32_t>(cv & ~pl.rest_mask);
-    x.gen_init = pl.init && init[x.traj];
+    x.gen_init = pl.init && init_v;
     x.tile_out = false;
     x.cdm = x.cdv = 0;
     if (aux && pl.cond)
     {
-        const Collapse cc = coll[d.layer_off + t];
-        x.tile_out = (x.B & cc.mask) != (cc.val & cc.mask & pl.rest_mask);
+        x.tile_out = (x.B & ccur.mask) != (ccur.val & ccur.mask & pl.rest_mask);
         const std::uint32_t lowmask = (1u << pl.c) - 1u, wmask = (1u << (pl.k - pl.c)) - 1u;
-        x.cdm = static_cast<std::uint32_t>((cc.mask & lowmask) | (((cc.mask >> pl.s) & wmask) << pl.c));
-        x.cdv = static_cast<std::uint32_t>((cc.val & lowmask) | (((cc.val >> pl.s) & wmask) << pl.c));
+        x.cdm = static_cast<std::uint32_t>((ccur.mask & lowmask) | (((ccur.mask >> pl.s) & wmask) << pl.c));
+        x.cdv = static_cast<std::uint32_t>((ccur.val & lowmask) | (((ccur.val >> pl.s) & wmask) << pl.c));
     }
     return true;
 }
