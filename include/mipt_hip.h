@@ -225,9 +225,9 @@ typedef struct mipt_hip_batch_config
     int tile_bits;      /* 0 = the autotuner's choice (mipt_hip_batch_autotune; 13 before it runs); else 12..14 */
     int max_hist_bits;  /* measured sites per fused histogram chunk (<= 10); 0 = default */
     int threads;        /* with tile_bits = 13: 128 selects 64-amplitude registers per thread; else 0 */
-    int bond_arith;     /* bond arithmetic of the tile sweeps: 0 = the autotuner's choice (VALU before it
-                           runs); 1 = VALU only; 2 = VALU and matrix cores; 3 = matrix cores only
-                           (v_mfma_f32_4x4x1_16b_f32) */
+    int bond_arith;     /* bond arithmetic of the tile sweeps: 0 = the autotuner's choice (matrix cores
+                           before it runs); 1 = VALU only; 2 = VALU and matrix cores; 3 = matrix cores
+                           only (v_mfma_f32_4x4x1_16b_f32) */
     int plan;           /* layer plan variant 0..3 (qubit-block assignment, see mipt_hip_batch_autotune);
                            replaced by the autotuner's choice once it has run */
 } mipt_hip_batch_config;

@@ -1907,8 +1907,9 @@ struct Batch
         if (cfg.tile_bits == 14) cf = Config{6, 256, 6};
         if (cfg.tile_bits == 13) cf = cfg.threads == 128 ? Config{6, 128, 5} : Config{5, 256, 5};
         if (cfg.tile_bits == 12) cf = Config{6, 64, 4};
-        // 64-register tiles are built with the split arithmetic only
-        if (cfg.bond_arith > 0 && cf.rb == 5) cf.mx = cfg.bond_arith - 1;
+        // 32-register tiles default to matrix-core bonds (the faster form
+        // measured on MI355X); 64-register tiles are built for the VALU only
+        if (cf.rb == 5) cf.mx = cfg.bond_arith > 0 ? cfg.bond_arith - 1 : 2;
         cf.plan = cfg.plan;
         return cf;
     }
