@@ -232,6 +232,9 @@ typedef struct mipt_hip_batch_config
                            replaced by the autotuner's choice once it has run */
 } mipt_hip_batch_config;
 
+/* The batch owns every device buffer its calls use (states, draws, histogram
+ * partials, and the entropy workspace of its readouts with its rocSOLVER
+ * handle); destroy synchronises its streams and releases all of it. */
 int mipt_hip_batch_create(const mipt_hip_batch_config *config, void *hip_stream, mipt_hip_batch **out);
 int mipt_hip_batch_destroy(mipt_hip_batch *batch);
 /* Allocates device-resident states and streams for `count` trajectories. */

@@ -1828,10 +1828,20 @@ struct Batch
     long gate_launches = 0;
     double gate_bytes = 0.0;
 
+    // the batch's own entropy workspace (RDM buffers, eigensolver panels, the
+    // rocSOLVER handle), created with the first readout, freed with the batch
+    Workspace *ews = nullptr;
+    Workspace &ew()
+    {
+        if (!ews) ews = workspace_create();
+        return *ews;
+    }
+
     ~Batch()
     {
         drop_graph();
         release();
+        workspace_destroy(ews);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (partials2) (void)hipFree(partials2);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -3019,12 +3029,12 @@ int mipt_hip_batch_observables(mipt_hip_batch *b, int traj, int tmi_block, doubl
     {
         std::vector<int> kept(d.L / 2);
         std::iota(kept.begin(), kept.end(), 0);
-        MIPT_HIP_TRY(entropy_bits_f32(psi, d.nq, kept.data(), static_cast<int>(kept.size()), b->stream, s_half));
+        MIPT_HIP_TRY(entropy_bits_f32(b->ew(), psi, d.nq, kept.data(), static_cast<int>(kept.size()), b->stream, s_half));
     }
     if (tmi)
     {
         const int block = tmi_block > 0 ? tmi_block : d.L / 4;
-        MIPT_HIP_TRY(tmi_bits_f32(psi, d.nq, d.L, block, 0, b->stream, tmi, nullptr));
+        MIPT_HIP_TRY(tmi_bits_f32(b->ew(), psi, d.nq, d.L, block, 0, b->stream, tmi, nullptr));
     }
     if (probe_entropy || bloch3)
     {
@@ -3036,7 +3046,7 @@ int mipt_hip_batch_observables(mipt_hip_batch *b, int traj, int tmi_block, doubl
         else
         {
             double r[4];
-            MIPT_HIP_TRY(rho1_f32(psi, d.nq, d.L, b->stream, r));
+            MIPT_HIP_TRY(rho1_f32(b->ew(), psi, d.nq, d.L, b->stream, r));
             const double tr = r[0] + r[1];
             const double a0 = r[0] / tr, a1 = r[1] / tr, x = r[2] / tr, y = r[3] / tr;
             if (bloch3)
@@ -3134,7 +3144,7 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cyc
                 std::vector<const float2 *> ps(ni);
                 for (int i = 0; i < ni; ++i) ps[i] = psis[items[k][i].traj_j];
                 std::vector<double> sv(ni);
-                MIPT_HIP_TRY(entropy_many_bits_f32(ps.data(), nq, by_size[k].second.data(), sz, ni, b->stream, sv.data()));
+                MIPT_HIP_TRY(entropy_many_bits_f32(b->ew(), ps.data(), nq, by_size[k].second.data(), sz, ni, b->stream, sv.data()));
                 for (int i = 0; i < ni; ++i)
                 {
                     const Item &it = items[k][i];
@@ -3153,7 +3163,7 @@ int mipt_hip_batch_observables_all(mipt_hip_batch *b, int tmi_block, int tmi_cyc
             for (int j = 0; j < cnt; ++j)
                 for (int q = 0; q < L / 2; ++q) kept[static_cast<std::size_t>(j) * (L / 2) + q] = q;
             std::vector<double> sv(cnt);
-            MIPT_HIP_TRY(entropy_many_bits_f32(psis.data(), nq, kept.data(), L / 2, cnt, b->stream, sv.data()));
+            MIPT_HIP_TRY(entropy_many_bits_f32(b->ew(), psis.data(), nq, kept.data(), L / 2, cnt, b->stream, sv.data()));
             for (int j = 0; j < cnt; ++j) s_half[idx[g0 + j]] = sv[j];
         }
         g0 = g1;
@@ -3200,7 +3210,7 @@ int mipt_hip_batch_translation_entropies(mipt_hip_batch *b, int min_kept, int ma
                 for (int i = 0; i < L; ++i)
                     for (int k = 0; k < l; ++k) kept[(static_cast<std::size_t>(j) * L + i) * l + k] = (i + k) % L;
             std::vector<double> v(static_cast<std::size_t>(cnt) * L);
-            MIPT_HIP_TRY(purity_many_f32(ps.data(), nq, kept.data(), l, cnt * L, b->stream, v.data()));
+            MIPT_HIP_TRY(purity_many_f32(b->ew(), ps.data(), nq, kept.data(), l, cnt * L, b->stream, v.data()));
             for (int j = 0; j < cnt; ++j)
             {
                 double acc = 0.0;
@@ -3215,7 +3225,7 @@ int mipt_hip_batch_translation_entropies(mipt_hip_batch *b, int min_kept, int ma
             {
                 if (l <= max_s1_kept)
                 {
-                    MIPT_HIP_TRY(entropy_many_bits_f32(ps.data(), nq, kept.data(), l, cnt * L, b->stream, v.data()));
+                    MIPT_HIP_TRY(entropy_many_bits_f32(b->ew(), ps.data(), nq, kept.data(), l, cnt * L, b->stream, v.data()));
                     for (int j = 0; j < cnt; ++j)
                     {
                         double acc = 0.0;
@@ -3242,7 +3252,7 @@ int mipt_hip_batch_tmi(mipt_hip_batch *b, int traj, int block, int cycle, int or
     }
     const TrajDesc &d = b->desc[traj];
     const int blk = block > 0 ? block : d.L / 4;
-    return tmi_bits_f32(b->a.states + d.state_off, d.nq, d.L, blk, cycle, b->stream, tmi, nullptr, order);
+    return tmi_bits_f32(b->ew(), b->a.states + d.state_off, d.nq, d.L, blk, cycle, b->stream, tmi, nullptr, order);
 }
 
 } // extern "C"

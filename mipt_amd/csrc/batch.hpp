@@ -59,20 +59,31 @@ struct DevArena
     std::size_t partial_cap = 0;         // doubles
 };
 
-// Entropy/observable helpers implemented in entropy.hip, used by the batch.
-int entropy_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
-int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream, double *out,
-                 double *terms, int order = 1);
+// Device workspace of the entropy paths (RDM buffers, eigensolver panels,
+// the rocSOLVER handle).  A batch owns one, created with the batch's first
+// readout and released with the batch; the context-free drop-in entry
+// points of the reference's signatures use one per calling thread.
+struct Workspace;
+Workspace *workspace_create();
+void workspace_destroy(Workspace *w);
+
+// Entropy/observable helpers implemented in entropy.hip, used by the batch
+// with its own workspace.
+int entropy_bits_f32(Workspace &w, const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream,
+                     double *out);
+int tmi_bits_f32(Workspace &w, const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream,
+                 double *out, double *terms, int order = 1);
 // Von Neumann entropies (bits) of many (state, kept set) items of one shape,
 // with batched eigensolves; kept is [count][kept_count]
-int entropy_many_bits_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count,
+int entropy_many_bits_f32(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count,
                           hipStream_t stream, double *out);
 // Tr(rho^2) of many (state, kept set) items of one shape
-int purity_many_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
-                    double *out);
+int purity_many_f32(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count,
+                    hipStream_t stream, double *out);
 // Renyi-2 entropy -log2 Tr(rho^2) of the kept set, bits
-int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out);
-int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4);
+int renyi2_bits_f32(Workspace &w, const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream,
+                    double *out);
+int rho1_f32(Workspace &w, const float2 *psi, int n, int qubit, hipStream_t stream, double *out4);
 
 // Batched FP64 Hermitian eigenvalues (eigen.hip): `count` m x m matrices at
 // mats + i * stride (m <= EIG_MAX_N; the lower triangle of the column-major

@@ -432,6 +432,8 @@ __global__ void rho1_kernel(const C *__restrict__ psi, int n, int q, double *__r
 }
 
 // ------------------------------------------------------------ host helpers
+} // namespace
+
 struct Workspace
 {
     float *ar = nullptr, *ai = nullptr;
@@ -464,9 +466,29 @@ struct Workspace
     std::size_t rowb_cap = 0, emb_cap = 0, psib_cap = 0;
 };
 
-// Process-lifetime per-thread workspace (never freed: the reference keeps its
-// workspaces thread_local too, and freeing at thread exit could run after the
-// HIP runtime is gone).
+Workspace *workspace_create() { return new Workspace(); }
+
+// Frees every buffer and the rocSOLVER handle (the caller has synchronised
+// the streams that used them).
+void workspace_destroy(Workspace *w)
+{
+    if (!w) return;
+    void *bufs[] = {w->ar, w->ai, w->part, w->rho, w->evals, w->e, w->info, w->row_off, w->red, w->rho_b, w->ev_b,
+                    w->e_b, w->info_b, w->eig_w, w->eig_d, w->eig_e, w->eig_s, w->rowb, w->emb, w->psib};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (w->handle) (void)rocblas_destroy_handle(w->handle);
+    delete w;
+}
+
+namespace
+{
+
+// Workspace of the context-free drop-in entry points (the reference's
+// signatures carry no context): one per calling thread for the process's
+// lifetime (never freed: the reference keeps its workspaces thread_local too,
+// and freeing at thread exit could run after the HIP runtime is gone).  A
+// batch passes its own workspace instead.
 thread_local Workspace *t_ws = nullptr;
 
 Workspace &ws()
@@ -533,9 +555,8 @@ std::atomic<int> g_eig_solver{0};
 
 // Entropies (bits) of `count` RDMs already in w.rho_b (stride m*m) with the
 // library's batched eigensolver; results to host out[count].
-int own_entropies(int m, int count, hipStream_t stream, double2 *mats, double *out)
+int own_entropies(Workspace &w, int m, int count, hipStream_t stream, double2 *mats, double *out)
 {
-    Workspace &w = ws();
     MIPT_HIP_TRY(grow(w.eig_w, w.eig_w_cap, static_cast<std::size_t>(count) * m * 32));
     MIPT_HIP_TRY(grow(w.eig_d, w.eig_d_cap, static_cast<std::size_t>(count) * m));
     MIPT_HIP_TRY(grow(w.eig_e, w.eig_e_cap, static_cast<std::size_t>(count) * m));
@@ -590,7 +611,7 @@ int gram_timing_end(hipStream_t stream, int slot)
 }
 
 // rho (FP64, device, row-major m x m) of the kept modes of psi.
-int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int &m_out,
+int build_rho(Workspace &w, const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int &m_out,
               double2 *dst = nullptr)
 {
     std::uint64_t kmask;
@@ -599,7 +620,6 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
         set_error("invalid kept-mode list");
         return E_ARG;
     }
-    Workspace &w = ws();
     const int m = 1 << count;
     const int env = n - count;
     const std::uint64_t env_mask = ((n >= 64) ? ~std::uint64_t{0} : ((std::uint64_t{1} << n) - 1)) & ~kmask;
@@ -658,9 +678,8 @@ int build_rho(const void *psi, bool f64, int n, const int *kept, int count, hipS
 }
 
 // Eigenvalues of w.rho (m x m) into host vector.
-int eigvalsh(int m, hipStream_t stream, std::vector<double> &lam)
+int eigvalsh(Workspace &w, int m, hipStream_t stream, std::vector<double> &lam)
 {
-    Workspace &w = ws();
     if (!w.handle)
     {
         const rocblas_status s = rocblas_create_handle(&w.handle);
@@ -703,7 +722,7 @@ double entropy_from_eigs(const std::vector<double> &lam)
     return std::abs(s) < 1.0e-12 ? 0.0 : s;
 }
 
-int entropy_impl(const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int order,
+int entropy_impl(Workspace &w, const void *psi, bool f64, int n, const int *kept, int count, hipStream_t stream, int order,
                  double *out)
 {
     if (!psi || !out)
@@ -723,10 +742,9 @@ int entropy_impl(const void *psi, bool f64, int n, const int *kept, int count, h
         return OK;
     }
     int m = 0;
-    MIPT_HIP_TRY(build_rho(psi, f64, n, kept, count, stream, m));
+    MIPT_HIP_TRY(build_rho(w, psi, f64, n, kept, count, stream, m));
     if (order == 2)
     {
-        Workspace &w = ws();
         const std::size_t n2 = static_cast<std::size_t>(m) * m;
         const int blocks = static_cast<int>(std::min<std::size_t>(1024, (n2 + 255) / 256));
         if (!w.red) MIPT_HIP_CHECK(hipMalloc(&w.red, 1024 * sizeof(double)));
@@ -749,24 +767,22 @@ int entropy_impl(const void *psi, bool f64, int n, const int *kept, int count, h
     if (g_eig_solver.load() == 0 && m <= 512)
     {
         // one workgroup per matrix: cheaper than rocSOLVER's launch sequence up to this size
-        Workspace &w = ws();
-        return own_entropies(m, 1, stream, w.rho, out);
+        return own_entropies(w, m, 1, stream, w.rho, out);
     }
     std::vector<double> lam;
-    MIPT_HIP_TRY(eigvalsh(m, stream, lam));
+    MIPT_HIP_TRY(eigvalsh(w, m, stream, lam));
     *out = entropy_from_eigs(lam);
     return OK;
 }
 
 template <typename C>
-int rho1_impl(const void *psi, int n, int q, hipStream_t stream, double *out4)
+int rho1_impl(Workspace &w, const void *psi, int n, int q, hipStream_t stream, double *out4)
 {
     if (!psi || !out4 || n < 1 || n >= 63 || q < 0 || q >= n)
     {
         set_error("rho1: bad arguments");
         return E_ARG;
     }
-    Workspace &w = ws();
     if (!w.red) MIPT_HIP_CHECK(hipMalloc(&w.red, 1024 * sizeof(double)));
     const std::uint64_t env = std::uint64_t{1} << (n - 1);
     const int blocks = static_cast<int>(std::min<std::uint64_t>(256, (env + 255) / 256));
@@ -781,7 +797,7 @@ int rho1_impl(const void *psi, int n, int q, hipStream_t stream, double *out4)
     return OK;
 }
 
-int tmi_impl(const void *psi, bool f64, int n, int L, int block, int cycle, hipStream_t stream, double *out,
+int tmi_impl(Workspace &w, const void *psi, bool f64, int n, int L, int block, int cycle, hipStream_t stream, double *out,
              double *terms, int order = 1)
 {
     if (!psi || !out || L < 4 || L > n || block < 1 || 3 * block > L || cycle < 0 || cycle >= L)
@@ -803,16 +819,15 @@ int tmi_impl(const void *psi, bool f64, int n, int L, int block, int cycle, hipS
     const std::vector<int> sets[7] = {A, B, Cc, D, cat(A, B), cat(A, Cc), cat(B, Cc)};
     double s[7];
     for (int i = 0; i < 7; ++i)
-        MIPT_HIP_TRY(entropy_impl(psi, f64, n, sets[i].data(), static_cast<int>(sets[i].size()), stream, order, &s[i]));
+        MIPT_HIP_TRY(entropy_impl(w, psi, f64, n, sets[i].data(), static_cast<int>(sets[i].size()), stream, order, &s[i]));
     *out = s[0] + s[1] + s[2] + s[3] - s[4] - s[5] - s[6];
     if (terms)
         for (int i = 0; i < 7; ++i) terms[i] = s[i];
     return OK;
 }
 
-int ensure_handle(hipStream_t stream)
+int ensure_handle(Workspace &w, hipStream_t stream)
 {
-    Workspace &w = ws();
     if (!w.handle)
     {
         const rocblas_status s = rocblas_create_handle(&w.handle);
@@ -831,11 +846,10 @@ int ensure_handle(hipStream_t stream)
 // eigensolver reads: one gather and one Gram launch for all items.  roff and
 // emask hold the uploaded tables; the caller keeps them until the stream
 // has passed this work.
-int build_rho_batch(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+int build_rho_batch(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
                     double2 *dst, std::vector<std::uint64_t> &roff, std::vector<std::uint64_t> &emask,
                     std::vector<std::uint64_t> &pv)
 {
-    Workspace &w = ws();
     const int m = 1 << kept_count, env = n - kept_count;
     const std::size_t K = std::size_t{1} << env;
     roff.assign(static_cast<std::size_t>(count) * m, 0);
@@ -887,9 +901,8 @@ int build_rho_batch(const float2 *const *psis, int n, const int *kept, int kept_
 
 // Release batch-sized workspaces above 2 GiB so later allocations (other
 // batches, other legs of a run) see the memory again.
-int trim_batch_workspace()
+int trim_batch_workspace(Workspace &w)
 {
-    Workspace &w = ws();
     const std::size_t held = w.rho_b_cap * 16 + (w.plane_cap + w.plane_cap_i) * 4 + w.eig_w_cap * 16;
     if (held <= (std::size_t{2} << 30)) return OK;
     MIPT_HIP_CHECK(hipFree(w.rho_b));
@@ -906,10 +919,9 @@ int trim_batch_workspace()
 // entropy_many on the library's eigensolver: chunks of whole GPU rounds,
 // each with one gather, one Gram, one tridiagonal reduction and one
 // eigenvalue launch.
-int entropy_many_own(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+int entropy_many_own(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
                      double *out)
 {
-    Workspace &w = ws();
     const int m = 1 << kept_count;
     const std::size_t K = std::size_t{1} << (n - kept_count);
     const std::size_t mm = static_cast<std::size_t>(m) * m;
@@ -927,11 +939,11 @@ int entropy_many_own(const float2 *const *psis, int n, const int *kept, int kept
     for (int c0 = 0; c0 < count; c0 += chunk)
     {
         const int nb = std::min(chunk, count - c0);
-        MIPT_HIP_TRY(build_rho_batch(psis + c0, n, kept + static_cast<std::size_t>(c0) * kept_count, kept_count, nb, stream,
+        MIPT_HIP_TRY(build_rho_batch(w, psis + c0, n, kept + static_cast<std::size_t>(c0) * kept_count, kept_count, nb, stream,
                                      w.rho_b, roff, emask, pv));
-        MIPT_HIP_TRY(own_entropies(m, nb, stream, w.rho_b, out + c0));  // synchronises the stream
+        MIPT_HIP_TRY(own_entropies(w, m, nb, stream, w.rho_b, out + c0));  // synchronises the stream
     }
-    return trim_batch_workspace();
+    return trim_batch_workspace(w);
 }
 
 // Von Neumann entropies (bits) of `count` items (state psis[i], kept set
@@ -940,7 +952,7 @@ int entropy_many_own(const float2 *const *psis, int n, const int *kept, int kept
 // library's batched eigensolver (one workgroup per matrix, eigen.hip) or,
 // when selected or beyond EIG_MAX_N, from one
 // rocsolver_zheevd_strided_batched call per chunk.
-int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+int entropy_many(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
                  double *out)
 {
     if (!psis || !out || count < 0 || (count > 0 && !kept))
@@ -953,21 +965,20 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
         for (int i = 0; i < count; ++i) out[i] = 0.0;
         return OK;
     }
-    Workspace &w = ws();
     const int m = 1 << kept_count;
     const std::size_t mm = static_cast<std::size_t>(m) * m;
     const bool own = g_eig_solver.load() == 0 && m <= EIG_MAX_N;
-    if (own) return entropy_many_own(psis, n, kept, kept_count, count, stream, out);
-    MIPT_HIP_TRY(ensure_handle(stream));
+    if (own) return entropy_many_own(w, psis, n, kept, kept_count, count, stream, out);
+    MIPT_HIP_TRY(ensure_handle(w, stream));
     if (m < 32)
     {
         // tiny RDMs on rocSOLVER: one solve each (cheap at this size)
         for (int i = 0; i < count; ++i)
         {
             int mo = 0;
-            MIPT_HIP_TRY(build_rho(psis[i], false, n, kept + static_cast<std::size_t>(i) * kept_count, kept_count, stream, mo));
+            MIPT_HIP_TRY(build_rho(w, psis[i], false, n, kept + static_cast<std::size_t>(i) * kept_count, kept_count, stream, mo));
             std::vector<double> lam1;
-            MIPT_HIP_TRY(eigvalsh(m, stream, lam1));
+            MIPT_HIP_TRY(eigvalsh(w, m, stream, lam1));
             out[i] = entropy_from_eigs(lam1);
         }
         return OK;
@@ -985,7 +996,7 @@ int entropy_many(const float2 *const *psis, int n, const int *kept, int kept_cou
         for (int i = 0; i < nb; ++i)
         {
             int mo = 0;
-            MIPT_HIP_TRY(build_rho(psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
+            MIPT_HIP_TRY(build_rho(w, psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
                                    kept_count, stream, mo, w.rho_b + i * mm));
         }
         const rocblas_status st = rocsolver_zheevd_strided_batched(
@@ -1036,7 +1047,7 @@ __global__ void purity_many_kernel(const double2 *__restrict__ rho, int m, doubl
 
 // Purities Tr(rho^2) of `count` (state, kept set) items of one shape (the
 // Renyi-2 counterpart of entropy_many: one strided RDM buffer per chunk).
-int purity_many(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+int purity_many(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
                 double *out)
 {
     if (!psis || !out || count < 0 || (count > 0 && !kept))
@@ -1049,7 +1060,6 @@ int purity_many(const float2 *const *psis, int n, const int *kept, int kept_coun
         for (int i = 0; i < count; ++i) out[i] = 1.0;
         return OK;
     }
-    Workspace &w = ws();
     const int m = 1 << kept_count;
     const std::size_t mm = static_cast<std::size_t>(m) * m;
     const int chunk = static_cast<int>(std::max<std::size_t>(1, std::min<std::size_t>(count, (std::size_t{1} << 30) / (mm * 16))));
@@ -1061,7 +1071,7 @@ int purity_many(const float2 *const *psis, int n, const int *kept, int kept_coun
         for (int i = 0; i < nb; ++i)
         {
             int mo = 0;
-            MIPT_HIP_TRY(build_rho(psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
+            MIPT_HIP_TRY(build_rho(w, psis[c0 + i], false, n, kept + static_cast<std::size_t>(c0 + i) * kept_count,
                                    kept_count, stream, mo, w.rho_b + i * mm));
         }
         hipLaunchKernelGGL(purity_many_kernel, dim3(nb), dim3(256), 0, stream, w.rho_b, m, w.ev_b);
@@ -1072,7 +1082,7 @@ int purity_many(const float2 *const *psis, int n, const int *kept, int kept_coun
     return OK;
 }
 
-int rdm_export(const void *psi, bool f64, int n, const int *kept, int count, double *host)
+int rdm_export(Workspace &w, const void *psi, bool f64, int n, const int *kept, int count, double *host)
 {
     if (!host || count > 12)
     {
@@ -1086,44 +1096,46 @@ int rdm_export(const void *psi, bool f64, int n, const int *kept, int count, dou
         host[1] = 0.0;
         return OK;
     }
-    MIPT_HIP_TRY(build_rho(psi, f64, n, kept, count, nullptr, m));
-    MIPT_HIP_CHECK(hipMemcpyAsync(host, ws().rho, static_cast<std::size_t>(m) * m * 16, hipMemcpyDeviceToHost, nullptr));
+    MIPT_HIP_TRY(build_rho(w, psi, f64, n, kept, count, nullptr, m));
+    MIPT_HIP_CHECK(hipMemcpyAsync(host, w.rho, static_cast<std::size_t>(m) * m * 16, hipMemcpyDeviceToHost, nullptr));
     MIPT_HIP_CHECK(hipStreamSynchronize(nullptr));
     return OK;
 }
 } // namespace
 
-int entropy_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out)
+int entropy_bits_f32(Workspace &w, const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream,
+                     double *out)
 {
-    return entropy_impl(psi, false, n, kept, kept_count, stream, 1, out);
+    return entropy_impl(w, psi, false, n, kept, kept_count, stream, 1, out);
 }
 
-int tmi_bits_f32(const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream, double *out,
-                 double *terms, int order)
+int tmi_bits_f32(Workspace &w, const float2 *psi, int n, int chain_sites, int block, int cycle, hipStream_t stream,
+                 double *out, double *terms, int order)
 {
-    return tmi_impl(psi, false, n, chain_sites, block, cycle, stream, out, terms, order);
+    return tmi_impl(w, psi, false, n, chain_sites, block, cycle, stream, out, terms, order);
 }
 
-int renyi2_bits_f32(const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream, double *out)
-{
-    return entropy_impl(psi, false, n, kept, kept_count, stream, 2, out);
-}
-
-int entropy_many_bits_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count,
-                          hipStream_t stream, double *out)
-{
-    return entropy_many(psis, n, kept, kept_count, count, stream, out);
-}
-
-int purity_many_f32(const float2 *const *psis, int n, const int *kept, int kept_count, int count, hipStream_t stream,
+int renyi2_bits_f32(Workspace &w, const float2 *psi, int n, const int *kept, int kept_count, hipStream_t stream,
                     double *out)
 {
-    return purity_many(psis, n, kept, kept_count, count, stream, out);
+    return entropy_impl(w, psi, false, n, kept, kept_count, stream, 2, out);
 }
 
-int rho1_f32(const float2 *psi, int n, int qubit, hipStream_t stream, double *out4)
+int entropy_many_bits_f32(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count,
+                          hipStream_t stream, double *out)
 {
-    return rho1_impl<float2>(psi, n, qubit, stream, out4);
+    return entropy_many(w, psis, n, kept, kept_count, count, stream, out);
+}
+
+int purity_many_f32(Workspace &w, const float2 *const *psis, int n, const int *kept, int kept_count, int count,
+                    hipStream_t stream, double *out)
+{
+    return purity_many(w, psis, n, kept, kept_count, count, stream, out);
+}
+
+int rho1_f32(Workspace &w, const float2 *psi, int n, int qubit, hipStream_t stream, double *out4)
+{
+    return rho1_impl<float2>(w, psi, n, qubit, stream, out4);
 }
 
 } // namespace mipt_hip
@@ -1145,40 +1157,40 @@ static int check_fermion(int fermion_trace)
 int mipt_hip_entropy_svd_f32(const void *s, int n, const int *k, int c, int f, double *out)
 {
     MIPT_HIP_TRY(check_fermion(f));
-    return entropy_impl(s, false, n, k, c, nullptr, 1, out);
+    return entropy_impl(ws(), s, false, n, k, c, nullptr, 1, out);
 }
 int mipt_hip_entropy_svd_f64(const void *s, int n, const int *k, int c, int f, double *out)
 {
     MIPT_HIP_TRY(check_fermion(f));
-    return entropy_impl(s, true, n, k, c, nullptr, 1, out);
+    return entropy_impl(ws(), s, true, n, k, c, nullptr, 1, out);
 }
 int mipt_hip_renyi2_f32(const void *s, int n, const int *k, int c, int f, double *out)
 {
     MIPT_HIP_TRY(check_fermion(f));
-    return entropy_impl(s, false, n, k, c, nullptr, 2, out);
+    return entropy_impl(ws(), s, false, n, k, c, nullptr, 2, out);
 }
 int mipt_hip_renyi2_f64(const void *s, int n, const int *k, int c, int f, double *out)
 {
     MIPT_HIP_TRY(check_fermion(f));
-    return entropy_impl(s, true, n, k, c, nullptr, 2, out);
+    return entropy_impl(ws(), s, true, n, k, c, nullptr, 2, out);
 }
 int mipt_hip_rdm_subsystem_f32(const void *s, int n, const int *k, int c, int f, double *rho)
 {
     MIPT_HIP_TRY(check_fermion(f));
-    return rdm_export(s, false, n, k, c, rho);
+    return rdm_export(ws(), s, false, n, k, c, rho);
 }
 int mipt_hip_rdm_subsystem_f64(const void *s, int n, const int *k, int c, int f, double *rho)
 {
     MIPT_HIP_TRY(check_fermion(f));
-    return rdm_export(s, true, n, k, c, rho);
+    return rdm_export(ws(), s, true, n, k, c, rho);
 }
 int mipt_hip_rho1_complex_f32(const void *s, int n, int q, double *rho)
 {
-    return rho1_impl<float2>(s, n, q, nullptr, rho);
+    return rho1_impl<float2>(ws(), s, n, q, nullptr, rho);
 }
 int mipt_hip_rho1_complex_f64(const void *s, int n, int q, double *rho)
 {
-    return rho1_impl<double2>(s, n, q, nullptr, rho);
+    return rho1_impl<double2>(ws(), s, n, q, nullptr, rho);
 }
 int mipt_hip_set_gram_profiling(int enable)
 {
@@ -1271,12 +1283,12 @@ int mipt_hip_set_eigensolver(int which)
 
 int mipt_hip_tmi_f32(const void *s, int n, int L, int block, int cycle, double *out, double *terms)
 {
-    return tmi_impl(s, false, n, L, block, cycle, nullptr, out, terms);
+    return tmi_impl(ws(), s, false, n, L, block, cycle, nullptr, out, terms);
 }
 
 int mipt_hip_tmi_f64(const void *s, int n, int L, int block, int cycle, double *out, double *terms)
 {
-    return tmi_impl(s, true, n, L, block, cycle, nullptr, out, terms);
+    return tmi_impl(ws(), s, true, n, L, block, cycle, nullptr, out, terms);
 }
 
 } // extern "C"
