@@ -176,3 +176,31 @@ def test_eigensolver_knob_refuses_bad_values(M):
         M.set_eigensolver(2)
     with pytest.raises(M.MiptHipError):
         M.entropy_hermitian_batched(np.zeros((1, 1025, 1025), np.complex128))
+
+
+def _device_free_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_batch_destroy_releases_its_readout_workspace(M):
+    # the batch owns the entropy workspace of its readouts: 8 trajectories x 3
+    # ten-site TMI regions = 24 RDMs of 1024 x 1024 with their Gram planes and
+    # eigensolver panels, ~0.6 GB (below the 2 GiB the library trims batch
+    # buffers to after a readout); it is held while the batch lives and
+    # released with it
+    free0 = _device_free_bytes()
+    b = _batch(M, 20, 4, 8, 9090)
+    try:
+        free_states = _device_free_bytes()
+        sh, _ = b.observables_all(5, 1)
+        assert np.all(np.isfinite(sh))
+        held = free_states - _device_free_bytes()
+        assert held > (400 << 20), f"the readout holds only {held / 2**20:.0f} MiB"
+    finally:
+        b.close()
+    assert _device_free_bytes() >= free0 - (64 << 20)
