@@ -341,6 +341,19 @@ int mipt_hip_batch_reset(mipt_hip_batch *batch);
  * flags ([layers][L]) and uniforms ([layers][L]) to host (any may be NULL). */
 int mipt_hip_batch_export_layers(mipt_hip_batch *batch, int traj, double *gates_ri, uint8_t *flags,
                                  double *uniforms);
+/* Tile plan of one layer (no counterpart in the reference; for callers and
+ * tests that check or inspect the sweep schedule on the host, no GPU needed):
+ * the sweeps the batch runs for layer `layer` of an nq-qubit state whose
+ * first L qubits form the chain (closed ring if `closed`, grid_x columns for
+ * a grid, 0 for a chain) under tile_bits 12..14 / threads as in
+ * mipt_hip_batch_config (tile_bits 13 with threads 128 or 256), layer plan
+ * variant 0..3.  out receives ints: nbonds, then (q0, q1) of every bond of
+ * the layer; npasses; per pass k, c, s, nstages; per stage use_lds, zpair,
+ * then (bond, q0, q1) for bond[0] and bond[1] (bond -1: none), reg_u[6],
+ * lane_u[6], wave_u[4].  n_out receives the count of ints; status -1 (and
+ * nothing written) when out_len is smaller. */
+int mipt_hip_layer_plan(int L, int nq, int closed, int grid_x, int layer, int tile_bits, int threads, int plan,
+                        int *out, int out_len, int *n_out);
 /* Tiled groups in flight: 2 (default) alternates groups between the batch
  * stream and a second stream so one group's small kernels overlap the
  * other's sweeps; 1 runs every group on the batch stream, so a sweep's

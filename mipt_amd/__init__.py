@@ -100,6 +100,7 @@ def lib():
     L.mipt_hip_batch_export_layers.argtypes = [vp, i, dp, u8p, dp]
     L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
     L.mipt_hip_batch_set_streams.argtypes = [vp, i]
+    L.mipt_hip_layer_plan.argtypes = [i, i, i, i, i, i, i, i, ip, i, ip]
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
@@ -313,6 +314,40 @@ class PlainEngine:
         out = ctypes.c_double()
         _check(lib().mipt_hip_engine_squared_norm(self._h, ctypes.byref(out)), "squared_norm")
         return out.value
+
+
+def layer_plan(L: int, nq: int, closed: bool, grid_x: int, layer: int, tile_bits: int = 13, threads: int = 0,
+               plan: int = 0) -> dict:
+    """The sweeps the batch runs for one layer (host only, no GPU): {"bonds":
+    [(q0, q1)], "passes": [{"k", "c", "s", "stages": [{"use_lds", "zpair",
+    "bonds": [(bond, q0, q1)] x 2, "reg_u", "lane_u", "wave_u"}]}]}."""
+    buf = np.zeros(1 << 14, dtype=np.int32)
+    n = ctypes.c_int()
+    _check(lib().mipt_hip_layer_plan(L, nq, int(bool(closed)), grid_x, layer, tile_bits, threads, plan, _ip(buf),
+                                     len(buf), ctypes.byref(n)), "layer_plan")
+    v = buf[: n.value].tolist()
+    pos = 0
+
+    def take(k):
+        nonlocal pos
+        out = v[pos:pos + k]
+        pos += k
+        return out
+
+    nb = take(1)[0]
+    flat = take(2 * nb)
+    bonds = [(flat[2 * b], flat[2 * b + 1]) for b in range(nb)]
+    passes = []
+    for _ in range(take(1)[0]):
+        k, c, s, ns = take(4)
+        stages = []
+        for _ in range(ns):
+            use_lds, zpair = take(2)
+            sb = [tuple(take(3)) for _ in range(2)]
+            stages.append({"use_lds": use_lds, "zpair": zpair, "bonds": sb, "reg_u": take(6), "lane_u": take(6),
+                           "wave_u": take(4)})
+        passes.append({"k": k, "c": c, "s": s, "stages": stages})
+    return {"bonds": bonds, "passes": passes}
 
 
 def stream_copy_calibration(log2_elems: int = 25) -> int:

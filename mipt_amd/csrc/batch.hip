@@ -2989,6 +2989,73 @@ int mipt_hip_batch_export_layers(mipt_hip_batch *b, int traj, double *g, uint8_t
     return b->export_layers(traj, g, f, u);
 }
 
+int mipt_hip_layer_plan(int L, int nq, int closed, int grid_x, int layer, int tile_bits, int threads, int plan,
+                        int *out, int out_len, int *n_out)
+{
+    if (!out || !n_out || L < 2 || nq < L || nq > 30 || layer < 0 || plan < 0 || plan > 3 || grid_x < 0 ||
+        (grid_x > 0 && (L % grid_x) != 0))
+    {
+        set_error("layer_plan: bad arguments");
+        return E_ARG;
+    }
+    Config cf;
+    if (tile_bits == 14 && threads == 0) cf = Config{6, 256, 6};
+    else if (tile_bits == 13 && (threads == 0 || threads == 256)) cf = Config{5, 256, 5};
+    else if (tile_bits == 13 && threads == 128) cf = Config{6, 128, 5};
+    else if (tile_bits == 12 && threads == 0) cf = Config{6, 64, 4};
+    else
+    {
+        set_error("layer_plan: tile_bits %d with threads %d is not a tile configuration", tile_bits, threads);
+        return E_ARG;
+    }
+    cf.plan = plan;
+    if (nq <= cf.k())
+    {
+        set_error("layer_plan: %d qubits fit one tile (the small kernel runs them)", nq);
+        return E_ARG;
+    }
+    LayerPlan lp;
+    MIPT_HIP_TRY(build_layer_plan(L, nq, closed, grid_x, layer, cf, lp));
+    std::vector<int> v;
+    const int nb = geo_bond_count(L, grid_x, layer, closed);
+    v.push_back(nb);
+    for (int b = 0; b < nb; ++b)
+    {
+        int q0, q1;
+        geo_bond(L, grid_x, layer, b, closed, q0, q1);
+        v.push_back(q0);
+        v.push_back(q1);
+    }
+    v.push_back(static_cast<int>(lp.passes.size()));
+    for (const PassPlan &pp : lp.passes)
+    {
+        v.insert(v.end(), {pp.k, pp.c, pp.s, pp.nstages});
+        for (int si = 0; si < pp.nstages; ++si)
+        {
+            const StagePlan &sp = pp.st[si];
+            v.push_back(sp.use_lds);
+            v.push_back(sp.zpair);
+            for (int j = 0; j < 2; ++j)
+            {
+                int q0 = -1, q1 = -1;
+                if (sp.bond[j] >= 0) geo_bond(L, grid_x, layer, sp.bond[j], closed, q0, q1);
+                v.insert(v.end(), {sp.bond[j], q0, q1});
+            }
+            v.insert(v.end(), sp.reg_u, sp.reg_u + 6);
+            v.insert(v.end(), sp.lane_u, sp.lane_u + 6);
+            v.insert(v.end(), sp.wave_u, sp.wave_u + 4);
+        }
+    }
+    *n_out = static_cast<int>(v.size());
+    if (static_cast<int>(v.size()) > out_len)
+    {
+        set_error("layer_plan: %zu ints needed, out_len %d", v.size(), out_len);
+        return E_ARG;
+    }
+    std::copy(v.begin(), v.end(), out);
+    return OK;
+}
+
 int mipt_hip_batch_set_streams(mipt_hip_batch *b, int streams)
 {
     if (!b || (streams != 1 && streams != 2))
