@@ -101,6 +101,7 @@ def lib():
     L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
     L.mipt_hip_batch_set_streams.argtypes = [vp, i]
     L.mipt_hip_layer_plan.argtypes = [i, i, i, i, i, i, i, i, ip, i, ip]
+    L.mipt_hip_set_layer_fusion.argtypes = [i]
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
     L.mipt_hip_batch_tmi.argtypes = [vp, i, i, i, i, dp]
@@ -314,6 +315,15 @@ class PlainEngine:
         out = ctypes.c_double()
         _check(lib().mipt_hip_engine_squared_norm(self._h, ctypes.byref(out)), "squared_norm")
         return out.value
+
+
+def set_layer_fusion(on) -> None:
+    """Process-wide: fuse a quiet layer's last sweep with the next layer's
+    first (True / 1, the default), or not (False / 0); None or -1 goes back to
+    MIPT_HIP_FUSE / the default."""
+    if on is None:
+        on = -1
+    _check(lib().mipt_hip_set_layer_fusion(int(on) if isinstance(on, bool) else on), "set_layer_fusion")
 
 
 def layer_plan(L: int, nq: int, closed: bool, grid_x: int, layer: int, tile_bits: int = 13, threads: int = 0,

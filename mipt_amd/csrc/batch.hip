@@ -29,6 +29,8 @@
 #include "state_ops.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <type_traits>
 #include <array>
 #include <cmath>
 #include <cstdlib>
@@ -610,6 +612,10 @@ struct PassPlan
     int zskip;         // leave tiles on the collapsed-out branch untouched (the next sweep zeroes them)
     std::uint64_t bonded;  // sites whose bonds this sweep applies
     std::uint64_t pmask;   // previous layer's collapsed sites to zero again at load (no rescale)
+    // layer fusion: this is the layer's last sweep, launched over the
+    // trajectories that measure nothing in this layer, which continue with the
+    // next layer's first sweep (PassArgs::next, the same window) on the tile
+    int fuse;
 };
 
 struct PassArgs
@@ -619,6 +625,7 @@ struct PassArgs
     int ntraj;             // trajectories in the group (tiles = ntraj * plan.ntiles)
     int layer;
     PassPlan plan;
+    PassPlan next;         // with plan.fuse: the next layer's first sweep (same window; its stages are used)
 };
 
 __device__ __forceinline__ std::uint32_t swz(std::uint32_t u)
@@ -1120,6 +1127,7 @@ struct TileCtx
     bool zero, gen_init, tile_out;
     std::uint32_t flags;          // the layer's measured sites
     std::uint32_t cdm, cdv;       // decided outcomes on local bits (u bits; overflow-chunk histograms)
+    std::uint64_t goff2;          // the next layer's first gate (fused launches)
 };
 
 // Read-only arrays of a sweep, from kernel arguments declared __restrict__ so
@@ -1162,6 +1170,7 @@ __device__ __forceinline__ bool tile_context(const PassArgs &args, const int *__
     const Collapse ccur = coll[lt];
     x.flags = fl;
     if (aux && pl.cond && __popc(x.flags) <= pl.chunk * MAX_HIST_BITS) return false;
+    x.goff2 = (d.gate_off + gate_prefix(d, t + 1)) * 16;
     x.L = d.L;
     x.nq = d.nq;
     x.B = pdep64(static_cast<std::uint64_t>(x.tile), pl.rest_mask);
@@ -1301,8 +1310,16 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     // are published by the first exchange's barrier (or one of its own)
     const float *mxtab = reinterpret_cast<const float *>(ldsb + (std::size_t{8} << (k - 1)));
     bool tab_ready = MX == 0;
-    if constexpr (MX > 0 && AUX == 0) fill_mx_table<NT>(pl, gates + cur.goff, const_cast<float *>(mxtab));
-    // moves the tile into stage sp's mapping (no move for an M0 stage)
+    constexpr bool FUSED = ZP >= 4;  // ZP 4..6: the next layer's first sweep follows, its qubit-0 form ZP - 4
+    if constexpr (MX > 0 && AUX == 0)
+    {
+        fill_mx_table<NT>(pl, gates + cur.goff, const_cast<float *>(mxtab));
+        if constexpr (FUSED)
+            fill_mx_table<NT>(args.next, gates + cur.goff2,
+                              const_cast<float *>(mxtab) + pl.nstages * (2 * MX_SLOT_BYTES / 4));
+    }
+    // moves the tile into stage sp's mapping (no move for an M0 stage: the
+    // first of a plan, and a fused continuation starts with an exchange)
     auto enter_stage = [&](const StagePlan &sp) {
         if (!tab_ready && !sp.use_lds) __syncthreads();
         tab_ready = true;
@@ -1322,70 +1339,84 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
             in_m0 = false;
         }
     };
-    constexpr bool zlast = ZP == 1 || ZP == 2;
-    const int nloop = AUX ? 0 : pl.nstages - (zlast ? 1 : 0);
-    for (int si = 0; si < nloop; ++si)
-    {
-        const StagePlan &sp = pl.st[si];
-        enter_stage(sp);
-        apply_stage_gates<NV, MX, ZP == 3 ? 3 : 0>(v, sp, gl, mxtab + si * (2 * MX_SLOT_BYTES / 4), lane);
-    }
-    if constexpr (zlast && !AUX)
-    {
-        const int si = pl.nstages - 1;
-        const StagePlan &sp = pl.st[si];
-        enter_stage(sp);
-        apply_stage_gates<NV, MX, ZP>(v, sp, gl, mxtab + si * (2 * MX_SLOT_BYTES / 4), lane);
-    }
-
-    // ---- the chunk's measured sites that are local to this tile (u bits)
-    std::uint32_t loc_u = 0;
-    // a layer without measured sites needs no partials (its finalize
-    // records the identity collapse without reading them)
-    const bool do_hist = pl.hist && !(pl.chunk == 0 && cur.flags == 0u);
-    if (do_hist)
-    {
-        int seen = 0;
-        for (int q = 0; q < cur.L; ++q)
+    // the stages of plan P, its qubit-0 form Z (the stage type of ztag) fixed
+    // at compile time, gates glx, coefficient-table slots from stage slot0
+    auto run_stages = [&](const PassPlan &P, auto ztag, const float2 *glx, int slot0) {
+        constexpr int Z = decltype(ztag)::value;
+        constexpr bool zlast = Z == 1 || Z == 2;
+        const int nloop = AUX ? 0 : P.nstages - (zlast ? 1 : 0);
+        for (int si = 0; si < nloop; ++si)
         {
-            if (!((cur.flags >> q) & 1u)) continue;
-            if (seen >= pl.chunk * MAX_HIST_BITS && seen < (pl.chunk + 1) * MAX_HIST_BITS)
+            const StagePlan &sp = P.st[si];
+            enter_stage(sp);
+            apply_stage_gates<NV, MX, Z == 3 ? 3 : 0>(v, sp, glx, mxtab + (slot0 + si) * (2 * MX_SLOT_BYTES / 4), lane);
+        }
+        if constexpr (zlast && !AUX)
+        {
+            const int si = P.nstages - 1;
+            const StagePlan &sp = P.st[si];
+            enter_stage(sp);
+            apply_stage_gates<NV, MX, Z>(v, sp, glx, mxtab + (slot0 + si) * (2 * MX_SLOT_BYTES / 4), lane);
+        }
+    };
+    run_stages(pl, std::integral_constant<int, FUSED ? 0 : ZP>{}, gl, 0);
+
+    // back to the load mapping, the store, the histogram partial
+    auto finish = [&]() {
+        // ---- the chunk's measured sites that are local to this tile (u bits)
+        std::uint32_t loc_u = 0;
+        // a layer without measured sites needs no partials (its finalize
+        // records the identity collapse without reading them)
+        const bool do_hist = pl.hist && !(pl.chunk == 0 && cur.flags == 0u);
+        if (do_hist)
+        {
+            int seen = 0;
+            for (int q = 0; q < cur.L; ++q)
             {
-                if (q < c) loc_u |= 1u << q;
-                else if (q >= s && q < s + (k - c)) loc_u |= 1u << (q - s + c);
+                if (!((cur.flags >> q) & 1u)) continue;
+                if (seen >= pl.chunk * MAX_HIST_BITS && seen < (pl.chunk + 1) * MAX_HIST_BITS)
+                {
+                    if (q < c) loc_u |= 1u << q;
+                    else if (q >= s && q < s + (k - c)) loc_u |= 1u << (q - s + c);
+                }
+                ++seen;
             }
-            ++seen;
         }
-    }
 
-    // ---- back to M0 through LDS
-    m0_terms(toff, T3m0, lane, wave);
-    if (!in_m0) lds_exchange<NV>(ldsb, v, T3, Q3, T3m0, Q3m0, lds_dirty);
+        // ---- back to M0 through LDS
+        m0_terms(toff, T3m0, lane, wave);
+        if (!in_m0) lds_exchange<NV>(ldsb, v, T3, Q3, T3m0, Q3m0, lds_dirty);
 
-    // ---- store (M0); before the histogram, so the tile's registers are
-    // free again as its FP64 weights are formed
-    if (pl.store)
-    {
-        char *tp = reinterpret_cast<char *>(cur.state + cur.B);
-#pragma unroll
-        for (int i = 0; i < NV / 2; ++i)
-            *reinterpret_cast<float4 *>(tp + ((toff + (static_cast<std::uint32_t>(i) << jshift)) << 3)) =
-                make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
-    }
-    // ---- FP64 histogram partial of this tile over the chunk's local sites
-    if (do_hist)
-    {
-        double *out = a.partials + (static_cast<std::size_t>(cur.tl) * pl.ntiles + cur.tile) * HSTRIDE;
-        if (cur.tile_out)
+        // ---- store (M0); before the histogram, so the tile's registers are
+        // free again as its FP64 weights are formed
+        if (pl.store)
         {
-            // the tile lies outside the decided outcomes: every bin is 0
-            const std::uint32_t bins = 1u << __popc(loc_u);
-            for (std::uint32_t bb = threadIdx.x; bb < bins; bb += NT) out[bb] = 0.0;
+            char *tp = reinterpret_cast<char *>(cur.state + cur.B);
+#pragma unroll
+            for (int i = 0; i < NV / 2; ++i)
+                *reinterpret_cast<float4 *>(tp + ((toff + (static_cast<std::uint32_t>(i) << jshift)) << 3)) =
+                    make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
         }
-        else
-            tile_histogram<NV, NT>(v, loc_u, AUX && pl.cond, cur.cdm, cur.cdv, lane, wave,
-                                   reinterpret_cast<double *>(lds), out);
-    }
+        // ---- FP64 histogram partial of this tile over the chunk's local sites
+        if (do_hist)
+        {
+            double *out = a.partials + (static_cast<std::size_t>(cur.tl) * pl.ntiles + cur.tile) * HSTRIDE;
+            if (cur.tile_out)
+            {
+                // the tile lies outside the decided outcomes: every bin is 0
+                const std::uint32_t bins = 1u << __popc(loc_u);
+                for (std::uint32_t bb = threadIdx.x; bb < bins; bb += NT) out[bb] = 0.0;
+            }
+            else
+                tile_histogram<NV, NT>(v, loc_u, AUX && pl.cond, cur.cdm, cur.cdv, lane, wave,
+                                       reinterpret_cast<double *>(lds), out);
+        }
+    };
+    // a fused launch holds only trajectories that measure nothing in this
+    // layer: the next layer's first sweep (same window, no collapse to apply,
+    // no histogram) follows on the tile in registers and LDS
+    if constexpr (FUSED) run_stages(args.next, std::integral_constant<int, ZP - 4>{}, gates + cur.goff2, pl.nstages);
+    finish();
 }
 
 // One sweep over the tiles of a group of trajectories.  AUX = 0: a gate sweep
@@ -1434,6 +1465,30 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs ar
     }
 }
 
+// Layer fusion's trajectory order: for layer t = t0 + blockIdx.x, the
+// group's trajectories (positions in list) that measure nothing in layer t
+// first, then the others, each in list order; out[blockIdx.x * n + ...].
+// q: n bytes of LDS.
+__global__ void __launch_bounds__(256) quiet_order_kernel(const int *__restrict__ list, int n,
+                                                          const TrajDesc *__restrict__ desc,
+                                                          const std::uint32_t *__restrict__ flags, int t0,
+                                                          int *__restrict__ out)
+{
+    extern __shared__ unsigned char q[];
+    const int t = t0 + static_cast<int>(blockIdx.x);
+    for (int j = threadIdx.x; j < n; j += 256) q[j] = flags[desc[list[j]].layer_off + t] == 0u ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int *o = out + static_cast<std::size_t>(blockIdx.x) * n;
+        int w = 0;
+        for (int j = 0; j < n; ++j)
+            if (q[j]) o[w++] = list[j];
+        for (int j = 0; j < n; ++j)
+            if (!q[j]) o[w++] = list[j];
+    }
+}
+
 // One workgroup per trajectory of the group: finalize_chunk after a
 // histogram sweep (the kernel boundary publishes every tile's partials).
 struct FinalizeArgs
@@ -1479,6 +1534,32 @@ struct Config
     int plan = 0;
     int k() const { return 6 + rb + (nt == 256 ? 2 : nt == 128 ? 1 : 0); }
 };
+
+// Layer fusion: mipt_hip_set_layer_fusion, else MIPT_HIP_FUSE (1 on, the
+// default; 0 off; any other value is refused).
+std::atomic<int> g_fuse{-1};  // -1: not set by call (or set back)
+
+int fuse_layers(bool &on)
+{
+    int v = g_fuse.load();
+    if (v < 0)
+    {
+        static const int env = [] {
+            const char *e = std::getenv("MIPT_HIP_FUSE");
+            if (!e || !*e || std::strcmp(e, "1") == 0) return 1;
+            if (std::strcmp(e, "0") == 0) return 0;
+            return -1;
+        }();
+        v = env;
+    }
+    if (v < 0)
+    {
+        set_error("MIPT_HIP_FUSE must be 0 or 1");
+        return E_ARG;
+    }
+    on = v == 1;
+    return OK;
+}
 
 // Build the sweeps of one layer parity for a chain of L sites in nq qubits.
 struct LayerPlan
@@ -1818,6 +1899,9 @@ struct Batch
     DevArena al{};             // arena view of that group (its own partials buffer)
     double *partials2 = nullptr;
     std::size_t partial2_cap = 0;
+    // per stream lane: each layer's trajectory order for layer fusion
+    int *fuse_lists[2] = {nullptr, nullptr};
+    std::size_t fuse_cap[2] = {0, 0};
 
     // Sampled HIP-event timing of the gate sweeps (pass kernels).
     int prof_every = 0;
@@ -1844,6 +1928,8 @@ struct Batch
         workspace_destroy(ews);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (partials2) (void)hipFree(partials2);
+        for (int *fl : fuse_lists)
+            if (fl) (void)hipFree(fl);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (side) (void)hipStreamDestroy(side);
@@ -1895,6 +1981,7 @@ struct Batch
     // second full evolve after the draws last changed, replayed afterwards
     bool graph_on = false;
     int graph_warm = 0;  // full eager evolves since the draws/config changed
+    int graph_fuse = -1; // layer fusion setting the graph was captured with
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 
@@ -2157,19 +2244,20 @@ struct Batch
     }
 
     template <int RB, int NT, int AUX, int MX, int ZP>
-    int launch_pass(const PassPlan &pl, const int *list, int ntraj, int layer)
+    int launch_pass(const PassPlan &pl, const PassPlan *next, const int *list, int ntraj, int layer)
     {
-        PassArgs pa{al, list, ntraj, layer, pl};
-        // half a tile, then the matrix-core coefficient table
+        PassArgs pa{al, list, ntraj, layer, pl, next ? *next : PassPlan{}};
+        // half a tile, then the matrix-core coefficient table (of both sweeps when fused)
+        const int slots = 2 * (pl.nstages + (next ? next->nstages : 0));
         const std::size_t lds = (std::size_t{1} << (pl.k - 1)) * sizeof(float2) +
-                                (MX > 0 ? static_cast<std::size_t>(2 * pl.nstages) * MX_SLOT_BYTES : 0);
+                                (MX > 0 ? static_cast<std::size_t>(slots) * MX_SLOT_BYTES : 0);
         static bool attr_set = false;
         if (!attr_set)
         {
             MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX, ZP>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                static_cast<int>((std::size_t{1} << 13) * sizeof(float2) +
-                                                                MX_MAX_SLOTS * MX_SLOT_BYTES)));
+                                                                2 * MX_MAX_SLOTS * MX_SLOT_BYTES)));
             attr_set = true;
         }
         const unsigned grid = static_cast<unsigned>(static_cast<long>(ntraj) * pl.ntiles);
@@ -2184,43 +2272,77 @@ struct Batch
     // Stage-free sweeps (AUX) have no bond arithmetic; gate sweeps of 2^13
     // tiles come in all three arithmetic mixes, the 64-register shapes on the
     // VALU (the matrix-core operands do not fit beside 64 amplitudes).
-    int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer)
+    // The stage holding a bond of qubit 0 is the plan's last (1 / 2: its
+    // form), or none (0); any other plan is refused.
+    int qubit0_form(const PassPlan &pl, int &zp)
+    {
+        zp = 0;
+        for (int si = 0; si < pl.nstages; ++si)
+            if (pl.st[si].zpair)
+            {
+                if (si != pl.nstages - 1)
+                {
+                    set_error("tile plan: the bond of qubit 0 is not in the last stage");
+                    return E_UNSUPPORTED;
+                }
+                zp = pl.st[si].zpair;
+            }
+        return OK;
+    }
+
+    // next (with pl.fuse): the next layer's first sweep, run in the same
+    // launch for the trajectories that measure nothing in this layer.
+    int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer,
+                 const PassPlan *next = nullptr)
     {
         const bool aux = pl.nstages == 0;
         if (cf.nt == 256 && cf.rb == 5)
         {
-            if (aux) return launch_pass<5, 256, 1, 0, 0>(pl, list, ntraj, layer);
-            // the planner puts the stage with a bond of qubit 0 last
+            if (aux) return launch_pass<5, 256, 1, 0, 0>(pl, nullptr, list, ntraj, layer);
             int zp = 0;
-            for (int si = 0; si < pl.nstages; ++si)
-                if (pl.st[si].zpair)
+            MIPT_HIP_TRY(qubit0_form(pl, zp));
+            if (pl.fuse)
+            {
+                int zn = 0;
+                MIPT_HIP_TRY(qubit0_form(*next, zn));
+                if (zp != 0 || cf.mx == 1)
                 {
-                    if (si != pl.nstages - 1)
-                    {
-                        set_error("tile plan: the bond of qubit 0 is not in the last stage");
-                        return E_UNSUPPORTED;
-                    }
-                    zp = pl.st[si].zpair;
+                    set_error("layer fusion needs a first sweep without the bond of qubit 0");
+                    return E_UNSUPPORTED;
                 }
+                if (cf.mx == 0)
+                    return zn == 0   ? launch_pass<5, 256, 0, 0, 4>(pl, next, list, ntraj, layer)
+                           : zn == 1 ? launch_pass<5, 256, 0, 0, 5>(pl, next, list, ntraj, layer)
+                                     : launch_pass<5, 256, 0, 0, 6>(pl, next, list, ntraj, layer);
+                return zn == 0   ? launch_pass<5, 256, 0, 2, 4>(pl, next, list, ntraj, layer)
+                       : zn == 1 ? launch_pass<5, 256, 0, 2, 5>(pl, next, list, ntraj, layer)
+                                 : launch_pass<5, 256, 0, 2, 6>(pl, next, list, ntraj, layer);
+            }
             if (cf.mx == 0)
-                return zp == 0   ? launch_pass<5, 256, 0, 0, 0>(pl, list, ntraj, layer)
-                       : zp == 1 ? launch_pass<5, 256, 0, 0, 1>(pl, list, ntraj, layer)
-                                 : launch_pass<5, 256, 0, 0, 2>(pl, list, ntraj, layer);
+                return zp == 0   ? launch_pass<5, 256, 0, 0, 0>(pl, nullptr, list, ntraj, layer)
+                       : zp == 1 ? launch_pass<5, 256, 0, 0, 1>(pl, nullptr, list, ntraj, layer)
+                                 : launch_pass<5, 256, 0, 0, 2>(pl, nullptr, list, ntraj, layer);
             if (cf.mx == 2)
-                return zp == 0   ? launch_pass<5, 256, 0, 2, 0>(pl, list, ntraj, layer)
-                       : zp == 1 ? launch_pass<5, 256, 0, 2, 1>(pl, list, ntraj, layer)
-                                 : launch_pass<5, 256, 0, 2, 2>(pl, list, ntraj, layer);
-            return zp == 0   ? launch_pass<5, 256, 0, 1, 0>(pl, list, ntraj, layer)
-                   : zp == 1 ? launch_pass<5, 256, 0, 1, 1>(pl, list, ntraj, layer)
-                             : launch_pass<5, 256, 0, 1, 2>(pl, list, ntraj, layer);
+                return zp == 0   ? launch_pass<5, 256, 0, 2, 0>(pl, nullptr, list, ntraj, layer)
+                       : zp == 1 ? launch_pass<5, 256, 0, 2, 1>(pl, nullptr, list, ntraj, layer)
+                                 : launch_pass<5, 256, 0, 2, 2>(pl, nullptr, list, ntraj, layer);
+            return zp == 0   ? launch_pass<5, 256, 0, 1, 0>(pl, nullptr, list, ntraj, layer)
+                   : zp == 1 ? launch_pass<5, 256, 0, 1, 1>(pl, nullptr, list, ntraj, layer)
+                             : launch_pass<5, 256, 0, 1, 2>(pl, nullptr, list, ntraj, layer);
+        }
+        if (pl.fuse)
+        {
+            set_error("layer fusion needs 2^13-amplitude tiles of 256 threads");
+            return E_UNSUPPORTED;
         }
         if (cf.nt == 256)
-            return aux ? launch_pass<6, 256, 1, 0, 0>(pl, list, ntraj, layer)
-                       : launch_pass<6, 256, 0, 0, 3>(pl, list, ntraj, layer);
+            return aux ? launch_pass<6, 256, 1, 0, 0>(pl, nullptr, list, ntraj, layer)
+                       : launch_pass<6, 256, 0, 0, 3>(pl, nullptr, list, ntraj, layer);
         if (cf.nt == 128)
-            return aux ? launch_pass<6, 128, 1, 0, 0>(pl, list, ntraj, layer)
-                       : launch_pass<6, 128, 0, 0, 3>(pl, list, ntraj, layer);
-        return aux ? launch_pass<6, 64, 1, 0, 0>(pl, list, ntraj, layer) : launch_pass<6, 64, 0, 0, 3>(pl, list, ntraj, layer);
+            return aux ? launch_pass<6, 128, 1, 0, 0>(pl, nullptr, list, ntraj, layer)
+                       : launch_pass<6, 128, 0, 0, 3>(pl, nullptr, list, ntraj, layer);
+        return aux ? launch_pass<6, 64, 1, 0, 0>(pl, nullptr, list, ntraj, layer)
+                   : launch_pass<6, 64, 0, 0, 3>(pl, nullptr, list, ntraj, layer);
     }
 
     // Tiled evolution of a group of same-size trajectories (nq > 14).
@@ -2242,6 +2364,39 @@ struct Batch
         bool attach_any = false;
         for (int i : grp)
             attach_any |= desc[i].probes > 0 && (desc[i].attach > 0 || (desc[i].probes > 1 && desc[i].stagger > 0));
+        // layer fusion (2^13 tiles of 256 threads, no reference attached
+        // mid-evolution): the layers' sweeps alternate their window order so
+        // that consecutive layers meet in one window; trajectories that
+        // measure nothing in layer t run layer t+1's first sweep in the same
+        // launch as layer t's last (one pass over their states fewer), the
+        // others run the two sweeps as usual.  Each launch gets a compacted
+        // list (quiet_order_kernel), so no workgroup idles.
+        bool fuse_env = false;
+        MIPT_HIP_TRY(fuse_layers(fuse_env));
+        const bool fuse_on = fuse_env && cf.rb == 5 && cf.nt == 256 && cf.mx != 1 && !attach_any && t1 - t0 > 1;
+        int *qorder = nullptr;
+        if (fuse_on)
+        {
+            const std::size_t need = static_cast<std::size_t>(ntraj) * (t1 - t0 - 1);
+            if (need > fuse_cap[lane])
+            {
+                if (fuse_lists[lane]) MIPT_HIP_CHECK(hipFree(fuse_lists[lane]));
+                fuse_lists[lane] = nullptr;
+                MIPT_HIP_CHECK(hipMalloc(&fuse_lists[lane], need * sizeof(int)));
+                fuse_cap[lane] = need;
+            }
+            qorder = fuse_lists[lane];
+            hipLaunchKernelGGL(quiet_order_kernel, dim3(t1 - t0 - 1), dim3(256), static_cast<std::size_t>(ntraj), ls, list,
+                               ntraj, static_cast<const TrajDesc *>(a.desc), static_cast<const std::uint32_t *>(a.flags),
+                               t0, qorder);
+            MIPT_HIP_CHECK(hipGetLastError());
+        }
+        const double layer_bytes = 16.0 * static_cast<double>(std::size_t{1} << d0.nq);
+        // the previous layer fused: its quiet trajectories already ran this
+        // layer's first sweep; the others are prev_rest[0 .. prev_nrest)
+        const int *prev_rest = nullptr;
+        int prev_nrest = ntraj;
+        bool fused_prev = false;
         for (int t = t0; t < t1; ++t)
         {
             const LayerPlan &LP = lp[t % P];
@@ -2260,36 +2415,83 @@ struct Batch
             int maxm = 0;
             for (int i : grp) maxm = std::max(maxm, __builtin_popcount(h_flags[desc[i].layer_off + t]));
             const int chunks = (maxm + MAX_HIST_BITS - 1) / MAX_HIST_BITS;
-            for (std::size_t p = 0; p < LP.passes.size(); ++p)
+            const int np = static_cast<int>(LP.passes.size());
+            int order[8];
+            for (int i = 0; i < np; ++i) order[i] = fuse_on && (t & 1) ? np - 1 - i : i;
+            const PassPlan &lastp = LP.passes[order[np - 1]];
+            // fusion into layer t+1: its first sweep has this layer's last
+            // window, neither holds the bond of qubit 0 in a way the fused
+            // kernel cannot take, and some trajectory measures nothing here
+            int nquiet = 0;
+            for (int i : grp) nquiet += h_flags[desc[i].layer_off + t] == 0u;
+            PassPlan nx{};
+            bool fuse = false;
+            if (fuse_on && t + 1 < t1 && nquiet > 0 && np > 1)
             {
-                PassPlan pl = LP.passes[p];
-                pl.collapse = (p == 0) ? 1 : 0;
-                pl.init = (p == 0 && t == 0) ? 1 : 0;
-                pl.hist = (p + 1 == LP.passes.size()) ? 1 : 0;  // chunk 0 always: it also records m == 0
+                const LayerPlan &LN = lp[(t + 1) % P];
+                const int nn = static_cast<int>(LN.passes.size());
+                nx = LN.passes[(t + 1) & 1 ? nn - 1 : 0];
+                int z1 = 0;
+                for (int si = 0; si < lastp.nstages; ++si) z1 |= lastp.st[si].zpair;
+                fuse = nn > 1 && nx.k == lastp.k && nx.c == lastp.c && nx.s == lastp.s && nx.rest_mask == lastp.rest_mask &&
+                       z1 == 0 && nx.nstages > 0 && nx.st[0].use_lds;
+            }
+            const int *quiet = fuse ? qorder + static_cast<std::size_t>(t - t0) * ntraj : nullptr;
+            const int *rest = fuse ? quiet + nquiet : list;
+            const int nrest = fuse ? ntraj - nquiet : ntraj;
+            auto launch = [&](PassPlan pl, const int *lst, int cnt, double passes, const PassPlan *nxt) {
+                if (cnt == 0) return static_cast<int>(OK);
+                // algorithmic bytes: one read + one write of the states per
+                // layer, shared among the layer's sweeps, for the sweeps run
+                int slot;
+                MIPT_HIP_TRY(timed_launch_begin(layer_bytes * cnt * passes / np, slot));
+                MIPT_HIP_TRY(run_pass(cf, pl, lst, cnt, t, nxt));
+                MIPT_HIP_TRY(timed_launch_end(slot));
+                return static_cast<int>(OK);
+            };
+            for (int i = 0; i < np; ++i)
+            {
+                PassPlan pl = LP.passes[order[i]];
+                pl.collapse = (i == 0) ? 1 : 0;
+                pl.init = (i == 0 && t == 0) ? 1 : 0;
+                pl.hist = (i + 1 == np) ? 1 : 0;  // chunk 0 always: it also records m == 0
                 pl.chunk = 0;
                 pl.cond = 0;
                 pl.store = 1;
                 // the collapse sweep skips zero-branch tiles when a later sweep
                 // of the layer re-applies the branch on the sites it leaves alone
-                pl.zskip = (p == 0 && LP.passes.size() > 1) ? 1 : 0;
-                pl.pmask = (p == 1) ? (((std::uint64_t{1} << d0.nq) - 1) & ~LP.passes[0].bonded) : 0;
-                // algorithmic bytes: one read + one write of the states per
-                // layer, shared among the layer's sweeps
-                const double bytes = 16.0 * static_cast<double>(std::size_t{1} << d0.nq) * ntraj /
-                                     static_cast<double>(LP.passes.size());
-                int slot;
-                MIPT_HIP_TRY(timed_launch_begin(bytes, slot));
-                MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
-                MIPT_HIP_TRY(timed_launch_end(slot));
+                pl.zskip = (i == 0 && np > 1) ? 1 : 0;
+                pl.pmask = (i == 1) ? (((std::uint64_t{1} << d0.nq) - 1) & ~LP.passes[order[0]].bonded) : 0;
+                pl.fuse = 0;
+                if (i == 0 && fused_prev) MIPT_HIP_TRY(launch(pl, prev_rest, prev_nrest, 1.0, nullptr));
+                else if (i + 1 == np && fuse)
+                {
+                    MIPT_HIP_TRY(launch(pl, rest, nrest, 1.0, nullptr));
+                    PassPlan pf = pl;
+                    pf.fuse = 1;
+                    MIPT_HIP_TRY(launch(pf, quiet, nquiet, 2.0, &nx));
+                }
+                else MIPT_HIP_TRY(launch(pl, list, ntraj, 1.0, nullptr));
             }
-            const PassPlan &last = LP.passes.back();
-            const Geom geo{last.rest_mask, last.ntiles, last.k, last.c, last.s};
-            FinalizeArgs fa{al, list, t, 0, geo};
-            hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, ls, fa);
-            MIPT_HIP_CHECK(hipGetLastError());
+            // the histogram partials of the layer's last sweep sit at the
+            // trajectories' positions in the list that sweep ran over
+            const Geom geo{lastp.rest_mask, lastp.ntiles, lastp.k, lastp.c, lastp.s};
+            if (nrest > 0)
+            {
+                FinalizeArgs fa{al, rest, t, 0, geo};
+                hipLaunchKernelGGL(finalize_kernel, dim3(nrest), dim3(256), 0, ls, fa);
+                MIPT_HIP_CHECK(hipGetLastError());
+            }
+            if (fuse)
+            {
+                // nothing measured: the identity collapse, no partials read
+                FinalizeArgs fq{al, quiet, t, 0, geo};
+                hipLaunchKernelGGL(finalize_kernel, dim3(nquiet), dim3(256), 0, ls, fq);
+                MIPT_HIP_CHECK(hipGetLastError());
+            }
             for (int ch = 1; ch < chunks; ++ch)
             {
-                PassPlan pl = last;
+                PassPlan pl = lastp;
                 pl.nstages = 0;
                 pl.collapse = 0;
                 pl.init = 0;
@@ -2297,11 +2499,15 @@ struct Batch
                 pl.chunk = ch;
                 pl.cond = 1;
                 pl.store = 0;
-                MIPT_HIP_TRY(run_pass(cf, pl, list, ntraj, t));
-                FinalizeArgs fc{al, list, t, ch, geo};
-                hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, ls, fc);
+                pl.fuse = 0;
+                MIPT_HIP_TRY(run_pass(cf, pl, rest, nrest, t));
+                FinalizeArgs fc{al, rest, t, ch, geo};
+                hipLaunchKernelGGL(finalize_kernel, dim3(nrest), dim3(256), 0, ls, fc);
                 MIPT_HIP_CHECK(hipGetLastError());
             }
+            fused_prev = fuse;
+            prev_rest = rest;
+            prev_nrest = nrest;
         }
         // apply the last layer's pending collapse
         if (t1 > t0) MIPT_HIP_TRY(close_layer(cf, lp[(t1 - 1) % P], list, ntraj, t1));
@@ -2338,6 +2544,15 @@ struct Batch
         }
         if (graph_on && t0 == 0 && t1_req < 0)
         {
+            // a change of the layer-fusion setting changes the launch sequence
+            // (and the buffers it needs): warm up and capture again
+            bool fz = false;
+            MIPT_HIP_TRY(fuse_layers(fz));
+            if (static_cast<int>(fz) != graph_fuse)
+            {
+                drop_graph();
+                graph_fuse = static_cast<int>(fz);
+            }
             if (graph_exec)
             {
                 MIPT_HIP_CHECK(hipGraphLaunch(graph_exec, stream));
@@ -2987,6 +3202,17 @@ int mipt_hip_batch_export_layers(mipt_hip_batch *b, int traj, double *g, uint8_t
 {
     if (!b) { set_error("batch_export_layers: null batch"); return E_ARG; }
     return b->export_layers(traj, g, f, u);
+}
+
+int mipt_hip_set_layer_fusion(int on)
+{
+    if (on < -1 || on > 1)
+    {
+        set_error("layer fusion must be 0 (off), 1 (on) or -1 (MIPT_HIP_FUSE / default), got %d", on);
+        return E_ARG;
+    }
+    g_fuse.store(on);
+    return OK;
 }
 
 int mipt_hip_layer_plan(int L, int nq, int closed, int grid_x, int layer, int tile_bits, int threads, int plan,

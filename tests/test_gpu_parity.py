@@ -180,6 +180,59 @@ def test_three_sweep_chains_match_oracle(M, n, plan):
         batch.close()
 
 
+def _oracle_closed(args):
+    n, layers, p, seed, closed = args
+    psi, out, p1, (g, f, u) = O.simulate(n, layers, p, seed, closed)
+    return psi, out, p1, f, u
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
+def test_layer_fusion_on_and_off_match_oracle(M, plan):
+    # With layer fusion a trajectory that measures nothing in layer t runs the
+    # first sweep of layer t + 1 in the same launch as the last sweep of layer
+    # t, and the sweeps of odd layers run in reverse window order.  Both
+    # settings against the oracle (device-drawn streams; quiet layers at every
+    # p here, all of them at p = 0), and the two device states against each
+    # other: L=20 rings, an L=20 chain, an L=18 ring and L=22 (three sweeps).
+    ps = [0.0, 0.01, 0.03, 0.06, 0.1, 0.2]
+    cases = [(20, 16, p, 3300 + i, True) for i, p in enumerate(ps)] + \
+            [(20, 12, 0.02, 3310, False), (18, 10, 0.03, 3311, True), (22, 8, 0.03, 3312, True)]
+    specs = [M.Spec(n, layers, p, seed, closed) for (n, layers, p, seed, closed) in cases]
+    got = {}
+    try:
+        for on in (0, 1):
+            print(f"plan {plan}: layer fusion {on}, {len(cases)} trajectories", flush=True)
+            M.set_layer_fusion(on)
+            batch = M.Batch(specs, plan=plan)
+            try:
+                batch.generate()
+                batch.evolve()
+                batch.synchronize()
+                got[on] = [(batch.outcomes(i), batch.get_state(i)) for i in range(len(cases))]
+            finally:
+                batch.close()
+    finally:
+        M.set_layer_fusion(None)
+    O.lib()
+    with threadpool_limits(1), ThreadPoolExecutor(min(POOL, len(cases))) as ex:
+        refs = list(ex.map(_oracle_closed, cases))
+    for i, (psi, out, p1, f, u) in enumerate(refs):
+        tag = f"plan {plan} traj {i} L={cases[i][0]} p={cases[i][2]}"
+        near = False
+        for on in (0, 1):
+            (o, flagged, gp1, _), st = got[on][i]
+            near |= check_outcomes(o, out, p1, u, f, f"{tag} fusion {on}")
+            m = f == 1
+            assert np.abs(gp1[m] - p1[m]).max(initial=0) < 1e-5, tag
+        if near:
+            continue
+        for on in (0, 1):
+            assert np.abs(got[on][i][1] - psi).max() < 1e-5, f"{tag} fusion {on}: amplitudes"
+        assert np.array_equal(got[0][i][0][0], got[1][i][0][0]), tag
+        assert np.abs(got[0][i][1] - got[1][i][1]).max() < 1e-5, tag
+
+
 @pytest.mark.timeout(600)
 def test_metric_workload_full_depth_32_trajectories(M):
     # L=20 ring, T=40, p in {0, 0.02, ..., 0.30}: two trajectories per p value
