@@ -119,7 +119,8 @@ def main(argv=None):
     ap.add_argument("--bond-arith", type=int, default=0,
                     help="1 VALU, 2 VALU + matrix cores, 3 matrix cores (0: the autotuner's choice)")
     ap.add_argument("--streams", type=int, default=2, help="tiled groups in flight during the timed steps")
-    ap.add_argument("--profile-every", type=int, default=7, help="odd, so both sweeps of a layer are sampled")
+    ap.add_argument("--profile-every", type=int, default=1,
+                    help="time every n-th gate sweep with HIP events (1: all, so the mean weighs every launch kind)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB plain copy (traffic-counter calibration)")
     ap.add_argument("--rdm-L", type=int, default=0,
@@ -205,7 +206,7 @@ def main(argv=None):
     value = layer_trajs / (ms_per_step / 1e3)
     # per-kernel timing: one more eager step outside the timed region, every
     # group on the batch stream (so no other group's kernels overlap a sweep)
-    # and every profile_every-th sweep bracketed by HIP events on that stream
+    # and every (profile_every-th) sweep bracketed by HIP events on that stream
     b.set_graph(False)
     b.set_streams(1)
     b.set_profiling(args.profile_every)
@@ -543,7 +544,7 @@ def c4_leg(M, hip, n=64, L=26, layers=52, p=0.16):
     # per-launch sweep timing on one stream (an extra run, outside the timed one)
     b.reset()
     b.set_streams(1)
-    b.set_profiling(5)
+    b.set_profiling(1)
     b.evolve()
     b.synchronize()
     st = b.pass_stats()
