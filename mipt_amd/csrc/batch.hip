@@ -613,9 +613,17 @@ struct PassPlan
     std::uint64_t bonded;  // sites whose bonds this sweep applies
     std::uint64_t pmask;   // previous layer's collapsed sites to zero again at load (no rescale)
     // layer fusion: this is the layer's last sweep, launched over the
-    // trajectories that measure nothing in this layer, which continue with the
-    // next layer's first sweep (PassArgs::next, the same window) on the tile
+    // trajectories that fuse this layer into the next, which continue with
+    // the next layer's first sweep (PassArgs::next, the same window) on the
+    // tile
     int fuse;
+    // with layer fusion a list entry ~i marks trajectory i as having run this
+    // layer's first sweep fused into the previous layer's last, before that
+    // layer's outcomes were drawn: with pend (the layer's second sweep) such
+    // a trajectory gets the previous collapse applied at load, as a first
+    // sweep applies it
+    int pend;
+    int pbase;  // the launch's i-th trajectory writes histogram partial record pbase + i
 };
 
 struct PassArgs
@@ -716,6 +724,7 @@ __device__ __forceinline__ void group_mfma(float2 (&v)[NV], float kv)
     v[o2] = make_float2(t1[0], t1[1]);
     v[i3] = make_float2(t1[2], t1[3]);
 }
+
 #undef MIPT_MFMA_STEP
 
 // Matrix-core coefficient table of a sweep in LDS: per bond slot (2 per
@@ -1161,7 +1170,9 @@ __device__ __forceinline__ bool tile_context(const PassArgs &args, const int *__
     // dependent rounds of scalar loads (trajectory; its descriptor and init
     // flag; its layer's flags and collapse records): the tile's global loads
     // wait for no more round trips than these.
-    x.traj = traj_list[x.tl];
+    const int entry = traj_list[x.tl];
+    const bool pending = entry < 0;
+    x.traj = pending ? ~entry : entry;
     const TrajDesc d = desc[x.traj];
     const int init_v = init[x.traj];
     const std::uint32_t lt = d.layer_off + static_cast<std::uint32_t>(t);
@@ -1180,7 +1191,7 @@ __device__ __forceinline__ bool tile_context(const PassArgs &args, const int *__
     // the layer's first sweep skipped tiles of its zero branch, that branch
     // on the pmask sites (untouched since, so still zero by definition);
     // selects rather than branches, so no load waits behind a branch
-    const bool use_c = pl.collapse && t > 0;
+    const bool use_c = (pl.collapse || (pl.pend && pending)) && t > 0;
     const bool use_p = !use_c && pl.pmask != 0 && t > 0;
     const std::uint64_t pm = use_c ? ~std::uint64_t{0} : (use_p ? pl.pmask : 0);
     const std::uint64_t cm = cprev.mask & pm, cv = cprev.val & cm;
@@ -1400,7 +1411,7 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
         // ---- FP64 histogram partial of this tile over the chunk's local sites
         if (do_hist)
         {
-            double *out = a.partials + (static_cast<std::size_t>(cur.tl) * pl.ntiles + cur.tile) * HSTRIDE;
+            double *out = a.partials + (static_cast<std::size_t>(pl.pbase + cur.tl) * pl.ntiles + cur.tile) * HSTRIDE;
             if (cur.tile_out)
             {
                 // the tile lies outside the decided outcomes: every bin is 0
@@ -1465,27 +1476,58 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs ar
     }
 }
 
+// Which trajectories fuse layer t into layer t+1 (their last sweep of layer
+// t and first of layer t+1 in one launch): those whose measured sites in
+// layer t (at most MAX_HIST_BITS of them) are none of the sites that layer
+// t+1's first sweep bonds.  Those bonds commute with layer t's projectors and
+// leave the joint distribution of its measured sites unchanged, so the
+// histogram is taken after them and the collapse waits for layer t+1's
+// second sweep.  Per layer parity j = t mod period: ok[j] (the two sweeps
+// share a window the fused kernel takes) and block[j] (layer t+1's first
+// sweep's bonded sites).  The host applies the same rule to its copy of the
+// flags to size the launches.
+struct FuseRule
+{
+    std::uint64_t block[4];
+    int ok[4];
+    int period;
+    int t1;  // end of the evolution range: its last layer fuses into nothing
+};
+
+__host__ __device__ inline bool fuses(const FuseRule &r, std::uint32_t flags, int t)
+{
+    const int j = t % r.period;
+    return t + 1 < r.t1 && r.ok[j] && __builtin_popcount(flags) <= MAX_HIST_BITS && (flags & r.block[j]) == 0;
+}
+
 // Layer fusion's trajectory order: for layer t = t0 + blockIdx.x, the
-// group's trajectories (positions in list) that measure nothing in layer t
-// first, then the others, each in list order; out[blockIdx.x * n + ...].
-// q: n bytes of LDS.
-__global__ void __launch_bounds__(256) quiet_order_kernel(const int *__restrict__ list, int n,
-                                                          const TrajDesc *__restrict__ desc,
-                                                          const std::uint32_t *__restrict__ flags, int t0,
-                                                          int *__restrict__ out)
+// group's trajectories (entries of list) that fuse layer t first, then the
+// others, each in list order, into out[blockIdx.x * n + ...]; a trajectory
+// that fused layer t-1 (t > t0) is entered as ~index (its collapse of layer
+// t-1 is still to apply).  q: n bytes of LDS.
+__global__ void __launch_bounds__(256) fuse_order_kernel(const int *__restrict__ list, int n,
+                                                         const TrajDesc *__restrict__ desc,
+                                                         const std::uint32_t *__restrict__ flags, int t0, FuseRule r,
+                                                         int *__restrict__ out)
 {
     extern __shared__ unsigned char q[];
     const int t = t0 + static_cast<int>(blockIdx.x);
-    for (int j = threadIdx.x; j < n; j += 256) q[j] = flags[desc[list[j]].layer_off + t] == 0u ? 1 : 0;
+    for (int j = threadIdx.x; j < n; j += 256)
+    {
+        const std::uint32_t off = desc[list[j]].layer_off;
+        const bool now = fuses(r, flags[off + t], t);
+        const bool before = t > t0 && fuses(r, flags[off + t - 1], t - 1);
+        q[j] = static_cast<unsigned char>((now ? 1 : 0) | (before ? 2 : 0));
+    }
     __syncthreads();
     if (threadIdx.x == 0)
     {
         int *o = out + static_cast<std::size_t>(blockIdx.x) * n;
         int w = 0;
         for (int j = 0; j < n; ++j)
-            if (q[j]) o[w++] = list[j];
+            if (q[j] & 1) o[w++] = (q[j] & 2) ? ~list[j] : list[j];
         for (int j = 0; j < n; ++j)
-            if (!q[j]) o[w++] = list[j];
+            if (!(q[j] & 1)) o[w++] = (q[j] & 2) ? ~list[j] : list[j];
     }
 }
 
@@ -1494,15 +1536,17 @@ __global__ void __launch_bounds__(256) quiet_order_kernel(const int *__restrict_
 struct FinalizeArgs
 {
     DevArena a;
-    const int *traj_list;
+    const int *traj_list;  // entries ~i (layer fusion's pending mark) name trajectory i
     int layer, chunk;
     Geom g;
+    int pbase;  // partial record of the list's first trajectory
 };
 
 __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f)
 {
     __shared__ double M[2 << MAX_HIST_BITS];
-    finalize_chunk(f.a, f.traj_list[blockIdx.x], blockIdx.x, f.layer, f.chunk, f.g, M);
+    const int entry = f.traj_list[blockIdx.x];
+    finalize_chunk(f.a, entry < 0 ? ~entry : entry, f.pbase + static_cast<int>(blockIdx.x), f.layer, f.chunk, f.g, M);
 }
 
 // Marks the layer's collapse as applied (identity), after the closing sweep.
@@ -2366,18 +2410,42 @@ struct Batch
             attach_any |= desc[i].probes > 0 && (desc[i].attach > 0 || (desc[i].probes > 1 && desc[i].stagger > 0));
         // layer fusion (2^13 tiles of 256 threads, no reference attached
         // mid-evolution): the layers' sweeps alternate their window order so
-        // that consecutive layers meet in one window; trajectories that
-        // measure nothing in layer t run layer t+1's first sweep in the same
-        // launch as layer t's last (one pass over their states fewer), the
-        // others run the two sweeps as usual.  Each launch gets a compacted
-        // list (quiet_order_kernel), so no workgroup idles.
+        // that consecutive layers meet in one window; a trajectory whose
+        // measured sites in layer t are not bonded in layer t+1's first sweep
+        // (fuses(), quiet ones among them) runs that sweep in the same launch
+        // as layer t's last (one pass over its state fewer), and its collapse
+        // of layer t at layer t+1's second sweep; the others run the two
+        // sweeps as usual.  Each launch gets a compacted list
+        // (fuse_order_kernel), so no workgroup idles.
         bool fuse_env = false;
         MIPT_HIP_TRY(fuse_layers(fuse_env));
         const bool fuse_on = fuse_env && cf.rb == 5 && cf.nt == 256 && cf.mx != 1 && !attach_any && t1 - t0 > 1;
+        FuseRule rule{};
+        rule.period = P;
+        rule.t1 = t1;
+        PassPlan nxp[4]{};
+        for (int j = 0; j < P && fuse_on; ++j)
+        {
+            // layer parity j: its last sweep and the next layer's first share a
+            // window, and neither holds the bond of qubit 0 in a way the fused
+            // kernel cannot take
+            const LayerPlan &LJ = lp[j];
+            const LayerPlan &LN = lp[(j + 1) % P];
+            const int np = static_cast<int>(LJ.passes.size()), nn = static_cast<int>(LN.passes.size());
+            if (np < 2 || nn < 2) continue;
+            const PassPlan &lastp = LJ.passes[(j & 1) ? 0 : np - 1];
+            const PassPlan &nx = LN.passes[((j + 1) & 1) ? nn - 1 : 0];
+            int z1 = 0;
+            for (int si = 0; si < lastp.nstages; ++si) z1 |= lastp.st[si].zpair;
+            rule.ok[j] = nx.k == lastp.k && nx.c == lastp.c && nx.s == lastp.s && nx.rest_mask == lastp.rest_mask &&
+                         z1 == 0 && nx.nstages > 0 && nx.st[0].use_lds;
+            rule.block[j] = nx.bonded;
+            nxp[j] = nx;
+        }
         int *qorder = nullptr;
         if (fuse_on)
         {
-            const std::size_t need = static_cast<std::size_t>(ntraj) * (t1 - t0 - 1);
+            const std::size_t need = static_cast<std::size_t>(ntraj) * (t1 - t0);
             if (need > fuse_cap[lane])
             {
                 if (fuse_lists[lane]) MIPT_HIP_CHECK(hipFree(fuse_lists[lane]));
@@ -2386,17 +2454,15 @@ struct Batch
                 fuse_cap[lane] = need;
             }
             qorder = fuse_lists[lane];
-            hipLaunchKernelGGL(quiet_order_kernel, dim3(t1 - t0 - 1), dim3(256), static_cast<std::size_t>(ntraj), ls, list,
+            hipLaunchKernelGGL(fuse_order_kernel, dim3(t1 - t0), dim3(256), static_cast<std::size_t>(ntraj), ls, list,
                                ntraj, static_cast<const TrajDesc *>(a.desc), static_cast<const std::uint32_t *>(a.flags),
-                               t0, qorder);
+                               t0, rule, qorder);
             MIPT_HIP_CHECK(hipGetLastError());
         }
         const double layer_bytes = 16.0 * static_cast<double>(std::size_t{1} << d0.nq);
-        // the previous layer fused: its quiet trajectories already ran this
-        // layer's first sweep; the others are prev_rest[0 .. prev_nrest)
-        const int *prev_rest = nullptr;
-        int prev_nrest = ntraj;
-        bool fused_prev = false;
+        // trajectories that fused layer t-1 (they lead that layer's order and
+        // already ran layer t's first sweep)
+        int prev_nf = 0;
         for (int t = t0; t < t1; ++t)
         {
             const LayerPlan &LP = lp[t % P];
@@ -2419,26 +2485,12 @@ struct Batch
             int order[8];
             for (int i = 0; i < np; ++i) order[i] = fuse_on && (t & 1) ? np - 1 - i : i;
             const PassPlan &lastp = LP.passes[order[np - 1]];
-            // fusion into layer t+1: its first sweep has this layer's last
-            // window, neither holds the bond of qubit 0 in a way the fused
-            // kernel cannot take, and some trajectory measures nothing here
-            int nquiet = 0;
-            for (int i : grp) nquiet += h_flags[desc[i].layer_off + t] == 0u;
-            PassPlan nx{};
-            bool fuse = false;
-            if (fuse_on && t + 1 < t1 && nquiet > 0 && np > 1)
-            {
-                const LayerPlan &LN = lp[(t + 1) % P];
-                const int nn = static_cast<int>(LN.passes.size());
-                nx = LN.passes[(t + 1) & 1 ? nn - 1 : 0];
-                int z1 = 0;
-                for (int si = 0; si < lastp.nstages; ++si) z1 |= lastp.st[si].zpair;
-                fuse = nn > 1 && nx.k == lastp.k && nx.c == lastp.c && nx.s == lastp.s && nx.rest_mask == lastp.rest_mask &&
-                       z1 == 0 && nx.nstages > 0 && nx.st[0].use_lds;
-            }
-            const int *quiet = fuse ? qorder + static_cast<std::size_t>(t - t0) * ntraj : nullptr;
-            const int *rest = fuse ? quiet + nquiet : list;
-            const int nrest = fuse ? ntraj - nquiet : ntraj;
+            // this layer's order (fusing trajectories first, the others from
+            // position nf), the same count the device derives from its flags
+            const int *ord = fuse_on ? qorder + static_cast<std::size_t>(t - t0) * ntraj : list;
+            int nf = 0;
+            if (fuse_on)
+                for (int i : grp) nf += fuses(rule, h_flags[desc[i].layer_off + t], t) ? 1 : 0;
             auto launch = [&](PassPlan pl, const int *lst, int cnt, double passes, const PassPlan *nxt) {
                 if (cnt == 0) return static_cast<int>(OK);
                 // algorithmic bytes: one read + one write of the states per
@@ -2463,32 +2515,32 @@ struct Batch
                 pl.zskip = (i == 0 && np > 1) ? 1 : 0;
                 pl.pmask = (i == 1) ? (((std::uint64_t{1} << d0.nq) - 1) & ~LP.passes[order[0]].bonded) : 0;
                 pl.fuse = 0;
-                if (i == 0 && fused_prev) MIPT_HIP_TRY(launch(pl, prev_rest, prev_nrest, 1.0, nullptr));
-                else if (i + 1 == np && fuse)
+                pl.pend = (i == 1) ? 1 : 0;
+                pl.pbase = 0;
+                if (i == 0)
                 {
-                    MIPT_HIP_TRY(launch(pl, rest, nrest, 1.0, nullptr));
+                    // those that fused layer t-1 ran this sweep already
+                    const int *others = prev_nf > 0 ? qorder + static_cast<std::size_t>(t - 1 - t0) * ntraj + prev_nf : ord;
+                    MIPT_HIP_TRY(launch(pl, others, ntraj - prev_nf, 1.0, nullptr));
+                }
+                else if (i + 1 == np && nf > 0)
+                {
+                    PassPlan pr = pl;
+                    pr.pbase = nf;
+                    MIPT_HIP_TRY(launch(pr, ord + nf, ntraj - nf, 1.0, nullptr));
                     PassPlan pf = pl;
                     pf.fuse = 1;
-                    MIPT_HIP_TRY(launch(pf, quiet, nquiet, 2.0, &nx));
+                    MIPT_HIP_TRY(launch(pf, ord, nf, 2.0, &nxp[t % P]));
                 }
-                else MIPT_HIP_TRY(launch(pl, list, ntraj, 1.0, nullptr));
+                else MIPT_HIP_TRY(launch(pl, ord, ntraj, 1.0, nullptr));
             }
             // the histogram partials of the layer's last sweep sit at the
-            // trajectories' positions in the list that sweep ran over
+            // trajectories' positions in the layer's order
             const Geom geo{lastp.rest_mask, lastp.ntiles, lastp.k, lastp.c, lastp.s};
-            if (nrest > 0)
-            {
-                FinalizeArgs fa{al, rest, t, 0, geo};
-                hipLaunchKernelGGL(finalize_kernel, dim3(nrest), dim3(256), 0, ls, fa);
-                MIPT_HIP_CHECK(hipGetLastError());
-            }
-            if (fuse)
-            {
-                // nothing measured: the identity collapse, no partials read
-                FinalizeArgs fq{al, quiet, t, 0, geo};
-                hipLaunchKernelGGL(finalize_kernel, dim3(nquiet), dim3(256), 0, ls, fq);
-                MIPT_HIP_CHECK(hipGetLastError());
-            }
+            FinalizeArgs fa{al, ord, t, 0, geo, 0};
+            hipLaunchKernelGGL(finalize_kernel, dim3(ntraj), dim3(256), 0, ls, fa);
+            MIPT_HIP_CHECK(hipGetLastError());
+            // more than MAX_HIST_BITS measured sites: never a fusing trajectory
             for (int ch = 1; ch < chunks; ++ch)
             {
                 PassPlan pl = lastp;
@@ -2500,14 +2552,14 @@ struct Batch
                 pl.cond = 1;
                 pl.store = 0;
                 pl.fuse = 0;
-                MIPT_HIP_TRY(run_pass(cf, pl, rest, nrest, t));
-                FinalizeArgs fc{al, rest, t, ch, geo};
-                hipLaunchKernelGGL(finalize_kernel, dim3(nrest), dim3(256), 0, ls, fc);
+                pl.pend = 0;
+                pl.pbase = nf;
+                MIPT_HIP_TRY(run_pass(cf, pl, ord + nf, ntraj - nf, t));
+                FinalizeArgs fc{al, ord + nf, t, ch, geo, nf};
+                hipLaunchKernelGGL(finalize_kernel, dim3(ntraj - nf), dim3(256), 0, ls, fc);
                 MIPT_HIP_CHECK(hipGetLastError());
             }
-            fused_prev = fuse;
-            prev_rest = rest;
-            prev_nrest = nrest;
+            prev_nf = nf;
         }
         // apply the last layer's pending collapse
         if (t1 > t0) MIPT_HIP_TRY(close_layer(cf, lp[(t1 - 1) % P], list, ntraj, t1));

@@ -189,13 +189,16 @@ def _oracle_closed(args):
 @pytest.mark.timeout(300)
 @pytest.mark.parametrize("plan", [0, 1, 2, 3])
 def test_layer_fusion_on_and_off_match_oracle(M, plan):
-    # With layer fusion a trajectory that measures nothing in layer t runs the
-    # first sweep of layer t + 1 in the same launch as the last sweep of layer
-    # t, and the sweeps of odd layers run in reverse window order.  Both
-    # settings against the oracle (device-drawn streams; quiet layers at every
-    # p here, all of them at p = 0), and the two device states against each
-    # other: L=20 rings, an L=20 chain, an L=18 ring and L=22 (three sweeps).
-    ps = [0.0, 0.01, 0.03, 0.06, 0.1, 0.2]
+    # With layer fusion a trajectory none of whose measured sites in layer t
+    # is bonded in layer t + 1's first sweep runs that sweep in the same
+    # launch as the last sweep of layer t (its histogram taken after it, its
+    # collapse applied at layer t + 1's second sweep), and the sweeps of odd
+    # layers run in reverse window order.  Both settings against the oracle
+    # (device-drawn streams; fusing layers at every p here, all of them at
+    # p = 0; at p = 0.6 layers beyond one 10-site histogram chunk, which never
+    # fuse, in the same group), and the two device states against each other:
+    # L=20 rings, an L=20 chain, an L=18 ring and L=22 (three sweeps).
+    ps = [0.0, 0.01, 0.03, 0.06, 0.1, 0.2, 0.6]
     cases = [(20, 16, p, 3300 + i, True) for i, p in enumerate(ps)] + \
             [(20, 12, 0.02, 3310, False), (18, 10, 0.03, 3311, True), (22, 8, 0.03, 3312, True)]
     specs = [M.Spec(n, layers, p, seed, closed) for (n, layers, p, seed, closed) in cases]
