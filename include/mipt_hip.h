@@ -343,11 +343,13 @@ int mipt_hip_batch_export_layers(mipt_hip_batch *batch, int traj, double *gates_
                                  double *uniforms);
 /* Layer fusion (no counterpart in the reference; process-wide, default on,
  * else MIPT_HIP_FUSE=0/1): the sweeps of consecutive layers alternate their
- * window order, and a trajectory that measures nothing in a layer runs the
- * next layer's first sweep in the same pass over its state as this layer's
- * last (2^13-amplitude tiles of 256 threads, no reference attached during
- * the evolution).  Results agree with the unfused schedule to rounding (the
- * commuting bonds of a layer meet in another order).  on = -1 goes back to
+ * window order, and a trajectory none of whose measured sites in a layer is
+ * bonded in the next layer's first sweep runs that sweep in the same pass
+ * over its state as this layer's last, the layer's collapse following at
+ * the next layer's second sweep (2^13-amplitude tiles of 256 threads, no
+ * reference attached during the evolution).  Results agree with the unfused
+ * schedule to rounding (commuting bonds and projectors meet in another
+ * order).  on = -1 goes back to
  * MIPT_HIP_FUSE / the default.  Status -1 for any other value. */
 int mipt_hip_set_layer_fusion(int on);
 /* Tile plan of one layer (no counterpart in the reference; for callers and

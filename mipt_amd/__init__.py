@@ -318,8 +318,9 @@ class PlainEngine:
 
 
 def set_layer_fusion(on) -> None:
-    """Process-wide: fuse a quiet layer's last sweep with the next layer's
-    first (True / 1, the default), or not (False / 0); None or -1 goes back to
+    """Process-wide: run a layer's last sweep and the next layer's first in
+    one launch for the trajectories whose measured sites the latter does not
+    bond (True / 1, the default), or not (False / 0); None or -1 goes back to
     MIPT_HIP_FUSE / the default."""
     if on is None:
         on = -1

@@ -1423,9 +1423,10 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
                                        reinterpret_cast<double *>(lds), out);
         }
     };
-    // a fused launch holds only trajectories that measure nothing in this
-    // layer: the next layer's first sweep (same window, no collapse to apply,
-    // no histogram) follows on the tile in registers and LDS
+    // a fused launch holds the trajectories that fuse this layer into the
+    // next: the next layer's first sweep (same window; its bonds touch none
+    // of this layer's measured sites, whose collapse waits) follows on the
+    // tile in registers and LDS, and the histogram is taken after it
     if constexpr (FUSED) run_stages(args.next, std::integral_constant<int, ZP - 4>{}, gates + cur.goff2, pl.nstages);
     finish();
 }
@@ -2335,7 +2336,7 @@ struct Batch
     }
 
     // next (with pl.fuse): the next layer's first sweep, run in the same
-    // launch for the trajectories that measure nothing in this layer.
+    // launch for the trajectories that fuse this layer into the next.
     int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer,
                  const PassPlan *next = nullptr)
     {
