@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "circuit layers/s at L=20 x 4096 traj, batched vs plain; S/TMI err vs FP64"
 HBM_PEAK_GBS = 8000.0
+FP32_MATRIX_PEAK_TFLOPS = 157.3
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pass_kernel_L20x112_traffic.md")
 
 
@@ -276,7 +277,14 @@ def main(argv=None):
                      "sweep_time_share": round(st["launches"] * st["avg_ms"] / single_stream_ms, 4)
                      if single_stream_ms > 0 else None,
                      "aggregate_sweep_GBps": round(aggregate, 1),
-                     "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4)},
+                     "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4),
+                     # the same launches as arithmetic: a dense 4x4 complex bond is 16 real
+                     # multiply-adds per amplitude, L/2 bonds per layer of the ring, so
+                     # 2 * (L/2) FLOP per algorithmic byte (16 B per amplitude and layer)
+                     "arith": {"flop_per_launch": round(st["bytes_avg"] * L),
+                               "achieved_TFLOPs": round(achieved * L / 1e3, 2),
+                               "peak_TFLOPs": FP32_MATRIX_PEAK_TFLOPS,
+                               "frac": round(achieved * L / 1e3 / FP32_MATRIX_PEAK_TFLOPS, 4)}},
         "autotune": tune,
         "graph": not args.no_graph,
         "plain_layers_per_s": None if plain_lps is None else round(plain_lps, 1),
