@@ -1241,7 +1241,8 @@ __device__ __forceinline__ void tile_load(const TileCtx &x, bool have, const flo
     {
         const std::uint32_t o = toff + (static_cast<std::uint32_t>(i) << jshift);
         const std::uint32_t off = (fetch && (o & m) == val) ? (o << 3) : 0x80000000u;
-        const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        // nontemporal (cache policy nt): a sweep reads each tile once
+        const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 2));
         v[2 * i] = make_float2(q.x, q.y);
         v[2 * i + 1] = make_float2(q.z, q.w);
     }
@@ -1399,14 +1400,18 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
         if (!in_m0) lds_exchange<NV>(ldsb, v, T3, Q3, T3m0, Q3m0, lds_dirty);
 
         // ---- store (M0); before the histogram, so the tile's registers are
-        // free again as its FP64 weights are formed
+        // free again as its FP64 weights are formed.  Nontemporal, as the
+        // loads: the tile is not read again before the next sweep, after
+        // every other tile of the launch (with both, the sweeps measured 2%
+        // faster)
         if (pl.store)
         {
             char *tp = reinterpret_cast<char *>(cur.state + cur.B);
 #pragma unroll
             for (int i = 0; i < NV / 2; ++i)
-                *reinterpret_cast<float4 *>(tp + ((toff + (static_cast<std::uint32_t>(i) << jshift)) << 3)) =
-                    make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+                __builtin_nontemporal_store(
+                    floatx4_t{v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y},
+                    reinterpret_cast<floatx4_t *>(tp + ((toff + (static_cast<std::uint32_t>(i) << jshift)) << 3)));
         }
         // ---- FP64 histogram partial of this tile over the chunk's local sites
         if (do_hist)

@@ -236,6 +236,44 @@ def test_layer_fusion_on_and_off_match_oracle(M, plan):
         assert np.abs(got[0][i][1] - got[1][i][1]).max() < 1e-5, tag
 
 
+def test_split_evolution_ranges_match_one_evolution(M):
+    # evolve(0, 3), evolve(3, 7), evolve(7, 12) against one evolve() of the
+    # same device draws, layer fusion on: a range's last layer fuses into
+    # nothing and its collapse is applied before the range returns, so no
+    # deferred collapse crosses a boundary; L=20 rings and an L=22 ring
+    # (three sweeps), and the oracle on top
+    cases = [(20, 12, p, 3400 + i, True) for i, p in enumerate([0.0, 0.03, 0.1, 0.25])] + [(22, 12, 0.05, 3410, True)]
+    specs = [M.Spec(n, layers, p, seed, closed) for (n, layers, p, seed, closed) in cases]
+    print(f"split ranges: {len(cases)} trajectories", flush=True)
+    runs = []
+    for split in (False, True):
+        batch = M.Batch(specs)
+        try:
+            batch.generate()
+            if split:
+                for a, b in ((0, 3), (3, 7), (7, 12)):
+                    batch.evolve(a, b)
+            else:
+                batch.evolve()
+            batch.synchronize()
+            runs.append([(batch.outcomes(i), batch.get_state(i)) for i in range(len(cases))])
+        finally:
+            batch.close()
+    O.lib()
+    with threadpool_limits(1), ThreadPoolExecutor(min(POOL, len(cases))) as ex:
+        refs = list(ex.map(_oracle_closed, cases))
+    for i, (psi, out, p1, f, u) in enumerate(refs):
+        tag = f"traj {i} L={cases[i][0]} p={cases[i][2]}"
+        near = False
+        for r in runs:
+            near |= check_outcomes(r[i][0][0], out, p1, u, f, tag)
+        if near:
+            continue
+        assert np.array_equal(runs[0][i][0][0], runs[1][i][0][0]), tag
+        for r in runs:
+            assert np.abs(r[i][1] - psi).max() < 1e-5, tag
+
+
 @pytest.mark.timeout(600)
 def test_metric_workload_full_depth_32_trajectories(M):
     # L=20 ring, T=40, p in {0, 0.02, ..., 0.30}: two trajectories per p value
