@@ -20,11 +20,13 @@
 //     in the last sweep (per-tile partials, fixed order), a finalize kernel
 //     turns them into sequential outcomes, and the collapse with FP64
 //     renormalisation is folded into the next layer's first sweep;
-//   * trajectories are processed in groups of about 3.5 GiB of state, two
+//   * trajectories are processed in groups of at most 4 GiB of state, two
 //     groups in flight on two streams: a sweep launch over one group is then
-//     57344 workgroups, whose last partial round of four workgroups per CU
-//     costs little (measured on MI355X at L=20 with matrix-core bonds: 448 MiB
-//     groups 179k layers/s, 896 MiB 188k, 1.75 GiB 192.2k, 3.5 GiB 192.6k).
+//     up to 65536 workgroups, whose last partial round of four workgroups per
+//     CU costs little (measured on MI355X at L=20 with matrix-core bonds:
+//     448 MiB groups 179k layers/s, 896 MiB 188k, 1.75 GiB 192.2k, 3.5 GiB
+//     192.6k; later, with layer fusion, 3.5 GiB 208.7k, 4 GiB 210.3k (4096
+//     trajectories in eight even groups), 8 GiB 209.5k).
 #include "batch.hpp"
 #include "state_ops.hpp"
 
@@ -2700,7 +2702,7 @@ struct Batch
                 static const std::size_t budget = [] {
                     const char *e = std::getenv("MIPT_HIP_GROUP_MIB");  // per-stream group budget
                     const long v = e ? std::atol(e) : 0;
-                    return static_cast<std::size_t>(v > 0 ? v : 3584) << 20;
+                    return static_cast<std::size_t>(v > 0 ? v : 4096) << 20;
                 }();
                 const int g = static_cast<int>(std::max<std::size_t>(1, budget / bytes));
                 const TrajDesc &d0 = desc[order[pos]];
