@@ -1260,12 +1260,25 @@ __device__ __forceinline__ std::uint32_t fresh_tid()
     return t;
 }
 
+// The stage structure every L=20 plan has at 2^13-amplitude tiles, by the
+// qubit-0 form z of the sweep: three stages, each applying bond[0]; stage 0
+// of a form-0 sweep works in the load mapping (no exchange), every other
+// stage starts with an exchange; bond[1] is applied in stages 0 and 1 (form
+// 0 and 1) or 0 and 2 (form 2); the bond of qubit 0 is stage 2's bond[0].
+// Sweep kernels with FIX run exactly this structure as straight-line code:
+// no run-time test of a stage's exchange or bonds, whose two outcomes would
+// leave the tile's registers paired differently (the bond on register bits
+// (3,4) pairs registers j, j|8 in its outputs, the one on (1,2) j, j|2) and
+// force copies back to one layout where they meet.
+__host__ __device__ constexpr bool fixed_lds(int z, int si) { return !(z == 0 && si == 0); }
+__host__ __device__ constexpr bool fixed_b1(int z, int si) { return z == 2 ? si != 1 : si != 2; }
+
 // Processes one tile whose amplitudes are in v (M0 mapping): the initial
 // state or the previous layer's collapse scale, the stages of the sweep plan,
 // back to M0, the store, and the FP64 histogram partial.  ZP 1 / 2: the last
 // stage holds the sweep's bond of qubit 0 in that form and the others none;
 // ZP 0: no stage has one; ZP 3: any stage may, in either form.
-template <int RB, int NT, int AUX, int MX, int ZP>
+template <int RB, int NT, int AUX, int MX, int ZP, int FIX>
 __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 *__restrict__ gates, const TileCtx &cur,
                                              float2 (&v)[1 << RB], std::uint32_t jshift,
                                              const std::uint32_t (&Q3m0)[6])
@@ -1334,29 +1347,52 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     }
     // moves the tile into stage sp's mapping (no move for an M0 stage: the
     // first of a plan, and a fused continuation starts with an exchange)
+    auto exchange_to = [&](const StagePlan &sp) {
+        const std::uint32_t tt = fresh_tid();
+        const std::uint32_t T3n =
+            thread_term<WB>(sp.lane_u, sp.wave_u, static_cast<int>(tt & 63u), static_cast<int>(tt >> 6)) << 3;
+        std::uint32_t Q3n[6];
+        Q3n[0] = 0;
+#pragma unroll
+        for (int r = 1; r < 6; ++r) Q3n[r] = swz((1u << sp.reg_u[r]) >> 1) << 3;
+        lds_exchange<NV>(ldsb, v, T3, Q3, T3n, Q3n, lds_dirty);
+        T3 = T3n;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) Q3[r] = Q3n[r];
+        in_m0 = false;
+    };
     auto enter_stage = [&](const StagePlan &sp) {
         if (!tab_ready && !sp.use_lds) __syncthreads();
         tab_ready = true;
-        if (sp.use_lds)
-        {
-            const std::uint32_t tt = fresh_tid();
-            const std::uint32_t T3n =
-                thread_term<WB>(sp.lane_u, sp.wave_u, static_cast<int>(tt & 63u), static_cast<int>(tt >> 6)) << 3;
-            std::uint32_t Q3n[6];
-            Q3n[0] = 0;
-#pragma unroll
-            for (int r = 1; r < 6; ++r) Q3n[r] = swz((1u << sp.reg_u[r]) >> 1) << 3;
-            lds_exchange<NV>(ldsb, v, T3, Q3, T3n, Q3n, lds_dirty);
-            T3 = T3n;
-#pragma unroll
-            for (int r = 0; r < 6; ++r) Q3[r] = Q3n[r];
-            in_m0 = false;
-        }
+        if (sp.use_lds) exchange_to(sp);
     };
     // the stages of plan P, its qubit-0 form Z (the stage type of ztag) fixed
     // at compile time, gates glx, coefficient-table slots from stage slot0
     auto run_stages = [&](const PassPlan &P, auto ztag, const float2 *glx, int slot0) {
         constexpr int Z = decltype(ztag)::value;
+        if constexpr (FIX != 0 && AUX == 0)
+        {
+            static_assert(Z <= 2 && MX != 0, "fixed stage structure: forms 0..2 on the matrix cores");
+            auto stage = [&](auto sidx) {
+                constexpr int si = decltype(sidx)::value;
+                constexpr int zf = si == 2 ? Z : 0;
+                const StagePlan &sp = P.st[si];
+                if constexpr (fixed_lds(Z, si)) exchange_to(sp);
+                else if (!tab_ready) __syncthreads();
+                tab_ready = true;
+                const float *tab = mxtab + (slot0 + si) * (2 * MX_SLOT_BYTES / 4);
+                const float2 *g0 = glx + 16 * sp.bond[0];
+                if constexpr (zf == 1) bond_on_regs<NV, 0, 1, MX>(v, g0, tab, lane);
+                else if constexpr (zf == 2) bond_on_regs<NV, 1, 0, MX>(v, g0, tab, lane);
+                else bond_on_regs<NV, 1, 2, MX>(v, g0, tab, lane);
+                if constexpr (fixed_b1(Z, si))
+                    bond_on_regs<NV, 3, 4, MX>(v, glx + 16 * sp.bond[1], tab + MX_SLOT_BYTES / 4, lane);
+            };
+            stage(std::integral_constant<int, 0>{});
+            stage(std::integral_constant<int, 1>{});
+            stage(std::integral_constant<int, 2>{});
+            return;
+        }
         constexpr bool zlast = Z == 1 || Z == 2;
         const int nloop = AUX ? 0 : P.nstages - (zlast ? 1 : 0);
         for (int si = 0; si < nloop; ++si)
@@ -1445,7 +1481,7 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
 // One workgroup per tile; with the half-tile LDS buffer four 2^13-amplitude
 // workgroups share a CU (64-register tiles: one), whose loads overlap one
 // another's LDS stages and bond arithmetic.
-template <int RB, int NT, int AUX, int MX, int ZP>
+template <int RB, int NT, int AUX, int MX, int ZP, int FIX = 0>
 __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
                                                                   const TrajDesc *__restrict__ desc,
                                                                   const std::uint32_t *__restrict__ flags,
@@ -1480,7 +1516,7 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs ar
     if (id < total && tile_context(args, in.traj_list, in.desc, in.flags, in.coll, in.init, in.gates, id, AUX != 0, x))
     {
         tile_load<NV>(x, true, args.a.states, toff, jshift, v);
-        process_tile<RB, NT, AUX, MX, ZP>(args, in.gates, x, v, jshift, Q3m0);
+        process_tile<RB, NT, AUX, MX, ZP, FIX>(args, in.gates, x, v, jshift, Q3m0);
     }
 }
 
@@ -2295,7 +2331,7 @@ struct Batch
         return OK;
     }
 
-    template <int RB, int NT, int AUX, int MX, int ZP>
+    template <int RB, int NT, int AUX, int MX, int ZP, int FIX = 0>
     int launch_pass(const PassPlan &pl, const PassPlan *next, const int *list, int ntraj, int layer)
     {
         PassArgs pa{al, list, ntraj, layer, pl, next ? *next : PassPlan{}};
@@ -2306,14 +2342,14 @@ struct Batch
         static bool attr_set = false;
         if (!attr_set)
         {
-            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX, ZP>),
+            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX, ZP, FIX>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                static_cast<int>((std::size_t{1} << 13) * sizeof(float2) +
                                                                 2 * MX_MAX_SLOTS * MX_SLOT_BYTES)));
             attr_set = true;
         }
         const unsigned grid = static_cast<unsigned>(static_cast<long>(ntraj) * pl.ntiles);
-        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX, ZP>), dim3(grid), dim3(NT), lds, ls, pa,
+        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX, ZP, FIX>), dim3(grid), dim3(NT), lds, ls, pa,
                            static_cast<const int *>(list), static_cast<const TrajDesc *>(al.desc),
                            static_cast<const std::uint32_t *>(al.flags), static_cast<const Collapse *>(al.coll),
                            static_cast<const int *>(al.init), static_cast<const float2 *>(al.gates));
@@ -2342,6 +2378,21 @@ struct Batch
         return OK;
     }
 
+    // Whether sweep plan pl of qubit-0 form z has the stage structure the FIX
+    // kernels run (fixed_lds / fixed_b1).
+    static bool has_fixed_structure(const PassPlan &pl, int z)
+    {
+        if (pl.nstages != 3) return false;
+        for (int si = 0; si < 3; ++si)
+        {
+            const StagePlan &sp = pl.st[si];
+            if ((sp.use_lds != 0) != fixed_lds(z, si) || sp.bond[0] < 0 || (sp.bond[1] >= 0) != fixed_b1(z, si) ||
+                sp.zpair != (si == 2 ? z : 0))
+                return false;
+        }
+        return true;
+    }
+
     // next (with pl.fuse): the next layer's first sweep, run in the same
     // launch for the trajectories that fuse this layer into the next.
     int run_pass(const Config &cf, const PassPlan &pl, const int *list, int ntraj, int layer,
@@ -2366,6 +2417,9 @@ struct Batch
                     return zn == 0   ? launch_pass<5, 256, 0, 0, 4>(pl, next, list, ntraj, layer)
                            : zn == 1 ? launch_pass<5, 256, 0, 0, 5>(pl, next, list, ntraj, layer)
                                      : launch_pass<5, 256, 0, 0, 6>(pl, next, list, ntraj, layer);
+                if (zn != 0 && has_fixed_structure(pl, 0) && has_fixed_structure(*next, zn))
+                    return zn == 1 ? launch_pass<5, 256, 0, 2, 5, 1>(pl, next, list, ntraj, layer)
+                                   : launch_pass<5, 256, 0, 2, 6, 1>(pl, next, list, ntraj, layer);
                 return zn == 0   ? launch_pass<5, 256, 0, 2, 4>(pl, next, list, ntraj, layer)
                        : zn == 1 ? launch_pass<5, 256, 0, 2, 5>(pl, next, list, ntraj, layer)
                                  : launch_pass<5, 256, 0, 2, 6>(pl, next, list, ntraj, layer);
@@ -2374,6 +2428,10 @@ struct Batch
                 return zp == 0   ? launch_pass<5, 256, 0, 0, 0>(pl, nullptr, list, ntraj, layer)
                        : zp == 1 ? launch_pass<5, 256, 0, 0, 1>(pl, nullptr, list, ntraj, layer)
                                  : launch_pass<5, 256, 0, 0, 2>(pl, nullptr, list, ntraj, layer);
+            if (cf.mx == 2 && has_fixed_structure(pl, zp))
+                return zp == 0   ? launch_pass<5, 256, 0, 2, 0, 1>(pl, nullptr, list, ntraj, layer)
+                       : zp == 1 ? launch_pass<5, 256, 0, 2, 1, 1>(pl, nullptr, list, ntraj, layer)
+                                 : launch_pass<5, 256, 0, 2, 2, 1>(pl, nullptr, list, ntraj, layer);
             if (cf.mx == 2)
                 return zp == 0   ? launch_pass<5, 256, 0, 2, 0>(pl, nullptr, list, ntraj, layer)
                        : zp == 1 ? launch_pass<5, 256, 0, 2, 1>(pl, nullptr, list, ntraj, layer)
