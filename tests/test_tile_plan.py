@@ -12,7 +12,8 @@ load/store mapping, and the one stage holding the bond of qubit 0 comes last
 (the sweep kernels fix its form at compile time and refuse other plans).
 The bond lists are checked against the reference's brickwork
 (`cpp/include/mipt/haar.hpp:233-243, 300-304`: pairs (2b + t mod 2, next)
-with the closing (n-1, 0) bond on a ring).
+with the closing (n-1, 0) bond on a ring).  Every sweep of the L=20 ring has
+the stage structure that the compile-time sweep kernels run.
 """
 import itertools
 
@@ -131,3 +132,39 @@ def test_layer_plan_refuses_bad_arguments(M):
         M.layer_plan(12, 12, True, 0, 0)             # fits one tile: the small kernel's case
     with pytest.raises(M.MiptHipError):
         M.layer_plan(20, 20, True, 3, 0)             # 3 does not divide 20
+
+
+def fixed_structure(ps):
+    """The stage structure the compile-time sweep kernels run (qubit-0 form z
+    of the last stage): three stages each applying bond[0]; stage 0 of a
+    form-0 sweep in the load mapping, every other stage after an exchange;
+    bond[1] in stages 0 and 1 (forms 0, 1) or 0 and 2 (form 2)."""
+    st = ps["stages"]
+    if len(st) != 3:
+        return False
+    z = st[2]["zpair"]
+    for si, x in enumerate(st):
+        lds = not (z == 0 and si == 0)
+        b1 = si != 1 if z == 2 else si != 2
+        if bool(x["use_lds"]) != lds or x["bonds"][0][0] < 0 or (x["bonds"][1][0] >= 0) != b1 or \
+                x["zpair"] != (z if si == 2 else 0):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("plan", range(4))
+def test_l20_ring_sweeps_have_the_compiled_stage_structure(M, plan):
+    # every sweep of the metric's workload (L=20 ring, both layer parities,
+    # every plan variant) runs on the kernels whose stage structure is fixed
+    # at compile time; a plan that lost it would fall back to the slower
+    # run-time stage loop without any other symptom
+    for layer in (0, 1):
+        p = M.layer_plan(20, 20, True, 0, layer, 13, 0, plan)
+        assert all(fixed_structure(ps) for ps in p["passes"]), (plan, layer)
+        # and the fused pairs: a layer's last sweep (form 0) continues into
+        # the next layer's first (odd layers run their windows in reverse)
+        nxt = M.layer_plan(20, 20, True, 0, layer + 1, 13, 0, plan)["passes"]
+        last = p["passes"][0 if layer & 1 else -1]
+        first = nxt[-1 if (layer + 1) & 1 else 0]
+        assert last["stages"][2]["zpair"] == 0 and first["stages"][2]["zpair"] in (1, 2)
+        assert last["s"] == first["s"]
