@@ -365,6 +365,19 @@ int mipt_hip_set_layer_fusion(int on);
  * nothing written) when out_len is smaller. */
 int mipt_hip_layer_plan(int L, int nq, int closed, int grid_x, int layer, int tile_bits, int threads, int plan,
                         int *out, int out_len, int *n_out);
+/* The sweep kernels compiled for a fixed stage structure (no counterpart in
+ * the reference; for tests and callers that check which plans get them, no
+ * GPU needed): out receives (zp, pat, pat2) per kernel.  zp 0..2: a sweep
+ * whose qubit-0 form (the zpair of its last stage) is zp and whose structure
+ * code is pat; zp 4..6: a fused launch, a form-0 sweep of code pat followed
+ * by the next layer's first sweep of form zp - 4 and code pat2.  The code of
+ * a plan of 1..3 stages: its stage count, plus per stage si the bits
+ * 1 << (2 + 3 si) (the stage starts with an exchange), 1 << (3 + 3 si)
+ * (bond[0] present), 1 << (4 + 3 si) (bond[1] present).  Used for 2^13-
+ * amplitude tiles of 256 threads with matrix-core bonds; other sweeps run a
+ * stage loop that reads the structure at run time.  n_out receives the count
+ * of ints; status -1 (nothing written) when out_len is smaller. */
+int mipt_hip_compiled_sweeps(int *out, int out_len, int *n_out);
 /* Tiled groups in flight: 2 (default) alternates groups between the batch
  * stream and a second stream so one group's small kernels overlap the
  * other's sweeps; 1 runs every group on the batch stream, so a sweep's

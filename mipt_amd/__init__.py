@@ -101,6 +101,7 @@ def lib():
     L.mipt_hip_batch_set_profiling.argtypes = [vp, i]
     L.mipt_hip_batch_set_streams.argtypes = [vp, i]
     L.mipt_hip_layer_plan.argtypes = [i, i, i, i, i, i, i, i, ip, i, ip]
+    L.mipt_hip_compiled_sweeps.argtypes = [ip, i, ip]
     L.mipt_hip_set_layer_fusion.argtypes = [i]
     L.mipt_hip_batch_pass_stats.argtypes = [vp, dp, ip, dp, ctypes.POINTER(ctypes.c_long), dp]
     L.mipt_hip_stream_copy.argtypes = [vp, vp, ctypes.c_uint64, vp]
@@ -359,6 +360,16 @@ def layer_plan(L: int, nq: int, closed: bool, grid_x: int, layer: int, tile_bits
                            "wave_u": take(4)})
         passes.append({"k": k, "c": c, "s": s, "stages": stages})
     return {"bonds": bonds, "passes": passes}
+
+
+def compiled_sweeps() -> list:
+    """The sweep kernels compiled for a fixed stage structure: [(zp, pat,
+    pat2)] (host only, no GPU)."""
+    buf = np.zeros(1024, dtype=np.int32)
+    n = ctypes.c_int()
+    _check(lib().mipt_hip_compiled_sweeps(_ip(buf), len(buf), ctypes.byref(n)), "compiled_sweeps")
+    v = buf[: n.value].tolist()
+    return [tuple(v[i:i + 3]) for i in range(0, len(v), 3)]
 
 
 def stream_copy_calibration(log2_elems: int = 25) -> int:

@@ -1260,25 +1260,44 @@ __device__ __forceinline__ std::uint32_t fresh_tid()
     return t;
 }
 
-// The stage structure every L=20 plan has at 2^13-amplitude tiles, by the
-// qubit-0 form z of the sweep: three stages, each applying bond[0]; stage 0
-// of a form-0 sweep works in the load mapping (no exchange), every other
-// stage starts with an exchange; bond[1] is applied in stages 0 and 1 (form
-// 0 and 1) or 0 and 2 (form 2); the bond of qubit 0 is stage 2's bond[0].
-// Sweep kernels with FIX run exactly this structure as straight-line code:
-// no run-time test of a stage's exchange or bonds, whose two outcomes would
-// leave the tile's registers paired differently (the bond on register bits
-// (3,4) pairs registers j, j|8 in its outputs, the one on (1,2) j, j|2) and
-// force copies back to one layout where they meet.
-__host__ __device__ constexpr bool fixed_lds(int z, int si) { return !(z == 0 && si == 0); }
-__host__ __device__ constexpr bool fixed_b1(int z, int si) { return z == 2 ? si != 1 : si != 2; }
+// Stage structure of a sweep plan as a compile-time code: bits 0-1 the
+// stage count (1..3); per stage si three bits from bit 2 + 3 si: an exchange
+// starts it (only stage 0 may work in the load mapping), bond[0] present,
+// bond[1] present.  The bond of qubit 0, when the sweep has one, is the last
+// stage's bond[0] (its form is the kernel's ZP).  Sweep kernels with a code
+// run exactly that structure as straight-line code: no run-time test of a
+// stage's exchange or bonds, whose two outcomes would leave the tile's
+// registers paired differently (the bond on register bits (3,4) pairs
+// registers j, j|8 in its outputs, the one on (1,2) j, j|2) and force copies
+// back to one layout where they meet.
+__host__ __device__ constexpr int pat_nst(int c) { return c & 3; }
+__host__ __device__ constexpr bool pat_lds(int c, int si) { return (c >> (2 + 3 * si)) & 1; }
+__host__ __device__ constexpr bool pat_b0(int c, int si) { return (c >> (3 + 3 * si)) & 1; }
+__host__ __device__ constexpr bool pat_b1(int c, int si) { return (c >> (4 + 3 * si)) & 1; }
+
+// The compiled-structure sweep kernels (matrix-core bonds, 2^13 tiles of 256
+// threads): zp 0..2 a sweep of that qubit-0 form with structure pat; zp 4..6
+// a fused launch, base structure pat (form 0), continuation pat2 of form
+// zp - 4.  They cover every sweep and fused pair of the L=20 ring and chain
+// plans and of the L=24 and L=26 rings (every plan variant); other sweeps
+// run the stage loop that reads the structure at run time.
+struct CompiledSweep
+{
+    int zp, pat, pat2;
+};
+constexpr CompiledSweep kCompiledSweeps[] = {
+    {0, 1019, 0}, {1, 1023, 0}, {2, 1919, 0}, {5, 1019, 1023}, {6, 1019, 1919},                  // L=20
+    {0, 250, 0},  {0, 254, 0},  {1, 254, 0},  {1, 891, 0},     {1, 1915, 0},   {2, 254, 0},  {2, 895, 0},  // L=24, 26
+    {4, 250, 254}, {6, 250, 254}, {6, 250, 895}, {4, 1019, 254},
+};
+constexpr int kNumCompiledSweeps = static_cast<int>(sizeof(kCompiledSweeps) / sizeof(kCompiledSweeps[0]));
 
 // Processes one tile whose amplitudes are in v (M0 mapping): the initial
 // state or the previous layer's collapse scale, the stages of the sweep plan,
 // back to M0, the store, and the FP64 histogram partial.  ZP 1 / 2: the last
 // stage holds the sweep's bond of qubit 0 in that form and the others none;
 // ZP 0: no stage has one; ZP 3: any stage may, in either form.
-template <int RB, int NT, int AUX, int MX, int ZP, int FIX>
+template <int RB, int NT, int AUX, int MX, int ZP, int PAT, int PAT2>
 __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 *__restrict__ gates, const TileCtx &cur,
                                              float2 (&v)[1 << RB], std::uint32_t jshift,
                                              const std::uint32_t (&Q3m0)[6])
@@ -1368,25 +1387,33 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     };
     // the stages of plan P, its qubit-0 form Z (the stage type of ztag) fixed
     // at compile time, gates glx, coefficient-table slots from stage slot0
-    auto run_stages = [&](const PassPlan &P, auto ztag, const float2 *glx, int slot0) {
+    auto run_stages = [&](const PassPlan &P, auto ztag, auto ptag, const float2 *glx, int slot0) {
         constexpr int Z = decltype(ztag)::value;
-        if constexpr (FIX != 0 && AUX == 0)
+        constexpr int C = decltype(ptag)::value;
+        if constexpr (C != 0 && AUX == 0)
         {
-            static_assert(Z <= 2 && MX != 0, "fixed stage structure: forms 0..2 on the matrix cores");
+            static_assert(Z <= 2 && MX != 0, "compiled stage structure: forms 0..2 on the matrix cores");
+            static_assert(pat_nst(C) >= 1 && (Z == 0 || pat_b0(C, pat_nst(C) - 1)), "qubit-0 bond is the last bond[0]");
             auto stage = [&](auto sidx) {
                 constexpr int si = decltype(sidx)::value;
-                constexpr int zf = si == 2 ? Z : 0;
-                const StagePlan &sp = P.st[si];
-                if constexpr (fixed_lds(Z, si)) exchange_to(sp);
-                else if (!tab_ready) __syncthreads();
-                tab_ready = true;
-                const float *tab = mxtab + (slot0 + si) * (2 * MX_SLOT_BYTES / 4);
-                const float2 *g0 = glx + 16 * sp.bond[0];
-                if constexpr (zf == 1) bond_on_regs<NV, 0, 1, MX>(v, g0, tab, lane);
-                else if constexpr (zf == 2) bond_on_regs<NV, 1, 0, MX>(v, g0, tab, lane);
-                else bond_on_regs<NV, 1, 2, MX>(v, g0, tab, lane);
-                if constexpr (fixed_b1(Z, si))
-                    bond_on_regs<NV, 3, 4, MX>(v, glx + 16 * sp.bond[1], tab + MX_SLOT_BYTES / 4, lane);
+                if constexpr (si < pat_nst(C))
+                {
+                    constexpr int zf = si == pat_nst(C) - 1 ? Z : 0;
+                    const StagePlan &sp = P.st[si];
+                    if constexpr (pat_lds(C, si)) exchange_to(sp);
+                    else if (!tab_ready) __syncthreads();
+                    tab_ready = true;
+                    const float *tab = mxtab + (slot0 + si) * (2 * MX_SLOT_BYTES / 4);
+                    if constexpr (pat_b0(C, si))
+                    {
+                        const float2 *g0 = glx + 16 * sp.bond[0];
+                        if constexpr (zf == 1) bond_on_regs<NV, 0, 1, MX>(v, g0, tab, lane);
+                        else if constexpr (zf == 2) bond_on_regs<NV, 1, 0, MX>(v, g0, tab, lane);
+                        else bond_on_regs<NV, 1, 2, MX>(v, g0, tab, lane);
+                    }
+                    if constexpr (pat_b1(C, si))
+                        bond_on_regs<NV, 3, 4, MX>(v, glx + 16 * sp.bond[1], tab + MX_SLOT_BYTES / 4, lane);
+                }
             };
             stage(std::integral_constant<int, 0>{});
             stage(std::integral_constant<int, 1>{});
@@ -1409,7 +1436,7 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
             apply_stage_gates<NV, MX, Z>(v, sp, glx, mxtab + (slot0 + si) * (2 * MX_SLOT_BYTES / 4), lane);
         }
     };
-    run_stages(pl, std::integral_constant<int, FUSED ? 0 : ZP>{}, gl, 0);
+    run_stages(pl, std::integral_constant<int, FUSED ? 0 : ZP>{}, std::integral_constant<int, PAT>{}, gl, 0);
 
     // back to the load mapping, the store, the histogram partial
     auto finish = [&]() {
@@ -1470,7 +1497,9 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
     // next: the next layer's first sweep (same window; its bonds touch none
     // of this layer's measured sites, whose collapse waits) follows on the
     // tile in registers and LDS, and the histogram is taken after it
-    if constexpr (FUSED) run_stages(args.next, std::integral_constant<int, ZP - 4>{}, gates + cur.goff2, pl.nstages);
+    if constexpr (FUSED)
+        run_stages(args.next, std::integral_constant<int, ZP - 4>{}, std::integral_constant<int, PAT2>{}, gates + cur.goff2,
+                   pl.nstages);
     finish();
 }
 
@@ -1481,7 +1510,7 @@ __device__ __forceinline__ void process_tile(const PassArgs &args, const float2 
 // One workgroup per tile; with the half-tile LDS buffer four 2^13-amplitude
 // workgroups share a CU (64-register tiles: one), whose loads overlap one
 // another's LDS stages and bond arithmetic.
-template <int RB, int NT, int AUX, int MX, int ZP, int FIX = 0>
+template <int RB, int NT, int AUX, int MX, int ZP, int PAT = 0, int PAT2 = 0>
 __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs args, const int *__restrict__ traj_list,
                                                                   const TrajDesc *__restrict__ desc,
                                                                   const std::uint32_t *__restrict__ flags,
@@ -1516,7 +1545,7 @@ __global__ void __launch_bounds__(NT, (RB == 6 ? 1 : 4)) pass_kernel(PassArgs ar
     if (id < total && tile_context(args, in.traj_list, in.desc, in.flags, in.coll, in.init, in.gates, id, AUX != 0, x))
     {
         tile_load<NV>(x, true, args.a.states, toff, jshift, v);
-        process_tile<RB, NT, AUX, MX, ZP, FIX>(args, in.gates, x, v, jshift, Q3m0);
+        process_tile<RB, NT, AUX, MX, ZP, PAT, PAT2>(args, in.gates, x, v, jshift, Q3m0);
     }
 }
 
@@ -2331,7 +2360,7 @@ struct Batch
         return OK;
     }
 
-    template <int RB, int NT, int AUX, int MX, int ZP, int FIX = 0>
+    template <int RB, int NT, int AUX, int MX, int ZP, int PAT = 0, int PAT2 = 0>
     int launch_pass(const PassPlan &pl, const PassPlan *next, const int *list, int ntraj, int layer)
     {
         PassArgs pa{al, list, ntraj, layer, pl, next ? *next : PassPlan{}};
@@ -2342,14 +2371,14 @@ struct Batch
         static bool attr_set = false;
         if (!attr_set)
         {
-            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX, ZP, FIX>),
+            MIPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pass_kernel<RB, NT, AUX, MX, ZP, PAT, PAT2>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                static_cast<int>((std::size_t{1} << 13) * sizeof(float2) +
                                                                 2 * MX_MAX_SLOTS * MX_SLOT_BYTES)));
             attr_set = true;
         }
         const unsigned grid = static_cast<unsigned>(static_cast<long>(ntraj) * pl.ntiles);
-        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX, ZP, FIX>), dim3(grid), dim3(NT), lds, ls, pa,
+        hipLaunchKernelGGL((pass_kernel<RB, NT, AUX, MX, ZP, PAT, PAT2>), dim3(grid), dim3(NT), lds, ls, pa,
                            static_cast<const int *>(list), static_cast<const TrajDesc *>(al.desc),
                            static_cast<const std::uint32_t *>(al.flags), static_cast<const Collapse *>(al.coll),
                            static_cast<const int *>(al.init), static_cast<const float2 *>(al.gates));
@@ -2378,19 +2407,38 @@ struct Batch
         return OK;
     }
 
-    // Whether sweep plan pl of qubit-0 form z has the stage structure the FIX
-    // kernels run (fixed_lds / fixed_b1).
-    static bool has_fixed_structure(const PassPlan &pl, int z)
+    // The stage-structure code of plan pl (pat_*), or 0 when it has none.
+    static int pattern_code(const PassPlan &pl)
     {
-        if (pl.nstages != 3) return false;
-        for (int si = 0; si < 3; ++si)
+        if (pl.nstages < 1 || pl.nstages > 3) return 0;
+        int c = pl.nstages;
+        for (int si = 0; si < pl.nstages; ++si)
         {
             const StagePlan &sp = pl.st[si];
-            if ((sp.use_lds != 0) != fixed_lds(z, si) || sp.bond[0] < 0 || (sp.bond[1] >= 0) != fixed_b1(z, si) ||
-                sp.zpair != (si == 2 ? z : 0))
-                return false;
+            if ((si > 0 && !sp.use_lds) || (sp.zpair && (si != pl.nstages - 1 || sp.bond[0] < 0))) return 0;
+            c |= ((sp.use_lds ? 1 : 0) << (2 + 3 * si)) | ((sp.bond[0] >= 0 ? 1 : 0) << (3 + 3 * si)) |
+                 ((sp.bond[1] >= 0 ? 1 : 0) << (4 + 3 * si));
         }
-        return true;
+        return c;
+    }
+
+    // Launches the compiled-structure kernel (matrix-core bonds) for (zp,
+    // pat, pat2) if it is one of kCompiledSweeps, else leaves found false.
+    template <int I = 0>
+    int launch_compiled(int zp, int pat, int pat2, const PassPlan &pl, const PassPlan *next, const int *list, int ntraj,
+                        int layer, bool &found)
+    {
+        if constexpr (I < kNumCompiledSweeps)
+        {
+            constexpr CompiledSweep k = kCompiledSweeps[I];
+            if (zp == k.zp && pat == k.pat && pat2 == k.pat2)
+            {
+                found = true;
+                return launch_pass<5, 256, 0, 2, k.zp, k.pat, k.pat2>(pl, next, list, ntraj, layer);
+            }
+            return launch_compiled<I + 1>(zp, pat, pat2, pl, next, list, ntraj, layer, found);
+        }
+        return OK;
     }
 
     // next (with pl.fuse): the next layer's first sweep, run in the same
@@ -2417,9 +2465,10 @@ struct Batch
                     return zn == 0   ? launch_pass<5, 256, 0, 0, 4>(pl, next, list, ntraj, layer)
                            : zn == 1 ? launch_pass<5, 256, 0, 0, 5>(pl, next, list, ntraj, layer)
                                      : launch_pass<5, 256, 0, 0, 6>(pl, next, list, ntraj, layer);
-                if (zn != 0 && has_fixed_structure(pl, 0) && has_fixed_structure(*next, zn))
-                    return zn == 1 ? launch_pass<5, 256, 0, 2, 5, 1>(pl, next, list, ntraj, layer)
-                                   : launch_pass<5, 256, 0, 2, 6, 1>(pl, next, list, ntraj, layer);
+                bool found = false;
+                MIPT_HIP_TRY(launch_compiled(4 + zn, pattern_code(pl), pattern_code(*next), pl, next, list, ntraj, layer,
+                                             found));
+                if (found) return OK;
                 return zn == 0   ? launch_pass<5, 256, 0, 2, 4>(pl, next, list, ntraj, layer)
                        : zn == 1 ? launch_pass<5, 256, 0, 2, 5>(pl, next, list, ntraj, layer)
                                  : launch_pass<5, 256, 0, 2, 6>(pl, next, list, ntraj, layer);
@@ -2428,10 +2477,12 @@ struct Batch
                 return zp == 0   ? launch_pass<5, 256, 0, 0, 0>(pl, nullptr, list, ntraj, layer)
                        : zp == 1 ? launch_pass<5, 256, 0, 0, 1>(pl, nullptr, list, ntraj, layer)
                                  : launch_pass<5, 256, 0, 0, 2>(pl, nullptr, list, ntraj, layer);
-            if (cf.mx == 2 && has_fixed_structure(pl, zp))
-                return zp == 0   ? launch_pass<5, 256, 0, 2, 0, 1>(pl, nullptr, list, ntraj, layer)
-                       : zp == 1 ? launch_pass<5, 256, 0, 2, 1, 1>(pl, nullptr, list, ntraj, layer)
-                                 : launch_pass<5, 256, 0, 2, 2, 1>(pl, nullptr, list, ntraj, layer);
+            if (cf.mx == 2)
+            {
+                bool found = false;
+                MIPT_HIP_TRY(launch_compiled(zp, pattern_code(pl), 0, pl, nullptr, list, ntraj, layer, found));
+                if (found) return OK;
+            }
             if (cf.mx == 2)
                 return zp == 0   ? launch_pass<5, 256, 0, 2, 0>(pl, nullptr, list, ntraj, layer)
                        : zp == 1 ? launch_pass<5, 256, 0, 2, 1>(pl, nullptr, list, ntraj, layer)
@@ -3330,6 +3381,23 @@ int mipt_hip_set_layer_fusion(int on)
         return E_ARG;
     }
     g_fuse.store(on);
+    return OK;
+}
+
+int mipt_hip_compiled_sweeps(int *out, int out_len, int *n_out)
+{
+    if (!out || !n_out || out_len < 3 * kNumCompiledSweeps)
+    {
+        set_error("compiled_sweeps: need room for %d ints", 3 * kNumCompiledSweeps);
+        return E_ARG;
+    }
+    for (int i = 0; i < kNumCompiledSweeps; ++i)
+    {
+        out[3 * i] = kCompiledSweeps[i].zp;
+        out[3 * i + 1] = kCompiledSweeps[i].pat;
+        out[3 * i + 2] = kCompiledSweeps[i].pat2;
+    }
+    *n_out = 3 * kNumCompiledSweeps;
     return OK;
 }
 

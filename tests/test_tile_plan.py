@@ -12,8 +12,8 @@ load/store mapping, and the one stage holding the bond of qubit 0 comes last
 (the sweep kernels fix its form at compile time and refuse other plans).
 The bond lists are checked against the reference's brickwork
 (`cpp/include/mipt/haar.hpp:233-243, 300-304`: pairs (2b + t mod 2, next)
-with the closing (n-1, 0) bond on a ring).  Every sweep of the L=20 ring has
-the stage structure that the compile-time sweep kernels run.
+with the closing (n-1, 0) bond on a ring).  Every sweep and fused pair of
+the benchmarked chains has a kernel compiled for its stage structure.
 """
 import itertools
 
@@ -134,37 +134,53 @@ def test_layer_plan_refuses_bad_arguments(M):
         M.layer_plan(20, 20, True, 3, 0)             # 3 does not divide 20
 
 
-def fixed_structure(ps):
-    """The stage structure the compile-time sweep kernels run (qubit-0 form z
-    of the last stage): three stages each applying bond[0]; stage 0 of a
-    form-0 sweep in the load mapping, every other stage after an exchange;
-    bond[1] in stages 0 and 1 (forms 0, 1) or 0 and 2 (form 2)."""
+
+def structure_code(ps):
+    """A sweep's stage-structure code as the compiled kernels key it: the
+    stage count, plus per stage si the bits 1 << (2 + 3 si) (exchange),
+    1 << (3 + 3 si) (bond[0]), 1 << (4 + 3 si) (bond[1]); None for a plan the
+    codes do not describe.  Returns (qubit-0 form, code)."""
     st = ps["stages"]
-    if len(st) != 3:
-        return False
-    z = st[2]["zpair"]
+    if not 1 <= len(st) <= 3:
+        return None
+    c = len(st)
     for si, x in enumerate(st):
-        lds = not (z == 0 and si == 0)
-        b1 = si != 1 if z == 2 else si != 2
-        if bool(x["use_lds"]) != lds or x["bonds"][0][0] < 0 or (x["bonds"][1][0] >= 0) != b1 or \
-                x["zpair"] != (z if si == 2 else 0):
-            return False
-    return True
+        if (si > 0 and not x["use_lds"]) or (x["zpair"] and (si != len(st) - 1 or x["bonds"][0][0] < 0)):
+            return None
+        c |= (int(bool(x["use_lds"])) << (2 + 3 * si)) | (int(x["bonds"][0][0] >= 0) << (3 + 3 * si)) | \
+             (int(x["bonds"][1][0] >= 0) << (4 + 3 * si))
+    return st[-1]["zpair"], c
 
 
-@pytest.mark.parametrize("plan", range(4))
-def test_l20_ring_sweeps_have_the_compiled_stage_structure(M, plan):
-    # every sweep of the metric's workload (L=20 ring, both layer parities,
-    # every plan variant) runs on the kernels whose stage structure is fixed
-    # at compile time; a plan that lost it would fall back to the slower
-    # run-time stage loop without any other symptom
-    for layer in (0, 1):
-        p = M.layer_plan(20, 20, True, 0, layer, 13, 0, plan)
-        assert all(fixed_structure(ps) for ps in p["passes"]), (plan, layer)
-        # and the fused pairs: a layer's last sweep (form 0) continues into
-        # the next layer's first (odd layers run their windows in reverse)
-        nxt = M.layer_plan(20, 20, True, 0, layer + 1, 13, 0, plan)["passes"]
-        last = p["passes"][0 if layer & 1 else -1]
-        first = nxt[-1 if (layer + 1) & 1 else 0]
-        assert last["stages"][2]["zpair"] == 0 and first["stages"][2]["zpair"] in (1, 2)
-        assert last["s"] == first["s"]
+@pytest.mark.parametrize("L,closed", [(20, True), (20, False), (24, True), (26, True)])
+def test_sweeps_of_the_benchmarked_chains_have_compiled_kernels(M, L, closed):
+    # every sweep of these chains (both layer parities, every plan variant)
+    # and every fused pair (a layer's last sweep, form 0, continuing into the
+    # next layer's first; odd layers run their windows in reverse) has a
+    # kernel compiled for its stage structure; a plan without one would fall
+    # back to the slower run-time stage loop with no other symptom
+    compiled = set(M.compiled_sweeps())
+    assert all(zp in (0, 1, 2, 4, 5, 6) for zp, _, _ in compiled)
+    for plan in range(4):
+        P = [M.layer_plan(L, L, closed, 0, layer, 13, 0, plan)["passes"] for layer in (0, 1)]
+        for layer in (0, 1):
+            for ps in P[layer]:
+                z, c = structure_code(ps)
+                assert (z, c, 0) in compiled, (plan, layer, ps["s"], z, c)
+            n = len(P[layer])
+            last = P[layer][0 if layer & 1 else n - 1]
+            nxt = P[1 - layer]
+            first = nxt[len(nxt) - 1 if (layer + 1) & 1 else 0]
+            assert last["s"] == first["s"] and first["stages"][0]["use_lds"]
+            (z0, c0), (z1, c1) = structure_code(last), structure_code(first)
+            assert z0 == 0 and (4 + z1, c0, c1) in compiled, (plan, layer, c0, z1, c1)
+
+
+def test_l20_structure_codes_are_the_three_known_ones(M):
+    # the codes of the metric's sweeps, written out: three stages each with
+    # bond[0]; form 0 starts in the load mapping, bond[1] in stages 0-1
+    # (forms 0, 1) or 0 and 2 (form 2)
+    codes = {structure_code(ps) for layer in (0, 1) for ps in M.layer_plan(20, 20, True, 0, layer)["passes"]}
+    assert codes == {(0, 3 | (0b110 << 2) | (0b111 << 5) | (0b011 << 8)),
+                     (1, 3 | (0b111 << 2) | (0b111 << 5) | (0b011 << 8)),
+                     (2, 3 | (0b111 << 2) | (0b011 << 5) | (0b111 << 8))}
