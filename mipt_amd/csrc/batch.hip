@@ -1279,8 +1279,9 @@ __host__ __device__ constexpr bool pat_b1(int c, int si) { return (c >> (4 + 3 *
 // threads): zp 0..2 a sweep of that qubit-0 form with structure pat; zp 4..6
 // a fused launch, base structure pat (form 0), continuation pat2 of form
 // zp - 4.  They cover every sweep and fused pair of the L=20 ring and chain
-// plans and of the L=24 and L=26 rings (every plan variant); other sweeps
-// run the stage loop that reads the structure at run time.
+// plans, of the L=24 and L=26 rings and of the 4 x 5 grid (every plan
+// variant); other sweeps run the stage loop that reads the structure at run
+// time.
 struct CompiledSweep
 {
     int zp, pat, pat2;
@@ -1289,6 +1290,7 @@ constexpr CompiledSweep kCompiledSweeps[] = {
     {0, 1019, 0}, {1, 1023, 0}, {2, 1919, 0}, {5, 1019, 1023}, {6, 1019, 1919},                  // L=20
     {0, 250, 0},  {0, 254, 0},  {1, 254, 0},  {1, 891, 0},     {1, 1915, 0},   {2, 254, 0},  {2, 895, 0},  // L=24, 26
     {4, 250, 254}, {6, 250, 254}, {6, 250, 895}, {4, 1019, 254},
+    {0, 29, 0},   {0, 242, 0},  {1, 110, 0},  {4, 242, 254},   {4, 1019, 29},  {5, 254, 1023},  // 4 x 5 grid
 };
 constexpr int kNumCompiledSweeps = static_cast<int>(sizeof(kCompiledSweeps) / sizeof(kCompiledSweeps[0]));
 
