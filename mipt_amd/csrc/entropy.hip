@@ -63,6 +63,24 @@ __global__ void gather_kernel(const C *__restrict__ psi, const std::uint64_t *__
     }
 }
 
+// Four consecutive K entries k0..k0+3 of row g of a plane (row length K)
+// into dst, entries past k1 or rows past m as 0: one 16-byte load when all
+// four are in range (K a power of two of at least 4 keeps it aligned).
+__device__ __forceinline__ void stage_k4(float *dst, const float *__restrict__ plane, int g, int m, int K, int k0, int k1)
+{
+    if (g < m && k0 + 3 < k1)
+    {
+        const float4 v = *reinterpret_cast<const float4 *>(plane + static_cast<std::size_t>(g) * K + k0);
+        dst[0] = v.x;
+        dst[1] = v.y;
+        dst[2] = v.z;
+        dst[3] = v.w;
+    }
+    else
+        for (int e = 0; e < 4; ++e)
+            dst[e] = (g < m && k0 + e < k1) ? plane[static_cast<std::size_t>(g) * K + k0 + e] : 0.f;
+}
+
 // Lower-triangle tile (bi >= bj) of rho over K range [k0, k1) of split `sp`;
 // FP64 partial written to part[sp][tile][64][64] as (re, im).
 __global__ void __launch_bounds__(256) gram_mfma_kernel(const float *__restrict__ ar, const float *__restrict__ ai,
@@ -98,15 +116,10 @@ __global__ void __launch_bounds__(256) gram_mfma_kernel(const float *__restrict_
     {
         // stage 64 rows x 16 k of the i-block and j-block
         const int gi = bi * GT + lrow, gj = bj * GT + lrow;
-        for (int e = 0; e < 4; ++e)
-        {
-            const int k = kk + lk + e;
-            const bool okk = k < k1;
-            sA_r[lrow][lk + e] = (gi < m && okk) ? ar[static_cast<std::size_t>(gi) * K + k] : 0.f;
-            sA_i[lrow][lk + e] = (gi < m && okk) ? ai[static_cast<std::size_t>(gi) * K + k] : 0.f;
-            sB_r[lrow][lk + e] = (gj < m && okk) ? ar[static_cast<std::size_t>(gj) * K + k] : 0.f;
-            sB_i[lrow][lk + e] = (gj < m && okk) ? ai[static_cast<std::size_t>(gj) * K + k] : 0.f;
-        }
+        stage_k4(&sA_r[lrow][lk], ar, gi, m, K, kk + lk, k1);
+        stage_k4(&sA_i[lrow][lk], ai, gi, m, K, kk + lk, k1);
+        stage_k4(&sB_r[lrow][lk], ar, gj, m, K, kk + lk, k1);
+        stage_k4(&sB_i[lrow][lk], ai, gj, m, K, kk + lk, k1);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4)
@@ -187,15 +200,10 @@ __global__ void __launch_bounds__(256) gram_mfma64_kernel(const float *__restric
     for (int kk = k0; kk < k1; kk += GK)
     {
         const int gi = bi * GT + lrow, gj = bj * GT + lrow;
-        for (int e = 0; e < 4; ++e)
-        {
-            const int k = kk + lk + e;
-            const bool okk = k < k1;
-            sA_r[lrow][lk + e] = (gi < m && okk) ? ar[static_cast<std::size_t>(gi) * K + k] : 0.f;
-            sA_i[lrow][lk + e] = (gi < m && okk) ? ai[static_cast<std::size_t>(gi) * K + k] : 0.f;
-            sB_r[lrow][lk + e] = (gj < m && okk) ? ar[static_cast<std::size_t>(gj) * K + k] : 0.f;
-            sB_i[lrow][lk + e] = (gj < m && okk) ? ai[static_cast<std::size_t>(gj) * K + k] : 0.f;
-        }
+        stage_k4(&sA_r[lrow][lk], ar, gi, m, K, kk + lk, k1);
+        stage_k4(&sA_i[lrow][lk], ai, gi, m, K, kk + lk, k1);
+        stage_k4(&sB_r[lrow][lk], ar, gj, m, K, kk + lk, k1);
+        stage_k4(&sB_i[lrow][lk], ai, gj, m, K, kk + lk, k1);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4)
@@ -312,15 +320,10 @@ __global__ void __launch_bounds__(256) gram_batch64_kernel(const float *__restri
     const int ga = bj * GT + lrow, gb = bi * GT + lrow;
     for (int kk = 0; kk < K; kk += GK)
     {
-        for (int e = 0; e < 4; ++e)
-        {
-            const int k = kk + lk + e;
-            const bool okk = k < K;
-            sA_r[lrow][lk + e] = (ga < m && okk) ? ar[static_cast<std::size_t>(ga) * K + k] : 0.f;
-            sA_i[lrow][lk + e] = (ga < m && okk) ? ai[static_cast<std::size_t>(ga) * K + k] : 0.f;
-            sB_r[lrow][lk + e] = (gb < m && okk) ? ar[static_cast<std::size_t>(gb) * K + k] : 0.f;
-            sB_i[lrow][lk + e] = (gb < m && okk) ? ai[static_cast<std::size_t>(gb) * K + k] : 0.f;
-        }
+        stage_k4(&sA_r[lrow][lk], ar, ga, m, K, kk + lk, K);
+        stage_k4(&sA_i[lrow][lk], ai, ga, m, K, kk + lk, K);
+        stage_k4(&sB_r[lrow][lk], ar, gb, m, K, kk + lk, K);
+        stage_k4(&sB_i[lrow][lk], ai, gb, m, K, kk + lk, K);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4)
