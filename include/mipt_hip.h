@@ -343,10 +343,10 @@ int mipt_hip_batch_export_layers(mipt_hip_batch *batch, int traj, double *gates_
                                  double *uniforms);
 /* Layer fusion (no counterpart in the reference; process-wide, default on,
  * else MIPT_HIP_FUSE=0/1): the sweeps of consecutive layers alternate their
- * window order, and a trajectory none of whose measured sites in a layer is
- * bonded in the next layer's first sweep runs that sweep in the same pass
- * over its state as this layer's last, the layer's collapse following at
- * the next layer's second sweep (2^13-amplitude tiles of 256 threads, no
+ * window order, and a trajectory that measures at most one site in a layer,
+ * not bonded in the next layer's first sweep, runs that sweep in the same
+ * pass over its state as this layer's last, the layer's collapse following
+ * at the next layer's second sweep (2^13-amplitude tiles of 256 threads, no
  * reference attached during the evolution).  Results agree with the unfused
  * schedule to rounding (commuting bonds and projectors meet in another
  * order).  on = -1 goes back to

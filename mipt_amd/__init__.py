@@ -320,9 +320,9 @@ class PlainEngine:
 
 def set_layer_fusion(on) -> None:
     """Process-wide: run a layer's last sweep and the next layer's first in
-    one launch for the trajectories whose measured sites the latter does not
-    bond (True / 1, the default), or not (False / 0); None or -1 goes back to
-    MIPT_HIP_FUSE / the default."""
+    one launch for the trajectories that measure at most one site in the
+    layer, not bonded by the latter (True / 1, the default), or not (False /
+    0); None or -1 goes back to MIPT_HIP_FUSE / the default."""
     if on is None:
         on = -1
     _check(lib().mipt_hip_set_layer_fusion(int(on) if isinstance(on, bool) else on), "set_layer_fusion")

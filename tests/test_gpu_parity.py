@@ -189,8 +189,8 @@ def _oracle_closed(args):
 @pytest.mark.timeout(300)
 @pytest.mark.parametrize("plan", [0, 1, 2, 3])
 def test_layer_fusion_on_and_off_match_oracle(M, plan):
-    # With layer fusion a trajectory none of whose measured sites in layer t
-    # is bonded in layer t + 1's first sweep runs that sweep in the same
+    # With layer fusion a trajectory that measures at most one site in layer
+    # t, not bonded in layer t + 1's first sweep, runs that sweep in the same
     # launch as the last sweep of layer t (its histogram taken after it, its
     # collapse applied at layer t + 1's second sweep), and the sweeps of odd
     # layers run in reverse window order.  Both settings against the oracle
