@@ -2537,12 +2537,12 @@ struct Batch
             attach_any |= desc[i].probes > 0 && (desc[i].attach > 0 || (desc[i].probes > 1 && desc[i].stagger > 0));
         // layer fusion (2^13 tiles of 256 threads, no reference attached
         // mid-evolution): the layers' sweeps alternate their window order so
-        // that consecutive layers meet in one window; a trajectory whose
-        // measured sites in layer t are not bonded in layer t+1's first sweep
-        // (fuses(), quiet ones among them) runs that sweep in the same launch
-        // as layer t's last (one pass over its state fewer), and its collapse
-        // of layer t at layer t+1's second sweep; the others run the two
-        // sweeps as usual.  Each launch gets a compacted list
+        // that consecutive layers meet in one window; a trajectory that
+        // measures at most FUSE_MAX_MEASURED sites in layer t, none bonded in
+        // layer t+1's first sweep (fuses()), runs that sweep in the same
+        // launch as layer t's last (one pass over its state fewer), and its
+        // collapse of layer t at layer t+1's second sweep; the others run
+        // the two sweeps as usual.  Each launch gets a compacted list
         // (fuse_order_kernel), so no workgroup idles.
         bool fuse_env = false;
         MIPT_HIP_TRY(fuse_layers(fuse_env));
