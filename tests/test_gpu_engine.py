@@ -414,8 +414,10 @@ def test_batch_probe_attach_after_scrambling(M):
 def test_batch_grid_matches_oracle(M):
     # 2D grids with the four-colour layering: 4 rows x 5 columns (20 qubits,
     # tile sweeps with windows chosen to cover the vertical bonds), 3 x 4
-    # (small kernel) and 5 x 4 with closed rows
-    cases = [(5, 4, 8, 0.12, 2001, False), (4, 3, 8, 0.2, 2002, False), (4, 5, 6, 0.3, 2003, True)]
+    # (small kernel) and 5 x 4 with closed rows; at p = 0.03 most layers of
+    # the 4 x 5 grid fuse into the next (its compiled fused sweeps)
+    cases = [(5, 4, 8, 0.12, 2001, False), (4, 3, 8, 0.2, 2002, False), (4, 5, 6, 0.3, 2003, True),
+             (5, 4, 12, 0.03, 2004, False)]
     specs = [M.Spec(gx * gy, layers, p, seed, closed, grid_x=gx) for (gx, gy, layers, p, seed, closed) in cases]
     print("batch 2D grids", flush=True)
     b = M.Batch(specs)
