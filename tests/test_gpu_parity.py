@@ -304,6 +304,30 @@ def test_metric_workload_full_depth_32_trajectories(M):
 
 
 @pytest.mark.timeout(600)
+def test_fused_three_sweep_rings_match_oracle(M):
+    # three sweeps per layer with most (L=24, p = 0.02) or all (L=26, p = 0)
+    # layers fused into the next: the compiled fused sweeps of these rings,
+    # the deferred collapse at the middle sweep, against the oracle
+    cases = [(24, 6, 0.02, 3501, True), (26, 4, 0.0, 3502, True)]
+    print(f"fused three-sweep rings: {cases}", flush=True)
+    specs = [M.Spec(n, layers, p, seed, closed) for (n, layers, p, seed, closed) in cases]
+    batch = M.Batch(specs)
+    try:
+        batch.generate()
+        batch.evolve()
+        batch.synchronize()
+        got = [(batch.outcomes(i), batch.get_state(i)) for i in range(len(cases))]
+    finally:
+        batch.close()
+    for i, case in enumerate(cases):
+        psi, out, p1, f, u = _oracle_closed(case)
+        (o, flagged, gp1, _), st = got[i]
+        near = check_outcomes(o, out, p1, u, f, f"L={case[0]}")
+        if not near:
+            assert np.abs(st - psi).max() < 1e-5, f"L={case[0]}"
+
+
+@pytest.mark.timeout(600)
 def test_l26_evolution_matches_oracle(M):
     # L=26 (2^26 amplitudes, 512 MiB complex64 on the device, 1 GiB complex128
     # in the oracle), four layers at p = 0.16 on the device-drawn stream
