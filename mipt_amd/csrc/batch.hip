@@ -2546,7 +2546,11 @@ struct Batch
         // (fuse_order_kernel), so no workgroup idles.
         bool fuse_env = false;
         MIPT_HIP_TRY(fuse_layers(fuse_env));
-        const bool fuse_on = fuse_env && cf.rb == 5 && cf.nt == 256 && cf.mx != 1 && !attach_any && t1 - t0 > 1;
+        // (the order kernel keeps a byte per trajectory of the group in LDS:
+        // groups beyond 64 KiB of them, possible only with a raised
+        // MIPT_HIP_GROUP_MIB, run unfused)
+        const bool fuse_on = fuse_env && cf.rb == 5 && cf.nt == 256 && cf.mx != 1 && !attach_any && t1 - t0 > 1 &&
+                             ntraj <= 65536;
         FuseRule rule{};
         rule.period = P;
         rule.t1 = t1;
