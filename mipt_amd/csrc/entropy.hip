@@ -292,11 +292,15 @@ __global__ void gather_batch_kernel(const float2 *const *__restrict__ psis, cons
 // covers 16 consecutive columns).  The eigensolver reads that triangle as
 // the lower triangle of the column-major conj(rho), which has the same
 // spectrum.  Exact products of the complex64 inputs, FP64 sums, real diagonal.
+template <int TS>
 __global__ void __launch_bounds__(256) gram_batch64_kernel(const float *__restrict__ ar_all,
                                                            const float *__restrict__ ai_all, int m, int K,
                                                            int ntiles_tri, double2 *__restrict__ rho_all)
 {
-    __shared__ float sA_r[GT][GK + 1], sA_i[GT][GK + 1], sB_r[GT][GK + 1], sB_i[GT][GK + 1];
+    // TS x TS output tiles (64, or 32 for RDMs of at most 32 rows); each wave
+    // takes a quadrant of QS x QS in 16 x 16 MFMA blocks
+    constexpr int QS = TS / 2, NX = QS / 16;
+    __shared__ float sA_r[TS][GK + 1], sA_i[TS][GK + 1], sB_r[TS][GK + 1], sB_i[TS][GK + 1];
     const int tile = blockIdx.x;
     const std::size_t item = blockIdx.y;
     const float *ar = ar_all + item * static_cast<std::size_t>(m) * K;
@@ -308,36 +312,39 @@ __global__ void __launch_bounds__(256) gram_batch64_kernel(const float *__restri
     const int bj = tile - bi * (bi + 1) / 2;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wi = wave >> 1, wj = wave & 1;
-    doublex4 accr[2][2], acci[2][2];
-    for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y)
+    doublex4 accr[NX][NX], acci[NX][NX];
+    for (int x = 0; x < NX; ++x)
+        for (int y = 0; y < NX; ++y)
         {
             accr[x][y] = doublex4{0, 0, 0, 0};
             acci[x][y] = doublex4{0, 0, 0, 0};
         }
     const int lrow = threadIdx.x >> 2;
     const int lk = (threadIdx.x & 3) * 4;
-    const int ga = bj * GT + lrow, gb = bi * GT + lrow;
+    const int ga = bj * TS + lrow, gb = bi * TS + lrow;
     for (int kk = 0; kk < K; kk += GK)
     {
-        stage_k4(&sA_r[lrow][lk], ar, ga, m, K, kk + lk, K);
-        stage_k4(&sA_i[lrow][lk], ai, ga, m, K, kk + lk, K);
-        stage_k4(&sB_r[lrow][lk], ar, gb, m, K, kk + lk, K);
-        stage_k4(&sB_i[lrow][lk], ai, gb, m, K, kk + lk, K);
+        if (lrow < TS)
+        {
+            stage_k4(&sA_r[lrow][lk], ar, ga, m, K, kk + lk, K);
+            stage_k4(&sA_i[lrow][lk], ai, ga, m, K, kk + lk, K);
+            stage_k4(&sB_r[lrow][lk], ar, gb, m, K, kk + lk, K);
+            stage_k4(&sB_i[lrow][lk], ai, gb, m, K, kk + lk, K);
+        }
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4)
         {
             const int kq = ks + (lane >> 4);
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+            for (int x = 0; x < NX; ++x)
             {
-                const int ra = wi * 32 + x * 16 + (lane & 15);
+                const int ra = wi * QS + x * 16 + (lane & 15);
                 const double a_r = sA_r[ra][kq], a_i = sA_i[ra][kq];
 #pragma unroll
-                for (int y = 0; y < 2; ++y)
+                for (int y = 0; y < NX; ++y)
                 {
-                    const int rb = wj * 32 + y * 16 + (lane & 15);
+                    const int rb = wj * QS + y * 16 + (lane & 15);
                     const double b_r = sB_r[rb][kq], b_i = sB_i[rb][kq];
                     accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, b_r, accr[x][y], 0, 0, 0);
                     accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, b_i, accr[x][y], 0, 0, 0);
@@ -348,12 +355,12 @@ __global__ void __launch_bounds__(256) gram_batch64_kernel(const float *__restri
         }
         __syncthreads();
     }
-    for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y)
+    for (int x = 0; x < NX; ++x)
+        for (int y = 0; y < NX; ++y)
             for (int e = 0; e < 4; ++e)
             {
-                const int a = bj * GT + wi * 32 + x * 16 + (lane >> 4) + 4 * e;
-                const int b = bi * GT + wj * 32 + y * 16 + (lane & 15);
+                const int a = bj * TS + wi * QS + x * 16 + (lane >> 4) + 4 * e;
+                const int b = bi * TS + wj * QS + y * 16 + (lane & 15);
                 if (a < m && b < m && a <= b)
                     rho[static_cast<std::size_t>(a) * m + b] = make_double2(accr[x][y][e], a == b ? 0.0 : acci[x][y][e]);
             }
@@ -891,12 +898,19 @@ int build_rho_batch(Workspace &w, const float2 *const *psis, int n, const int *k
     hipLaunchKernelGGL(gather_batch_kernel, dim3(gblocks), dim3(256), 0, stream,
                        reinterpret_cast<const float2 *const *>(w.psib), w.rowb, w.emb, kept_count, env, count, w.ar, w.ai);
     MIPT_HIP_CHECK(hipGetLastError());
-    const int nb = (m + GT - 1) / GT;
+    // RDMs of at most 32 rows take 32 x 32 tiles (a 64 x 64 tile would be
+    // three quarters padding)
+    const int ts = m <= 32 ? 32 : GT;
+    const int nb = (m + ts - 1) / ts;
     const int ntri = nb * (nb + 1) / 2;
     int slot = -1;
     MIPT_HIP_TRY(gram_timing_begin(stream, 4.0 * m * (m + 1.0) * static_cast<double>(K) * count, slot));
-    hipLaunchKernelGGL(gram_batch64_kernel, dim3(static_cast<unsigned>(ntri), static_cast<unsigned>(count)), dim3(256), 0,
-                       stream, w.ar, w.ai, m, static_cast<int>(K), ntri, dst);
+    if (ts == 32)
+        hipLaunchKernelGGL(gram_batch64_kernel<32>, dim3(static_cast<unsigned>(ntri), static_cast<unsigned>(count)),
+                           dim3(256), 0, stream, w.ar, w.ai, m, static_cast<int>(K), ntri, dst);
+    else
+        hipLaunchKernelGGL(gram_batch64_kernel<GT>, dim3(static_cast<unsigned>(ntri), static_cast<unsigned>(count)),
+                           dim3(256), 0, stream, w.ar, w.ai, m, static_cast<int>(K), ntri, dst);
     MIPT_HIP_CHECK(hipGetLastError());
     MIPT_HIP_TRY(gram_timing_end(stream, slot));
     return OK;
