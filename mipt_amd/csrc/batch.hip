@@ -961,14 +961,15 @@ __device__ __forceinline__ void tile_histogram(const float2 (&v)[NV], std::uint3
             for (int j = 0; j < NV; ++j)
                 if (!(j & (1 << r))) w[j] += w[j | (1 << r)];
         }
-    // unmeasured lane bits: butterfly over the surviving registers
+    // unmeasured lane bits: butterfly over the surviving registers (one
+    // uniform test per register, then its exchanges in increasing lane bit)
 #pragma unroll
-    for (int b = 0; b < 6; ++b)
-        if (!((ml >> b) & 1u))
+    for (int j = 0; j < NV; ++j)
+        if ((static_cast<std::uint32_t>(j) & ~mr) == 0u)
         {
 #pragma unroll
-            for (int j = 0; j < NV; ++j)
-                if ((static_cast<std::uint32_t>(j) & ~mr) == 0u) w[j] += __shfl_xor(w[j], 1 << b);
+            for (int b = 0; b < 6; ++b)
+                if (!((ml >> b) & 1u)) w[j] += __shfl_xor(w[j], 1 << b);
         }
     const int nlow = __popc(loc_u & nw_mask);
     const std::uint32_t stride = 1u << nlow;
